@@ -1,0 +1,152 @@
+"""Golden cases shared by make_goldens.py (writes fixtures by running the
+reference ompVC built from /root/reference in the authoring container) and the
+tests (regenerate the same inputs, run the oracle or the GPU driver, compare).
+
+Inputs are regenerated deterministically from the product's generators; their
+sha256 digests are recorded in each case's manifest so a generator change is
+caught before any comparison.
+"""
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(REPO, "flowvc-1_amd"))
+
+from fvcpy import deck, host  # noqa: E402
+
+DG = dict(Dimensions=2, Data_MeshType=0, Data_InFilePrefix="dg", Data_TRes=21, Data_TDelta=0.5, Data_TMin=0.0,
+          Data_TPeriodic=0, Data_MeshBounds_XMin=0.0, Data_MeshBounds_XMax=2.0, Data_MeshBounds_YMin=0.0,
+          Data_MeshBounds_YMax=1.0, Data_MeshBounds_ZMin=0.0, Data_MeshBounds_ZMax=0.0, Int_Type=1,
+          Int_TimeStep=0.01)
+PIPE = dict(Dimensions=3, Data_MeshType=1, Data_InFilePrefix="pipe", Data_TRes=51, Data_TDelta=0.02, Data_TMin=0.0,
+            Data_TPeriodic=1, Data_MeshBounds_XMin=-1.0, Data_MeshBounds_XMax=1.0, Data_MeshBounds_YMin=-1.0,
+            Data_MeshBounds_YMax=1.0, Data_MeshBounds_ZMin=0.0, Data_MeshBounds_ZMax=10.0, Int_Type=1,
+            Int_TimeStep=0.001, LocalSearchChecking=1)
+
+
+def gen_dg(d):
+    """Double gyre A=0.1, eps=0.25, omega=2 pi/10 on 64x32 nodes, 21 slices t=0..10."""
+    host.write_double_gyre(d, "dg", 64, 32, 21, 0.5)
+
+
+PIPE_DIMS = (12, 12, 40)
+
+
+def gen_pipe(d):
+    """Kuhn-split pipe R=1 L=10 on 12x12x40 voxels; 50 noisy slices (sigma 0.01, seed 1234)."""
+    t = host.pipe(*PIPE_DIMS, 1.0, 10.0)
+    xyz, conn, adj = host.tube_mesh(t)
+    host.write_unstruct_mesh(d, "pipe", xyz, conn, adj)
+    for s in range(50):
+        host.write_vel_slice(os.path.join(d, "pipe_vel.%d.bin" % s), s * 0.02,
+                             host.tube_velocity(t, xyz, s * 0.02, 0.01, 1234, s))
+    host.write_binary_list(os.path.join(d, "seeds.bin"), host.tube_seeds(t, 1000, 0.0, 10.0 / 3.0, 42))
+    # location probes: interior, face, edge, vertex, outside and far-apart points
+    pts = list(host.tube_seeds(t, 40, 0.0, 10.0, 7))
+    dx, dz = 2.0 / PIPE_DIMS[0], 10.0 / PIPE_DIMS[2]
+    for i in range(2, 10):
+        x = -1.0 + i * dx
+        pts.append([x, x, 3 * dz])                      # on Kuhn diagonal faces (x=y)
+        pts.append([x, -1.0 + 5 * dx, 7 * dz + 0.3 * dz])  # on voxel face planes
+        pts.append([x, -1.0 + 6 * dx, 11 * dz])          # on voxel edges
+        pts.append([-1.0 + 5 * dx, -1.0 + 6 * dx, i * dz])  # on mesh vertices
+    pts += [[0.0, 0.0, -0.5], [1.5, 0.0, 5.0], [0.0, 0.0, 10.5], [0.99, 0.99, 2.0], [0.0, 0.0, 0.0],
+            [0.0, 0.0, 10.0], [-1.0 + 6 * dx, 0.0, 0.0]]
+    with open(os.path.join(d, "probes.txt"), "w") as f:
+        f.write("%d\n" % len(pts))
+        for p in pts:
+            f.write("%.17g %.17g %.17g\n" % tuple(p))
+
+
+CASES = {
+    # 2-D Cartesian FTLE over T=10 (the CPU-runnable workload's physics, 128x64 grid)
+    "dg_ftle": dict(gen=gen_dg, outputs=["dgftle.0.bin", "dgftle_noT.0.bin", "dgftle_Cartesian.bin"],
+                    deck=dict(DG, Output_TStart=0.0, Output_TRes=1, Output_TDelta=1.0, Int_TimeDirection=1,
+                              FTLE_Compute=1, FTLE_MeshBounds_XMin=0.0, FTLE_MeshBounds_XMax=2.0,
+                              FTLE_MeshBounds_YMin=0.0, FTLE_MeshBounds_YMax=1.0, FTLE_MeshBounds_XRes=128,
+                              FTLE_MeshBounds_YRes=64, FTLE_MeshBounds_ZRes=1, FTLE_IntTLength=10.0,
+                              FTLE_OutFilePrefix="dgftle", FTLE_ICFile="dgftle_ic.bin")),
+    # backward-time FTLE, 3 slides, 64x32 grid
+    "dg_ftle_back": dict(gen=gen_dg, outputs=["dgb.0.bin", "dgb.1.bin", "dgb.2.bin", "dgb_noT.0.bin",
+                                              "dgb_noT.1.bin", "dgb_noT.2.bin", "dgb_Cartesian.bin"],
+                         deck=dict(DG, Output_TStart=10.0, Output_TRes=3, Output_TDelta=1.0, Int_TimeDirection=-1,
+                                   FTLE_Compute=1, FTLE_MeshBounds_XMin=0.0, FTLE_MeshBounds_XMax=2.0,
+                                   FTLE_MeshBounds_YMin=0.0, FTLE_MeshBounds_YMax=1.0, FTLE_MeshBounds_XRes=64,
+                                   FTLE_MeshBounds_YRes=32, FTLE_MeshBounds_ZRes=1, FTLE_IntTLength=4.0,
+                                   FTLE_OutFilePrefix="dgb", FTLE_ICFile="dgb_ic.bin")),
+    # 2-D Cartesian tracers from a generated 64x32 grid, 6 outputs.  The reference
+    # leaves LeftDomainTime of generated Cartesian tracers uninitialised
+    # (tracers.c:1340-1532); 2048 records (160 KB) come from a fresh zeroed
+    # allocation, which the restatement reproduces by writing 0.
+    "dg_tracers": dict(gen=gen_dg, outputs=["dgtr.%d.bin" % k for k in range(6)],
+                       deck=dict(DG, Output_TStart=0.0, Output_TRes=6, Output_TDelta=2.0, Int_TimeDirection=1,
+                                 Trace_Compute=1, Trace_GenerateMesh=1, Trace_MeshBounds_XMin=0.0,
+                                 Trace_MeshBounds_XMax=2.0, Trace_MeshBounds_YMin=0.0, Trace_MeshBounds_YMax=1.0,
+                                 Trace_MeshBounds_XRes=64, Trace_MeshBounds_YRes=32, Trace_MeshBounds_ZRes=1,
+                                 Trace_IntTLength=10.0, Trace_OutFilePrefix="dgtr")),
+    # 3-D unstructured tracers: 1000 seeds, one period, 11 outputs, exits
+    "pipe_tracers": dict(gen=gen_pipe, outputs=["ptr.%d.bin" % k for k in range(11)], ic="ptr.IC",
+                         deck=dict(PIPE, Output_TStart=0.0, Output_TRes=11, Output_TDelta=0.1, Int_TimeDirection=1,
+                                   Trace_Compute=1, Trace_GenerateMesh=0, Trace_InFile="seeds.bin",
+                                   Trace_InFileFormat=4, Trace_IntTLength=1.0, Trace_OutFilePrefix="ptr")),
+    # point location chain on special points (faces, edges, vertices, outside), no integration
+    "pipe_locate": dict(gen=gen_pipe, outputs=[], ic="ploc.IC",
+                        deck=dict(PIPE, Output_TStart=0.0, Output_TRes=1, Output_TDelta=0.1, Int_TimeDirection=1,
+                                  Trace_Compute=1, Trace_GenerateMesh=0, Trace_InFile="probes.txt",
+                                  Trace_InFileFormat=1, Trace_IntTLength=0.0, Trace_OutFilePrefix="ploc")),
+    # 3-D unstructured FTLE with outside masks and early exits, 2 slides
+    "pipe_ftle": dict(gen=gen_pipe, outputs=["pft.0.bin", "pft.1.bin", "pft_noT.0.bin", "pft_noT.1.bin",
+                                             "pft_Cartesian.bin"],
+                      deck=dict(PIPE, Output_TStart=0.0, Output_TRes=2, Output_TDelta=0.1, Int_TimeDirection=1,
+                                FTLE_Compute=1, FTLE_MeshBounds_XMin=-1.0, FTLE_MeshBounds_XMax=1.0,
+                                FTLE_MeshBounds_YMin=-1.0, FTLE_MeshBounds_YMax=1.0, FTLE_MeshBounds_ZMin=0.0,
+                                FTLE_MeshBounds_ZMax=10.0, FTLE_MeshBounds_XRes=12, FTLE_MeshBounds_YRes=12,
+                                FTLE_MeshBounds_ZRes=30, FTLE_IntTLength=0.5, FTLE_OutFilePrefix="pft",
+                                FTLE_ICFile="pft_ic.bin")),
+}
+
+# loop choice of ompVC depends on the host processor count (parameters.c:398-421);
+# the fixtures were written on an 8-processor container
+GOLDEN_NPROC = 8
+
+
+def sha256(path):
+    h = hashlib.sha256()
+    with open(path, "rb") as f:
+        for chunk in iter(lambda: f.read(1 << 20), b""):
+            h.update(chunk)
+    return h.hexdigest()
+
+
+def prepare(case, workdir):
+    """Generate inputs for `case` under workdir/data and return (deck_path, out_dir)."""
+    c = CASES[case]
+    data = os.path.join(workdir, "data")
+    out = os.path.join(workdir, "out")
+    os.makedirs(data, exist_ok=True)
+    os.makedirs(out, exist_ok=True)
+    c["gen"](data)
+    dk = dict(c["deck"], Path_Data=data + "/", Path_Output=out + "/")
+    deck_path = os.path.join(workdir, case + ".in")
+    deck.write_deck(deck_path, **dk)
+    return deck_path, data, out
+
+
+def input_digests(data):
+    return {f: sha256(os.path.join(data, f)) for f in sorted(os.listdir(data))
+            if not f.endswith(".IC") and not f.endswith("_ic.bin") and "_slide" not in f and "_OUT" not in f
+            and not f.startswith("pft.") and not f.startswith("dgftle.") and not f.startswith("dgb.")}
+
+
+def fixture_dir(case):
+    return os.path.join(HERE, case)
+
+
+def load_manifest(case):
+    with open(os.path.join(fixture_dir(case), "manifest.json")) as f:
+        return json.load(f)
