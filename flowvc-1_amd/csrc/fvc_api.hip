@@ -1,0 +1,877 @@
+/*
+ * fvc_api.hip - the C ABI of include/fvc.h.
+ *
+ * Device layout (per context):
+ *   X64  double4[num_nodes]  node coordinates (FP64, exact walk decisions)
+ *   X32  float4[num_nodes]   node coordinates minus the mesh centre (FP32 path)
+ *   conn int4[num_elems], adj int4[num_elems]
+ *   spatial hash: uniform cells over the padded node bounding box, each cell
+ *     listing (ascending) the elements whose padded bounding box meets it
+ *   slices: one buffer per resident slot; double4 / float4 / 4 x 16-bit per node
+ * Particles are SoA (X,Y,Z,V*,elem,left,left_time,...) for coalesced state I/O.
+ */
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <vector>
+
+#include "fvc_internal.h"
+
+using namespace fvc;
+
+/* ------------------------------------------------------------------------ */
+/* helpers                                                                   */
+/* ------------------------------------------------------------------------ */
+namespace fvc {
+
+int check_err(fvc_ctx *c, const char *what) {
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        c->last_error = std::string(what) + ": " + hipGetErrorString(e);
+        return FVC_E_HIP;
+    }
+    int code = 0;
+    int64_t where = -1;
+    if (hipMemcpy(&code, c->d_err, sizeof code, hipMemcpyDeviceToHost) != hipSuccess) return FVC_E_HIP;
+    if (code) {
+        hipMemcpy(&where, c->d_where, sizeof where, hipMemcpyDeviceToHost);
+        int zero = 0;
+        hipMemcpy(c->d_err, &zero, sizeof zero, hipMemcpyHostToDevice);
+        char buf[256];
+        snprintf(buf, sizeof buf, "%s: %s at point %lld", what, fvc_strerror(code), (long long)where);
+        c->last_error = buf;
+    }
+    return code;
+}
+
+void *scratch(fvc_ctx *c, size_t bytes) {
+    if (bytes <= c->scratch_bytes) return c->scratch;
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return nullptr;
+    if (c->scratch) hipFree(c->scratch);
+    c->scratch = nullptr;
+    size_t nb = std::max(bytes, (size_t)1 << 20);
+    if (hipMalloc(&c->scratch, nb) != hipSuccess) {
+        c->scratch_bytes = 0;
+        return nullptr;
+    }
+    c->scratch_bytes = nb;
+    return c->scratch;
+}
+
+DevField bind_field(fvc_ctx *c, int lo, int hi, double tmin, double tmax) {
+    DevField f;
+    f.lo = c->slices[lo];
+    f.hi = c->slices[hi];
+    f.scale_lo = c->slice_scale[lo];
+    f.scale_hi = c->slice_scale[hi];
+    f.tmin = tmin;
+    f.tmax = tmax;
+    return f;
+}
+
+} // namespace fvc
+
+static int set_err(fvc_ctx *c, int code, const char *msg) {
+    if (c) c->last_error = msg;
+    return code;
+}
+
+extern "C" const char *fvc_strerror(int code) {
+    switch (code) {
+    case FVC_OK: return "ok";
+    case FVC_E_INVALID: return "invalid argument";
+    case FVC_E_NOMEM: return "out of memory";
+    case FVC_E_HIP: return "HIP runtime error";
+    case FVC_E_INDETERMINATE_NEIGHBOR: return "Indeterminate neighbor";
+    case FVC_E_TLOC_RANGE: return "tloc must be between 0 and 1";
+    case FVC_E_ELEM_NEG1: return "Attempting to interpolate velocity at point with Element_Index = -1";
+    case FVC_E_WALK_LIMIT: return "walk exceeded its hop limit";
+    case FVC_E_EIG: return "Too many iterations in eigenvalue solver";
+    case FVC_E_ADMISSION: return "request does not fit free device memory";
+    case FVC_E_STATE: return "call out of sequence";
+    case FVC_E_LEFT: return "Attempting to integrate point that already left domain";
+    case FVC_E_NODEVICE: return "no usable GPU device";
+    default: return "unknown error";
+    }
+}
+
+extern "C" const char *fvc_last_error(const fvc_ctx *c) { return c ? c->last_error.c_str() : "no context"; }
+
+template <class T>
+static int dev_alloc(fvc_ctx *c, T **p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    if (hipMalloc((void **)p, sizeof(T) * count) != hipSuccess) {
+        c->last_error = "hipMalloc failed";
+        return FVC_E_NOMEM;
+    }
+    c->bytes_used += (int64_t)(sizeof(T) * count);
+    return FVC_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* spatial hash (host build, ascending element lists)                        */
+/* ------------------------------------------------------------------------ */
+static int build_hash(fvc_ctx *c, const fvc_mesh_desc *md) {
+    const int dims = md->dims;
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int64_t n = 0; n < md->num_nodes; n++)
+        for (int a = 0; a < 3; a++) {
+            lo[a] = std::min(lo[a], md->coords[3 * n + a]);
+            hi[a] = std::max(hi[a], md->coords[3 * n + a]);
+        }
+    double ext = 0.0;
+    for (int a = 0; a < dims; a++) ext = std::max(ext, hi[a] - lo[a]);
+    const double pad = 1e-6 * ext + 1e-12;
+    for (int a = 0; a < 3; a++) {
+        lo[a] -= pad;
+        hi[a] += pad;
+    }
+    if (dims == 2) {
+        lo[2] = -1.0;
+        hi[2] = 1.0;
+    }
+    double vol = 1.0;
+    for (int a = 0; a < dims; a++) vol *= (hi[a] - lo[a]);
+    /* cell edge ~ mean element size, offset from integer multiples to avoid
+     * aligning with structured meshes */
+    double h = 1.17 * pow(vol / (double)std::max<int64_t>(md->num_elems, 1), 1.0 / dims);
+    int res[3] = {1, 1, 1};
+    for (int a = 0; a < dims; a++) {
+        res[a] = (int)std::max(1.0, std::min(4096.0, ceil((hi[a] - lo[a]) / h)));
+    }
+    if (c->mesh.cres[0] < 0) { /* unused */
+    }
+    int64_t ncells = (int64_t)res[0] * res[1] * res[2];
+    double inv[3];
+    for (int a = 0; a < 3; a++) inv[a] = res[a] / (hi[a] - lo[a]);
+    const int nn = dims == 3 ? 4 : 3;
+    auto cell_range = [&](int64_t e, int clo[3], int chi[3]) {
+        double blo[3] = {1e300, 1e300, 1e300}, bhi[3] = {-1e300, -1e300, -1e300};
+        for (int q = 0; q < nn; q++) {
+            int64_t v = md->conn[4 * e + q];
+            for (int a = 0; a < 3; a++) {
+                blo[a] = std::min(blo[a], md->coords[3 * v + a]);
+                bhi[a] = std::max(bhi[a], md->coords[3 * v + a]);
+            }
+        }
+        for (int a = 0; a < 3; a++) {
+            if (a >= dims) {
+                clo[a] = chi[a] = 0;
+                continue;
+            }
+            clo[a] = std::max(0, (int)floor((blo[a] - pad - lo[a]) * inv[a]));
+            chi[a] = std::min(res[a] - 1, (int)floor((bhi[a] + pad - lo[a]) * inv[a]));
+        }
+    };
+    std::vector<int> start(ncells + 1, 0);
+    int clo[3], chi[3];
+    for (int64_t e = 0; e < md->num_elems; e++) {
+        cell_range(e, clo, chi);
+        for (int k = clo[2]; k <= chi[2]; k++)
+            for (int j = clo[1]; j <= chi[1]; j++)
+                for (int i = clo[0]; i <= chi[0]; i++) start[((int64_t)k * res[1] + j) * res[0] + i + 1]++;
+    }
+    for (int64_t q = 0; q < ncells; q++) start[q + 1] += start[q];
+    int64_t total = start[ncells];
+    if (total > INT32_MAX) return set_err(c, FVC_E_NOMEM, "spatial hash too large");
+    std::vector<int> fill(start.begin(), start.end() - 1);
+    std::vector<int> elems(total);
+    for (int64_t e = 0; e < md->num_elems; e++) { /* ascending e -> ascending lists */
+        cell_range(e, clo, chi);
+        for (int k = clo[2]; k <= chi[2]; k++)
+            for (int j = clo[1]; j <= chi[1]; j++)
+                for (int i = clo[0]; i <= chi[0]; i++) elems[fill[((int64_t)k * res[1] + j) * res[0] + i]++] = (int)e;
+    }
+    int rc;
+    if ((rc = dev_alloc(c, &c->cell_start, ncells + 1))) return rc;
+    if ((rc = dev_alloc(c, &c->cell_elems, total))) return rc;
+    if (hipMemcpy(c->cell_start, start.data(), sizeof(int) * (ncells + 1), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->cell_elems, elems.data(), sizeof(int) * std::max<int64_t>(total, 1), hipMemcpyHostToDevice) !=
+            hipSuccess)
+        return set_err(c, FVC_E_HIP, "hash upload failed");
+    c->ncells = ncells;
+    c->nentries = total;
+    c->mesh.cell_start = c->cell_start;
+    c->mesh.cell_elems = c->cell_elems;
+    for (int a = 0; a < 3; a++) {
+        c->mesh.hlo[a] = lo[a];
+        c->mesh.hinv[a] = inv[a];
+        c->mesh.hres[a] = res[a];
+    }
+    return FVC_OK;
+}
+
+/* FP32 guard band: bound on |natural coordinate error| of the FP32 path,
+ * from the largest inverse-map row norm over elements and the coordinate
+ * magnitude after centring. */
+static float guard_band(const fvc_mesh_desc *md, const double centre[3]) {
+    const int dims = md->dims;
+    double maxinv = 0.0, maxc = 0.0;
+    for (int64_t n = 0; n < md->num_nodes; n++)
+        for (int a = 0; a < dims; a++) maxc = std::max(maxc, fabs(md->coords[3 * n + a] - centre[a]));
+    for (int64_t e = 0; e < md->num_elems; e++) {
+        const int32_t *nd = md->conn + 4 * e;
+        double E[3][3] = {{0}};
+        for (int q = 1; q <= dims; q++)
+            for (int a = 0; a < dims; a++) E[a][q - 1] = md->coords[3 * nd[q] + a] - md->coords[3 * nd[0] + a];
+        double rowsum;
+        if (dims == 3) {
+            double det = E[0][0] * (E[1][1] * E[2][2] - E[1][2] * E[2][1]) -
+                         E[0][1] * (E[1][0] * E[2][2] - E[1][2] * E[2][0]) +
+                         E[0][2] * (E[1][0] * E[2][1] - E[1][1] * E[2][0]);
+            if (det == 0.0) return 1.0f;
+            double inv[3][3];
+            inv[0][0] = (E[1][1] * E[2][2] - E[1][2] * E[2][1]) / det;
+            inv[0][1] = (E[0][2] * E[2][1] - E[0][1] * E[2][2]) / det;
+            inv[0][2] = (E[0][1] * E[1][2] - E[0][2] * E[1][1]) / det;
+            inv[1][0] = (E[1][2] * E[2][0] - E[1][0] * E[2][2]) / det;
+            inv[1][1] = (E[0][0] * E[2][2] - E[0][2] * E[2][0]) / det;
+            inv[1][2] = (E[0][2] * E[1][0] - E[0][0] * E[1][2]) / det;
+            inv[2][0] = (E[1][0] * E[2][1] - E[1][1] * E[2][0]) / det;
+            inv[2][1] = (E[0][1] * E[2][0] - E[0][0] * E[2][1]) / det;
+            inv[2][2] = (E[0][0] * E[1][1] - E[0][1] * E[1][0]) / det;
+            rowsum = 0.0;
+            for (int r = 0; r < 3; r++) rowsum = std::max(rowsum, fabs(inv[r][0]) + fabs(inv[r][1]) + fabs(inv[r][2]));
+            double l0 = 0.0;
+            for (int a = 0; a < 3; a++) l0 += fabs(inv[0][a] + inv[1][a] + inv[2][a]);
+            rowsum = std::max(rowsum, l0);
+        } else {
+            double det = E[0][0] * E[1][1] - E[0][1] * E[1][0];
+            if (det == 0.0) return 1.0f;
+            rowsum = std::max(fabs(E[1][1]) + fabs(E[0][1]), fabs(E[1][0]) + fabs(E[0][0])) / fabs(det);
+            rowsum = std::max(rowsum, (fabs(E[1][1] - E[1][0]) + fabs(E[0][0] - E[0][1])) / fabs(det));
+        }
+        maxinv = std::max(maxinv, rowsum);
+    }
+    /* node rounding (2 ulp-halves per edge) and query rounding, FP32 solve error;
+     * safety factor 8 over the first-order bound */
+    double eps = ldexp(1.0, -24);
+    double g = 8.0 * maxinv * maxc * eps * (2.0 * 4.0 + 8.0);
+    return (float)std::min(0.25, std::max(g, 1e-6));
+}
+
+/* ------------------------------------------------------------------------ */
+/* context                                                                   */
+/* ------------------------------------------------------------------------ */
+extern "C" int fvc_create(const fvc_mesh_desc *md, const fvc_params *pp, fvc_ctx **out) {
+    if (!md || !pp || !out) return FVC_E_INVALID;
+    *out = nullptr;
+    if ((md->dims != 2 && md->dims != 3) || (md->mesh_type != 0 && md->mesh_type != 1)) return FVC_E_INVALID;
+    if (pp->precision < 0 || pp->precision > 3 || pp->max_slices < 2) return FVC_E_INVALID;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= pp->device || pp->device < 0) return FVC_E_NODEVICE;
+    if (hipSetDevice(pp->device) != hipSuccess) return FVC_E_NODEVICE;
+    fvc_ctx *c = new fvc_ctx();
+    c->device = pp->device;
+    c->precision = pp->precision;
+    c->max_slices = pp->max_slices;
+    c->slices.assign(pp->max_slices, nullptr);
+    c->slice_scale.assign(pp->max_slices, 1.0f);
+    c->slice_time.assign(pp->max_slices, 0.0);
+    memset(&c->mesh, 0, sizeof c->mesh);
+    DevMesh &m = c->mesh;
+    m.dims = md->dims;
+    m.mesh_type = md->mesh_type;
+    for (int a = 0; a < 6; a++) m.bounds[a] = md->bounds[a];
+    m.walk_limit = pp->walk_limit > 0 ? pp->walk_limit : (1 << 20);
+    int rc = FVC_OK;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return FVC_E_HIP;
+    }
+    if ((rc = dev_alloc(c, &c->d_err, 1)) || (rc = dev_alloc(c, &c->d_where, 1)) || (rc = dev_alloc(c, &c->d_steps, 1))) {
+        fvc_destroy(c);
+        return rc;
+    }
+    hipMemset(c->d_err, 0, sizeof(int));
+    if (md->mesh_type == 0) {
+        /* Cartesian grid (mesh.c:96-121) */
+        for (int a = 0; a < 3; a++) {
+            m.cmin[a] = md->cart_min[a];
+            m.cres[a] = md->cart_res[a];
+        }
+        if (m.cres[0] < 2 || m.cres[1] < 2 || md->cart_min[0] >= md->cart_max[0] || md->cart_min[1] >= md->cart_max[1] ||
+            (md->dims == 3 && (m.cres[2] < 2 || md->cart_min[2] >= md->cart_max[2]))) {
+            fvc_destroy(c);
+            return FVC_E_INVALID;
+        }
+        m.cdel[0] = (md->cart_max[0] - md->cart_min[0]) / (md->cart_res[0] - 1);
+        m.cdel[1] = (md->cart_max[1] - md->cart_min[1]) / (md->cart_res[1] - 1);
+        if (md->dims == 3)
+            m.cdel[2] = (md->cart_max[2] - md->cart_min[2]) / (md->cart_res[2] - 1);
+        else {
+            m.cmin[2] = 0;
+            m.cdel[2] = 0;
+            m.cres[2] = 1;
+        }
+        c->nvel = (int64_t)m.cres[0] * m.cres[1] * m.cres[2];
+    } else {
+        if (md->num_nodes <= 0 || md->num_elems <= 0 || !md->coords || !md->conn || !md->adj ||
+            md->num_elems > INT32_MAX || md->num_nodes > INT32_MAX) {
+            fvc_destroy(c);
+            return FVC_E_INVALID;
+        }
+        const int nn = md->dims == 3 ? 4 : 3;
+        for (int64_t e = 0; e < md->num_elems; e++)
+            for (int q = 0; q < 4; q++) {
+                int32_t v = md->conn[4 * e + q], a = md->adj[4 * e + q];
+                if ((q < nn && (v < 0 || v >= md->num_nodes)) || a < -1 || a >= md->num_elems) {
+                    fvc_destroy(c);
+                    return set_err(nullptr, FVC_E_INVALID, "");
+                }
+            }
+        m.nn = md->num_nodes;
+        m.ne = md->num_elems;
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int64_t n = 0; n < md->num_nodes; n++)
+            for (int a = 0; a < 3; a++) {
+                lo[a] = std::min(lo[a], md->coords[3 * n + a]);
+                hi[a] = std::max(hi[a], md->coords[3 * n + a]);
+            }
+        for (int a = 0; a < 3; a++) m.centre[a] = 0.5 * (lo[a] + hi[a]);
+        std::vector<double4> x64(md->num_nodes);
+        std::vector<float4> x32(md->num_nodes);
+        for (int64_t n = 0; n < md->num_nodes; n++) {
+            const double *p = md->coords + 3 * n;
+            x64[n] = make_double4(p[0], p[1], p[2], 0.0);
+            x32[n] = make_float4((float)(p[0] - m.centre[0]), (float)(p[1] - m.centre[1]),
+                                 (float)(p[2] - m.centre[2]), 0.0f);
+        }
+        std::vector<int4> cn(md->num_elems), ad(md->num_elems);
+        for (int64_t e = 0; e < md->num_elems; e++) {
+            const int32_t *q = md->conn + 4 * e, *a = md->adj + 4 * e;
+            /* 2-D meshes store -1 in the fourth slot; keep a valid id for loads */
+            cn[e] = make_int4(q[0], q[1], q[2], nn == 4 ? q[3] : q[0]);
+            ad[e] = make_int4(a[0], a[1], a[2], nn == 4 ? a[3] : -1);
+        }
+        if ((rc = dev_alloc(c, &c->X64, md->num_nodes)) || (rc = dev_alloc(c, &c->X32, md->num_nodes)) ||
+            (rc = dev_alloc(c, &c->conn, md->num_elems)) || (rc = dev_alloc(c, &c->adj, md->num_elems))) {
+            fvc_destroy(c);
+            return rc;
+        }
+        hipMemcpy(c->X64, x64.data(), sizeof(double4) * md->num_nodes, hipMemcpyHostToDevice);
+        hipMemcpy(c->X32, x32.data(), sizeof(float4) * md->num_nodes, hipMemcpyHostToDevice);
+        hipMemcpy(c->conn, cn.data(), sizeof(int4) * md->num_elems, hipMemcpyHostToDevice);
+        hipMemcpy(c->adj, ad.data(), sizeof(int4) * md->num_elems, hipMemcpyHostToDevice);
+        m.X64 = c->X64;
+        m.X32 = c->X32;
+        m.conn = c->conn;
+        m.adj = c->adj;
+        m.guard = guard_band(md, m.centre);
+        if ((rc = build_hash(c, md))) {
+            fvc_destroy(c);
+            return rc;
+        }
+        c->nvel = md->num_nodes;
+    }
+    c->vel_stride = c->precision == FVC_PREC_F64 ? 32 : (c->precision == FVC_PREC_F32 ? 16 : 8);
+    if (hipDeviceSynchronize() != hipSuccess) {
+        fvc_destroy(c);
+        return FVC_E_HIP;
+    }
+    *out = c;
+    return FVC_OK;
+}
+
+extern "C" void fvc_destroy(fvc_ctx *c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (void *s : c->slices)
+        if (s) hipFree(s);
+    hipFree(c->X64);
+    hipFree(c->X32);
+    hipFree(c->conn);
+    hipFree(c->adj);
+    hipFree(c->cell_start);
+    hipFree(c->cell_elems);
+    hipFree(c->d_err);
+    hipFree(c->d_where);
+    hipFree(c->d_steps);
+    if (c->scratch) hipFree(c->scratch);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" int fvc_sync(fvc_ctx *c) {
+    if (!c) return FVC_E_INVALID;
+    return hipStreamSynchronize(c->stream) == hipSuccess ? FVC_OK : FVC_E_HIP;
+}
+
+extern "C" int fvc_memory(fvc_ctx *c, int64_t *used, int64_t *free_bytes) {
+    if (!c) return FVC_E_INVALID;
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) return FVC_E_HIP;
+    if (used) *used = c->bytes_used;
+    if (free_bytes) *free_bytes = (int64_t)fr;
+    return FVC_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* slices                                                                    */
+/* ------------------------------------------------------------------------ */
+static inline uint16_t to_bf16(float v) { /* round to nearest even */
+    uint32_t u;
+    memcpy(&u, &v, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+static inline uint16_t to_f16(float v) {
+    _Float16 h = (_Float16)v;
+    uint16_t r;
+    memcpy(&r, &h, 2);
+    return r;
+}
+
+extern "C" int fvc_load_slice(fvc_ctx *c, int32_t slot, double timestamp, const double *uvw, int64_t n) {
+    if (!c || !uvw || slot < 0 || slot >= c->max_slices) return FVC_E_INVALID;
+    if (n != c->nvel) return set_err(c, FVC_E_INVALID, "slice length differs from the node count");
+    if (!c->slices[slot]) {
+        void *p = nullptr;
+        if (hipMalloc(&p, c->vel_stride * (size_t)n) != hipSuccess) return set_err(c, FVC_E_NOMEM, "slice alloc");
+        c->bytes_used += (int64_t)(c->vel_stride * (size_t)n);
+        c->slices[slot] = p;
+    }
+    std::vector<unsigned char> host(c->vel_stride * (size_t)n);
+    float scale = 1.0f;
+    if (c->precision == FVC_PREC_F64) {
+        double4 *h = (double4 *)host.data();
+        for (int64_t i = 0; i < n; i++) h[i] = make_double4(uvw[3 * i], uvw[3 * i + 1], uvw[3 * i + 2], 0.0);
+    } else if (c->precision == FVC_PREC_F32) {
+        float4 *h = (float4 *)host.data();
+        for (int64_t i = 0; i < n; i++)
+            h[i] = make_float4((float)uvw[3 * i], (float)uvw[3 * i + 1], (float)uvw[3 * i + 2], 0.0f);
+    } else {
+        double mx = 0.0;
+        for (int64_t i = 0; i < 3 * n; i++) mx = std::max(mx, fabs(uvw[i]));
+        scale = mx > 0.0 ? (float)mx : 1.0f;
+        const double inv = 1.0 / (double)scale;
+        uint16_t *h = (uint16_t *)host.data();
+        for (int64_t i = 0; i < n; i++)
+            for (int q = 0; q < 4; q++) {
+                float v = q < 3 ? (float)(uvw[3 * i + q] * inv) : 0.0f;
+                h[4 * i + q] = c->precision == FVC_PREC_BF16 ? to_bf16(v) : to_f16(v);
+            }
+    }
+    if (hipMemcpyAsync(c->slices[slot], host.data(), host.size(), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+        return set_err(c, FVC_E_HIP, "slice upload");
+    c->slice_scale[slot] = scale;
+    c->slice_time[slot] = timestamp;
+    return FVC_OK;
+}
+
+static int check_slots(fvc_ctx *c, int lo, int hi, double tmin, double tmax) {
+    if (lo < 0 || hi < 0 || lo >= c->max_slices || hi >= c->max_slices || !c->slices[lo] || !c->slices[hi])
+        return set_err(c, FVC_E_STATE, "velocity slot not loaded");
+    if (!(tmax > tmin)) return set_err(c, FVC_E_INVALID, "empty loaded interval");
+    return FVC_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* particles                                                                 */
+/* ------------------------------------------------------------------------ */
+extern "C" int fvc_particles_create(fvc_ctx *c, int64_t cap, fvc_particles **out) {
+    if (!c || !out || cap <= 0) return FVC_E_INVALID;
+    *out = nullptr;
+    fvc_particles *p = new fvc_particles();
+    p->ctx = c;
+    p->cap = cap;
+    size_t bytes = (size_t)cap * (8 * 8 + 3 * 4) + 256;
+    if (hipMalloc(&p->buf, bytes) != hipSuccess) {
+        delete p;
+        return set_err(c, FVC_E_NOMEM, "particle alloc");
+    }
+    c->bytes_used += (int64_t)bytes;
+    double *d = (double *)p->buf;
+    for (int a = 0; a < 3; a++) p->X[a] = d + a * cap;
+    for (int a = 0; a < 3; a++) p->V[a] = d + (3 + a) * cap;
+    p->left_time = d + 6 * cap;
+    p->scalar = d + 7 * cap;
+    int *iv = (int *)(d + 8 * cap);
+    p->elem = iv;
+    p->aux = iv + cap;
+    p->left = iv + 2 * cap;
+    hipMemsetAsync(p->buf, 0, bytes, c->stream);
+    hipStreamSynchronize(c->stream);
+    *out = p;
+    return FVC_OK;
+}
+
+extern "C" void fvc_particles_destroy(fvc_particles *p) {
+    if (!p) return;
+    hipStreamSynchronize(p->ctx->stream);
+    hipFree(p->buf);
+    delete p;
+}
+
+extern "C" int64_t fvc_particles_capacity(const fvc_particles *p) { return p ? p->cap : 0; }
+
+extern "C" int fvc_particles_set(fvc_particles *p, int64_t begin, const fvc_point *recs, int64_t n) {
+    if (!p || !recs || begin < 0 || n < 0 || begin + n > p->cap) return FVC_E_INVALID;
+    fvc_ctx *c = p->ctx;
+    std::vector<double> d(8 * (size_t)n);
+    std::vector<int> iv(3 * (size_t)n);
+    for (int64_t i = 0; i < n; i++) {
+        for (int a = 0; a < 3; a++) {
+            d[a * n + i] = recs[i].X[a];
+            d[(3 + a) * n + i] = recs[i].V[a];
+        }
+        d[6 * n + i] = recs[i].LeftDomainTime;
+        d[7 * n + i] = recs[i].Scalar;
+        iv[i] = recs[i].ElementIndex;
+        iv[n + i] = recs[i].AuxElementIndex;
+        iv[2 * n + i] = recs[i].LeftDomain;
+    }
+    double *dst[8] = {p->X[0], p->X[1], p->X[2], p->V[0], p->V[1], p->V[2], p->left_time, p->scalar};
+    for (int a = 0; a < 8; a++)
+        FVC_HIP(hipMemcpyAsync(dst[a] + begin, d.data() + a * n, 8 * n, hipMemcpyHostToDevice, c->stream));
+    int *idst[3] = {p->elem, p->aux, p->left};
+    for (int a = 0; a < 3; a++)
+        FVC_HIP(hipMemcpyAsync(idst[a] + begin, iv.data() + a * n, 4 * n, hipMemcpyHostToDevice, c->stream));
+    FVC_HIP(hipStreamSynchronize(c->stream));
+    return FVC_OK;
+}
+
+extern "C" int fvc_particles_get(fvc_particles *p, int64_t begin, fvc_point *recs, int64_t n) {
+    if (!p || !recs || begin < 0 || n < 0 || begin + n > p->cap) return FVC_E_INVALID;
+    fvc_ctx *c = p->ctx;
+    std::vector<double> d(8 * (size_t)n);
+    std::vector<int> iv(3 * (size_t)n);
+    double *src[8] = {p->X[0], p->X[1], p->X[2], p->V[0], p->V[1], p->V[2], p->left_time, p->scalar};
+    for (int a = 0; a < 8; a++)
+        FVC_HIP(hipMemcpyAsync(d.data() + a * n, src[a] + begin, 8 * n, hipMemcpyDeviceToHost, c->stream));
+    int *isrc[3] = {p->elem, p->aux, p->left};
+    for (int a = 0; a < 3; a++)
+        FVC_HIP(hipMemcpyAsync(iv.data() + a * n, isrc[a] + begin, 4 * n, hipMemcpyDeviceToHost, c->stream));
+    FVC_HIP(hipStreamSynchronize(c->stream));
+    for (int64_t i = 0; i < n; i++) {
+        for (int a = 0; a < 3; a++) {
+            recs[i].X[a] = d[a * n + i];
+            recs[i].V[a] = d[(3 + a) * n + i];
+        }
+        recs[i].LeftDomainTime = d[6 * n + i];
+        recs[i].Scalar = d[7 * n + i];
+        recs[i].ElementIndex = iv[i];
+        recs[i].AuxElementIndex = iv[n + i];
+        recs[i].LeftDomain = iv[2 * n + i];
+        recs[i].pad_ = 0;
+    }
+    return FVC_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* integration                                                               */
+/* ------------------------------------------------------------------------ */
+extern "C" int fvc_launch_velocity(fvc_ctx *c, fvc_particles *p, int64_t begin, int64_t count, double t, int32_t lo,
+                                   int32_t hi, double tmin, double tmax) {
+    if (!c || !p || begin < 0 || count < 0 || begin + count > p->cap) return FVC_E_INVALID;
+    int rc = check_slots(c, lo, hi, tmin, tmax);
+    if (rc) return rc;
+    return launch_velocity(c, ptrs(p), begin, count, t, bind_field(c, lo, hi, tmin, tmax));
+}
+
+extern "C" int fvc_advect(fvc_ctx *c, fvc_particles *p, int64_t begin, int64_t count, double t1, double t2, int32_t lo,
+                          int32_t hi, double tmin, double tmax, const fvc_int_params *ip) {
+    if (!c || !p || !ip || begin < 0 || count < 0 || begin + count > p->cap) return FVC_E_INVALID;
+    if (ip->int_type != FVC_INT_RK4 && ip->int_type != FVC_INT_EULER)
+        return set_err(c, FVC_E_INVALID, "integrator not available in fvc_advect");
+    if (!(ip->time_step > 1.0e-10)) return set_err(c, FVC_E_INVALID, "Int_TimeStep must be positive");
+    int rc = check_slots(c, lo, hi, tmin, tmax);
+    if (rc) return rc;
+    return launch_advect(c, ptrs(p), nullptr, begin, count, t1, t2, bind_field(c, lo, hi, tmin, tmax), *ip);
+}
+
+extern "C" int fvc_last_step_count(fvc_ctx *c, int64_t *n) {
+    if (!c || !n) return FVC_E_INVALID;
+    *n = c->last_steps;
+    return FVC_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* location                                                                   */
+/* ------------------------------------------------------------------------ */
+extern "C" int fvc_locate(fvc_ctx *c, const double *xyz, const int32_t *guess, int64_t n, int32_t mode, int32_t *elem) {
+    if (!c || !xyz || !elem || n < 0 || mode < 0 || mode > 2 || (mode != FVC_LOCATE_GLOBAL && !guess))
+        return FVC_E_INVALID;
+    if (c->mesh.mesh_type != 1) return set_err(c, FVC_E_INVALID, "location needs an unstructured mesh");
+    if (n == 0) return FVC_OK;
+    double *dx = nullptr;
+    int *dg = nullptr, *dout = nullptr;
+    FVC_HIP(hipMallocAsync((void **)&dx, sizeof(double) * 3 * n, c->stream));
+    FVC_HIP(hipMallocAsync((void **)&dg, sizeof(int) * n, c->stream));
+    FVC_HIP(hipMallocAsync((void **)&dout, sizeof(int) * n, c->stream));
+    FVC_HIP(hipMemcpyAsync(dx, xyz, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
+    if (guess) FVC_HIP(hipMemcpyAsync(dg, guess, sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
+    int rc = launch_locate(c, dx, guess ? dg : nullptr, n, mode, dout);
+    if (!rc) {
+        FVC_HIP(hipMemcpyAsync(elem, dout, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
+        FVC_HIP(hipStreamSynchronize(c->stream));
+    }
+    hipFreeAsync(dx, c->stream);
+    hipFreeAsync(dg, c->stream);
+    hipFreeAsync(dout, c->stream);
+    hipStreamSynchronize(c->stream);
+    return rc;
+}
+
+static int seed_common(fvc_ctx *c, int chain, const double *xyz, int64_t n, const double mn[3], const double del[3],
+                       const int32_t res[3], int lsc, fvc_point *out) {
+#pragma clang fp contract(off)
+    double *dx = nullptr, *dlt = nullptr;
+    int *de = nullptr, *dl = nullptr;
+    if (chain == 0) {
+        FVC_HIP(hipMallocAsync((void **)&dx, sizeof(double) * 3 * n, c->stream));
+        FVC_HIP(hipMemcpyAsync(dx, xyz, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
+    }
+    FVC_HIP(hipMallocAsync((void **)&dlt, sizeof(double) * n, c->stream));
+    FVC_HIP(hipMallocAsync((void **)&de, sizeof(int) * n, c->stream));
+    FVC_HIP(hipMallocAsync((void **)&dl, sizeof(int) * n, c->stream));
+    int rc = launch_seed_chain(c, chain, n, dx, mn, del, res, lsc, de, dl, dlt);
+    if (!rc) {
+        std::vector<int> e(n), l(n);
+        std::vector<double> lt(n);
+        FVC_HIP(hipMemcpyAsync(e.data(), de, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
+        FVC_HIP(hipMemcpyAsync(l.data(), dl, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
+        FVC_HIP(hipMemcpyAsync(lt.data(), dlt, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+        FVC_HIP(hipStreamSynchronize(c->stream));
+        for (int64_t i = 0; i < n; i++) {
+            fvc_point &p = out[i];
+            memset(&p, 0, sizeof p);
+            if (chain == 0) {
+                p.X[0] = xyz[3 * i];
+                p.X[1] = xyz[3 * i + 1];
+                p.X[2] = c->mesh.dims == 2 ? 0.0 : xyz[3 * i + 2];
+            } else {
+                int ii = (int)(i % res[0]), jj = (int)((i / res[0]) % res[1]), kk = (int)(i / ((int64_t)res[0] * res[1]));
+                p.X[0] = mn[0] + ii * del[0];
+                p.X[1] = mn[1] + jj * del[1];
+                p.X[2] = mn[2] + kk * del[2];
+            }
+            p.ElementIndex = c->mesh.mesh_type == 1 ? e[i] : 0;
+            p.AuxElementIndex = chain == 0 && !l[i] ? e[i] : 0;
+            p.LeftDomain = l[i];
+            p.LeftDomainTime = lt[i];
+        }
+    }
+    if (dx) hipFreeAsync(dx, c->stream);
+    hipFreeAsync(dlt, c->stream);
+    hipFreeAsync(de, c->stream);
+    hipFreeAsync(dl, c->stream);
+    hipStreamSynchronize(c->stream);
+    return rc;
+}
+
+extern "C" int fvc_seed_list(fvc_ctx *c, const double *xyz, int64_t n, int32_t lsc, fvc_point *out) {
+    if (!c || !xyz || !out || n < 0) return FVC_E_INVALID;
+    if (n == 0) return FVC_OK;
+    std::vector<double> x(3 * (size_t)n);
+    for (int64_t i = 0; i < n; i++) {
+        x[3 * i] = xyz[3 * i];
+        x[3 * i + 1] = xyz[3 * i + 1];
+        x[3 * i + 2] = c->mesh.dims == 2 ? 0.0 : xyz[3 * i + 2]; /* tracers.c:1033 */
+    }
+    return seed_common(c, 0, x.data(), n, nullptr, nullptr, nullptr, lsc, out);
+}
+
+extern "C" int fvc_seed_grid(fvc_ctx *c, const double mn[3], const double del[3], const int32_t res[3], int32_t lsc,
+                             fvc_point *out) {
+    if (!c || !mn || !del || !res || !out || res[0] < 1 || res[1] < 1 || res[2] < 1) return FVC_E_INVALID;
+    int64_t n = (int64_t)res[0] * res[1] * res[2];
+    return seed_common(c, 1, nullptr, n, mn, del, res, lsc, out);
+}
+
+/* ------------------------------------------------------------------------ */
+/* FTLE slides                                                               */
+/* ------------------------------------------------------------------------ */
+__global__ void k_ftle_grid(double *X0, double *X1, double *X2, int *elem, int *left, double *lt, int64_t n, double mn0,
+                            double mn1, double mn2, double d0, double d1, double d2, int r1, int r2,
+                            const int *celem, const int *cleft, const double *clt) {
+#pragma clang fp contract(off)
+    int64_t P = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (P >= n) return;
+    int k = (int)(P % r2), j = (int)((P / r2) % r1), i = (int)(P / ((int64_t)r2 * r1));
+    X0[P] = mn0 + i * d0;
+    X1[P] = mn1 + j * d1;
+    X2[P] = mn2 + k * d2;
+    elem[P] = celem[P];
+    left[P] = cleft[P];
+    lt[P] = clt[P];
+}
+
+extern "C" int fvc_ftle_create(fvc_ctx *c, const double mn[3], const double mx[3], const int32_t res[3], int32_t lsc,
+                               double start_time, double stop_time, fvc_ftle **out) {
+    if (!c || !mn || !mx || !res || !out) return FVC_E_INVALID;
+    *out = nullptr;
+    const int dims = c->mesh.dims;
+    if (res[0] < 2 || res[1] < 2 || (dims == 3 && res[2] < 2)) return FVC_E_INVALID;
+    fvc_ftle *f = new fvc_ftle();
+    f->ctx = c;
+    for (int a = 0; a < 3; a++) {
+        f->mn[a] = mn[a];
+        f->mx[a] = mx[a];
+        f->res[a] = res[a];
+    }
+    /* parameters.c:310-320 */
+    f->del[0] = (f->mx[0] - f->mn[0]) / (f->res[0] - 1);
+    f->del[1] = (f->mx[1] - f->mn[1]) / (f->res[1] - 1);
+    if (dims == 3)
+        f->del[2] = (f->mx[2] - f->mn[2]) / (f->res[2] - 1);
+    else {
+        f->mn[2] = f->mx[2] = 0.0;
+        f->del[2] = 0.0;
+        f->res[2] = 1;
+    }
+    f->n = (int64_t)f->res[0] * f->res[1] * f->res[2];
+    f->start_time = start_time;
+    f->stop_time = stop_time;
+    int rc = fvc_particles_create(c, f->n, &f->pts);
+    if (rc) {
+        delete f;
+        return rc;
+    }
+    size_t n = (size_t)f->n;
+    size_t bytes = n * (3 * 4 + 3 * 8 + 3 * 8 + 3 * 8 + 4 + 4 + 4 + 8 + 4) + 1024;
+    if (hipMalloc(&f->buf, bytes) != hipSuccess) {
+        fvc_ftle_destroy(f);
+        return set_err(c, FVC_E_NOMEM, "FTLE alloc");
+    }
+    c->bytes_used += (int64_t)bytes;
+    double *d = (double *)f->buf;
+    f->wt = d;
+    f->wot = d + n;
+    for (int a = 0; a < 3; a++) {
+        f->Xn[a] = d + (2 + a) * n;
+        f->Vn[a] = d + (5 + a) * n;
+    }
+    f->exit_time = d + 8 * n;
+    int *iv = (int *)(d + 9 * n);
+    f->have = iv;
+    f->elem_new = iv + n;
+    f->exited = iv + 2 * n;
+    f->advected = iv + 3 * n;
+    f->active = iv + 4 * n;
+    /* location chain of InitializeFTLEArray, then the grid records */
+    int *ce = f->elem_new, *cl = f->exited; /* reuse staging as chain output */
+    double *clt = f->exit_time;
+    rc = launch_seed_chain(c, 2, f->n, nullptr, f->mn, f->del, f->res, lsc, ce, cl, clt);
+    if (rc) {
+        fvc_ftle_destroy(f);
+        return rc;
+    }
+    fvc_particles *p = f->pts;
+    k_ftle_grid<<<(unsigned)((f->n + 255) / 256), 256, 0, c->stream>>>(
+        p->X[0], p->X[1], p->X[2], p->elem, p->left, p->left_time, f->n, f->mn[0], f->mn[1], f->mn[2], f->del[0],
+        f->del[1], f->del[2], f->res[1], f->res[2], ce, cl, clt);
+    if ((rc = launch_ftle_mask(f))) {
+        fvc_ftle_destroy(f);
+        return rc;
+    }
+    /* points inside = not left */
+    int *flags = (int *)scratch(c, sizeof(int) * n);
+    if (!flags) {
+        fvc_ftle_destroy(f);
+        return FVC_E_NOMEM;
+    }
+    hipStreamSynchronize(c->stream);
+    std::vector<int> lh(n);
+    hipMemcpy(lh.data(), p->left, sizeof(int) * n, hipMemcpyDeviceToHost);
+    int64_t inside = 0;
+    for (size_t q = 0; q < n; q++) inside += !lh[q];
+    f->inside = inside;
+    *out = f;
+    return check_err(c, "fvc_ftle_create");
+}
+
+extern "C" void fvc_ftle_destroy(fvc_ftle *f) {
+    if (!f) return;
+    hipStreamSynchronize(f->ctx->stream);
+    if (f->buf) hipFree(f->buf);
+    fvc_particles_destroy(f->pts);
+    delete f;
+}
+
+extern "C" int fvc_ftle_frame(fvc_ftle *f, int32_t dir, int32_t lo, int32_t hi, double tmin, double tmax,
+                              const fvc_int_params *ip, int32_t *done) {
+    if (!f || !ip || (dir != 1 && dir != -1)) return FVC_E_INVALID;
+    fvc_ctx *c = f->ctx;
+    if (done) *done = 0;
+    if (f->status == 2) return FVC_OK;
+    const double T = 1.0e-10;
+    if (!(f->status == 1 || (f->start_time <= tmax + T && f->start_time >= tmin - T))) return FVC_OK;
+    int rc = check_slots(c, lo, hi, tmin, tmax);
+    if (rc) return rc;
+    if (ip->int_type != FVC_INT_RK4 && ip->int_type != FVC_INT_EULER) return FVC_E_INVALID;
+    double t1;
+    if (f->status == 0) {
+        f->status = 1;
+        t1 = f->start_time;
+        rc = launch_velocity(c, ptrs(f->pts), 0, f->n, f->start_time, bind_field(c, lo, hi, tmin, tmax));
+        if (rc) return rc;
+    } else {
+        t1 = dir > 0 ? tmin : tmax;
+    }
+    double t2 = dir > 0 ? (tmax < f->stop_time ? tmax : f->stop_time) : (tmin > f->stop_time ? tmin : f->stop_time);
+    rc = launch_ftle_frame(f, lo, hi, tmin, tmax, t1, t2, *ip);
+    if (rc) return rc;
+    if (((dir > 0) ? (f->stop_time - tmax) : (tmin - f->stop_time)) < T) f->status = 2;
+    if (done) *done = f->status == 2 ? 2 : 1;
+    return FVC_OK;
+}
+
+extern "C" int fvc_ftle_output(fvc_ftle *f, double int_t, double *wt, double *wot) {
+    if (!f) return FVC_E_INVALID;
+    fvc_ctx *c = f->ctx;
+    int rc = launch_ftle_fill(f, int_t);
+    if (rc) return rc;
+    size_t n = (size_t)f->n;
+    std::vector<double> a(n), b(n);
+    FVC_HIP(hipMemcpy(a.data(), f->wt, sizeof(double) * n, hipMemcpyDeviceToHost));
+    FVC_HIP(hipMemcpy(b.data(), f->wot, sizeof(double) * n, hipMemcpyDeviceToHost));
+    /* i-major storage -> k,j,i output order (ftle.c:1302-1315) */
+    int64_t q = 0;
+    for (int k = 0; k < f->res[2]; k++)
+        for (int j = 0; j < f->res[1]; j++)
+            for (int i = 0; i < f->res[0]; i++, q++) {
+                int64_t P = ((int64_t)i * f->res[1] + j) * f->res[2] + k;
+                if (wt) wt[q] = a[P];
+                if (wot) wot[q] = b[P];
+            }
+    return FVC_OK;
+}
+
+extern "C" int fvc_ftle_points(fvc_ftle *f, int64_t *n, int64_t *inside) {
+    if (!f) return FVC_E_INVALID;
+    if (n) *n = f->n;
+    if (inside) *inside = f->inside;
+    return FVC_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Z-order                                                                    */
+/* ------------------------------------------------------------------------ */
+extern "C" int fvc_zsort(fvc_ctx *c, const double *xyz, int64_t n, const double lo[3], const double hi[3],
+                         int32_t *perm) {
+    if (!c || !xyz || !lo || !hi || !perm || n < 0) return FVC_E_INVALID;
+    if (n == 0) return FVC_OK;
+    double *dx = nullptr;
+    int *dp = nullptr;
+    FVC_HIP(hipMallocAsync((void **)&dx, sizeof(double) * 3 * n, c->stream));
+    FVC_HIP(hipMallocAsync((void **)&dp, sizeof(int) * n, c->stream));
+    FVC_HIP(hipMemcpyAsync(dx, xyz, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
+    int rc = launch_zsort(c, dx, n, lo, hi, dp);
+    if (!rc) {
+        FVC_HIP(hipMemcpyAsync(perm, dp, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
+        FVC_HIP(hipStreamSynchronize(c->stream));
+    }
+    hipFreeAsync(dx, c->stream);
+    hipFreeAsync(dp, c->stream);
+    hipStreamSynchronize(c->stream);
+    return rc;
+}

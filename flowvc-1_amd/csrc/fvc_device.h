@@ -1,0 +1,695 @@
+/*
+ * fvc_device.h - gfx950 device code of the advection path: point location,
+ * interpolation and the per-particle integrators.
+ *
+ * One lane owns one particle for a whole time interval; its state (FP64
+ * position, walk elements of the four RK4 stage probes) lives in registers.
+ * Mesh geometry and velocity are random gathers (no MFMA: this is a sparse
+ * stencil, not a contraction).
+ *
+ * Precision modes (template MODE):
+ *   P_F64  : FP64 everything, expressions in the reference's order with FP
+ *            contraction off, so results are bit-identical to ompVC's.
+ *   P_F32  : FP32 node coordinates (relative to the mesh centre) and FP32
+ *            velocity; FP64 position/time state.  Every walk decision (accept,
+ *            exit face) is taken in FP32 only when it is farther than a
+ *            guard band from its threshold; otherwise it is recomputed with
+ *            the FP64 formula, so the element sequence of a walk equals the
+ *            FP64 walk's for the same query point.
+ *   P_BF16/P_F16 : as P_F32 with velocity stored in 16 bits times a
+ *            per-slice scale.
+ */
+#pragma once
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+#include <stdint.h>
+
+namespace fvc {
+
+constexpr double TINY = 1.0e-10; /* ompVC/macros.h:17 */
+
+enum Mode { P_F64 = 0, P_F32 = 1, P_BF16 = 2, P_F16 = 3 };
+enum Accept { ACC_SEARCH = 0, ACC_INTERP = 1 }; /* d > -TINY (mesh.c:549) / d+TINY >= 0 (velocity.c:545) */
+
+/* device error codes (match include/fvc.h) */
+enum { E_OK = 0, E_INDET = -4, E_TLOC = -5, E_ELEM = -6, E_WALK = -7, E_EIG = -8, E_LEFT = -11 };
+
+struct DevMesh {
+    int dims, mesh_type;
+    double bounds[6];
+    /* Cartesian velocity grid */
+    double cmin[3], cdel[3];
+    int cres[3];
+    /* unstructured */
+    const double4 *__restrict__ X64;
+    const float4 *__restrict__ X32; /* node - centre, FP32 */
+    const int4 *__restrict__ conn;
+    const int4 *__restrict__ adj;
+    double centre[3];
+    int64_t nn, ne;
+    float guard;    /* FP32 decision guard band on natural coordinates */
+    int walk_limit; /* hop limit */
+    /* spatial hash */
+    const int *__restrict__ cell_start;
+    const int *__restrict__ cell_elems;
+    double hlo[3], hinv[3];
+    int hres[3];
+};
+
+/* Velocity slices bound for one interval: slot lo at tmin, hi at tmax. */
+struct DevField {
+    const void *lo, *hi;
+    float scale_lo, scale_hi;
+    double tmin, tmax;
+};
+
+struct ErrBuf {
+    int *code;     /* first error */
+    int64_t *where;
+};
+
+__device__ inline void raise_err(ErrBuf e, int code, int64_t where) {
+    if (atomicCAS(e.code, 0, code) == 0) *e.where = where;
+}
+
+/* ------------------------------------------------------------------------ */
+/* FP64 natural coordinates in the reference's expression order               */
+/* (ompVC/mesh.c:527-547, velocity.c:497-517)                                */
+/* ------------------------------------------------------------------------ */
+__device__ __forceinline__ void nat3_exact(const DevMesh &m, int e, double x, double y, double z, double &r, double &s,
+                                           double &t) {
+#pragma clang fp contract(off)
+    int4 nd = m.conn[e];
+    double4 p0 = m.X64[nd.x], p1 = m.X64[nd.y], p2 = m.X64[nd.z], p3 = m.X64[nd.w];
+    double x0 = p0.x, y0 = p0.y, z0 = p0.z, x1 = p1.x, y1 = p1.y, z1 = p1.z;
+    double x2 = p2.x, y2 = p2.y, z2 = p2.z, x3 = p3.x, y3 = p3.y, z3 = p3.z;
+    double c11 = (z3 - z0) * (y2 - y3) - (z2 - z3) * (y3 - y0);
+    double c21 = (z3 - z0) * (y0 - y1) - (z0 - z1) * (y3 - y0);
+    double c31 = (z1 - z2) * (y0 - y1) - (z0 - z1) * (y1 - y2);
+    double c12 = (x3 - x0) * (z2 - z3) - (x2 - x3) * (z3 - z0);
+    double c22 = (x3 - x0) * (z0 - z1) - (x0 - x1) * (z3 - z0);
+    double c32 = (x1 - x2) * (z0 - z1) - (x0 - x1) * (z1 - z2);
+    double c13 = (y3 - y0) * (x2 - x3) - (y2 - y3) * (x3 - x0);
+    double c23 = (y3 - y0) * (x0 - x1) - (y0 - y1) * (x3 - x0);
+    double c33 = (y1 - y2) * (x0 - x1) - (y0 - y1) * (x1 - x2);
+    double det = (x1 - x0) * ((y2 - y0) * (z3 - z0) - (z2 - z0) * (y3 - y0)) +
+                 (x2 - x0) * ((y0 - y1) * (z3 - z0) - (z0 - z1) * (y3 - y0)) +
+                 (x3 - x0) * ((y1 - y0) * (z2 - z0) - (z1 - z0) * (y2 - y0));
+    double dx = x - x0, dy = y - y0, dz = z - z0;
+    r = (c11 * dx + c12 * dy + c13 * dz) / det;
+    s = (c21 * dx + c22 * dy + c23 * dz) / det;
+    t = (c31 * dx + c32 * dy + c33 * dz) / det;
+}
+
+__device__ __forceinline__ void nat2_exact(const DevMesh &m, int e, double x, double y, double &r, double &s) {
+#pragma clang fp contract(off)
+    int4 nd = m.conn[e];
+    double4 p0 = m.X64[nd.x], p1 = m.X64[nd.y], p2 = m.X64[nd.z];
+    double x0 = p0.x, y0 = p0.y, x1 = p1.x, y1 = p1.y, x2 = p2.x, y2 = p2.y;
+    double c11 = y2 - y0, c12 = x0 - x2, c21 = y0 - y1, c22 = x1 - x0;
+    double det = (x1 - x0) * (y2 - y0) - (x2 - x0) * (y1 - y0);
+    r = (c11 * (x - x0) + c12 * (y - y0)) / det;
+    s = (c21 * (x - x0) + c22 * (y - y0)) / det;
+}
+
+__device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
+
+/* exact decision on natural coordinates; returns 1 accept, 0 move (slot set), -1 indeterminate */
+template <int DIM>
+__device__ __forceinline__ int decide_exact(double r, double s, double t, int acc, int &slot) {
+#pragma clang fp contract(off)
+    double d, l0;
+    if (DIM == 3) {
+        l0 = 1 - r - s - t;
+        d = dmin(r, dmin(s, dmin(t, l0)));
+    } else {
+        l0 = 1 - r - s;
+        d = dmin(r, dmin(s, l0));
+    }
+    bool in = acc == ACC_SEARCH ? (d > -TINY) : ((d + TINY) >= 0);
+    if (in) return 1;
+    if (fabs(r - d) < TINY) slot = 0;
+    else if (fabs(s - d) < TINY) slot = 1;
+    else if (DIM == 3 && fabs(t - d) < TINY) slot = 2;
+    else if (fabs(d - l0) < TINY) slot = DIM == 3 ? 3 : 2;
+    else return -1;
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* FP32 natural coordinates relative to node 0 (Cramer's rule)               */
+/* ------------------------------------------------------------------------ */
+__device__ __forceinline__ void nat3_fast(const DevMesh &m, int4 nd, double x, double y, double z, float &r, float &s,
+                                          float &t) {
+    float4 p0 = m.X32[nd.x], p1 = m.X32[nd.y], p2 = m.X32[nd.z], p3 = m.X32[nd.w];
+    float qx = (float)((x - m.centre[0]) - (double)p0.x);
+    float qy = (float)((y - m.centre[1]) - (double)p0.y);
+    float qz = (float)((z - m.centre[2]) - (double)p0.z);
+    float ax = p1.x - p0.x, ay = p1.y - p0.y, az = p1.z - p0.z;
+    float bx = p2.x - p0.x, by = p2.y - p0.y, bz = p2.z - p0.z;
+    float cx = p3.x - p0.x, cy = p3.y - p0.y, cz = p3.z - p0.z;
+    float n1x = by * cz - bz * cy, n1y = bz * cx - bx * cz, n1z = bx * cy - by * cx; /* b x c */
+    float n2x = cy * az - cz * ay, n2y = cz * ax - cx * az, n2z = cx * ay - cy * ax; /* c x a */
+    float n3x = ay * bz - az * by, n3y = az * bx - ax * bz, n3z = ax * by - ay * bx; /* a x b */
+    float inv = 1.0f / (ax * n1x + ay * n1y + az * n1z);
+    r = (qx * n1x + qy * n1y + qz * n1z) * inv;
+    s = (qx * n2x + qy * n2y + qz * n2z) * inv;
+    t = (qx * n3x + qy * n3y + qz * n3z) * inv;
+}
+
+__device__ __forceinline__ void nat2_fast(const DevMesh &m, int4 nd, double x, double y, float &r, float &s) {
+    float4 p0 = m.X32[nd.x], p1 = m.X32[nd.y], p2 = m.X32[nd.z];
+    float qx = (float)((x - m.centre[0]) - (double)p0.x);
+    float qy = (float)((y - m.centre[1]) - (double)p0.y);
+    float ax = p1.x - p0.x, ay = p1.y - p0.y, bx = p2.x - p0.x, by = p2.y - p0.y;
+    float inv = 1.0f / (ax * by - ay * bx);
+    r = (qx * by - qy * bx) * inv;
+    s = (ax * qy - ay * qx) * inv;
+}
+
+/* FP32 decision: 1 accept, 0 move (slot set), 2 undecided (escalate) */
+template <int DIM>
+__device__ __forceinline__ int decide_fast(float r, float s, float t, float g, int &slot) {
+    float l0 = DIM == 3 ? 1.0f - r - s - t : 1.0f - r - s;
+    float v[4] = {r, s, DIM == 3 ? t : l0, l0};
+    const int n = DIM == 3 ? 4 : 3;
+    int a = 0;
+    float d = v[0];
+    for (int q = 1; q < n; q++)
+        if (v[q] < d) {
+            d = v[q];
+            a = q;
+        }
+    if (d > g) return 1;
+    if (d >= -g) return 2;
+    float second = 3.0e38f;
+    for (int q = 0; q < n; q++)
+        if (q != a && v[q] < second) second = v[q];
+    if (second - d <= 2.0f * g) return 2;
+    slot = a;
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* walk (ompVC/mesh.c:496-611 and velocity.c:523-600)                       */
+/* ------------------------------------------------------------------------ */
+/* Walks from `e` towards (x,y,z).  Returns the accepting element, -1 when an
+ * adjacency of -1 is crossed, or -2 on error (err set).  Natural coordinates
+ * of the accepting element are returned in r,s,t (for interpolation). */
+template <int MODE, int DIM>
+__device__ __forceinline__ int walk(const DevMesh &m, double x, double y, double z, int e, int acc, double &r, double &s,
+                                    double &t, int &err) {
+    for (int hop = 0; hop < m.walk_limit; hop++) {
+        int slot = -1, dec;
+        if constexpr (MODE == P_F64) {
+            if (DIM == 3) nat3_exact(m, e, x, y, z, r, s, t);
+            else {
+                nat2_exact(m, e, x, y, r, s);
+                t = 0.0;
+            }
+            dec = decide_exact<DIM>(r, s, t, acc, slot);
+        } else {
+            int4 nd = m.conn[e];
+            float fr, fs, ft = 0.0f;
+            if (DIM == 3) nat3_fast(m, nd, x, y, z, fr, fs, ft);
+            else nat2_fast(m, nd, x, y, fr, fs);
+            dec = decide_fast<DIM>(fr, fs, ft, m.guard, slot);
+            if (dec == 2) {
+                if (DIM == 3) nat3_exact(m, e, x, y, z, r, s, t);
+                else {
+                    nat2_exact(m, e, x, y, r, s);
+                    t = 0.0;
+                }
+                dec = decide_exact<DIM>(r, s, t, acc, slot);
+            } else {
+                r = fr;
+                s = fs;
+                t = ft;
+            }
+        }
+        if (dec == 1) return e;
+        if (dec < 0) {
+            err = E_INDET;
+            return -2;
+        }
+        const int *ap = reinterpret_cast<const int *>(m.adj + e);
+        e = ap[slot];
+        if (e == -1) return -1;
+    }
+    err = E_WALK;
+    return -2;
+}
+
+/* ------------------------------------------------------------------------ */
+/* velocity loads                                                            */
+/* ------------------------------------------------------------------------ */
+template <int MODE>
+struct VelT {
+    using T = double;
+};
+template <>
+struct VelT<P_F32> {
+    using T = float;
+};
+template <>
+struct VelT<P_BF16> {
+    using T = float;
+};
+template <>
+struct VelT<P_F16> {
+    using T = float;
+};
+
+template <int MODE>
+__device__ __forceinline__ void load_vel(const void *base, float scale, int64_t n, typename VelT<MODE>::T out[3]) {
+    if constexpr (MODE == P_F64) {
+        double4 v = reinterpret_cast<const double4 *>(base)[n];
+        out[0] = v.x;
+        out[1] = v.y;
+        out[2] = v.z;
+    } else if constexpr (MODE == P_F32) {
+        float4 v = reinterpret_cast<const float4 *>(base)[n];
+        out[0] = v.x;
+        out[1] = v.y;
+        out[2] = v.z;
+    } else {
+        uint2 w = reinterpret_cast<const uint2 *>(base)[n];
+        unsigned short h0 = (unsigned short)(w.x & 0xffffu), h1 = (unsigned short)(w.x >> 16);
+        unsigned short h2 = (unsigned short)(w.y & 0xffffu);
+        if constexpr (MODE == P_BF16) {
+            out[0] = __uint_as_float((unsigned)h0 << 16) * scale;
+            out[1] = __uint_as_float((unsigned)h1 << 16) * scale;
+            out[2] = __uint_as_float((unsigned)h2 << 16) * scale;
+        } else {
+            out[0] = __half2float(__ushort_as_half(h0)) * scale;
+            out[1] = __half2float(__ushort_as_half(h1)) * scale;
+            out[2] = __half2float(__ushort_as_half(h2)) * scale;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* unstructured interpolation (ompVC/velocity.c:459-621)                     */
+/* ------------------------------------------------------------------------ */
+/* Probe state: element (walk start, updated in place) and a left flag. */
+struct Probe {
+    int elem;
+    int left;
+};
+
+template <int MODE, int DIM>
+__device__ __forceinline__ void vel_unstruct(const DevMesh &m, const DevField &f, double tloc, double x, double y,
+                                             double z, Probe &p, double out[3], int &err) {
+    if (p.elem < 0 || p.elem >= m.ne) {
+        err = E_ELEM;
+        out[0] = out[1] = out[2] = 0.0;
+        return;
+    }
+    double r, s, t;
+    int e = walk<MODE, DIM>(m, x, y, z, p.elem, ACC_INTERP, r, s, t, err);
+    if (e == -2) {
+        out[0] = out[1] = out[2] = 0.0;
+        return;
+    }
+    if (e == -1) {
+        p.elem = -1;
+        p.left = 1;
+        out[0] = out[1] = out[2] = 0.0;
+        return;
+    }
+    p.elem = e;
+    int4 nd = m.conn[e];
+    using VT = typename VelT<MODE>::T;
+    if constexpr (MODE == P_F64) {
+#pragma clang fp contract(off)
+        double a[4][3], b[4][3], n[4][3];
+        const int nn = DIM == 3 ? 4 : 3;
+        int ids[4] = {nd.x, nd.y, nd.z, nd.w};
+        for (int q = 0; q < nn; q++) {
+            load_vel<MODE>(f.lo, 1.0f, ids[q], a[q]);
+            load_vel<MODE>(f.hi, 1.0f, ids[q], b[q]);
+            for (int c = 0; c < 3; c++) n[q][c] = (1 - tloc) * a[q][c] + tloc * b[q][c];
+        }
+        double ar = fabs(r), as = fabs(s), at = fabs(t);
+        if (DIM == 3) {
+            for (int c = 0; c < 3; c++)
+                out[c] = n[0][c] + (n[1][c] - n[0][c]) * ar + (n[2][c] - n[0][c]) * as + (n[3][c] - n[0][c]) * at;
+        } else {
+            for (int c = 0; c < 2; c++) out[c] = n[0][c] + (n[1][c] - n[0][c]) * ar + (n[2][c] - n[0][c]) * as;
+            out[2] = 0.0;
+        }
+    } else {
+        const int nn = DIM == 3 ? 4 : 3;
+        int ids[4] = {nd.x, nd.y, nd.z, nd.w};
+        float tl = (float)tloc;
+        float n[4][3];
+        for (int q = 0; q < nn; q++) {
+            VT a[3], b[3];
+            load_vel<MODE>(f.lo, f.scale_lo, ids[q], a);
+            load_vel<MODE>(f.hi, f.scale_hi, ids[q], b);
+            for (int c = 0; c < 3; c++) n[q][c] = a[c] + tl * (b[c] - a[c]);
+        }
+        float ar = fabsf((float)r), as = fabsf((float)s), at = fabsf((float)t);
+        for (int c = 0; c < 3; c++) {
+            float v = n[0][c] + (n[1][c] - n[0][c]) * ar + (n[2][c] - n[0][c]) * as;
+            if (DIM == 3) v += (n[3][c] - n[0][c]) * at;
+            out[c] = (DIM == 2 && c == 2) ? 0.0 : (double)v;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Cartesian domain tests and interpolation (ompVC/velocity.c:257-457,624-669) */
+/* ------------------------------------------------------------------------ */
+__device__ __forceinline__ int outside_domain(const DevMesh &m, double x, double y, double z) {
+#pragma clang fp contract(off)
+    if (x < (m.bounds[0] - TINY) || x > (m.bounds[1] + TINY)) return 1;
+    if (y < (m.bounds[2] - TINY) || y > (m.bounds[3] + TINY)) return 1;
+    if (z < (m.bounds[4] - TINY) || z > (m.bounds[5] + TINY)) return 1;
+    return 0;
+}
+
+template <int DIM>
+__device__ __forceinline__ int outside_cart_vel(const DevMesh &m, const double X[3]) {
+#pragma clang fp contract(off)
+    for (int a = 0; a < DIM; a++) {
+        double q = (X[a] - m.cmin[a]) / m.cdel[a];
+        int lo = (int)floor(q), hi = (int)ceil(q);
+        if (lo < 0 || hi > m.cres[a] - 1) return 1;
+    }
+    return 0;
+}
+
+template <int MODE, int DIM>
+__device__ __forceinline__ void vel_cart(const DevMesh &m, const DevField &f, double tloc, const double X[3], int &left,
+                                         double out[3]) {
+#pragma clang fp contract(off)
+    if (outside_cart_vel<DIM>(m, X)) {
+        out[0] = out[1] = out[2] = 0.0;
+        left = 1;
+        return;
+    }
+    int cell[3] = {0, 0, 0};
+    double loc[3] = {0.0, 0.0, 0.0};
+    for (int a = 0; a < DIM; a++) {
+        cell[a] = (int)floor((X[a] - m.cmin[a]) / m.cdel[a]);
+        if (cell[a] == m.cres[a] - 1) {
+            cell[a] = m.cres[a] - 2;
+            loc[a] = 1.0;
+        } else {
+            loc[a] = (X[a] - m.cmin[a] - cell[a] * m.cdel[a]) / m.cdel[a];
+        }
+    }
+    const int64_t nx = m.cres[0], ny = m.cres[1];
+    const int i = cell[0], j = cell[1], k = cell[2], k1 = DIM == 3 ? k + 1 : k;
+    int64_t c[8] = {((int64_t)k * ny + j) * nx + i,           ((int64_t)k1 * ny + j) * nx + i,
+                    ((int64_t)k * ny + j + 1) * nx + i,       ((int64_t)k1 * ny + j + 1) * nx + i,
+                    ((int64_t)k * ny + j) * nx + i + 1,       ((int64_t)k1 * ny + j) * nx + i + 1,
+                    ((int64_t)k * ny + j + 1) * nx + i + 1,   ((int64_t)k1 * ny + j + 1) * nx + i + 1};
+    if constexpr (MODE == P_F64) {
+        double tl = tloc, xl = loc[0], yl = loc[1], zl = loc[2];
+        /* weights w[slot][corner], corner bits (x,y,z) = (4,2,1); products left to right */
+        double w0[8], w1[8];
+        for (int q = 0; q < 8; q++) {
+            double wx = (q & 4) ? xl : (1 - xl), wy = (q & 2) ? yl : (1 - yl), wz = (q & 1) ? zl : (1 - zl);
+            w1[q] = (tl) * wx * wy * wz;
+            w0[q] = (1 - tl) * wx * wy * wz;
+        }
+        double A[8][3], B[8][3];
+        for (int q = 0; q < 8; q++) {
+            load_vel<MODE>(f.lo, 1.0f, c[q], A[q]);
+            load_vel<MODE>(f.hi, 1.0f, c[q], B[q]);
+        }
+        for (int cc = 0; cc < 3; cc++) {
+            double acc = A[0][cc] * w0[0];
+            for (int q = 1; q < 8; q++) acc = acc + A[q][cc] * w0[q];
+            for (int q = 0; q < 8; q++) acc = acc + B[q][cc] * w1[q];
+            out[cc] = acc;
+        }
+    } else {
+        using VT = typename VelT<MODE>::T;
+        float tl = (float)tloc, xl = (float)loc[0], yl = (float)loc[1], zl = (float)loc[2];
+        float acc[3] = {0.f, 0.f, 0.f};
+        const int nq = DIM == 3 ? 8 : 8;
+        for (int q = 0; q < nq; q++) {
+            if (DIM == 2 && (q & 1)) continue; /* 2-D: the k+1 plane repeats k with zero weight */
+            float wx = (q & 4) ? xl : (1.f - xl), wy = (q & 2) ? yl : (1.f - yl);
+            float wz = DIM == 3 ? ((q & 1) ? zl : (1.f - zl)) : 1.f;
+            float w = wx * wy * wz;
+            VT a[3], b[3];
+            load_vel<MODE>(f.lo, f.scale_lo, c[q], a);
+            load_vel<MODE>(f.hi, f.scale_hi, c[q], b);
+            for (int cc = 0; cc < 3; cc++) acc[cc] += w * (a[cc] + tl * (b[cc] - a[cc]));
+        }
+        out[0] = acc[0];
+        out[1] = acc[1];
+        out[2] = DIM == 3 ? (double)acc[2] : 0.0;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* particle state and integrators (ompVC/integration.c)                     */
+/* ------------------------------------------------------------------------ */
+struct PState {
+    double X[3];
+    double V[3];
+    int elem;
+    int left;
+    double left_time;
+};
+
+__device__ __forceinline__ double sign_of(double a, double b) { return b >= 0.0 ? fabs(a) : -fabs(a); }
+
+/* GetVelocity on the particle itself (side effects on elem / left). */
+template <int MODE, int DIM>
+__device__ __forceinline__ void get_velocity(const DevMesh &m, const DevField &f, double tq, PState &p, double out[3],
+                                             int &err) {
+#pragma clang fp contract(off)
+    double tloc = (tq - f.tmin) / (f.tmax - f.tmin);
+    if (m.mesh_type == 1) {
+        if (p.elem < 0 || p.elem >= m.ne) {
+            err = E_ELEM;
+            return;
+        }
+        if ((tloc > 1 + TINY) || (tloc < 0 - TINY)) {
+            err = E_TLOC;
+            return;
+        }
+        Probe pr{p.elem, p.left};
+        vel_unstruct<MODE, DIM>(m, f, tloc, p.X[0], p.X[1], p.X[2], pr, out, err);
+        p.elem = pr.elem;
+        p.left = pr.left;
+    } else {
+        int lft = 0;
+        if (outside_cart_vel<DIM>(m, p.X)) {
+            out[0] = out[1] = out[2] = 0.0;
+            p.left = 1;
+            return;
+        }
+        if ((tloc > 1 + TINY) || (tloc < 0 - TINY)) {
+            err = E_TLOC;
+            return;
+        }
+        vel_cart<MODE, DIM>(m, f, tloc, p.X, lft, out);
+        if (lft) p.left = 1;
+    }
+}
+
+/* velocity at a probe position for a stage (probe owns its element / flag) */
+template <int MODE, int DIM, int UNSTRUCT>
+__device__ __forceinline__ void stage_velocity(const DevMesh &m, const DevField &f, double tq, const double X[3],
+                                               Probe &pr, double out[3], int &err) {
+#pragma clang fp contract(off)
+    double tloc = (tq - f.tmin) / (f.tmax - f.tmin);
+    if (UNSTRUCT) {
+        if (pr.elem == -1) {
+            err = E_ELEM;
+            return;
+        }
+        if ((tloc > 1 + TINY) || (tloc < 0 - TINY)) {
+            err = E_TLOC;
+            return;
+        }
+        vel_unstruct<MODE, DIM>(m, f, tloc, X[0], X[1], X[2], pr, out, err);
+    } else {
+        if (outside_cart_vel<DIM>(m, X)) {
+            out[0] = out[1] = out[2] = 0.0;
+            pr.left = 1;
+            return;
+        }
+        if ((tloc > 1 + TINY) || (tloc < 0 - TINY)) {
+            err = E_TLOC;
+            return;
+        }
+        int lft = 0;
+        vel_cart<MODE, DIM>(m, f, tloc, X, lft, out);
+        if (lft) pr.left = 1;
+    }
+}
+
+/* RK4 (ompVC/integration.c:317-462).  Returns the stop time; counts steps. */
+template <int MODE, int DIM, int UNSTRUCT>
+__device__ double rk4(const DevMesh &m, const DevField &f, PState &p, double t0, double t1, double dt, int extrap,
+                      int &err, uint32_t &nsteps) {
+#pragma clang fp contract(off)
+    if (p.left) {
+        err = E_LEFT;
+        return t1;
+    }
+    if (UNSTRUCT && (p.elem < 0 || p.elem >= m.ne)) {
+        err = E_ELEM;
+        return t1;
+    }
+    Probe mp[4];
+    for (int q = 0; q < 4; q++) {
+        mp[q].elem = p.elem;
+        mp[q].left = 0;
+    }
+    double tc = t0, h = sign_of(dt, t1 - t0);
+    const double hmin = 1e-12 * fabs(dt);
+    while ((tc - t1) * (t1 - t0) < 0.0) {
+        if ((tc + h - t1) * (tc + h - t0) > 0.0) h = sign_of(t1 - tc, t1 - t0);
+        double k1[3], k2[3], k3[3], k4[3], Y[3];
+        stage_velocity<MODE, DIM, UNSTRUCT>(m, f, tc, p.X, mp[0], k1, err);
+        if (err) return t1;
+        for (int c = 0; c < 3; c++) Y[c] = p.X[c] + 0.5 * k1[c] * h;
+        stage_velocity<MODE, DIM, UNSTRUCT>(m, f, tc + h / 2, Y, mp[1], k2, err);
+        if (err) return t1;
+        for (int c = 0; c < 3; c++) Y[c] = p.X[c] + 0.5 * k2[c] * h;
+        stage_velocity<MODE, DIM, UNSTRUCT>(m, f, tc + h / 2, Y, mp[2], k3, err);
+        if (err) return t1;
+        for (int c = 0; c < 3; c++) Y[c] = p.X[c] + k3[c] * h;
+        stage_velocity<MODE, DIM, UNSTRUCT>(m, f, tc + h, Y, mp[3], k4, err);
+        if (err) return t1;
+        double inc[3];
+        for (int c = 0; c < 3; c++) {
+            inc[c] = h * (k1[c] + 2 * k2[c] + 2 * k3[c] + k4[c]) / 6;
+            p.X[c] = p.X[c] + inc[c];
+        }
+        tc = tc + h;
+        if (fabs(h) > hmin) nsteps++;
+        bool out;
+        if (UNSTRUCT) {
+            double r, s, t;
+            int e = walk<MODE, DIM>(m, p.X[0], p.X[1], p.X[2], p.elem, ACC_SEARCH, r, s, t, err);
+            if (e == -2) return t1;
+            p.elem = e;
+            out = e < 0;
+        } else {
+            out = outside_domain(m, p.X[0], p.X[1], p.X[2]);
+        }
+        if (out || mp[0].left || mp[1].left || mp[2].left || mp[3].left) {
+            p.left = 1;
+            p.left_time = tc - h;
+            for (int c = 0; c < 3; c++) p.X[c] = p.X[c] - inc[c];
+            return p.left_time;
+        }
+        if (extrap) {
+            get_velocity<MODE, DIM>(m, f, tc, p, p.V, err);
+            if (err) return t1;
+        }
+    }
+    return t1;
+}
+
+/* Euler (ompVC/integration.c:222-311) */
+template <int MODE, int DIM, int UNSTRUCT>
+__device__ double euler(const DevMesh &m, const DevField &f, PState &p, double t0, double t1, double dt, int extrap,
+                        int &err, uint32_t &nsteps) {
+#pragma clang fp contract(off)
+    if (p.left) {
+        err = E_LEFT;
+        return t1;
+    }
+    if (UNSTRUCT && (p.elem < 0 || p.elem >= m.ne)) {
+        err = E_ELEM;
+        return t1;
+    }
+    double tc = t0, h = sign_of(dt, t1 - t0);
+    const double hmin = 1e-12 * fabs(dt);
+    while ((tc - t1) * (t1 - t0) < 0.0) {
+        if ((tc + h - t1) * (tc + h - t0) > 0.0) h = sign_of(t1 - tc, t1 - t0);
+        double ve[3];
+        get_velocity<MODE, DIM>(m, f, tc, p, ve, err);
+        if (err) return t1;
+        for (int c = 0; c < 3; c++) p.X[c] = p.X[c] + h * ve[c];
+        tc = tc + h;
+        if (fabs(h) > hmin) nsteps++;
+        bool out;
+        if (UNSTRUCT) {
+            double r, s, t;
+            int e = p.elem < 0 ? -1 : walk<MODE, DIM>(m, p.X[0], p.X[1], p.X[2], p.elem, ACC_SEARCH, r, s, t, err);
+            if (e == -2) return t1;
+            p.elem = e;
+            out = e < 0;
+        } else {
+            out = outside_domain(m, p.X[0], p.X[1], p.X[2]);
+        }
+        if (out) {
+            p.left = 1;
+            p.left_time = tc - h;
+            for (int c = 0; c < 3; c++) p.X[c] = p.X[c] - h * ve[c];
+            return p.left_time;
+        }
+        if (extrap) {
+            get_velocity<MODE, DIM>(m, f, tc, p, p.V, err);
+            if (err) return t1;
+        }
+    }
+    return t1;
+}
+
+/* Advect (ompVC/integration.c:22-45) */
+template <int MODE, int DIM, int UNSTRUCT>
+__device__ __forceinline__ double advect(const DevMesh &m, const DevField &f, PState &p, double t1, double t2, int itype,
+                                         double dt, int extrap, int &err, uint32_t &nsteps) {
+#pragma clang fp contract(off)
+    if (p.left && extrap) {
+        for (int c = 0; c < 3; c++) p.X[c] = p.X[c] + p.V[c] * (t2 - t1);
+        return t2;
+    }
+    double stop = itype == 1 ? rk4<MODE, DIM, UNSTRUCT>(m, f, p, t1, t2, dt, extrap, err, nsteps)
+                             : euler<MODE, DIM, UNSTRUCT>(m, f, p, t1, t2, dt, extrap, err, nsteps);
+    if ((fabs(stop - t2) > TINY) && extrap)
+        for (int c = 0; c < 3; c++) p.X[c] = p.X[c] + p.V[c] * (t2 - stop);
+    return stop;
+}
+
+/* ------------------------------------------------------------------------ */
+/* largest eigenvalue of a symmetric 3x3, cyclic Jacobi rotations (FP64)     */
+/* (Golub & Van Loan, Matrix Computations, sec. 8.5)                          */
+/* ------------------------------------------------------------------------ */
+__device__ __forceinline__ double max_eig_sym3(double a[3][3], int &err) {
+#pragma clang fp contract(off)
+    for (int i = 0; i < 3; i++)
+        for (int j = i + 1; j < 3; j++) a[i][j] = a[j][i] = 0.5 * (a[i][j] + a[j][i]);
+    for (int sweep = 0; sweep < 64; sweep++) {
+        double off = fabs(a[0][1]) + fabs(a[0][2]) + fabs(a[1][2]);
+        double diag = fabs(a[0][0]) + fabs(a[1][1]) + fabs(a[2][2]);
+        if (off == 0.0 || off <= 1e-300 || off < 1e-18 * diag) break;
+        if (sweep == 63) err = E_EIG;
+#pragma unroll
+        for (int pq = 0; pq < 3; pq++) {
+            int p = pq == 2 ? 1 : 0, q = pq == 0 ? 1 : 2;
+            double apq = a[p][q];
+            if (apq == 0.0) continue;
+            double theta = (a[q][q] - a[p][p]) / (2.0 * apq);
+            double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+            if (!isfinite(theta) || fabs(theta) > 1e150) t = 0.5 / theta;
+            double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+            double app = a[p][p], aqq = a[q][q];
+            a[p][p] = app - t * apq;
+            a[q][q] = aqq + t * apq;
+            a[p][q] = a[q][p] = 0.0;
+            int r = 3 - p - q;
+            double arp = a[r][p], arq = a[r][q];
+            a[r][p] = a[p][r] = c * arp - s * arq;
+            a[r][q] = a[q][r] = s * arp + c * arq;
+        }
+    }
+    double m = a[0][0] > a[1][1] ? a[0][0] : a[1][1];
+    return m > a[2][2] ? m : a[2][2];
+}
+
+} // namespace fvc

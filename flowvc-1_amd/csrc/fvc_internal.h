@@ -1,0 +1,128 @@
+/*
+ * fvc_internal.h - shared between the C ABI (fvc_api.hip) and the kernel
+ * launchers (fvc_kernels.hip).  Not part of the public interface.
+ */
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "fvc.h"
+#include "fvc_device.h"
+
+struct fvc_ctx {
+    int device;
+    int precision;
+    hipStream_t stream;
+    fvc::DevMesh mesh;
+    /* device allocations */
+    double4 *X64;
+    float4 *X32;
+    int4 *conn, *adj;
+    int *cell_start, *cell_elems;
+    int64_t ncells, nentries;
+    /* resident velocity slices */
+    int max_slices;
+    std::vector<void *> slices;
+    std::vector<float> slice_scale;
+    std::vector<double> slice_time;
+    int64_t nvel;       /* nodes per slice */
+    size_t vel_stride;  /* bytes per node */
+    /* errors and counters */
+    int *d_err;
+    int64_t *d_where;
+    unsigned long long *d_steps;
+    int64_t last_steps;
+    std::string last_error;
+    int64_t bytes_used;
+    /* scratch (grown on demand) */
+    void *scratch;
+    size_t scratch_bytes;
+};
+
+struct fvc_particles {
+    fvc_ctx *ctx;
+    int64_t cap;
+    void *buf;
+    double *X[3];
+    double *V[3];
+    int *elem, *aux, *left;
+    double *left_time, *scalar;
+};
+
+struct fvc_ftle {
+    fvc_ctx *ctx;
+    double mn[3], mx[3], del[3];
+    int32_t res[3];
+    int64_t n;
+    double start_time, stop_time;
+    int status; /* 0 unlaunched, 1 launched, 2 complete */
+    fvc_particles *pts;
+    /* per point */
+    int *have;
+    double *wt, *wot;
+    /* staging of one frame */
+    double *Xn[3];
+    int *elem_new;
+    int *exited;   /* 1 when the point left during this frame */
+    int *advected; /* 1 when the point was integrated this frame */
+    double *exit_time;
+    double *Vn[3];
+    int *active;    /* compacted indices of points to integrate */
+    int64_t nactive;
+    void *buf;
+    int64_t inside;
+};
+
+/* launchers (fvc_kernels.hip); all return FVC_OK or a negative code */
+namespace fvc {
+struct PtrSet {
+    double *X[3];
+    double *V[3];
+    int *elem;
+    int *left;
+    double *left_time;
+};
+
+inline PtrSet ptrs(fvc_particles *p) {
+    PtrSet s;
+    for (int c = 0; c < 3; c++) {
+        s.X[c] = p->X[c];
+        s.V[c] = p->V[c];
+    }
+    s.elem = p->elem;
+    s.left = p->left;
+    s.left_time = p->left_time;
+    return s;
+}
+
+DevField bind_field(fvc_ctx *c, int lo, int hi, double tmin, double tmax);
+
+int launch_velocity(fvc_ctx *c, PtrSet p, int64_t begin, int64_t count, double t, const DevField &f);
+int launch_advect(fvc_ctx *c, PtrSet p, const int *index, int64_t begin, int64_t count, double t1, double t2,
+                  const DevField &f, const fvc_int_params &ip);
+int launch_locate(fvc_ctx *c, const double *dxyz, const int *dguess, int64_t n, int mode, int *dout);
+/* chain: 0 = point list (ReadInTraceMesh), 1 = tracer grid (GenerateTracerMesh), 2 = FTLE grid */
+int launch_seed_chain(fvc_ctx *c, int chain, int64_t n, const double *dxyz, const double mn[3], const double del[3],
+                      const int32_t res[3], int lsc, int *d_elem, int *d_left, double *d_left_time);
+int launch_ftle_frame(fvc_ftle *f, int lo, int hi, double tmin, double tmax, double t1, double t2,
+                      const fvc_int_params &ip);
+int launch_ftle_fill(fvc_ftle *f, double int_t);
+int launch_ftle_mask(fvc_ftle *f);
+int launch_compact(fvc_ctx *c, const int *flags, int64_t n, int *out_idx, int64_t *count);
+int launch_zsort(fvc_ctx *c, const double *dxyz, int64_t n, const double lo[3], const double hi[3], int *dperm);
+
+int check_err(fvc_ctx *c, const char *what);
+void *scratch(fvc_ctx *c, size_t bytes);
+} // namespace fvc
+
+#define FVC_HIP(call)                                                                                                  \
+    do {                                                                                                               \
+        hipError_t e_ = (call);                                                                                        \
+        if (e_ != hipSuccess) {                                                                                        \
+            c->last_error = std::string(#call) + ": " + hipGetErrorString(e_);                                         \
+            return e_ == hipErrorOutOfMemory ? FVC_E_NOMEM : FVC_E_HIP;                                                \
+        }                                                                                                              \
+    } while (0)

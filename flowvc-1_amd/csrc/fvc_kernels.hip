@@ -1,0 +1,1011 @@
+/*
+ * fvc_kernels.hip - gfx950 kernels of the advection path and their launchers.
+ *
+ *  k_advect        Advect over an interval, one lane per particle, in place
+ *                  (the loop of ompVC/tracers.c:224-229).
+ *  k_launch_vel    GetVelocity at release time with its side effects.
+ *  k_ftle_*        one frame of ComputeFTLE for a slide: staged advection,
+ *                  early FTLE for exits, update, gradient + largest eigenvalue.
+ *  k_locate_*      walk / spatial-hash global search (lowest passing index).
+ *  k_seed_*        exact seed-location chains: a parallel pass finds, per
+ *                  point, the lowest containing element and whether more than
+ *                  one element contains it; one wave then replays the
+ *                  reference's sequential chain, walking only where the
+ *                  result can depend on the chain's guess.
+ *  k_morton/k_rs_* Z-order keys and a stable LSD radix sort (wave64 ballot
+ *                  ranking).
+ *  k_cmp_*         stable stream compaction (ballot + block prefix).
+ * Determinism: no output depends on atomic ordering; counters are integer sums.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "fvc_internal.h"
+
+namespace fvc {
+
+constexpr int BLOCK = 256;
+
+/* ------------------------------------------------------------------------ */
+/* helpers                                                                   */
+/* ------------------------------------------------------------------------ */
+__device__ inline void add_steps(unsigned long long *ctr, uint32_t ns) {
+    __shared__ unsigned int red[BLOCK / 64];
+    unsigned long long v = ns;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) red[w] = (unsigned int)v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long s = 0;
+        for (int q = 0; q < (int)(blockDim.x >> 6); q++) s += red[q];
+        if (s) atomicAdd(ctr, s);
+    }
+}
+
+__device__ inline void load_state(const PtrSet &p, int64_t i, PState &s) {
+    for (int c = 0; c < 3; c++) {
+        s.X[c] = p.X[c][i];
+        s.V[c] = p.V[c][i];
+    }
+    s.elem = p.elem[i];
+    s.left = p.left[i];
+    s.left_time = p.left_time[i];
+}
+
+__device__ inline void store_state(const PtrSet &p, int64_t i, const PState &s) {
+    for (int c = 0; c < 3; c++) {
+        p.X[c][i] = s.X[c];
+        p.V[c][i] = s.V[c];
+    }
+    p.elem[i] = s.elem;
+    p.left[i] = s.left;
+    p.left_time[i] = s.left_time;
+}
+
+/* ------------------------------------------------------------------------ */
+/* advect / launch velocity                                                   */
+/* ------------------------------------------------------------------------ */
+template <int MODE, int DIM, int UNS>
+__global__ __launch_bounds__(BLOCK) void k_advect(DevMesh m, DevField f, PtrSet p, const int *__restrict__ index,
+                                                 int64_t begin, int64_t count, double t1, double t2, int itype,
+                                                 double dt, int extrap, ErrBuf eb, unsigned long long *steps) {
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t ns = 0;
+    if (q < count) {
+        int64_t i = index ? (int64_t)index[q] : begin + q;
+        double lt = p.left_time[i];
+        int lf = p.left[i];
+        if (lt > -TINY && (!lf || extrap)) {
+            PState s;
+            load_state(p, i, s);
+            int err = 0;
+            advect<MODE, DIM, UNS>(m, f, s, t1, t2, itype, dt, extrap, err, ns);
+            if (err)
+                raise_err(eb, err, i);
+            else
+                store_state(p, i, s);
+        }
+    }
+    add_steps(steps, ns);
+}
+
+template <int MODE, int DIM, int UNS>
+__global__ __launch_bounds__(BLOCK) void k_launch_vel(DevMesh m, DevField f, PtrSet p, int64_t begin, int64_t count,
+                                                     double t, ErrBuf eb) {
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= count) return;
+    int64_t i = begin + q;
+    if (p.left[i]) return;
+    PState s;
+    load_state(p, i, s);
+    int err = 0;
+    double v[3];
+    get_velocity<MODE, DIM>(m, f, t, s, v, err);
+    if (err) {
+        raise_err(eb, err, i);
+        return;
+    }
+    for (int c = 0; c < 3; c++) s.V[c] = v[c];
+    store_state(p, i, s);
+}
+
+#define FVC_DISPATCH(MODE_, DIM_, UNS_, KERNEL, ...)                                                                   \
+    do {                                                                                                               \
+        if (UNS_) {                                                                                                    \
+            if (DIM_ == 3) {                                                                                           \
+                switch (MODE_) {                                                                                       \
+                case P_F64: KERNEL<P_F64, 3, 1> __VA_ARGS__; break;                                                     \
+                case P_F32: KERNEL<P_F32, 3, 1> __VA_ARGS__; break;                                                     \
+                case P_BF16: KERNEL<P_BF16, 3, 1> __VA_ARGS__; break;                                                   \
+                default: KERNEL<P_F16, 3, 1> __VA_ARGS__; break;                                                        \
+                }                                                                                                      \
+            } else {                                                                                                   \
+                switch (MODE_) {                                                                                       \
+                case P_F64: KERNEL<P_F64, 2, 1> __VA_ARGS__; break;                                                     \
+                case P_F32: KERNEL<P_F32, 2, 1> __VA_ARGS__; break;                                                     \
+                case P_BF16: KERNEL<P_BF16, 2, 1> __VA_ARGS__; break;                                                   \
+                default: KERNEL<P_F16, 2, 1> __VA_ARGS__; break;                                                        \
+                }                                                                                                      \
+            }                                                                                                          \
+        } else {                                                                                                       \
+            if (DIM_ == 3) {                                                                                           \
+                switch (MODE_) {                                                                                       \
+                case P_F64: KERNEL<P_F64, 3, 0> __VA_ARGS__; break;                                                     \
+                case P_F32: KERNEL<P_F32, 3, 0> __VA_ARGS__; break;                                                     \
+                case P_BF16: KERNEL<P_BF16, 3, 0> __VA_ARGS__; break;                                                   \
+                default: KERNEL<P_F16, 3, 0> __VA_ARGS__; break;                                                        \
+                }                                                                                                      \
+            } else {                                                                                                   \
+                switch (MODE_) {                                                                                       \
+                case P_F64: KERNEL<P_F64, 2, 0> __VA_ARGS__; break;                                                     \
+                case P_F32: KERNEL<P_F32, 2, 0> __VA_ARGS__; break;                                                     \
+                case P_BF16: KERNEL<P_BF16, 2, 0> __VA_ARGS__; break;                                                   \
+                default: KERNEL<P_F16, 2, 0> __VA_ARGS__; break;                                                        \
+                }                                                                                                      \
+            }                                                                                                          \
+        }                                                                                                              \
+    } while (0)
+
+static inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
+
+static ErrBuf errbuf(fvc_ctx *c) { return ErrBuf{c->d_err, c->d_where}; }
+
+int launch_velocity(fvc_ctx *c, PtrSet p, int64_t begin, int64_t count, double t, const DevField &f) {
+    if (count <= 0) return FVC_OK;
+    FVC_DISPATCH(c->precision, c->mesh.dims, c->mesh.mesh_type, k_launch_vel,
+                 <<<grid_for(count, BLOCK), BLOCK, 0, c->stream>>>(c->mesh, f, p, begin, count, t, errbuf(c)));
+    FVC_HIP(hipGetLastError());
+    return check_err(c, "fvc_launch_velocity");
+}
+
+int launch_advect(fvc_ctx *c, PtrSet p, const int *index, int64_t begin, int64_t count, double t1, double t2,
+                  const DevField &f, const fvc_int_params &ip) {
+    FVC_HIP(hipMemsetAsync(c->d_steps, 0, sizeof(unsigned long long), c->stream));
+    if (count > 0) {
+        int block = ip.block > 0 ? ip.block : BLOCK;
+        if (block != BLOCK) block = BLOCK; /* kernels are compiled for 256-lane blocks */
+        FVC_DISPATCH(c->precision, c->mesh.dims, c->mesh.mesh_type, k_advect,
+                     <<<grid_for(count, block), block, 0, c->stream>>>(c->mesh, f, p, index, begin, count, t1, t2,
+                                                                       ip.int_type == FVC_INT_RK4 ? 1 : 0, ip.time_step,
+                                                                       ip.extrapolate, errbuf(c), c->d_steps));
+        FVC_HIP(hipGetLastError());
+    }
+    unsigned long long steps = 0;
+    FVC_HIP(hipMemcpyAsync(&steps, c->d_steps, sizeof steps, hipMemcpyDeviceToHost, c->stream));
+    int rc = check_err(c, "fvc_advect");
+    c->last_steps = (int64_t)steps;
+    return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* location                                                                   */
+/* ------------------------------------------------------------------------ */
+/* lowest-index element with d > -TINY (mesh.c:321-363) and whether a second
+ * element also passes; cell lists are ascending, so the first pass is the
+ * lowest index. */
+template <int DIM>
+__device__ int hash_global(const DevMesh &m, double x, double y, double z, int &second) {
+    second = -1;
+    double fx = (x - m.hlo[0]) * m.hinv[0], fy = (y - m.hlo[1]) * m.hinv[1];
+    double fz = DIM == 3 ? (z - m.hlo[2]) * m.hinv[2] : 0.0;
+    if (!(fx >= 0.0 && fy >= 0.0 && fz >= 0.0)) return -1;
+    if (!(fx < (double)m.hres[0] && fy < (double)m.hres[1] && fz < (double)m.hres[2])) return -1;
+    int ix = (int)fx, iy = (int)fy, iz = (int)fz;
+    int64_t cell = ((int64_t)iz * m.hres[1] + iy) * m.hres[0] + ix;
+    int first = -1;
+    for (int q = m.cell_start[cell]; q < m.cell_start[cell + 1]; q++) {
+        int e = m.cell_elems[q];
+        double r, s, t = 0.0;
+        if (DIM == 3) nat3_exact(m, e, x, y, z, r, s, t);
+        else nat2_exact(m, e, x, y, r, s);
+        int slot;
+        if (decide_exact<DIM>(r, s, t, ACC_SEARCH, slot) == 1) {
+            if (first < 0) first = e;
+            else {
+                second = e;
+                break;
+            }
+        }
+    }
+    return first;
+}
+
+template <int DIM>
+__device__ int local_search(const DevMesh &m, double x, double y, double z, int guess, int &err) {
+    double r, s, t;
+    int e = walk<P_F64, DIM>(m, x, y, z, guess, ACC_SEARCH, r, s, t, err);
+    return e == -2 ? -1 : e;
+}
+
+template <int DIM>
+__global__ __launch_bounds__(BLOCK) void k_locate(DevMesh m, const double *__restrict__ xyz, const int *__restrict__ guess,
+                                                 int64_t n, int mode, int *__restrict__ out, ErrBuf eb) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double x = xyz[3 * i], y = xyz[3 * i + 1], z = DIM == 3 ? xyz[3 * i + 2] : 0.0;
+    int res = -1;
+    if (mode != FVC_LOCATE_GLOBAL) {
+        int g = guess[i];
+        if (g >= 0 && g < m.ne) {
+            int err = 0;
+            res = local_search<DIM>(m, x, y, z, g, err);
+            if (err) {
+                raise_err(eb, err, i);
+                res = -1;
+            }
+        }
+    }
+    if (mode == FVC_LOCATE_GLOBAL || (mode == FVC_LOCATE_WALK_GLOBAL && res < 0)) {
+        int second;
+        res = hash_global<DIM>(m, x, y, z, second);
+    }
+    out[i] = res;
+}
+
+int launch_locate(fvc_ctx *c, const double *dxyz, const int *dguess, int64_t n, int mode, int *dout) {
+    if (n <= 0) return FVC_OK;
+    if (c->mesh.dims == 3)
+        k_locate<3><<<grid_for(n, BLOCK), BLOCK, 0, c->stream>>>(c->mesh, dxyz, dguess, n, mode, dout, errbuf(c));
+    else
+        k_locate<2><<<grid_for(n, BLOCK), BLOCK, 0, c->stream>>>(c->mesh, dxyz, dguess, n, mode, dout, errbuf(c));
+    FVC_HIP(hipGetLastError());
+    return check_err(c, "fvc_locate");
+}
+
+/* ------------------------------------------------------------------------ */
+/* seed chains                                                                */
+/* ------------------------------------------------------------------------ */
+enum { SF_OUTB = 1, SF_MULTI = 2 };
+
+struct ChainGeom {
+    int chain;
+    const double *xyz; /* list chain */
+    double mn[3], del[3];
+    int res[3];
+};
+
+/* chain-order point i -> coordinates (ftle.c:412-420, tracers.c:1344-1346) */
+__device__ inline void chain_point(const ChainGeom &g, int64_t i, double X[3]) {
+#pragma clang fp contract(off)
+    if (g.chain == 0) {
+        X[0] = g.xyz[3 * i];
+        X[1] = g.xyz[3 * i + 1];
+        X[2] = g.xyz[3 * i + 2];
+        return;
+    }
+    int ii, jj, kk;
+    if (g.chain == 1) { /* k, j, i order: x fastest */
+        ii = (int)(i % g.res[0]);
+        jj = (int)((i / g.res[0]) % g.res[1]);
+        kk = (int)(i / ((int64_t)g.res[0] * g.res[1]));
+    } else { /* FTLE: i, j, k order: z fastest */
+        kk = (int)(i % g.res[2]);
+        jj = (int)((i / g.res[2]) % g.res[1]);
+        ii = (int)(i / ((int64_t)g.res[2] * g.res[1]));
+    }
+    X[0] = g.mn[0] + ii * g.del[0];
+    X[1] = g.mn[1] + jj * g.del[1];
+    X[2] = g.mn[2] + kk * g.del[2];
+}
+
+template <int DIM>
+__global__ __launch_bounds__(BLOCK) void k_seed_prepare(DevMesh m, ChainGeom g, int64_t n, int2 *__restrict__ info) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double X[3];
+    chain_point(g, i, X);
+    int flags = 0, emin = -1;
+    if (outside_domain(m, X[0], X[1], X[2]))
+        flags |= SF_OUTB;
+    else if (m.mesh_type == 1) {
+        int second;
+        emin = hash_global<DIM>(m, X[0], X[1], X[2], second);
+        if (second >= 0) flags |= SF_MULTI;
+    }
+    info[i] = make_int2(emin, flags);
+}
+
+/* One wave replays the chain; lanes hold 64 consecutive points' data and the
+ * per-point step is executed uniformly by the whole wave. */
+template <int DIM>
+__global__ __launch_bounds__(64) void k_seed_chain(DevMesh m, ChainGeom g, int64_t n, const int2 *__restrict__ info,
+                                                  int lsc, int seed0, int *__restrict__ out_elem,
+                                                  int *__restrict__ out_left, double *__restrict__ out_lt, ErrBuf eb) {
+    const int lane = threadIdx.x;
+    const bool uns = m.mesh_type == 1;
+    int found = 0;                 /* list chain */
+    int seed = seed0, guess = seed0 >= 0 ? seed0 : -1;
+    int iguess = -1, jguess = -1;  /* FTLE chain */
+    for (int64_t base = 0; base < n; base += 64) {
+        int2 my = base + lane < n ? info[base + lane] : make_int2(-1, SF_OUTB);
+        int r_elem = -1, r_left = 0;
+        double r_lt = 0.0;
+        int lim = (int)min((int64_t)64, n - base);
+        for (int l = 0; l < lim; l++) {
+            int64_t i = base + l;
+            int emin = __shfl(my.x, l, 64), flags = __shfl(my.y, l, 64);
+            int elem = -1, left = (flags & SF_OUTB) ? 1 : 0;
+            double lt = 0.0;
+            int kk = 0, jj = 0;
+            if (g.chain == 2) {
+                kk = (int)(i % g.res[2]);
+                jj = (int)((i / g.res[2]) % g.res[1]);
+                if (kk == 0) {
+                    jguess = -1;
+                    if (jj == 0) iguess = -1;
+                }
+            }
+            if (!left && uns) {
+                double X[3];
+                chain_point(g, i, X);
+                const bool direct = lsc && !(flags & SF_MULTI); /* result cannot depend on the guess */
+                int err = 0, index = -1;
+                if (g.chain == 0) {
+                    if (!found) {
+                        index = emin;
+                    } else {
+                        index = direct ? emin : local_search<DIM>(m, X[0], X[1], X[2], guess, err);
+                        if (!direct && index < 0 && lsc) index = emin;
+                    }
+                    if (index < 0) {
+                        left = 1;
+                        elem = -1;
+                    } else {
+                        elem = index;
+                        guess = index;
+                        found++;
+                    }
+                } else {
+                    if (seed < 0) {
+                        seed = emin;
+                        if (seed < 0) {
+                            left = 1;
+                            elem = -1;
+                            lt = -1.0;
+                        } else {
+                            elem = seed;
+                            guess = seed;
+                            if (g.chain == 2) jguess = iguess = seed;
+                        }
+                    } else {
+                        if (direct)
+                            index = emin;
+                        else {
+                            index = local_search<DIM>(m, X[0], X[1], X[2], guess, err);
+                            if (index < 0 && !err) index = local_search<DIM>(m, X[0], X[1], X[2], seed, err);
+                            if (index < 0 && !err && lsc) index = emin;
+                        }
+                        if (index < 0) {
+                            left = 1;
+                            elem = -1;
+                            lt = -1.0;
+                        } else {
+                            elem = index;
+                            guess = index;
+                            if (g.chain == 2) {
+                                if (jguess < 0) jguess = index;
+                                if (iguess < 0) iguess = index;
+                            }
+                        }
+                    }
+                }
+                if (err && lane == 0) raise_err(eb, err, i);
+            }
+            if (g.chain == 0) lt = left ? -1.0 : 0.0;
+            if (g.chain == 2 && left) lt = -1.0;
+            if (g.chain == 2 && kk == g.res[2] - 1) {
+                if (jguess >= 0) guess = jguess;
+                if (jj == g.res[1] - 1 && iguess >= 0) guess = iguess;
+            }
+            if (lane == l) {
+                r_elem = elem;
+                r_left = left;
+                r_lt = lt;
+            }
+        }
+        if (base + lane < n) {
+            out_elem[base + lane] = r_elem;
+            out_left[base + lane] = r_left;
+            out_lt[base + lane] = r_lt;
+        }
+    }
+}
+
+int launch_seed_chain(fvc_ctx *c, int chain, int64_t n, const double *dxyz, const double mn[3], const double del[3],
+                      const int32_t res[3], int lsc, int *d_elem, int *d_left, double *d_left_time) {
+    if (n <= 0) return FVC_OK;
+    ChainGeom g;
+    g.chain = chain;
+    g.xyz = dxyz;
+    for (int a = 0; a < 3; a++) {
+        g.mn[a] = mn ? mn[a] : 0.0;
+        g.del[a] = del ? del[a] : 0.0;
+        g.res[a] = res ? res[a] : 1;
+    }
+    int seed0 = -1;
+    if (chain != 0 && c->mesh.mesh_type == 1) {
+        /* centre seed: global search at the middle grid point (ftle.c:456-465, tracers.c:1318-1322) */
+        double xs[3] = {g.mn[0] + ((int)(g.res[0] / 2)) * g.del[0], g.mn[1] + ((int)(g.res[1] / 2)) * g.del[1],
+                        g.mn[2] + ((int)(g.res[2] / 2)) * g.del[2]};
+        double *dxs = (double *)scratch(c, 64);
+        if (!dxs) return FVC_E_NOMEM;
+        FVC_HIP(hipMemcpyAsync(dxs, xs, sizeof xs, hipMemcpyHostToDevice, c->stream));
+        int *dres = (int *)(dxs + 4);
+        int rc = launch_locate(c, dxs, nullptr, 1, FVC_LOCATE_GLOBAL, dres);
+        if (rc) return rc;
+        FVC_HIP(hipMemcpyAsync(&seed0, dres, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        FVC_HIP(hipStreamSynchronize(c->stream));
+    }
+    int2 *info = (int2 *)scratch(c, sizeof(int2) * (size_t)n);
+    if (!info) return FVC_E_NOMEM;
+    if (c->mesh.dims == 3) {
+        k_seed_prepare<3><<<grid_for(n, BLOCK), BLOCK, 0, c->stream>>>(c->mesh, g, n, info);
+        k_seed_chain<3><<<1, 64, 0, c->stream>>>(c->mesh, g, n, info, lsc, seed0, d_elem, d_left, d_left_time,
+                                                 errbuf(c));
+    } else {
+        k_seed_prepare<2><<<grid_for(n, BLOCK), BLOCK, 0, c->stream>>>(c->mesh, g, n, info);
+        k_seed_chain<2><<<1, 64, 0, c->stream>>>(c->mesh, g, n, info, lsc, seed0, d_elem, d_left, d_left_time,
+                                                 errbuf(c));
+    }
+    FVC_HIP(hipGetLastError());
+    return check_err(c, "seed chain");
+}
+
+/* ------------------------------------------------------------------------ */
+/* FTLE                                                                       */
+/* ------------------------------------------------------------------------ */
+struct FtleDev {
+    int res[3];
+    double del[3];
+    double start_time;
+    PtrSet p;
+    int *have;
+    double *wt, *wot;
+    double *Xn[3], *Vn[3];
+    int *elem_new, *exited, *advected;
+    double *exit_time;
+};
+
+static FtleDev ftle_dev(fvc_ftle *f) {
+    FtleDev d;
+    for (int a = 0; a < 3; a++) {
+        d.res[a] = f->res[a];
+        d.del[a] = f->del[a];
+        d.Xn[a] = f->Xn[a];
+        d.Vn[a] = f->Vn[a];
+    }
+    d.start_time = f->start_time;
+    d.p = ptrs(f->pts);
+    d.have = f->have;
+    d.wt = f->wt;
+    d.wot = f->wot;
+    d.elem_new = f->elem_new;
+    d.exited = f->exited;
+    d.advected = f->advected;
+    d.exit_time = f->exit_time;
+    return d;
+}
+
+/* staged advection of the active points (ftle.c:109-169 without the early part) */
+template <int MODE, int DIM, int UNS>
+__global__ __launch_bounds__(BLOCK) void k_ftle_advect(DevMesh m, DevField f, FtleDev d, const int *__restrict__ active,
+                                                      int64_t nactive, double t1, double t2, int itype, double dt,
+                                                      int extrap, ErrBuf eb, unsigned long long *steps) {
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t ns = 0;
+    if (q < nactive) {
+        int64_t i = active[q];
+        PState s;
+        load_state(d.p, i, s);
+        int err = 0;
+        advect<MODE, DIM, UNS>(m, f, s, t1, t2, itype, dt, extrap, err, ns);
+        if (err) raise_err(eb, err, i);
+        d.advected[i] = 1;
+        int ex = s.left && !extrap;
+        d.exited[i] = ex;
+        d.exit_time[i] = s.left_time;
+        for (int c = 0; c < 3; c++) {
+            d.Xn[c][i] = s.X[c];
+            d.Vn[c][i] = s.V[c];
+        }
+        /* with extrapolation the left flag travels through elem_new's sign bit slot */
+        d.elem_new[i] = s.elem;
+        if (extrap) d.p.left[i] = s.left;
+    }
+    add_steps(steps, ns);
+}
+
+/* Advect_FTLEPoint (ftle.c:1205-1227) from the frame-start state; the element
+ * is the one the reference's array holds when the trigger is processed. */
+template <int MODE, int DIM, int UNS>
+__device__ void advect_copy(const DevMesh &m, const DevField &f, const FtleDev &d, int64_t idx, int64_t trig, double t1,
+                            double t2, int itype, double dt, PState &s, int &err) {
+    for (int c = 0; c < 3; c++) {
+        s.X[c] = d.p.X[c][idx];
+        s.V[c] = 0.0;
+    }
+    s.left = d.p.left[idx];
+    bool updated = idx < trig && d.advected[idx] && !d.exited[idx];
+    s.elem = updated ? d.elem_new[idx] : d.p.elem[idx];
+    s.left_time = 0.0;
+    uint32_t ns = 0;
+    advect<MODE, DIM, UNS>(m, f, s, t1, t2, itype, dt, 0, err, ns);
+}
+
+__device__ inline void ftle_value(double D[3][3], double dt_int, double &wt, double &wot, int &err, bool zero_time) {
+#pragma clang fp contract(off)
+    double C[3][3];
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+            C[a][b] = 0;
+            for (int c = 0; c < 3; c++) C[a][b] += D[c][a] * D[c][b];
+        }
+    double lam = max_eig_sym3(C, err);
+    if (lam > 1.0 - TINY) {
+        if (zero_time) {
+            wt = 0.0;
+            wot = 0.0;
+        } else {
+            wt = (0.5 * log(lam)) / dt_int;
+            wot = 0.5 * log(lam);
+        }
+    } else {
+        wt = 0.0;
+        wot = 0.0;
+    }
+}
+
+/* gradient entries from lo/hi neighbour positions per axis (ftle.c:994-1020) */
+__device__ inline void gradient(const double lo[3][3], const double hi[3][3], const double fac[3], const double del[3],
+                                int dims, double D[3][3]) {
+#pragma clang fp contract(off)
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) D[a][b] = 0.0;
+    if (dims == 3) {
+        D[0][2] = (hi[2][0] - lo[2][0]) / (fac[2] * del[2]);
+        D[1][2] = (hi[2][1] - lo[2][1]) / (fac[2] * del[2]);
+        D[2][0] = (hi[0][2] - lo[0][2]) / (fac[0] * del[0]);
+        D[2][1] = (hi[1][2] - lo[1][2]) / (fac[1] * del[1]);
+        D[2][2] = (hi[2][2] - lo[2][2]) / (fac[2] * del[2]);
+    }
+    D[0][0] = (hi[0][0] - lo[0][0]) / (fac[0] * del[0]);
+    D[0][1] = (hi[1][0] - lo[1][0]) / (fac[1] * del[1]);
+    D[1][0] = (hi[0][1] - lo[0][1]) / (fac[0] * del[0]);
+    D[1][1] = (hi[1][1] - lo[1][1]) / (fac[1] * del[1]);
+}
+
+/* GetFTLEForPointEarly (ftle.c:788-1048).  The trigger of point P is the
+ * first exiting point, in the reference's loop order, among P and its six
+ * neighbours; te is the trigger's exit time. */
+template <int MODE, int DIM, int UNS>
+__global__ __launch_bounds__(BLOCK) void k_ftle_early(DevMesh m, DevField f, FtleDev d, int64_t n, double t1, int itype,
+                                                     double dt, ErrBuf eb) {
+#pragma clang fp contract(off)
+    int64_t P = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (P >= n || d.have[P]) return;
+    const int64_t sk = 1, sj = d.res[2], si = (int64_t)d.res[2] * d.res[1];
+    int k = (int)(P % d.res[2]), j = (int)((P / d.res[2]) % d.res[1]), i = (int)(P / si);
+    int64_t nb[6] = {P - si, P + si, P - sj, P + sj, P - sk, P + sk};
+    bool has[6] = {i > 0, i < d.res[0] - 1, j > 0, j < d.res[1] - 1, k > 0, k < d.res[2] - 1};
+    int64_t trig = d.exited[P] ? P : INT64_MAX;
+    for (int q = 0; q < 6; q++)
+        if (has[q] && d.exited[nb[q]] && nb[q] < trig) trig = nb[q];
+    if (trig == INT64_MAX) return;
+    const double te = d.exit_time[trig];
+    int err = 0;
+    double lt[6];
+    for (int q = 0; q < 6; q++) {
+        lt[q] = te;
+        if (!has[q]) continue;
+        PState s;
+        advect_copy<MODE, DIM, UNS>(m, f, d, nb[q], trig, t1, te, itype, dt, s, err);
+        if (s.left) lt[q] = s.left_time;
+    }
+    double mt = dmin(te, dmin(lt[5], dmin(lt[4], dmin(lt[3], dmin(lt[2], dmin(lt[1], lt[0]))))));
+    PState self;
+    advect_copy<MODE, DIM, UNS>(m, f, d, P, trig, t1, mt, itype, dt, self, err);
+    double lo[3][3], hi[3][3], fac[3] = {1.0, 1.0, 1.0};
+    int cc[3] = {i, j, k};
+    for (int ax = 0; ax < DIM; ax++) {
+        int last = d.res[ax] - 1;
+        PState a, b;
+        if (cc[ax] == 0) {
+            advect_copy<MODE, DIM, UNS>(m, f, d, nb[2 * ax + 1], trig, t1, mt, itype, dt, b, err);
+            for (int c = 0; c < 3; c++) {
+                lo[ax][c] = self.X[c];
+                hi[ax][c] = b.X[c];
+            }
+        } else if (cc[ax] == last) {
+            advect_copy<MODE, DIM, UNS>(m, f, d, nb[2 * ax], trig, t1, mt, itype, dt, a, err);
+            for (int c = 0; c < 3; c++) {
+                lo[ax][c] = a.X[c];
+                hi[ax][c] = self.X[c];
+            }
+        } else {
+            advect_copy<MODE, DIM, UNS>(m, f, d, nb[2 * ax], trig, t1, mt, itype, dt, a, err);
+            advect_copy<MODE, DIM, UNS>(m, f, d, nb[2 * ax + 1], trig, t1, mt, itype, dt, b, err);
+            for (int c = 0; c < 3; c++) {
+                lo[ax][c] = a.X[c];
+                hi[ax][c] = b.X[c];
+            }
+            fac[ax] = 2.0;
+        }
+    }
+    double D[3][3];
+    gradient(lo, hi, fac, d.del, DIM, D);
+    double wt, wot;
+    ftle_value(D, fabs(mt - d.start_time), wt, wot, err, fabs(mt - d.start_time) < TINY);
+    if (err) raise_err(eb, err, P);
+    d.wt[P] = wt;
+    d.wot[P] = wot;
+    d.have[P] = 1;
+}
+
+/* position update (ftle.c:160-169, UpdateFTLELocations ftle.c:1229-1242) */
+__global__ __launch_bounds__(BLOCK) void k_ftle_update(FtleDev d, const int *__restrict__ active, int64_t nactive,
+                                                      int uns, int extrap) {
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nactive) return;
+    int64_t i = active[q];
+    if (d.exited[i]) {
+        d.p.left[i] = 1;
+    } else {
+        for (int c = 0; c < 3; c++) {
+            d.p.X[c][i] = d.Xn[c][i];
+            d.p.V[c][i] = d.Vn[c][i];
+        }
+        if (uns) d.p.elem[i] = d.elem_new[i];
+    }
+    d.exited[i] = 0;
+    d.advected[i] = 0;
+}
+
+/* GetFTLEForPoint (ftle.c:1050-1203) */
+__global__ __launch_bounds__(BLOCK) void k_ftle_fill(FtleDev d, int64_t n, int dims, double int_t, ErrBuf eb) {
+#pragma clang fp contract(off)
+    int64_t P = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (P >= n || d.have[P]) return;
+    if (int_t < TINY) {
+        d.wt[P] = 0.0;
+        d.wot[P] = 0.0;
+        return;
+    }
+    const int64_t st[3] = {(int64_t)d.res[2] * d.res[1], d.res[2], 1};
+    int cc[3] = {(int)(P / st[0]), (int)((P / d.res[2]) % d.res[1]), (int)(P % d.res[2])};
+    double lo[3][3], hi[3][3], fac[3] = {1.0, 1.0, 1.0};
+    for (int ax = 0; ax < dims; ax++) {
+        int64_t l = P, h = P;
+        if (cc[ax] == 0)
+            h = P + st[ax];
+        else if (cc[ax] == d.res[ax] - 1)
+            l = P - st[ax];
+        else {
+            l = P - st[ax];
+            h = P + st[ax];
+            fac[ax] = 2.0;
+        }
+        for (int c = 0; c < 3; c++) {
+            lo[ax][c] = d.p.X[c][l];
+            hi[ax][c] = d.p.X[c][h];
+        }
+    }
+    double D[3][3];
+    gradient(lo, hi, fac, d.del, dims, D);
+    int err = 0;
+    double wt, wot;
+    ftle_value(D, int_t, wt, wot, err, false);
+    if (err) raise_err(eb, err, P);
+    d.wt[P] = wt;
+    d.wot[P] = wot;
+}
+
+/* outside masks after location (ftle.c:425-452, 484-501, 521-572) */
+__global__ __launch_bounds__(BLOCK) void k_ftle_mask(FtleDev d, int64_t n) {
+    int64_t P = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (P >= n) return;
+    const int64_t si = (int64_t)d.res[2] * d.res[1], sj = d.res[2];
+    int k = (int)(P % d.res[2]), j = (int)((P / d.res[2]) % d.res[1]), i = (int)(P / si);
+    int own = d.p.left[P];
+    int have = own;
+    if (i > 0 && d.p.left[P - si]) have = 1;
+    if (i < d.res[0] - 1 && d.p.left[P + si]) have = 1;
+    if (j > 0 && d.p.left[P - sj]) have = 1;
+    if (j < d.res[1] - 1 && d.p.left[P + sj]) have = 1;
+    if (k > 0 && d.p.left[P - 1]) have = 1;
+    if (k < d.res[2] - 1 && d.p.left[P + 1]) have = 1;
+    d.have[P] = have;
+    d.wt[P] = own ? -1.0 : 0.0;
+    d.wot[P] = own ? -1.0 : 0.0;
+    d.exited[P] = 0;
+    d.advected[P] = 0;
+}
+
+/* active list: points integrated this frame (ftle.c:113) */
+__global__ __launch_bounds__(BLOCK) void k_ftle_flags(FtleDev d, int64_t n, int extrap, int *__restrict__ flags) {
+    int64_t P = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (P >= n) return;
+    flags[P] = d.p.left_time[P] > -TINY && (!d.p.left[P] || extrap);
+}
+
+int launch_ftle_mask(fvc_ftle *f) {
+    fvc_ctx *c = f->ctx;
+    k_ftle_mask<<<grid_for(f->n, BLOCK), BLOCK, 0, c->stream>>>(ftle_dev(f), f->n);
+    FVC_HIP(hipGetLastError());
+    return FVC_OK;
+}
+
+int launch_ftle_frame(fvc_ftle *f, int lo, int hi, double tmin, double tmax, double t1, double t2,
+                      const fvc_int_params &ip) {
+    fvc_ctx *c = f->ctx;
+    DevField fld = bind_field(c, lo, hi, tmin, tmax);
+    FtleDev d = ftle_dev(f);
+    int itype = ip.int_type == FVC_INT_RK4 ? 1 : 0;
+    int *flags = (int *)scratch(c, sizeof(int) * (size_t)f->n);
+    if (!flags) return FVC_E_NOMEM;
+    k_ftle_flags<<<grid_for(f->n, BLOCK), BLOCK, 0, c->stream>>>(d, f->n, ip.extrapolate, flags);
+    FVC_HIP(hipGetLastError());
+    int rc = launch_compact(c, flags, f->n, f->active, &f->nactive);
+    if (rc) return rc;
+    FVC_HIP(hipMemsetAsync(c->d_steps, 0, sizeof(unsigned long long), c->stream));
+    if (f->nactive > 0) {
+        FVC_DISPATCH(c->precision, c->mesh.dims, c->mesh.mesh_type, k_ftle_advect,
+                     <<<grid_for(f->nactive, BLOCK), BLOCK, 0, c->stream>>>(c->mesh, fld, d, f->active, f->nactive,
+                                                                            t1, t2, itype, ip.time_step,
+                                                                            ip.extrapolate, errbuf(c), c->d_steps));
+        FVC_HIP(hipGetLastError());
+        if (!ip.extrapolate) {
+            FVC_DISPATCH(c->precision, c->mesh.dims, c->mesh.mesh_type, k_ftle_early,
+                         <<<grid_for(f->n, BLOCK), BLOCK, 0, c->stream>>>(c->mesh, fld, d, f->n, t1, itype,
+                                                                          ip.time_step, errbuf(c)));
+            FVC_HIP(hipGetLastError());
+        }
+        k_ftle_update<<<grid_for(f->nactive, BLOCK), BLOCK, 0, c->stream>>>(d, f->active, f->nactive,
+                                                                            c->mesh.mesh_type, ip.extrapolate);
+        FVC_HIP(hipGetLastError());
+    }
+    unsigned long long steps = 0;
+    FVC_HIP(hipMemcpyAsync(&steps, c->d_steps, sizeof steps, hipMemcpyDeviceToHost, c->stream));
+    rc = check_err(c, "fvc_ftle_frame");
+    c->last_steps = (int64_t)steps;
+    return rc;
+}
+
+int launch_ftle_fill(fvc_ftle *f, double int_t) {
+    fvc_ctx *c = f->ctx;
+    k_ftle_fill<<<grid_for(f->n, BLOCK), BLOCK, 0, c->stream>>>(ftle_dev(f), f->n, c->mesh.dims, int_t, errbuf(c));
+    FVC_HIP(hipGetLastError());
+    return check_err(c, "fvc_ftle_output");
+}
+
+/* ------------------------------------------------------------------------ */
+/* stable compaction: indices of nonzero flags, in order                      */
+/* ------------------------------------------------------------------------ */
+constexpr int CMP_TILE = 4096; /* items per block */
+
+__global__ __launch_bounds__(BLOCK) void k_cmp_count(const int *__restrict__ flags, int64_t n, int *__restrict__ counts) {
+    int64_t base = (int64_t)blockIdx.x * CMP_TILE;
+    int cnt = 0;
+    for (int q = threadIdx.x; q < CMP_TILE; q += BLOCK) {
+        int64_t i = base + q;
+        if (i < n && flags[i]) cnt++;
+    }
+    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
+    __shared__ int red[BLOCK / 64];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) counts[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+/* exclusive scan of a small int array in one block (in place); total at a[n] */
+__global__ __launch_bounds__(1024) void k_scan_small(int *__restrict__ a, int n) {
+    __shared__ int carry;
+    __shared__ int tmp[1024];
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < n; base += 1024) {
+        int i = base + threadIdx.x;
+        int v = i < n ? a[i] : 0;
+        tmp[threadIdx.x] = v;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            int add = threadIdx.x >= off ? tmp[threadIdx.x - off] : 0;
+            __syncthreads();
+            tmp[threadIdx.x] += add;
+            __syncthreads();
+        }
+        if (i < n) a[i] = carry + tmp[threadIdx.x] - v;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry += tmp[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) a[n] = carry;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_cmp_scatter(const int *__restrict__ flags, int64_t n,
+                                                      const int *__restrict__ offs, int *__restrict__ out) {
+    __shared__ int wsum[BLOCK / 64];
+    __shared__ int run;
+    if (threadIdx.x == 0) run = offs[blockIdx.x];
+    __syncthreads();
+    int64_t base = (int64_t)blockIdx.x * CMP_TILE;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int r = 0; r < CMP_TILE; r += BLOCK) {
+        int64_t i = base + r + threadIdx.x;
+        int fl = (i < n && flags[i]) ? 1 : 0;
+        unsigned long long b = __ballot(fl);
+        int before = __popcll(b & ((1ull << lane) - 1ull));
+        if (lane == 0) wsum[w] = __popcll(b);
+        __syncthreads();
+        int pre = 0;
+        for (int q = 0; q < w; q++) pre += wsum[q];
+        if (fl) out[run + pre + before] = (int)i;
+        __syncthreads();
+        if (threadIdx.x == 0) run += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        __syncthreads();
+    }
+}
+
+int launch_compact(fvc_ctx *c, const int *flags, int64_t n, int *out_idx, int64_t *count) {
+    int nb = (int)((n + CMP_TILE - 1) / CMP_TILE);
+    if (nb == 0) {
+        *count = 0;
+        return FVC_OK;
+    }
+    int *offs = nullptr;
+    FVC_HIP(hipMallocAsync((void **)&offs, sizeof(int) * (nb + 1), c->stream));
+    k_cmp_count<<<nb, BLOCK, 0, c->stream>>>(flags, n, offs);
+    k_scan_small<<<1, 1024, 0, c->stream>>>(offs, nb);
+    k_cmp_scatter<<<nb, BLOCK, 0, c->stream>>>(flags, n, offs, out_idx);
+    int total = 0;
+    FVC_HIP(hipMemcpyAsync(&total, offs + nb, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    FVC_HIP(hipFreeAsync(offs, c->stream));
+    FVC_HIP(hipStreamSynchronize(c->stream));
+    FVC_HIP(hipGetLastError());
+    *count = total;
+    return FVC_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Z-order keys + stable LSD radix sort                                       */
+/* ------------------------------------------------------------------------ */
+__device__ inline uint64_t spread3(uint64_t v) {
+    v &= 0x1fffffull;
+    v = (v | (v << 32)) & 0x1f00000000ffffull;
+    v = (v | (v << 16)) & 0x1f0000ff0000ffull;
+    v = (v | (v << 8)) & 0x100f00f00f00f00full;
+    v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
+    v = (v | (v << 2)) & 0x1249249249249249ull;
+    return v;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_morton(const double *__restrict__ xyz, int64_t n, double lo0, double lo1,
+                                                 double lo2, double s0, double s1, double s2, uint64_t *__restrict__ keys,
+                                                 int *__restrict__ vals) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double lo[3] = {lo0, lo1, lo2}, sc[3] = {s0, s1, s2};
+    uint64_t q[3];
+    for (int a = 0; a < 3; a++) {
+        double v = (xyz[3 * i + a] - lo[a]) * sc[a];
+        v = v < 0.0 ? 0.0 : (v > 2097151.0 ? 2097151.0 : v);
+        q[a] = (uint64_t)v;
+    }
+    /* x bit most significant within each level, then y, then z */
+    keys[i] = (spread3(q[0]) << 2) | (spread3(q[1]) << 1) | spread3(q[2]);
+    vals[i] = (int)i;
+}
+
+constexpr int RS_TILE = 4096;
+
+__global__ __launch_bounds__(BLOCK) void k_rs_hist(const uint64_t *__restrict__ keys, int64_t n, int shift, int nblocks,
+                                                  int *__restrict__ hist) {
+    __shared__ int h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    for (int q = threadIdx.x; q < RS_TILE; q += BLOCK) {
+        int64_t i = base + q;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255], 1);
+    }
+    __syncthreads();
+    hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(BLOCK) void k_rs_scatter(const uint64_t *__restrict__ kin, const int *__restrict__ vin,
+                                                     int64_t n, int shift, int nblocks, const int *__restrict__ offs,
+                                                     uint64_t *__restrict__ kout, int *__restrict__ vout) {
+    __shared__ int run[256];
+    __shared__ int wcnt[BLOCK / 64][256];
+    run[threadIdx.x] = offs[threadIdx.x * nblocks + blockIdx.x];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    for (int r = 0; r < RS_TILE; r += BLOCK) {
+        for (int q = 0; q < BLOCK / 64; q++) wcnt[q][threadIdx.x] = 0;
+        __syncthreads();
+        int64_t i = base + r + threadIdx.x;
+        bool valid = i < n;
+        uint64_t key = valid ? kin[i] : 0;
+        int val = valid ? vin[i] : 0;
+        int dig = valid ? (int)((key >> shift) & 255) : 256;
+        /* lanes of this wave with the same digit */
+        unsigned long long same = __ballot(valid);
+        for (int b = 0; b < 9; b++) {
+            unsigned long long bb = __ballot((dig >> b) & 1);
+            same &= ((dig >> b) & 1) ? bb : ~bb;
+        }
+        int before = __popcll(same & ((1ull << lane) - 1ull));
+        if (valid && before == 0) wcnt[w][dig] = __popcll(same);
+        __syncthreads();
+        if (valid) {
+            int pos = run[dig] + before;
+            for (int q = 0; q < w; q++) pos += wcnt[q][dig];
+            kout[pos] = key;
+            vout[pos] = val;
+        }
+        __syncthreads();
+        int tot = 0;
+        for (int q = 0; q < BLOCK / 64; q++) tot += wcnt[q][threadIdx.x];
+        run[threadIdx.x] += tot;
+        __syncthreads();
+    }
+}
+
+/* exclusive scan over the digit-major histogram (hist[d*nb + b]) */
+__global__ __launch_bounds__(1024) void k_rs_scan(int *__restrict__ a, int total) { /* single block */
+    __shared__ int carry;
+    __shared__ int tmp[1024];
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < total; base += 1024) {
+        int i = base + threadIdx.x;
+        int v = i < total ? a[i] : 0;
+        tmp[threadIdx.x] = v;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            int add = threadIdx.x >= off ? tmp[threadIdx.x - off] : 0;
+            __syncthreads();
+            tmp[threadIdx.x] += add;
+            __syncthreads();
+        }
+        if (i < total) a[i] = carry + tmp[threadIdx.x] - v;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry += tmp[1023];
+        __syncthreads();
+    }
+}
+
+int launch_zsort(fvc_ctx *c, const double *dxyz, int64_t n, const double lo[3], const double hi[3], int *dperm) {
+    if (n <= 0) return FVC_OK;
+    if (n > INT32_MAX / 2) return FVC_E_INVALID;
+    double sc[3];
+    for (int a = 0; a < 3; a++) sc[a] = hi[a] > lo[a] ? 2097152.0 / (hi[a] - lo[a]) : 0.0;
+    int nb = (int)((n + RS_TILE - 1) / RS_TILE);
+    uint64_t *k0 = nullptr, *k1 = nullptr;
+    int *v0 = nullptr, *v1 = nullptr, *hist = nullptr;
+    FVC_HIP(hipMallocAsync((void **)&k0, sizeof(uint64_t) * n, c->stream));
+    FVC_HIP(hipMallocAsync((void **)&k1, sizeof(uint64_t) * n, c->stream));
+    FVC_HIP(hipMallocAsync((void **)&v0, sizeof(int) * n, c->stream));
+    FVC_HIP(hipMallocAsync((void **)&v1, sizeof(int) * n, c->stream));
+    FVC_HIP(hipMallocAsync((void **)&hist, sizeof(int) * 256 * (size_t)nb, c->stream));
+    k_morton<<<grid_for(n, BLOCK), BLOCK, 0, c->stream>>>(dxyz, n, lo[0], lo[1], lo[2], sc[0], sc[1], sc[2], k0, v0);
+    for (int shift = 0; shift < 64; shift += 8) {
+        k_rs_hist<<<nb, BLOCK, 0, c->stream>>>(k0, n, shift, nb, hist);
+        k_rs_scan<<<1, 1024, 0, c->stream>>>(hist, 256 * nb);
+        k_rs_scatter<<<nb, BLOCK, 0, c->stream>>>(k0, v0, n, shift, nb, hist, k1, v1);
+        std::swap(k0, k1);
+        std::swap(v0, v1);
+    }
+    FVC_HIP(hipGetLastError());
+    FVC_HIP(hipMemcpyAsync(dperm, v0, sizeof(int) * n, hipMemcpyDeviceToDevice, c->stream));
+    FVC_HIP(hipFreeAsync(k0, c->stream));
+    FVC_HIP(hipFreeAsync(k1, c->stream));
+    FVC_HIP(hipFreeAsync(v0, c->stream));
+    FVC_HIP(hipFreeAsync(v1, c->stream));
+    FVC_HIP(hipFreeAsync(hist, c->stream));
+    FVC_HIP(hipStreamSynchronize(c->stream));
+    return FVC_OK;
+}
+
+} // namespace fvc

@@ -1,0 +1,262 @@
+"""ctypes bindings of libfvc.so, the C ABI of include/fvc.h (HIP path on gfx950).
+
+Loading fails loudly when the library is not built; there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import PKG_DIR
+from .host import POINT_DTYPE
+
+OK = 0
+PREC = {"f64": 0, "f32": 1, "bf16": 2, "f16": 3}
+LOCATE_WALK, LOCATE_GLOBAL, LOCATE_WALK_GLOBAL = 0, 1, 2
+INT_EULER, INT_RK4, INT_AB2, INT_AB3 = 0, 1, 3, 4
+
+_LIB = None
+
+
+class MeshDesc(ctypes.Structure):
+    _fields_ = [("dims", ctypes.c_int32), ("mesh_type", ctypes.c_int32), ("bounds", ctypes.c_double * 6),
+                ("cart_min", ctypes.c_double * 3), ("cart_max", ctypes.c_double * 3), ("cart_res", ctypes.c_int32 * 3),
+                ("num_nodes", ctypes.c_int64), ("coords", ctypes.c_void_p), ("num_elems", ctypes.c_int64),
+                ("conn", ctypes.c_void_p), ("adj", ctypes.c_void_p)]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int32), ("max_slices", ctypes.c_int32), ("device", ctypes.c_int32),
+                ("hash_cells", ctypes.c_int32), ("walk_limit", ctypes.c_int32)]
+
+
+class IntParams(ctypes.Structure):
+    _fields_ = [("int_type", ctypes.c_int32), ("extrapolate", ctypes.c_int32), ("time_step", ctypes.c_double),
+                ("block", ctypes.c_int32), ("steps_per_launch", ctypes.c_int32)]
+
+
+class FvcError(RuntimeError):
+    def __init__(self, code, what, detail=""):
+        self.code = code
+        super().__init__("%s failed: %s (%d) %s" % (what, lib().fvc_strerror(code).decode(), code, detail))
+
+
+def lib_path():
+    return os.path.join(PKG_DIR, "libfvc.so")
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = lib_path()
+        if not os.path.exists(path):
+            raise RuntimeError("libfvc.so is not built (run `make -C flowvc-1_amd`); the HIP path has no fallback")
+        L = ctypes.CDLL(path)
+        vp, i32, i64, d = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+        L.fvc_strerror.restype = ctypes.c_char_p
+        L.fvc_strerror.argtypes = [ctypes.c_int]
+        L.fvc_last_error.restype = ctypes.c_char_p
+        L.fvc_last_error.argtypes = [vp]
+        L.fvc_create.argtypes = [ctypes.POINTER(MeshDesc), ctypes.POINTER(Params), ctypes.POINTER(vp)]
+        L.fvc_destroy.argtypes = [vp]
+        L.fvc_sync.argtypes = [vp]
+        L.fvc_memory.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]
+        L.fvc_load_slice.argtypes = [vp, i32, d, vp, i64]
+        L.fvc_particles_create.argtypes = [vp, i64, ctypes.POINTER(vp)]
+        L.fvc_particles_destroy.argtypes = [vp]
+        L.fvc_particles_set.argtypes = [vp, i64, vp, i64]
+        L.fvc_particles_get.argtypes = [vp, i64, vp, i64]
+        L.fvc_particles_capacity.restype = i64
+        L.fvc_particles_capacity.argtypes = [vp]
+        L.fvc_launch_velocity.argtypes = [vp, vp, i64, i64, d, i32, i32, d, d]
+        L.fvc_advect.argtypes = [vp, vp, i64, i64, d, d, i32, i32, d, d, ctypes.POINTER(IntParams)]
+        L.fvc_last_step_count.argtypes = [vp, ctypes.POINTER(i64)]
+        L.fvc_locate.argtypes = [vp, vp, vp, i64, i32, vp]
+        L.fvc_seed_list.argtypes = [vp, vp, i64, i32, vp]
+        L.fvc_seed_grid.argtypes = [vp, vp, vp, vp, i32, vp]
+        L.fvc_ftle_create.argtypes = [vp, vp, vp, vp, i32, d, d, ctypes.POINTER(vp)]
+        L.fvc_ftle_destroy.argtypes = [vp]
+        L.fvc_ftle_frame.argtypes = [vp, i32, i32, i32, d, d, ctypes.POINTER(IntParams), ctypes.POINTER(i32)]
+        L.fvc_ftle_output.argtypes = [vp, d, vp, vp]
+        L.fvc_ftle_points.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]
+        L.fvc_zsort.argtypes = [vp, vp, i64, vp, vp, vp]
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def _chk(rc, what, ctx=None):
+    if rc != OK:
+        detail = lib().fvc_last_error(ctx).decode() if ctx else ""
+        raise FvcError(rc, what, detail)
+
+
+class Context:
+    """A device context: mesh, spatial hash and resident velocity slices."""
+
+    def __init__(self, dims, bounds, precision="f64", max_slices=2, coords=None, conn=None, adj=None,
+                 cart_min=None, cart_max=None, cart_res=None, walk_limit=0, device=0):
+        L = lib()
+        md = MeshDesc()
+        md.dims = dims
+        md.bounds[:] = list(bounds)
+        self._keep = []
+        if coords is not None:
+            md.mesh_type = 1
+            coords = np.ascontiguousarray(coords, np.float64)
+            conn = np.ascontiguousarray(conn, np.int32)
+            adj = np.ascontiguousarray(adj, np.int32)
+            self._keep += [coords, conn, adj]
+            md.num_nodes, md.num_elems = coords.shape[0], conn.shape[0]
+            md.coords, md.conn, md.adj = _p(coords), _p(conn), _p(adj)
+            self.nvel = coords.shape[0]
+        else:
+            md.mesh_type = 0
+            md.cart_min[:] = list(cart_min)
+            md.cart_max[:] = list(cart_max)
+            md.cart_res[:] = list(cart_res)
+            self.nvel = int(cart_res[0] * cart_res[1] * (cart_res[2] if dims == 3 else 1))
+        pp = Params(PREC[precision], max_slices, device, 0, walk_limit)
+        h = ctypes.c_void_p()
+        _chk(L.fvc_create(ctypes.byref(md), ctypes.byref(pp), ctypes.byref(h)), "fvc_create")
+        self.h = h
+        self.dims = dims
+        self.precision = precision
+        self.mesh_type = md.mesh_type
+
+    def close(self):
+        if self.h:
+            lib().fvc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def load_slice(self, slot, t, uvw):
+        uvw = np.ascontiguousarray(uvw, np.float64)
+        _chk(lib().fvc_load_slice(self.h, slot, t, _p(uvw), uvw.shape[0]), "fvc_load_slice", self.h)
+
+    def memory(self):
+        u, f = ctypes.c_int64(), ctypes.c_int64()
+        _chk(lib().fvc_memory(self.h, ctypes.byref(u), ctypes.byref(f)), "fvc_memory", self.h)
+        return u.value, f.value
+
+    def locate(self, xyz, guess=None, mode=LOCATE_WALK):
+        xyz = np.ascontiguousarray(xyz, np.float64)
+        out = np.empty(xyz.shape[0], np.int32)
+        g = None if guess is None else np.ascontiguousarray(guess, np.int32)
+        _chk(lib().fvc_locate(self.h, _p(xyz), _p(g), xyz.shape[0], mode, _p(out)), "fvc_locate", self.h)
+        return out
+
+    def seed_list(self, xyz, lsc=1):
+        xyz = np.ascontiguousarray(xyz, np.float64)
+        out = np.zeros(xyz.shape[0], POINT_DTYPE)
+        _chk(lib().fvc_seed_list(self.h, _p(xyz), xyz.shape[0], lsc, _p(out)), "fvc_seed_list", self.h)
+        return out
+
+    def seed_grid(self, mn, dl, res, lsc=1):
+        mn, dl = np.asarray(mn, np.float64), np.asarray(dl, np.float64)
+        res = np.asarray(res, np.int32)
+        out = np.zeros(int(np.prod(res)), POINT_DTYPE)
+        _chk(lib().fvc_seed_grid(self.h, _p(mn), _p(dl), _p(res), lsc, _p(out)), "fvc_seed_grid", self.h)
+        return out
+
+    def zsort(self, xyz, lo, hi):
+        xyz = np.ascontiguousarray(xyz, np.float64)
+        lo, hi = np.asarray(lo, np.float64), np.asarray(hi, np.float64)
+        perm = np.empty(xyz.shape[0], np.int32)
+        _chk(lib().fvc_zsort(self.h, _p(xyz), xyz.shape[0], _p(lo), _p(hi), _p(perm)), "fvc_zsort", self.h)
+        return perm
+
+    def last_steps(self):
+        n = ctypes.c_int64()
+        lib().fvc_last_step_count(self.h, ctypes.byref(n))
+        return n.value
+
+
+class Particles:
+    """Device-resident LagrangianPoint state."""
+
+    def __init__(self, ctx, capacity):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        _chk(lib().fvc_particles_create(ctx.h, capacity, ctypes.byref(h)), "fvc_particles_create", ctx.h)
+        self.h = h
+        self.capacity = capacity
+
+    def close(self):
+        if self.h:
+            lib().fvc_particles_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set(self, recs, begin=0):
+        recs = np.ascontiguousarray(recs, POINT_DTYPE)
+        _chk(lib().fvc_particles_set(self.h, begin, _p(recs), recs.shape[0]), "fvc_particles_set", self.ctx.h)
+
+    def get(self, n=None, begin=0):
+        n = self.capacity - begin if n is None else n
+        out = np.zeros(n, POINT_DTYPE)
+        _chk(lib().fvc_particles_get(self.h, begin, _p(out), n), "fvc_particles_get", self.ctx.h)
+        return out
+
+    def launch_velocity(self, t, lo, hi, tmin, tmax, begin=0, count=None):
+        count = self.capacity - begin if count is None else count
+        _chk(lib().fvc_launch_velocity(self.ctx.h, self.h, begin, count, t, lo, hi, tmin, tmax),
+             "fvc_launch_velocity", self.ctx.h)
+
+    def advect(self, t1, t2, lo, hi, tmin, tmax, dt, int_type=INT_RK4, extrapolate=0, begin=0, count=None):
+        count = self.capacity - begin if count is None else count
+        ip = IntParams(int_type, extrapolate, dt, 0, 0)
+        _chk(lib().fvc_advect(self.ctx.h, self.h, begin, count, t1, t2, lo, hi, tmin, tmax, ctypes.byref(ip)),
+             "fvc_advect", self.ctx.h)
+        return self.ctx.last_steps()
+
+
+class FtleSlide:
+    def __init__(self, ctx, mn, mx, res, lsc, start, stop):
+        self.ctx = ctx
+        mn, mx = np.asarray(mn, np.float64), np.asarray(mx, np.float64)
+        res = np.asarray(res, np.int32)
+        h = ctypes.c_void_p()
+        _chk(lib().fvc_ftle_create(ctx.h, _p(mn), _p(mx), _p(res), lsc, start, stop, ctypes.byref(h)),
+             "fvc_ftle_create", ctx.h)
+        self.h = h
+        n, ins = ctypes.c_int64(), ctypes.c_int64()
+        lib().fvc_ftle_points(h, ctypes.byref(n), ctypes.byref(ins))
+        self.n, self.inside = n.value, ins.value
+
+    def frame(self, direction, lo, hi, tmin, tmax, dt, int_type=INT_RK4, extrapolate=0):
+        ip = IntParams(int_type, extrapolate, dt, 0, 0)
+        done = ctypes.c_int32()
+        _chk(lib().fvc_ftle_frame(self.h, direction, lo, hi, tmin, tmax, ctypes.byref(ip), ctypes.byref(done)),
+             "fvc_ftle_frame", self.ctx.h)
+        return done.value
+
+    def output(self, int_t):
+        wt = np.empty(self.n, np.float64)
+        wot = np.empty(self.n, np.float64)
+        _chk(lib().fvc_ftle_output(self.h, int_t, _p(wt), _p(wot)), "fvc_ftle_output", self.ctx.h)
+        return wt, wot
+
+    def close(self):
+        if self.h:
+            lib().fvc_ftle_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,430 @@
+/*
+ * flowvc_main.c - ompVC-compatible driver over the MI355X C ABI (include/fvc.h).
+ *
+ *   flowvc <deck.in> [--precision f64|f32|bf16|f16] [--nproc N] [--quiet]
+ *
+ * Reads the same 97-entry deck and binary mesh/velocity files as ompflowVC and
+ * writes the same FTLE (<prefix>.<n>.bin, _noT, _Cartesian.bin, .info) and
+ * tracer (<prefix>.<k>.bin, <prefix>.IC) files.  The frame loop follows
+ * ompVC/main.c:170-227; FTLE slides follow ftle.c:24-377 and tracer releases
+ * tracers.c:25-289.  Velocity slices stay resident on the device: each data
+ * file is uploaded once into its own slot (the reference re-reads two slices
+ * per frame, velocity.c:61-234).  --nproc emulates the host processor count
+ * that selects ompVC's tracer output loop (parameters.c:411-421).
+ * Modes outside the advection path (inertial particles, RKF, staggered
+ * release, exposure/residence time, strain rate, VelOut, normal flow) stop
+ * with an error.
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#include <unistd.h>
+
+#include "fvc.h"
+#include "fvc_host.h"
+
+#define TINY FVC_TINY
+
+static void fatal(const char *fmt, const char *a) {
+    fprintf(stderr, "Error: ");
+    fprintf(stderr, fmt, a);
+    fprintf(stderr, "\n");
+    exit(1);
+}
+
+static void check(int rc, fvc_ctx *c, const char *what) {
+    if (rc) {
+        fprintf(stderr, "Error: %s: %s (%s)\n", what, fvc_strerror(rc), c ? fvc_last_error(c) : "");
+        exit(1);
+    }
+}
+
+static double now_s(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+typedef struct {
+    fvc_ctx *c;
+    const fvc_deck *d;
+    int64_t n;
+    int nslots;
+    int *slot_file; /* file index held by each slot, -1 empty */
+    double *buf;
+    int next_victim;
+    double io_s;
+} slices_t;
+
+static int file_of(const fvc_deck *d, int frame) {
+    int mod = (int)fmod(frame, d->Data_TRes - 1); /* velocity.c:77-80 */
+    if (mod < 0) mod += d->Data_TRes - 1;
+    return mod;
+}
+
+/* slot holding `frame`, loading the file when absent; `keep` is not evicted */
+static int slot_for(slices_t *s, int frame, int keep) {
+    int f = file_of(s->d, frame);
+    for (int q = 0; q < s->nslots; q++)
+        if (s->slot_file[q] == f) return q;
+    int q = -1;
+    for (int k = 0; k < s->nslots; k++)
+        if (s->slot_file[k] < 0) {
+            q = k;
+            break;
+        }
+    if (q < 0) {
+        q = s->next_victim;
+        if (q == keep) q = (q + 1) % s->nslots;
+        s->next_victim = (q + 1) % s->nslots;
+    }
+    char path[3 * FVC_STR];
+    snprintf(path, sizeof path, "%s%s_vel.%d.bin", s->d->Path_Data, s->d->Data_InFilePrefix,
+             s->d->Data_SuffixTMin + f * s->d->Data_SuffixTDelta);
+    double t0 = now_s(), ts;
+    if (fvc_read_vel_slice(path, &ts, s->buf, s->n)) fatal("Could not load velocity data from %s", path);
+    check(fvc_load_slice(s->c, q, ts, s->buf, s->n), s->c, "fvc_load_slice");
+    s->io_s += now_s() - t0;
+    s->slot_file[q] = f;
+    return q;
+}
+
+static void write_info(const fvc_deck *d) {
+    char path[3 * FVC_STR];
+    snprintf(path, sizeof path, "%s%s.info", d->Path_Output, d->FTLE_OutFilePrefix);
+    FILE *o = fopen(path, "w");
+    if (!o) fatal("Could not open file %s", path);
+    /* ftle.c:1253-1270 */
+    fprintf(o, "Int_Type\t\t=\t%10d\n", d->Int_Type);
+    fprintf(o, "Int_TimeStep\t\t=\t%10g\n", d->Int_TimeStep);
+    fprintf(o, "Int_Accuracy\t\t=\t%10g\n", d->Int_Accuracy);
+    fprintf(o, "Int_MinTimeStep\t\t=\t%10g\n", d->Int_MinTimeStep);
+    fprintf(o, "Int_MaxTimeStep\t\t=\t%10g\n", d->Int_MaxTimeStep);
+    fprintf(o, "Int_TimeDirection\t=\t%10d\n", d->Int_TimeDirection);
+    fprintf(o, "Int_NormalFlow\t\t=\t%10d\n", d->Int_NormalFlow);
+    fprintf(o, "Int_NormalFlowScaling\t=\t%10g\n\n", d->Int_NormalFlowScaling);
+    fprintf(o, "LocalSearchChecking\t=\t%10d\n\n", d->LocalSearchChecking);
+    fprintf(o, "FTLE_GenerateMesh\t=\t%10d\n", d->FTLE_GenerateMesh);
+    fprintf(o, "FTLE_ICFile\t\t=\t%s\n", d->FTLE_ICFile);
+    fprintf(o, "FTLE_IntTLength\t\t=\t%10g\n", d->FTLE_IntTLength);
+    fclose(o);
+    snprintf(path, sizeof path, "%s%s_Cartesian.bin", d->Path_Output, d->FTLE_OutFilePrefix);
+    double mn[3] = {d->FTLE_Cart.XMin, d->FTLE_Cart.YMin, d->FTLE_Cart.ZMin};
+    double mx[3] = {d->FTLE_Cart.XMax, d->FTLE_Cart.YMax, d->FTLE_Cart.ZMax};
+    int32_t res[3] = {d->FTLE_Cart.XRes, d->FTLE_Cart.YRes, d->FTLE_Cart.ZRes};
+    if (fvc_write_cart_header(path, mn, mx, res)) fatal("Could not open file %s", path);
+}
+
+static void write_ftle(const fvc_deck *d, int outnum, double start, const double *wt, const double *wot, int64_t n) {
+    char path[3 * FVC_STR];
+    for (int which = 0; which < 2; which++) {
+        snprintf(path, sizeof path, "%s%s%s.%d.bin", d->Path_Output, d->FTLE_OutFilePrefix, which ? "_noT" : "",
+                 outnum);
+        FILE *o = fopen(path, "wb");
+        if (!o) fatal("Could not open file %s", path);
+        fwrite(&start, 8, 1, o);
+        fwrite(which ? wot : wt, 8, (size_t)n, o);
+        fclose(o);
+    }
+}
+
+static double *read_seed_list(const fvc_deck *d, int64_t *n) {
+    char path[3 * FVC_STR];
+    snprintf(path, sizeof path, "%s%s", d->Path_Data, d->Trace_InFile);
+    double *xyz = NULL;
+    if (d->Trace_InFileFormat == 4) { /* tracers.c:1020-1035 */
+        FILE *f = fopen(path, "rb");
+        int32_t cnt;
+        if (!f || fread(&cnt, 4, 1, f) != 1) fatal("Could not open file %s", path);
+        xyz = (double *)calloc((size_t)3 * cnt, sizeof(double));
+        for (int q = 0; q < cnt; q++)
+            if (fread(xyz + 3 * q, 8, (size_t)d->Dimensions, f) != (size_t)d->Dimensions)
+                fatal("Could not read tracer coordinates from %s", path);
+        fclose(f);
+        *n = cnt;
+    } else if (d->Trace_InFileFormat == 1) { /* tracers.c:1048-1063 */
+        FILE *f = fopen(path, "r");
+        int cnt;
+        if (!f || fscanf(f, "%d", &cnt) != 1) fatal("Could not open file %s", path);
+        xyz = (double *)calloc((size_t)3 * cnt, sizeof(double));
+        for (int q = 0; q < cnt; q++)
+            for (int a = 0; a < d->Dimensions; a++)
+                if (fscanf(f, "%lf", xyz + 3 * q + a) != 1) fatal("Could not read tracer coordinates from %s", path);
+        fclose(f);
+        *n = cnt;
+    } else {
+        fatal("Unsupported value for Trace_InFileFormat in %s", path);
+    }
+    return xyz;
+}
+
+int main(int argc, char **argv) {
+    const char *deck_path = NULL;
+    int precision = FVC_PREC_F64, nproc = (int)sysconf(_SC_NPROCESSORS_ONLN), quiet = 0;
+    for (int a = 1; a < argc; a++) {
+        if (!strcmp(argv[a], "--precision") && a + 1 < argc) {
+            const char *p = argv[++a];
+            precision = !strcmp(p, "f32") ? FVC_PREC_F32 : !strcmp(p, "bf16") ? FVC_PREC_BF16
+                      : !strcmp(p, "f16") ? FVC_PREC_F16 : FVC_PREC_F64;
+        } else if (!strcmp(argv[a], "--nproc") && a + 1 < argc) {
+            nproc = atoi(argv[++a]);
+        } else if (!strcmp(argv[a], "--quiet")) {
+            quiet = 1;
+        } else {
+            deck_path = argv[a];
+        }
+    }
+    if (!deck_path) {
+        fprintf(stderr, "usage: %s <deck.in> [--precision f64|f32|bf16|f16] [--nproc N] [--quiet]\n", argv[0]);
+        return 2;
+    }
+    double t_start = now_s();
+    fvc_deck d;
+    char err[512];
+    if (fvc_deck_read(deck_path, &d, err, sizeof err) || fvc_deck_check(&d, err, sizeof err) ||
+        fvc_deck_derive(&d, err, sizeof err))
+        fatal("%s", err);
+    if (d.legacy_96 && !quiet) fprintf(stderr, "Note: deck has 96 entries (no Num_threads); read as Num_threads = 0\n");
+    if (d.Particle_Radius > TINY || d.Int_Type == 2 || d.Trace_ReleaseStrategy == 1 || d.Trace_CETCompute ||
+        d.Trace_RTCompute || d.Trace_APCompute || d.VelOut_Compute || d.Int_NormalFlow)
+        fatal("%s", "deck selects a mode outside the advection path (inertial, RKF, staggered, CET, RT, AP, VelOut, "
+                    "normal flow)");
+    const int dir = d.Int_TimeDirection;
+
+    /* mesh (mesh.c:21-213) */
+    fvc_mesh_desc md;
+    memset(&md, 0, sizeof md);
+    md.dims = d.Dimensions;
+    md.mesh_type = d.Data_MeshType;
+    memcpy(md.bounds, d.Bounds, sizeof md.bounds);
+    double *xyz = NULL;
+    int32_t *conn = NULL, *adj = NULL;
+    char path[3 * FVC_STR];
+    if (d.Data_MeshType == 0) {
+        snprintf(path, sizeof path, "%s%s_Cartesian.bin", d.Path_Data, d.Data_InFilePrefix);
+        if (fvc_read_cart_header(path, md.cart_min, md.cart_max, md.cart_res)) fatal("Could not open file %s", path);
+    } else {
+        int64_t nn, ne;
+        if (fvc_read_unstruct_counts(d.Path_Data, d.Data_InFilePrefix, &nn, &ne))
+            fatal("Could not open mesh files with prefix %s", d.Data_InFilePrefix);
+        xyz = (double *)malloc(sizeof(double) * 3 * (size_t)nn);
+        conn = (int32_t *)malloc(sizeof(int32_t) * 4 * (size_t)ne);
+        adj = (int32_t *)malloc(sizeof(int32_t) * 4 * (size_t)ne);
+        int rc = fvc_read_unstruct_mesh(d.Path_Data, d.Data_InFilePrefix, xyz, conn, adj);
+        if (rc == -3) fatal("%s", "Incompatible number of elements listed in connectivity and adjacency files");
+        if (rc) fatal("Could not read mesh files with prefix %s", d.Data_InFilePrefix);
+        md.num_nodes = nn;
+        md.num_elems = ne;
+        md.coords = xyz;
+        md.conn = conn;
+        md.adj = adj;
+    }
+    fvc_params pp = {precision, d.Data_TRes - 1 >= 2 ? d.Data_TRes - 1 : 2, 0, 0, 0};
+    fvc_ctx *c = NULL;
+    check(fvc_create(&md, &pp, &c), NULL, "fvc_create");
+    slices_t sl;
+    memset(&sl, 0, sizeof sl);
+    sl.c = c;
+    sl.d = &d;
+    sl.n = d.Data_MeshType == 0 ? (int64_t)md.cart_res[0] * md.cart_res[1] * (d.Dimensions == 3 ? md.cart_res[2] : 1)
+                                : md.num_nodes;
+    sl.nslots = pp.max_slices;
+    sl.slot_file = (int *)malloc(sizeof(int) * sl.nslots);
+    for (int q = 0; q < sl.nslots; q++) sl.slot_file[q] = -1;
+    sl.buf = (double *)malloc(sizeof(double) * 3 * (size_t)sl.n);
+    fvc_int_params ip = {d.Int_Type == 1 ? FVC_INT_RK4 : FVC_INT_EULER, d.Int_Extrapolate, d.Int_TimeStep, 0, 0};
+    double t_compute = 0.0;
+    int64_t steps_total = 0;
+
+    if (d.FTLE_Compute) {
+        if (!d.FTLE_GenerateMesh) fatal("%s", "FTLE_GenerateMesh = 0 (IC reuse) is not supported by this driver");
+        int ns = d.Output_TRes;
+        fvc_ftle **f = (fvc_ftle **)calloc((size_t)ns, sizeof(fvc_ftle *));
+        double mn[3] = {d.FTLE_Bounds[0], d.FTLE_Bounds[2], d.FTLE_Bounds[4]};
+        double mx[3] = {d.FTLE_Bounds[1], d.FTLE_Bounds[3], d.FTLE_Bounds[5]};
+        int32_t res[3] = {d.FTLE_Res[0], d.FTLE_Res[1], d.FTLE_Res[2]};
+        for (int ss = 0; ss < ns; ss++) {
+            /* ftle.c:672-685 */
+            double start = d.Output_TStart + ss * dir * d.Output_TDelta, stop;
+            if (!d.Data_TPeriodic) {
+                stop = fmin(d.Output_TStart + ss * dir * d.Output_TDelta + (double)dir * d.FTLE_IntTLength, d.Data_TMax);
+                stop = fmax(stop, d.Data_TMin);
+                if (fabs(stop - start) < d.FTLE_IntTLength - TINY && !quiet)
+                    fprintf(stderr, "Warning: FTLE at time %f will have a maximum integration length of %f\n", start,
+                            fabs(stop - start));
+            } else {
+                stop = d.Output_TStart + ss * dir * d.Output_TDelta + (double)dir * d.FTLE_IntTLength;
+            }
+            check(fvc_ftle_create(c, mn, mx, res, d.LocalSearchChecking, start, stop, &f[ss]), c, "fvc_ftle_create");
+        }
+        int64_t npts = 0, inside = 0;
+        fvc_ftle_points(f[0], &npts, &inside);
+        if (!quiet) printf("  %lld of %lld located in domain\n", (long long)inside, (long long)npts);
+        double *wt = (double *)malloc(sizeof(double) * (size_t)npts), *wot = (double *)malloc(sizeof(double) * (size_t)npts);
+        int written_first = 0, numout = 0;
+        for (int df = d.Data_FirstFrame; df != d.Data_LastFrame; df += dir) {
+            int flo = dir > 0 ? df : df - 1;
+            int lo = slot_for(&sl, flo, -1);
+            int hi = slot_for(&sl, flo + 1, lo);
+            double tmin = d.Data_TMin + flo * d.Data_TDelta, tmax = tmin + d.Data_TDelta;
+            for (int ss = 0; ss < ns; ss++) {
+                int32_t done = 0;
+                double t0 = now_s();
+                check(fvc_ftle_frame(f[ss], dir, lo, hi, tmin, tmax, &ip, &done), c, "fvc_ftle_frame");
+                if (done == 0) continue; /* complete earlier, or not yet launched */
+                int64_t st = 0;
+                fvc_last_step_count(c, &st);
+                steps_total += st;
+                t_compute += now_s() - t0;
+                if (done == 2) {
+                    if ((ss == 0 && !d.FTLE_ComputeVariation) || (d.FTLE_ComputeVariation && !written_first)) {
+                        write_info(&d);
+                        written_first = 1;
+                    }
+                    check(fvc_ftle_output(f[ss], d.FTLE_IntTLength, wt, wot), c, "fvc_ftle_output");
+                    int outnum = d.FTLE_ComputeVariation ? numout++ : (dir > 0 ? ss : d.Output_TRes - 1 - ss);
+                    double start = d.Output_TStart + ss * dir * d.Output_TDelta;
+                    write_ftle(&d, outnum, start, wt, wot, npts);
+                    if (!quiet) printf("Stopping FTLE output slide %d and writing to output file.\n", ss);
+                } else if (d.FTLE_ComputeVariation && !fmod(df - d.Data_FirstFrame, d.FTLE_VariationOutFreq)) {
+                    double start = d.Output_TStart + ss * dir * d.Output_TDelta;
+                    double t2 = dir > 0 ? tmax : tmin; /* slide not complete: t2 is the loaded bound */
+                    if (!written_first) {
+                        write_info(&d);
+                        written_first = 1;
+                    }
+                    check(fvc_ftle_output(f[ss], fabs(t2 - start), wt, wot), c, "fvc_ftle_output");
+                    write_ftle(&d, numout++, start, wt, wot, npts);
+                }
+            }
+        }
+        free(wt);
+        free(wot);
+        for (int ss = 0; ss < ns; ss++) fvc_ftle_destroy(f[ss]);
+        free(f);
+    } else if (d.Trace_Compute) {
+        /* seeds: ReadInTraceMesh / GenerateTracerMesh location chains */
+        int64_t n = 0;
+        fvc_point *ic = NULL;
+        if (d.Trace_GenerateMesh) {
+            double mn[3] = {d.Trace_Cart.XMin, d.Trace_Cart.YMin, d.Trace_Cart.ZMin};
+            double del[3] = {d.Trace_Cart.XDelta, d.Trace_Cart.YDelta, d.Trace_Cart.ZDelta};
+            int32_t res[3] = {d.Trace_Cart.XRes, d.Trace_Cart.YRes, d.Trace_Cart.ZRes};
+            n = (int64_t)res[0] * res[1] * res[2];
+            ic = (fvc_point *)calloc((size_t)n, sizeof(fvc_point));
+            check(fvc_seed_grid(c, mn, del, res, d.LocalSearchChecking, ic), c, "fvc_seed_grid");
+        } else {
+            if (d.Trace_MultipleInFiles || d.Trace_InFileFormat == 0)
+                fatal("%s", "Trace_MultipleInFiles / INITIALIZED input are not supported by this driver");
+            double *sx = read_seed_list(&d, &n);
+            ic = (fvc_point *)calloc((size_t)n, sizeof(fvc_point));
+            check(fvc_seed_list(c, sx, n, d.LocalSearchChecking, ic), c, "fvc_seed_list");
+            free(sx);
+        }
+        snprintf(path, sizeof path, "%s%s.IC", d.Path_Data, d.Trace_OutFilePrefix);
+        FILE *o = fopen(path, "wb");
+        if (!o) fatal("Could not open file %s", path);
+        int32_t n32 = (int32_t)n;
+        fwrite(&n32, 4, 1, o);
+        fwrite(ic, sizeof(fvc_point), (size_t)n, o);
+        fclose(o);
+        const int nl = d.Trace_NumLaunchTimes;
+        fvc_particles *P = NULL;
+        check(fvc_particles_create(c, n * nl > 0 ? n * nl : 1, &P), c, "fvc_particles_create");
+        double *rstart = (double *)malloc(sizeof(double) * nl), *rstop = (double *)malloc(sizeof(double) * nl);
+        int *status = (int *)calloc((size_t)nl, sizeof(int));
+        for (int ss = 0; ss < nl; ss++) {
+            check(fvc_particles_set(P, (int64_t)ss * n, ic, n), c, "fvc_particles_set");
+            rstart[ss] = d.Output_TStart + dir * ss * d.Trace_LaunchTimeSpacing; /* tracers.c:1287-1291 */
+            rstop[ss] = dir > 0 ? fmin(rstart[ss] + d.Trace_IntTLength, d.Output_TEnd)
+                                : fmax(rstart[ss] - d.Trace_IntTLength, d.Output_TEnd);
+        }
+        int upper = nl >= nproc / 2;
+        int64_t *cnt = (int64_t *)calloc((size_t)d.Output_TRes, sizeof(int64_t));
+        FILE **of = (FILE **)calloc((size_t)d.Output_TRes, sizeof(FILE *));
+        fvc_point *tmp = (fvc_point *)malloc(sizeof(fvc_point) * (size_t)(n > 0 ? n : 1));
+        int outframe = 0;
+        double nextout = d.Output_TStart;
+        for (int df = d.Data_FirstFrame; df != d.Data_LastFrame; df += dir) {
+            int flo = dir > 0 ? df : df - 1;
+            int lo = slot_for(&sl, flo, -1);
+            int hi = slot_for(&sl, flo + 1, lo);
+            double tmin = d.Data_TMin + flo * d.Data_TDelta, tmax = tmin + d.Data_TDelta;
+            double t1 = dir > 0 ? tmin : tmax, t2;
+            while (((dir > 0) ? (t1 - tmax) : (tmin - t1)) < 0) {
+                t2 = dir > 0 ? fmin(nextout, tmax) : fmax(tmin, nextout);
+                for (int ss = 0; ss < nl; ss++) {
+                    if (status[ss] != 2 && (status[ss] == 1 || (rstart[ss] <= fmax(t1, t2) && rstart[ss] >= fmin(t1, t2)))) {
+                        double t0 = now_s();
+                        if (status[ss] == 0) {
+                            status[ss] = 1;
+                            t1 = rstart[ss];
+                            check(fvc_launch_velocity(c, P, (int64_t)ss * n, n, rstart[ss], lo, hi, tmin, tmax), c,
+                                  "fvc_launch_velocity");
+                        }
+                        check(fvc_advect(c, P, (int64_t)ss * n, n, t1, t2, lo, hi, tmin, tmax, &ip), c, "fvc_advect");
+                        int64_t st = 0;
+                        fvc_last_step_count(c, &st);
+                        steps_total += st;
+                        t_compute += now_s() - t0;
+                        if (fabs(nextout - t2) < TINY) {
+                            check(fvc_particles_get(P, (int64_t)ss * n, tmp, n), c, "fvc_particles_get");
+                            for (int64_t q = 0; q < n; q++) {
+                                if (tmp[q].LeftDomainTime > -TINY && (!tmp[q].LeftDomain || d.Trace_AlwaysOutput)) {
+                                    if (!of[outframe]) {
+                                        snprintf(path, sizeof path, "%s%s.%d.bin", d.Path_Output, d.Trace_OutFilePrefix,
+                                                 outframe);
+                                        of[outframe] = fopen(path, "wb");
+                                        if (!of[outframe]) fatal("Could not open file %s", path);
+                                        double tt = d.Output_TStart + outframe * dir * d.Output_TDelta;
+                                        fwrite(&tt, 8, 1, of[outframe]);
+                                    }
+                                    fwrite(tmp[q].X, 8, 3, of[outframe]);
+                                    cnt[outframe]++;
+                                }
+                            }
+                        }
+                    }
+                    if (((dir > 0) ? (rstop[ss] - t2) : (t2 - rstop[ss])) < TINY) status[ss] = 2;
+                }
+                if (fabs(t2 - nextout) < TINY) {
+                    nextout += dir * d.Output_TDelta;
+                    outframe++;
+                    if (outframe >= d.Output_TRes) break;
+                }
+                t1 = t2;
+            }
+        }
+        /* OutputTracers_OMP writes every frame file (tracers.c:1700-1761); OutputTracers only non-empty ones */
+        for (int i = 0; i < d.Output_TRes; i++) {
+            if (!of[i] && upper) {
+                snprintf(path, sizeof path, "%s%s.%d.bin", d.Path_Output, d.Trace_OutFilePrefix, i);
+                of[i] = fopen(path, "wb");
+                if (!of[i]) fatal("Could not open file %s", path);
+                double tt = d.Output_TStart + i * dir * d.Output_TDelta;
+                fwrite(&tt, 8, 1, of[i]);
+            }
+            if (of[i]) fclose(of[i]);
+        }
+        free(tmp);
+        free(cnt);
+        free(of);
+        free(rstart);
+        free(rstop);
+        free(status);
+        free(ic);
+        fvc_particles_destroy(P);
+    }
+    if (!quiet)
+        printf("Simulation complete: %.3f s total, %.3f s on the GPU path, %.3f s velocity I/O, %lld particle-steps\n",
+               now_s() - t_start, t_compute, sl.io_s, (long long)steps_total);
+    fvc_destroy(c);
+    free(sl.slot_file);
+    free(sl.buf);
+    free(xyz);
+    free(conn);
+    free(adj);
+    return 0;
+}

@@ -1,0 +1,188 @@
+/*
+ * fvc.h - C ABI of the MI355X (gfx950) flowVC advection path.
+ *
+ * ompVC has no plugin layer: its hot path is the per-point loop that calls
+ * Advect() (ompVC/integration.h:13) from ComputeFTLE (ompVC/ftle.c:109-173) and
+ * Compute_Tracer (ompVC/tracers.c:217-229), with the mesh and the two loaded
+ * velocity slices in globals (ompVC/globals.h:34-67) and errors reported by
+ * FatalError -> exit(1) (ompVC/io.c:15-52).  This ABI replaces those loops one
+ * level up, with batched calls over device-resident state.  Plain pointers and
+ * sizes only; host arrays are lent for the duration of a call.
+ *
+ * Errors: every call returns FVC_OK (0) or a negative code; fvc_strerror()
+ * names it.  A caller keeping ompVC's behaviour turns non-zero into exit(1).
+ * Threading: one host thread per context; calls on one context are ordered on
+ * the context's stream and are synchronous unless named *_async.
+ */
+#ifndef FVC_H
+#define FVC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define FVC_OK 0
+#define FVC_E_INVALID (-1)                /* bad argument */
+#define FVC_E_NOMEM (-2)                  /* device or host allocation failed */
+#define FVC_E_HIP (-3)                    /* HIP runtime error */
+#define FVC_E_INDETERMINATE_NEIGHBOR (-4) /* ompVC/mesh.c:561, velocity.c:597 */
+#define FVC_E_TLOC_RANGE (-5)             /* ompVC/velocity.c:277, :478 */
+#define FVC_E_ELEM_NEG1 (-6)              /* ompVC/velocity.c:472, integration.c:335 */
+#define FVC_E_WALK_LIMIT (-7)             /* walk exceeded the hop limit (no reference counterpart) */
+#define FVC_E_EIG (-8)                    /* ompVC/mymath.c:107 "Too many iterations" */
+#define FVC_E_ADMISSION (-9)              /* serving: request does not fit free device memory */
+#define FVC_E_STATE (-10)                 /* call out of sequence */
+#define FVC_E_LEFT (-11)                  /* ompVC/integration.c:326 integrate a point that left */
+#define FVC_E_NODEVICE (-12)              /* no usable gfx950 device */
+
+/* ---- enums -------------------------------------------------------------- */
+enum fvc_mesh_type { FVC_CARTESIAN = 0, FVC_UNSTRUCTURED = 1 };
+/* storage/arithmetic of the velocity field and element geometry */
+enum fvc_precision {
+    FVC_PREC_F64 = 0,  /* FP64 throughout, bit-identical to ompVC's arithmetic */
+    FVC_PREC_F32 = 1,  /* FP32 geometry+velocity, FP64 state, exact FP64 walk decisions */
+    FVC_PREC_BF16 = 2, /* as F32 with velocity stored bf16 x per-slice scale */
+    FVC_PREC_F16 = 3   /* as F32 with velocity stored fp16 x per-slice scale */
+};
+/* Int_Type (ompVC/integration.c:32-37) plus multistep extensions */
+enum fvc_int_type { FVC_INT_EULER = 0, FVC_INT_RK4 = 1, FVC_INT_AB2 = 3, FVC_INT_AB3 = 4 };
+/* point location modes (ompVC/mesh.c:304-398, :496-611) */
+enum fvc_locate_mode {
+    FVC_LOCATE_WALK = 0,        /* Get_Element_Local_Search(X, guess): -1 when the walk leaves */
+    FVC_LOCATE_GLOBAL = 1,      /* Get_Element_Global_Search(X): lowest-index containing element */
+    FVC_LOCATE_WALK_GLOBAL = 2  /* walk, then global search when the walk leaves */
+};
+
+/* ---- records (byte layouts of ompVC/structs.h:16-59) -------------------- */
+typedef struct fvc_point { /* LagrangianPoint: 80 bytes */
+    double X[3];
+    double V[3];
+    int32_t ElementIndex;
+    int32_t AuxElementIndex;
+    int32_t LeftDomain;
+    int32_t pad_;
+    double LeftDomainTime;
+    double Scalar;
+} fvc_point;
+
+typedef struct fvc_mesh_desc {
+    int32_t dims;      /* Dimensions: 2 or 3 */
+    int32_t mesh_type; /* Data_MeshType */
+    double bounds[6];  /* Data_MeshBounds X/Y/Z Min,Max (ompVC/velocity.c:654-669) */
+    /* Cartesian velocity grid, <prefix>_Cartesian.bin (ompVC/mesh.c:67-122) */
+    double cart_min[3], cart_max[3];
+    int32_t cart_res[3];
+    /* unstructured: <prefix>_coordinates / _connectivity / _adjacency.bin
+     * (ompVC/mesh.c:124-213); 2-D meshes carry 3 node ids and -1 per element */
+    int64_t num_nodes;
+    const double *coords;  /* num_nodes x 3 */
+    int64_t num_elems;
+    const int32_t *conn;   /* num_elems x 4 */
+    const int32_t *adj;    /* num_elems x 4 */
+} fvc_mesh_desc;
+
+typedef struct fvc_params {
+    int32_t precision;       /* enum fvc_precision */
+    int32_t max_slices;      /* resident velocity slices (>= 2) */
+    int32_t device;          /* HIP device ordinal */
+    int32_t hash_cells;      /* spatial-hash cells along the longest axis; 0 = auto */
+    int32_t walk_limit;      /* hop limit of one walk; 0 = 1<<20 */
+} fvc_params;
+
+typedef struct fvc_int_params {
+    int32_t int_type;    /* enum fvc_int_type */
+    int32_t extrapolate; /* Int_Extrapolate (ompVC/integration.c:27-43) */
+    double time_step;    /* Int_TimeStep */
+    int32_t block;       /* threads per block; 0 = default */
+    int32_t steps_per_launch; /* 0 = whole interval in one launch */
+} fvc_int_params;
+
+typedef struct fvc_ctx fvc_ctx;
+typedef struct fvc_particles fvc_particles;
+typedef struct fvc_ftle fvc_ftle;
+
+/* ---- context ------------------------------------------------------------- */
+/* Upload the mesh (LoadMeshData, ompVC/mesh.c:21-43) and build the device
+ * spatial hash.  Replaces the mesh globals of ompVC/globals.h:38-47. */
+int fvc_create(const fvc_mesh_desc *mesh, const fvc_params *params, fvc_ctx **out);
+void fvc_destroy(fvc_ctx *ctx);
+const char *fvc_strerror(int code);
+/* Message of the last failure on this context (HIP error text, offending index). */
+const char *fvc_last_error(const fvc_ctx *ctx);
+int fvc_sync(fvc_ctx *ctx);
+/* Device memory in use by the context, and free device memory (bytes). */
+int fvc_memory(fvc_ctx *ctx, int64_t *used, int64_t *free_bytes);
+
+/* Store one velocity slice, n x (u,v,w) FP64 as read from <prefix>_vel.<n>.bin
+ * (ompVC/velocity.c:126-141, 184-211), into resident slot `slot`.  Replaces the
+ * two-slot ring of LoadCartVelDataFrame / LoadUnstructVelDataFrame
+ * (ompVC/velocity.c:61-234): callers name the two slots to interpolate between.
+ * BF16/F16 precision stores the slice scaled by its max-abs value. */
+int fvc_load_slice(fvc_ctx *ctx, int32_t slot, double timestamp, const double *uvw, int64_t n);
+
+/* ---- particles (device-resident LagrangianPoint state) ------------------- */
+int fvc_particles_create(fvc_ctx *ctx, int64_t capacity, fvc_particles **out);
+void fvc_particles_destroy(fvc_particles *p);
+/* copy n host records into positions [begin, begin+n) / back out */
+int fvc_particles_set(fvc_particles *p, int64_t begin, const fvc_point *recs, int64_t n);
+int fvc_particles_get(fvc_particles *p, int64_t begin, fvc_point *recs, int64_t n);
+int64_t fvc_particles_capacity(const fvc_particles *p);
+
+/* GetVelocity at launch for points with !LeftDomain (ompVC/tracers.c:202-215,
+ * ftle.c:80-93): sets V and, as the reference's side effects, LeftDomain and
+ * ElementIndex.  Slices `lo`,`hi` hold the data at tmin,tmax. */
+int fvc_launch_velocity(fvc_ctx *ctx, fvc_particles *p, int64_t begin, int64_t count, double t,
+                        int32_t lo, int32_t hi, double tmin, double tmax);
+
+/* Advect(&pt, t1, t2) (ompVC/integration.c:22-45) for every point of
+ * [begin, begin+count) with LeftDomainTime > -TINY && (!LeftDomain ||
+ * extrapolate), the filter of ompVC/tracers.c:224 and ftle.c:113.  Slices lo,hi
+ * hold the loaded interval [tmin, tmax] (main.c:188-195). */
+int fvc_advect(fvc_ctx *ctx, fvc_particles *p, int64_t begin, int64_t count, double t1, double t2,
+               int32_t lo, int32_t hi, double tmin, double tmax, const fvc_int_params *ip);
+/* Points advanced by the last fvc_advect (particle-steps, the step in which a
+ * point exits included, floating-point micro-steps below 1e-12*h excluded). */
+int fvc_last_step_count(fvc_ctx *ctx, int64_t *particle_steps);
+
+/* ---- point location -------------------------------------------------------- */
+/* n query points xyz (n x 3, host), guesses (walk modes), results elem (host).
+ * Element indices are those ompVC returns for the same (X, guess). */
+int fvc_locate(fvc_ctx *ctx, const double *xyz, const int32_t *guess, int64_t n, int32_t mode, int32_t *elem);
+/* Seed location chains of ReadInTraceMesh (ompVC/tracers.c:1111-1251) for a
+ * point list, and GenerateTracerMesh (tracers.c:1298-1532) for a grid: writes
+ * X, LeftDomain, ElementIndex and LeftDomainTime of n records (grid: k,j,i
+ * order).  Exact chain semantics; see fvc_seed_grid for the grid form. */
+int fvc_seed_list(fvc_ctx *ctx, const double *xyz, int64_t n, int32_t local_search_checking, fvc_point *out);
+int fvc_seed_grid(fvc_ctx *ctx, const double mn[3], const double del[3], const int32_t res[3],
+                  int32_t local_search_checking, fvc_point *out);
+
+/* ---- FTLE (ompVC/ftle.c) ---------------------------------------------------- */
+/* Grid generation, outside masks and the location chain of InitializeFTLEArray
+ * (ompVC/ftle.c:383-609) for one output slide. */
+int fvc_ftle_create(fvc_ctx *ctx, const double mn[3], const double mx[3], const int32_t res[3],
+                    int32_t local_search_checking, double start_time, double stop_time, fvc_ftle **out);
+void fvc_ftle_destroy(fvc_ftle *f);
+/* One frame of ComputeFTLE for this slide (ompVC/ftle.c:68-214): launch when
+ * not yet launched, advect, early FTLE for exits (ftle.c:788-1048) with the
+ * reference's loop-order semantics, position update.  *done receives 0 when
+ * the slide was not integrated in this interval (complete, or not launched
+ * yet), 1 when it was, 2 when it was and reached its stop time. */
+int fvc_ftle_frame(fvc_ftle *f, int32_t direction, int32_t lo, int32_t hi, double tmin, double tmax,
+                   const fvc_int_params *ip, int32_t *done);
+/* Missing values via GetFTLEForPoint (ftle.c:1050-1203) with integration time
+ * int_t, then FTLEwT / FTLEwoT in k,j,i order (i fastest) as OutputFTLE writes. */
+int fvc_ftle_output(fvc_ftle *f, double int_t, double *ftle_wt, double *ftle_wot);
+int fvc_ftle_points(fvc_ftle *f, int64_t *n, int64_t *inside);
+
+/* ---- Z-order seed sort (cudaVC/ZOrderSort/main.cpp:86-101 ordering) ------- */
+/* Sort n points by the interleaved-bit (x most significant) order of their
+ * coordinates quantised to 21 bits on [lo, hi]; perm receives the stable order. */
+int fvc_zsort(fvc_ctx *ctx, const double *xyz, int64_t n, const double lo[3], const double hi[3], int32_t *perm);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

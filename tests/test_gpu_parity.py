@@ -1,0 +1,172 @@
+"""GPU kernels through the C ABI against the oracle (CPU FP64 restatement) on
+the same seeded inputs: element indices bit-exact for identical query points
+(walk from a guess, lowest-index global search) in every precision mode; FP64
+trajectories bit-exact; FP32/BF16/FP16 trajectories within stated tolerances;
+Z-order permutation ordered by the reference comparator."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import cases
+from fvcpy import fvc, host
+
+sys.path.insert(0, os.path.join(cases.REPO, "oracle"))
+import oracle  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+PIPE_BOUNDS = [-1.0, 1.0, -1.0, 1.0, 0.0, 10.0]
+
+
+@pytest.fixture(scope="module")
+def pipe():
+    t = host.pipe(12, 12, 40, 1.0, 10.0)
+    xyz, conn, adj = host.tube_mesh(t)
+    slices = [host.tube_velocity(t, xyz, s * 0.02, 0.01, 1234, s) for s in range(3)]
+    return t, xyz, conn, adj, slices
+
+
+def query_points(t, n, seed):
+    rng = np.random.default_rng(seed)
+    pts = host.tube_seeds(t, n, 0.0, 10.0, seed)
+    # add points on element faces/edges/vertices (exact grid coordinates) and outside
+    g = rng.integers(0, 13, size=(n // 4, 3)).astype(np.float64)
+    grid = np.stack([-1 + g[:, 0] * 2 / 12, -1 + g[:, 1] * 2 / 12, g[:, 2] * 10 / 40], 1)
+    out = rng.uniform(-1.5, 1.5, size=(n // 8, 3)) * [1, 1, 8]
+    return np.concatenate([pts, grid, out])
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32", "bf16"])
+def test_locate_walk_and_global_bit_exact(pipe, precision):
+    t, xyz, conn, adj, _ = pipe
+    ctx = fvc.Context(3, PIPE_BOUNDS, precision, 2, coords=xyz, conn=conn, adj=adj)
+    om = oracle.Mesh(3, PIPE_BOUNDS, coords=xyz, conn=conn, adj=adj)
+    q = query_points(t, 4000, 11)
+    rng = np.random.default_rng(5)
+    guess = rng.integers(0, conn.shape[0], q.shape[0]).astype(np.int32)
+    print("locate %s n=%d" % (precision, q.shape[0]), flush=True)
+    g_gpu = ctx.locate(q, guess, fvc.LOCATE_WALK)
+    g_ref, rc = om.locate_local(q, guess)
+    assert rc == 0
+    np.testing.assert_array_equal(g_gpu, g_ref)
+    glob_gpu = ctx.locate(q, None, fvc.LOCATE_GLOBAL)
+    glob_ref = om.locate_global(q)
+    np.testing.assert_array_equal(glob_gpu, glob_ref)
+    both = ctx.locate(q, guess, fvc.LOCATE_WALK_GLOBAL)
+    np.testing.assert_array_equal(both, np.where(g_ref >= 0, g_ref, glob_ref))
+    assert (glob_ref >= 0).sum() > q.shape[0] // 2
+
+
+def test_seed_list_chain_matches_oracle(pipe):
+    t, xyz, conn, adj, _ = pipe
+    ctx = fvc.Context(3, PIPE_BOUNDS, "f64", 2, coords=xyz, conn=conn, adj=adj)
+    om = oracle.Mesh(3, PIPE_BOUNDS, coords=xyz, conn=conn, adj=adj)
+    q = query_points(t, 2000, 3)
+    for lsc in (1, 0):
+        a = ctx.seed_list(q, lsc)
+        b, rc = om.seed_list(q, lsc)
+        assert rc == 0
+        np.testing.assert_array_equal(a["LeftDomain"], b["LeftDomain"])
+        np.testing.assert_array_equal(a["ElementIndex"][a["LeftDomain"] == 0], b["ElementIndex"][b["LeftDomain"] == 0])
+        np.testing.assert_array_equal(a["LeftDomainTime"], b["LeftDomainTime"])
+
+
+def _advect_both(pipe, precision, n=3000, t1=0.0, t2=0.02, dt=1e-3):
+    t, xyz, conn, adj, slices = pipe
+    ctx = fvc.Context(3, PIPE_BOUNDS, precision, 4, coords=xyz, conn=conn, adj=adj)
+    om = oracle.Mesh(3, PIPE_BOUNDS, coords=xyz, conn=conn, adj=adj)
+    for s in range(2):
+        ctx.load_slice(s, s * 0.02, slices[s])
+        om.set_slot(s, slices[s])
+    om.set_interval(0.0, 0.02)
+    seeds = host.tube_seeds(t, n, 0.0, 10.0, 99)
+    recs, rc = om.seed_list(seeds, 1)
+    assert rc == 0
+    ref = recs.copy()
+    P = fvc.Particles(ctx, n)
+    P.set(recs)
+    print("advect %s n=%d" % (precision, n), flush=True)
+    steps = P.advect(t1, t2, 0, 1, 0.0, 0.02, dt)
+    got = P.get()
+    assert om.advect(ref, t1, t2, dt) == 0
+    return got, ref, steps
+
+
+def test_advect_f64_bit_exact(pipe):
+    got, ref, steps = _advect_both(pipe, "f64")
+    np.testing.assert_array_equal(got["X"], ref["X"])
+    np.testing.assert_array_equal(got["ElementIndex"], ref["ElementIndex"])
+    np.testing.assert_array_equal(got["LeftDomain"], ref["LeftDomain"])
+    np.testing.assert_array_equal(got["LeftDomainTime"], ref["LeftDomainTime"])
+    inside = (ref["LeftDomainTime"] > -1e-10) & True
+    assert steps >= 20 * (inside.sum() - ref["LeftDomain"].sum())
+
+
+@pytest.mark.parametrize("precision,tol", [("f32", 1e-5), ("bf16", 2e-3), ("f16", 2e-4)])
+def test_advect_reduced_precision_within_tolerance(pipe, precision, tol):
+    """Position error after one frame (20 RK4 steps); tol is absolute (domain 10)."""
+    got, ref, _ = _advect_both(pipe, precision)
+    same = (got["LeftDomain"] == ref["LeftDomain"])
+    assert same.mean() > 0.99
+    live = same & (ref["LeftDomain"] == 0) & (ref["LeftDomainTime"] > -1e-10)
+    err = np.abs(got["X"][live] - ref["X"][live]).max()
+    assert err <= tol, err
+
+
+def test_advect_cartesian_f64_bit_exact():
+    nx, ny = 64, 32
+    s0 = host.double_gyre_slice(nx, ny, 0.0)
+    s1 = host.double_gyre_slice(nx, ny, 0.5)
+    b = [0.0, 2.0, 0.0, 1.0, 0.0, 0.0]
+    ctx = fvc.Context(2, b, "f64", 2, cart_min=[0, 0, 0], cart_max=[2, 1, 0], cart_res=[nx, ny, 1])
+    om = oracle.Mesh(2, b, cart_min=[0, 0, 0], cart_max=[2, 1, 0], cart_res=[nx, ny, 1])
+    ctx.load_slice(0, 0.0, s0)
+    ctx.load_slice(1, 0.5, s1)
+    om.set_slot(0, s0)
+    om.set_slot(1, s1)
+    om.set_interval(0.0, 0.5)
+    rng = np.random.default_rng(1)
+    n = 5000
+    recs = np.zeros(n, oracle.POINT_DTYPE)
+    recs["X"][:, 0] = rng.uniform(0, 2, n)
+    recs["X"][:, 1] = rng.uniform(0, 1, n)
+    recs["X"][:10, 0] = 2.0  # boundary points (floor/ceil exits)
+    ref = recs.copy()
+    P = fvc.Particles(ctx, n)
+    P.set(recs)
+    P.advect(0.0, 0.5, 0, 1, 0.0, 0.5, 0.01)
+    got = P.get()
+    assert om.advect(ref, 0.0, 0.5, 0.01) == 0
+    np.testing.assert_array_equal(got["X"], ref["X"])
+    np.testing.assert_array_equal(got["LeftDomain"], ref["LeftDomain"])
+    np.testing.assert_array_equal(got["LeftDomainTime"], ref["LeftDomainTime"])
+
+
+def _zorder_less(a, b):
+    """cudaVC/ZOrderSort/main.cpp:86-101 comparator on integer coordinates."""
+    j, x = 0, 0
+    for k in range(3):
+        y = int(a[k]) ^ int(b[k])
+        if x < y and x < (x ^ y):
+            j, x = k, y
+    return a[j] < b[j]
+
+
+def test_zsort_order_matches_reference_comparator(pipe):
+    t, xyz, conn, adj, _ = pipe
+    ctx = fvc.Context(3, PIPE_BOUNDS, "f32", 2, coords=xyz, conn=conn, adj=adj)
+    rng = np.random.default_rng(2)
+    n = 5000
+    pts = rng.uniform([-1, -1, 0], [1, 1, 10], size=(n, 3))
+    pts[:50] = pts[50:100]  # duplicates: stable order by index
+    lo, hi = np.array([-1.0, -1.0, 0.0]), np.array([1.0, 1.0, 10.0])
+    perm = ctx.zsort(pts, lo, hi)
+    assert sorted(perm.tolist()) == list(range(n))
+    q = np.clip(np.floor((pts - lo) * (2097152.0 / (hi - lo))), 0, 2097151).astype(np.int64)
+    qs = q[perm]
+    for i in range(n - 1):
+        assert not _zorder_less(qs[i + 1], qs[i]), i
+        if (qs[i] == qs[i + 1]).all():
+            assert perm[i] < perm[i + 1]
