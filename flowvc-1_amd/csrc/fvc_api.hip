@@ -287,6 +287,10 @@ extern "C" int fvc_create(const fvc_mesh_desc *md, const fvc_params *pp, fvc_ctx
         fvc_destroy(c);
         return rc;
     }
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        fvc_destroy(c);
+        return FVC_E_HIP;
+    }
     hipMemset(c->d_err, 0, sizeof(int));
     if (md->mesh_type == 0) {
         /* Cartesian grid (mesh.c:96-121) */
@@ -393,8 +397,44 @@ extern "C" void fvc_destroy(fvc_ctx *c) {
     hipFree(c->d_where);
     hipFree(c->d_steps);
     if (c->scratch) hipFree(c->scratch);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
+}
+
+extern "C" int fvc_set_timing(fvc_ctx *c, int32_t enable) {
+    if (!c) return FVC_E_INVALID;
+    c->timing = enable ? 1 : 0;
+    return FVC_OK;
+}
+
+extern "C" int fvc_kernel_time(fvc_ctx *c, double *ms, int64_t *launches, int64_t *steps, int32_t reset) {
+    if (!c) return FVC_E_INVALID;
+    if (ms) *ms = c->t_ms;
+    if (launches) *launches = c->t_launches;
+    if (steps) *steps = c->t_steps;
+    if (reset) {
+        c->t_ms = 0.0;
+        c->t_launches = 0;
+        c->t_steps = 0;
+    }
+    return FVC_OK;
+}
+
+extern "C" int fvc_copy_probe(fvc_ctx *c, int64_t bytes) {
+    if (!c || bytes <= 0) return FVC_E_INVALID;
+    void *a = nullptr, *b = nullptr;
+    FVC_HIP(hipMalloc(&a, (size_t)bytes));
+    if (hipMalloc(&b, (size_t)bytes) != hipSuccess) {
+        hipFree(a);
+        return FVC_E_NOMEM;
+    }
+    hipMemsetAsync(a, 1, (size_t)bytes, c->stream);
+    int rc = launch_copy_probe(c, b, a, bytes);
+    hipFree(a);
+    hipFree(b);
+    return rc;
 }
 
 extern "C" int fvc_sync(fvc_ctx *c) {
@@ -563,6 +603,40 @@ extern "C" int fvc_particles_get(fvc_particles *p, int64_t begin, fvc_point *rec
         recs[i].LeftDomain = iv[2 * n + i];
         recs[i].pad_ = 0;
     }
+    return FVC_OK;
+}
+
+static int particles_d2d(fvc_particles *dst, int64_t db, const fvc_particles *src, int64_t sb, int64_t n) {
+    fvc_ctx *c = dst->ctx;
+    double *ds[8] = {dst->X[0], dst->X[1], dst->X[2], dst->V[0], dst->V[1], dst->V[2], dst->left_time, dst->scalar};
+    double *ss[8] = {src->X[0], src->X[1], src->X[2], src->V[0], src->V[1], src->V[2], src->left_time, src->scalar};
+    for (int a = 0; a < 8; a++)
+        FVC_HIP(hipMemcpyAsync(ds[a] + db, ss[a] + sb, 8 * n, hipMemcpyDeviceToDevice, c->stream));
+    int *di[3] = {dst->elem, dst->aux, dst->left};
+    int *si[3] = {src->elem, src->aux, src->left};
+    for (int a = 0; a < 3; a++)
+        FVC_HIP(hipMemcpyAsync(di[a] + db, si[a] + sb, 4 * n, hipMemcpyDeviceToDevice, c->stream));
+    FVC_HIP(hipStreamSynchronize(c->stream));
+    return FVC_OK;
+}
+
+extern "C" int fvc_particles_copy(fvc_particles *dst, int64_t db, const fvc_particles *src, int64_t sb, int64_t n) {
+    if (!dst || !src || dst->ctx != src->ctx || n < 0 || db < 0 || sb < 0 || db + n > dst->cap || sb + n > src->cap)
+        return FVC_E_INVALID;
+    if (n == 0) return FVC_OK;
+    return particles_d2d(dst, db, src, sb, n);
+}
+
+extern "C" int fvc_particles_get_xyz(fvc_particles *p, int64_t begin, double *xyz, int32_t *left, int64_t n) {
+    if (!p || !xyz || begin < 0 || n < 0 || begin + n > p->cap) return FVC_E_INVALID;
+    fvc_ctx *c = p->ctx;
+    std::vector<double> d(3 * (size_t)n);
+    for (int a = 0; a < 3; a++)
+        FVC_HIP(hipMemcpyAsync(d.data() + a * n, p->X[a] + begin, 8 * n, hipMemcpyDeviceToHost, c->stream));
+    if (left) FVC_HIP(hipMemcpyAsync(left, p->left + begin, 4 * n, hipMemcpyDeviceToHost, c->stream));
+    FVC_HIP(hipStreamSynchronize(c->stream));
+    for (int64_t i = 0; i < n; i++)
+        for (int a = 0; a < 3; a++) xyz[3 * i + a] = d[a * n + i];
     return FVC_OK;
 }
 
@@ -773,12 +847,20 @@ extern "C" int fvc_ftle_create(fvc_ctx *c, const double mn[3], const double mx[3
         fvc_ftle_destroy(f);
         return rc;
     }
-    /* points inside = not left */
-    int *flags = (int *)scratch(c, sizeof(int) * n);
-    if (!flags) {
+    /* snapshot of the initial state for fvc_ftle_reset */
+    if ((rc = fvc_particles_create(c, f->n, &f->init_pts)) || (rc = particles_d2d(f->init_pts, 0, p, 0, f->n))) {
         fvc_ftle_destroy(f);
-        return FVC_E_NOMEM;
+        return rc;
     }
+    if (hipMalloc((void **)&f->init_have, sizeof(int) * n) != hipSuccess ||
+        hipMalloc((void **)&f->init_w, sizeof(double) * 2 * n) != hipSuccess) {
+        fvc_ftle_destroy(f);
+        return set_err(c, FVC_E_NOMEM, "FTLE snapshot alloc");
+    }
+    c->bytes_used += (int64_t)(n * (4 + 16));
+    hipMemcpyAsync(f->init_have, f->have, sizeof(int) * n, hipMemcpyDeviceToDevice, c->stream);
+    hipMemcpyAsync(f->init_w, f->wt, sizeof(double) * 2 * n, hipMemcpyDeviceToDevice, c->stream);
+    /* points inside = not left */
     hipStreamSynchronize(c->stream);
     std::vector<int> lh(n);
     hipMemcpy(lh.data(), p->left, sizeof(int) * n, hipMemcpyDeviceToHost);
@@ -793,8 +875,26 @@ extern "C" void fvc_ftle_destroy(fvc_ftle *f) {
     if (!f) return;
     hipStreamSynchronize(f->ctx->stream);
     if (f->buf) hipFree(f->buf);
+    if (f->init_have) hipFree(f->init_have);
+    if (f->init_w) hipFree(f->init_w);
+    fvc_particles_destroy(f->init_pts);
     fvc_particles_destroy(f->pts);
     delete f;
+}
+
+extern "C" int fvc_ftle_reset(fvc_ftle *f) {
+    if (!f || !f->init_pts) return FVC_E_INVALID;
+    fvc_ctx *c = f->ctx;
+    size_t n = (size_t)f->n;
+    int rc = particles_d2d(f->pts, 0, f->init_pts, 0, f->n);
+    if (rc) return rc;
+    FVC_HIP(hipMemcpyAsync(f->have, f->init_have, sizeof(int) * n, hipMemcpyDeviceToDevice, c->stream));
+    FVC_HIP(hipMemcpyAsync(f->wt, f->init_w, sizeof(double) * 2 * n, hipMemcpyDeviceToDevice, c->stream));
+    FVC_HIP(hipMemsetAsync(f->exited, 0, sizeof(int) * n, c->stream));
+    FVC_HIP(hipMemsetAsync(f->advected, 0, sizeof(int) * n, c->stream));
+    FVC_HIP(hipStreamSynchronize(c->stream));
+    f->status = 0;
+    return FVC_OK;
 }
 
 extern "C" int fvc_ftle_frame(fvc_ftle *f, int32_t dir, int32_t lo, int32_t hi, double tmin, double tmax,

@@ -40,6 +40,11 @@ struct fvc_ctx {
     /* scratch (grown on demand) */
     void *scratch;
     size_t scratch_bytes;
+    /* kernel timing (HIP events on `stream`) */
+    int timing;
+    hipEvent_t ev0, ev1;
+    double t_ms;
+    int64_t t_launches, t_steps;
 };
 
 struct fvc_particles {
@@ -74,6 +79,10 @@ struct fvc_ftle {
     int64_t nactive;
     void *buf;
     int64_t inside;
+    /* state right after creation, for fvc_ftle_reset */
+    fvc_particles *init_pts;
+    int *init_have;
+    double *init_w;
 };
 
 /* launchers (fvc_kernels.hip); all return FVC_OK or a negative code */
@@ -114,8 +123,26 @@ int launch_ftle_mask(fvc_ftle *f);
 int launch_compact(fvc_ctx *c, const int *flags, int64_t n, int *out_idx, int64_t *count);
 int launch_zsort(fvc_ctx *c, const double *dxyz, int64_t n, const double lo[3], const double hi[3], int *dperm);
 
+int launch_copy_probe(fvc_ctx *c, void *dst, const void *src, int64_t bytes);
+
 int check_err(fvc_ctx *c, const char *what);
 void *scratch(fvc_ctx *c, size_t bytes);
+/* bracket the hot kernel launch with events when timing is on */
+inline void timing_begin(fvc_ctx *c) {
+    if (c->timing) (void)hipEventRecord(c->ev0, c->stream);
+}
+inline void timing_end(fvc_ctx *c) {
+    if (c->timing) (void)hipEventRecord(c->ev1, c->stream);
+}
+inline void timing_collect(fvc_ctx *c, int64_t steps) {
+    if (!c->timing) return;
+    float ms = 0.0f;
+    if (hipEventSynchronize(c->ev1) == hipSuccess && hipEventElapsedTime(&ms, c->ev0, c->ev1) == hipSuccess) {
+        c->t_ms += ms;
+        c->t_launches++;
+        c->t_steps += steps;
+    }
+}
 } // namespace fvc
 
 #define FVC_HIP(call)                                                                                                  \

@@ -166,16 +166,19 @@ int launch_advect(fvc_ctx *c, PtrSet p, const int *index, int64_t begin, int64_t
     if (count > 0) {
         int block = ip.block > 0 ? ip.block : BLOCK;
         if (block != BLOCK) block = BLOCK; /* kernels are compiled for 256-lane blocks */
+        timing_begin(c);
         FVC_DISPATCH(c->precision, c->mesh.dims, c->mesh.mesh_type, k_advect,
                      <<<grid_for(count, block), block, 0, c->stream>>>(c->mesh, f, p, index, begin, count, t1, t2,
                                                                        ip.int_type == FVC_INT_RK4 ? 1 : 0, ip.time_step,
                                                                        ip.extrapolate, errbuf(c), c->d_steps));
+        timing_end(c);
         FVC_HIP(hipGetLastError());
     }
     unsigned long long steps = 0;
     FVC_HIP(hipMemcpyAsync(&steps, c->d_steps, sizeof steps, hipMemcpyDeviceToHost, c->stream));
     int rc = check_err(c, "fvc_advect");
     c->last_steps = (int64_t)steps;
+    if (count > 0) timing_collect(c, (int64_t)steps);
     return rc;
 }
 
@@ -750,10 +753,12 @@ int launch_ftle_frame(fvc_ftle *f, int lo, int hi, double tmin, double tmax, dou
     if (rc) return rc;
     FVC_HIP(hipMemsetAsync(c->d_steps, 0, sizeof(unsigned long long), c->stream));
     if (f->nactive > 0) {
+        timing_begin(c);
         FVC_DISPATCH(c->precision, c->mesh.dims, c->mesh.mesh_type, k_ftle_advect,
                      <<<grid_for(f->nactive, BLOCK), BLOCK, 0, c->stream>>>(c->mesh, fld, d, f->active, f->nactive,
                                                                             t1, t2, itype, ip.time_step,
                                                                             ip.extrapolate, errbuf(c), c->d_steps));
+        timing_end(c);
         FVC_HIP(hipGetLastError());
         if (!ip.extrapolate) {
             FVC_DISPATCH(c->precision, c->mesh.dims, c->mesh.mesh_type, k_ftle_early,
@@ -769,7 +774,26 @@ int launch_ftle_frame(fvc_ftle *f, int lo, int hi, double tmin, double tmax, dou
     FVC_HIP(hipMemcpyAsync(&steps, c->d_steps, sizeof steps, hipMemcpyDeviceToHost, c->stream));
     rc = check_err(c, "fvc_ftle_frame");
     c->last_steps = (int64_t)steps;
+    if (f->nactive > 0) timing_collect(c, (int64_t)steps);
     return rc;
+}
+
+/* plain streaming copy, 16 B per lane (HBM byte-counter calibration) */
+__global__ __launch_bounds__(BLOCK) void k_copy_probe(uint4 *__restrict__ dst, const uint4 *__restrict__ src,
+                                                     int64_t n16) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n16; i += stride) dst[i] = src[i];
+}
+
+int launch_copy_probe(fvc_ctx *c, void *dst, const void *src, int64_t bytes) {
+    int64_t n16 = bytes / 16;
+    unsigned grid = (unsigned)std::min<int64_t>((n16 + BLOCK - 1) / BLOCK, 65536);
+    if (grid == 0) return FVC_OK;
+    k_copy_probe<<<grid, BLOCK, 0, c->stream>>>((uint4 *)dst, (const uint4 *)src, n16);
+    FVC_HIP(hipGetLastError());
+    FVC_HIP(hipStreamSynchronize(c->stream));
+    return FVC_OK;
 }
 
 int launch_ftle_fill(fvc_ftle *f, double int_t) {

@@ -80,6 +80,12 @@ def lib():
         L.fvc_ftle_output.argtypes = [vp, d, vp, vp]
         L.fvc_ftle_points.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]
         L.fvc_zsort.argtypes = [vp, vp, i64, vp, vp, vp]
+        L.fvc_particles_copy.argtypes = [vp, i64, vp, i64, i64]
+        L.fvc_particles_get_xyz.argtypes = [vp, i64, vp, vp, i64]
+        L.fvc_ftle_reset.argtypes = [vp]
+        L.fvc_set_timing.argtypes = [vp, i32]
+        L.fvc_kernel_time.argtypes = [vp, ctypes.POINTER(d), ctypes.POINTER(i64), ctypes.POINTER(i64), i32]
+        L.fvc_copy_probe.argtypes = [vp, i64]
         _LIB = L
     return _LIB
 
@@ -179,6 +185,19 @@ class Context:
         lib().fvc_last_step_count(self.h, ctypes.byref(n))
         return n.value
 
+    def set_timing(self, enable=True):
+        _chk(lib().fvc_set_timing(self.h, 1 if enable else 0), "fvc_set_timing", self.h)
+
+    def kernel_time(self, reset=False):
+        """(milliseconds, launches, particle-steps) of timed integration kernels."""
+        ms, nl, ns = ctypes.c_double(), ctypes.c_int64(), ctypes.c_int64()
+        _chk(lib().fvc_kernel_time(self.h, ctypes.byref(ms), ctypes.byref(nl), ctypes.byref(ns), 1 if reset else 0),
+             "fvc_kernel_time", self.h)
+        return ms.value, nl.value, ns.value
+
+    def copy_probe(self, nbytes):
+        _chk(lib().fvc_copy_probe(self.h, nbytes), "fvc_copy_probe", self.h)
+
 
 class Particles:
     """Device-resident LagrangianPoint state."""
@@ -210,6 +229,17 @@ class Particles:
         out = np.zeros(n, POINT_DTYPE)
         _chk(lib().fvc_particles_get(self.h, begin, _p(out), n), "fvc_particles_get", self.ctx.h)
         return out
+
+    def copy_from(self, src, n=None, begin=0, src_begin=0):
+        n = src.capacity - src_begin if n is None else n
+        _chk(lib().fvc_particles_copy(self.h, begin, src.h, src_begin, n), "fvc_particles_copy", self.ctx.h)
+
+    def get_xyz(self, n=None, begin=0):
+        n = self.capacity - begin if n is None else n
+        xyz = np.empty((n, 3), np.float64)
+        left = np.empty(n, np.int32)
+        _chk(lib().fvc_particles_get_xyz(self.h, begin, _p(xyz), _p(left), n), "fvc_particles_get_xyz", self.ctx.h)
+        return xyz, left
 
     def launch_velocity(self, t, lo, hi, tmin, tmax, begin=0, count=None):
         count = self.capacity - begin if count is None else count
@@ -243,6 +273,9 @@ class FtleSlide:
         _chk(lib().fvc_ftle_frame(self.h, direction, lo, hi, tmin, tmax, ctypes.byref(ip), ctypes.byref(done)),
              "fvc_ftle_frame", self.ctx.h)
         return done.value
+
+    def reset(self):
+        _chk(lib().fvc_ftle_reset(self.h), "fvc_ftle_reset", self.ctx.h)
 
     def output(self, int_t):
         wt = np.empty(self.n, np.float64)

@@ -130,6 +130,12 @@ void fvc_particles_destroy(fvc_particles *p);
 int fvc_particles_set(fvc_particles *p, int64_t begin, const fvc_point *recs, int64_t n);
 int fvc_particles_get(fvc_particles *p, int64_t begin, fvc_point *recs, int64_t n);
 int64_t fvc_particles_capacity(const fvc_particles *p);
+/* device-to-device copy of n records (a release re-launched from resident
+ * initial conditions, as CreateNewTraceLaunch copies the .IC file,
+ * ompVC/tracers.c:1579-1632) */
+int fvc_particles_copy(fvc_particles *dst, int64_t dst_begin, const fvc_particles *src, int64_t src_begin, int64_t n);
+/* positions only, n x 3 doubles (output frames, ompVC/tracers.c:243-252) */
+int fvc_particles_get_xyz(fvc_particles *p, int64_t begin, double *xyz, int32_t *left, int64_t n);
 
 /* GetVelocity at launch for points with !LeftDomain (ompVC/tracers.c:202-215,
  * ftle.c:80-93): sets V and, as the reference's side effects, LeftDomain and
@@ -176,6 +182,19 @@ int fvc_ftle_frame(fvc_ftle *f, int32_t direction, int32_t lo, int32_t hi, doubl
  * int_t, then FTLEwT / FTLEwoT in k,j,i order (i fastest) as OutputFTLE writes. */
 int fvc_ftle_output(fvc_ftle *f, double int_t, double *ftle_wt, double *ftle_wot);
 int fvc_ftle_points(fvc_ftle *f, int64_t *n, int64_t *inside);
+/* Restore the slide to its state right after fvc_ftle_create (unlaunched,
+ * located grid, masks), from a device-resident copy. */
+int fvc_ftle_reset(fvc_ftle *f);
+
+/* ---- measurement ------------------------------------------------------------ */
+/* When enabled, every launch of the integration kernels (fvc_advect's and the
+ * FTLE frame's advection kernel) is bracketed by HIP events on the context's
+ * stream; fvc_kernel_time returns the summed milliseconds, launch count and
+ * particle-steps since the last reset. */
+int fvc_set_timing(fvc_ctx *ctx, int32_t enable);
+int fvc_kernel_time(fvc_ctx *ctx, double *ms, int64_t *launches, int64_t *particle_steps, int32_t reset);
+/* A plain device copy of `bytes` bytes (HBM traffic calibration for counters). */
+int fvc_copy_probe(fvc_ctx *ctx, int64_t bytes);
 
 /* ---- Z-order seed sort (cudaVC/ZOrderSort/main.cpp:86-101 ordering) ------- */
 /* Sort n points by the interleaved-bit (x most significant) order of their

@@ -96,13 +96,15 @@ int oc_local_search(const oc_mesh *m, const double X[3], int guess, int *err);
 int oc_global_search(const oc_mesh *m, const double X[3]);
 int oc_velocity(const oc_mesh *m, double tq, oc_point *p, double out[3]); /* returns error code */
 double oc_advect(const oc_mesh *m, const oc_int *ip, oc_point *p, double t1, double t2, int *err);
+double oc_advect_counted(const oc_mesh *m, const oc_int *ip, oc_point *p, double t1, double t2, int *err,
+                         int64_t *steps);
 double oc_max_eig_sym3(const double a[3][3], int *err);
 
 /* batches (OpenMP over points; nthreads <= 0 leaves the runtime default) */
 int oc_locate_local_batch(const oc_mesh *m, int64_t n, const double *xyz, const int32_t *guess, int32_t *out);
 int oc_locate_global_batch(const oc_mesh *m, int64_t n, const double *xyz, int32_t *out, int nthreads);
 int oc_advect_batch(const oc_mesh *m, const oc_int *ip, int64_t n, oc_point *pts, double t1, double t2,
-                    int nthreads, int64_t chunk);
+                    int nthreads, int64_t chunk, int64_t *steps);
 int oc_velocity_batch(const oc_mesh *m, double tq, int64_t n, oc_point *pts, double *uvw);
 int oc_eig_batch(int64_t n, const double *mats, double *lam);
 

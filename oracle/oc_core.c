@@ -358,7 +358,7 @@ int oc_velocity(const oc_mesh *m, double tq, oc_point *p, double out[3]) {
 static inline double sign_of(double a, double b) { return b >= 0.0 ? fabs(a) : -fabs(a); }
 
 /* integration.c:222-311 (tracer side computations omitted: not on the path) */
-static double euler(const oc_mesh *m, oc_point *p, double t0, double t1, double dt, int extrap, int *err) {
+static double euler(const oc_mesh *m, oc_point *p, double t0, double t1, double dt, int extrap, int *err, int64_t *ns) {
     if (p->LeftDomain) {
         *err = OC_E_LEFT;
         return t1;
@@ -379,6 +379,7 @@ static double euler(const oc_mesh *m, oc_point *p, double t0, double t1, double 
         p->X[1] = p->X[1] + h * ve[1];
         p->X[2] = p->X[2] + h * ve[2];
         tc = tc + h;
+        if (fabs(h) > 1e-12 * fabs(dt)) (*ns)++;
         if (m->mesh_type == 1) {
             /* a walk that already left yields -1 without an out-of-range read */
             p->ElementIndex = p->ElementIndex < 0 ? -1 : oc_local_search(m, p->X, p->ElementIndex, err);
@@ -404,7 +405,7 @@ static double euler(const oc_mesh *m, oc_point *p, double t0, double t1, double 
 }
 
 /* integration.c:317-462 */
-static double rk4(const oc_mesh *m, oc_point *p, double t0, double t1, double dt, int extrap, int *err) {
+static double rk4(const oc_mesh *m, oc_point *p, double t0, double t1, double dt, int extrap, int *err, int64_t *ns) {
     oc_point mp[4];
     if (p->LeftDomain) {
         *err = OC_E_LEFT;
@@ -441,6 +442,7 @@ static double rk4(const oc_mesh *m, oc_point *p, double t0, double t1, double dt
         if ((rc = oc_velocity(m, tc + h, &mp[3], k4))) goto fail;
         for (int c = 0; c < 3; c++) p->X[c] = p->X[c] + h * (k1[c] + 2 * k2[c] + 2 * k3[c] + k4[c]) / 6;
         tc = tc + h;
+        if (fabs(h) > 1e-12 * fabs(dt)) (*ns)++;
         if (m->mesh_type == 1) {
             p->ElementIndex = oc_local_search(m, p->X, p->ElementIndex, err);
             if (*err) return t1;
@@ -464,13 +466,21 @@ fail:
 
 /* integration.c:22-45 (inertial particles and RKF are outside this path) */
 double oc_advect(const oc_mesh *m, const oc_int *ip, oc_point *p, double t1, double t2, int *err) {
+    int64_t ns = 0;
+    return oc_advect_counted(m, ip, p, t1, t2, err, &ns);
+}
+
+/* particle-steps counted as the GPU path counts them: every executed step,
+ * the exit step included, floating-point micro-steps (|h| <= 1e-12 dt) not */
+double oc_advect_counted(const oc_mesh *m, const oc_int *ip, oc_point *p, double t1, double t2, int *err,
+                         int64_t *ns) {
     *err = OC_OK;
     if (p->LeftDomain && ip->extrapolate) {
         for (int c = 0; c < 3; c++) p->X[c] = p->X[c] + p->V[c] * (t2 - t1);
         return t2;
     }
-    double stop = ip->int_type == 1 ? rk4(m, p, t1, t2, ip->time_step, ip->extrapolate, err)
-                                    : euler(m, p, t1, t2, ip->time_step, ip->extrapolate, err);
+    double stop = ip->int_type == 1 ? rk4(m, p, t1, t2, ip->time_step, ip->extrapolate, err, ns)
+                                    : euler(m, p, t1, t2, ip->time_step, ip->extrapolate, err, ns);
     if ((fabs(stop - t2) > TINY) && ip->extrapolate)
         for (int c = 0; c < 3; c++) p->X[c] = p->X[c] + p->V[c] * (t2 - stop);
     return stop;
@@ -535,18 +545,21 @@ int oc_locate_global_batch(const oc_mesh *m, int64_t n, const double *xyz, int32
 
 /* LOWERLOOP tracer integration (tracers.c:217-229): dynamic chunks over points */
 int oc_advect_batch(const oc_mesh *m, const oc_int *ip, int64_t n, oc_point *pts, double t1, double t2,
-                    int nthreads, int64_t chunk) {
+                    int nthreads, int64_t chunk, int64_t *steps) {
     int rc = OC_OK;
+    int64_t total = 0;
     if (chunk <= 0) chunk = 64;
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
-#pragma omp parallel for schedule(dynamic, chunk)
+#pragma omp parallel for schedule(dynamic, chunk) reduction(+ : total)
 #endif
     for (int64_t q = 0; q < n; q++) {
         oc_point *p = &pts[q];
         if (p->LeftDomainTime > -TINY && (!p->LeftDomain || ip->extrapolate)) {
             int err = OC_OK;
-            oc_advect(m, ip, p, t1, t2, &err);
+            int64_t ns = 0;
+            oc_advect_counted(m, ip, p, t1, t2, &err, &ns);
+            total += ns;
             if (err) {
 #ifdef _OPENMP
 #pragma omp critical
@@ -556,6 +569,7 @@ int oc_advect_batch(const oc_mesh *m, const oc_int *ip, int64_t n, oc_point *pts
         }
     }
     (void)nthreads;
+    if (steps) *steps = total;
     return rc;
 }
 

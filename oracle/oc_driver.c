@@ -879,7 +879,7 @@ int oc_run_deck(const char *deck_path, int nproc, char *msg, int msglen) {
                                     if ((rc = oc_velocity(m, r->start, &r->pts[q], r->pts[q].V))) goto done;
                                 }
                         }
-                        if ((rc = oc_advect_batch(m, &ip, n, r->pts, t1, t2, 1, 64))) goto done;
+                        if ((rc = oc_advect_batch(m, &ip, n, r->pts, t1, t2, 1, 64, NULL))) goto done;
                         if (fabs(nextout - t2) < TINY) {
                             for (int64_t q = 0; q < n; q++) {
                                 const oc_point *p = &r->pts[q];

@@ -44,7 +44,7 @@ def lib():
         L.oc_mesh_set_interval.argtypes = [vp, d, d]
         L.oc_locate_local_batch.argtypes = [vp, i64, vp, vp, vp]
         L.oc_locate_global_batch.argtypes = [vp, i64, vp, vp, ctypes.c_int]
-        L.oc_advect_batch.argtypes = [vp, ctypes.POINTER(OcInt), i64, vp, d, d, ctypes.c_int, i64]
+        L.oc_advect_batch.argtypes = [vp, ctypes.POINTER(OcInt), i64, vp, d, d, ctypes.c_int, i64, ctypes.POINTER(i64)]
         L.oc_velocity_batch.argtypes = [vp, d, i64, vp, vp]
         L.oc_eig_batch.argtypes = [i64, vp, vp]
         L.oc_seed_list.argtypes = [vp, i64, vp, ctypes.c_int, vp]
@@ -102,7 +102,9 @@ class Mesh:
         """Advect records in place (the LOWERLOOP filter applies); returns the error code."""
         assert pts.dtype == POINT_DTYPE and pts.flags.c_contiguous
         ip = OcInt(int_type, dt, extrapolate)
-        return lib().oc_advect_batch(self.h, ctypes.byref(ip), pts.shape[0], _p(pts), t1, t2, nthreads, chunk)
+        self.last_steps = ctypes.c_int64(0)
+        return lib().oc_advect_batch(self.h, ctypes.byref(ip), pts.shape[0], _p(pts), t1, t2, nthreads, chunk,
+                                     ctypes.byref(self.last_steps))
 
     def velocity(self, pts, tq):
         out = np.zeros((pts.shape[0], 3))
