@@ -523,7 +523,7 @@ extern "C" int fvc_particles_create(fvc_ctx *c, int64_t cap, fvc_particles **out
     fvc_particles *p = new fvc_particles();
     p->ctx = c;
     p->cap = cap;
-    size_t bytes = (size_t)cap * (8 * 8 + 3 * 4) + 256;
+    size_t bytes = (size_t)cap * (14 * 8 + 4 * 4) + 256;
     if (hipMalloc(&p->buf, bytes) != hipSuccess) {
         delete p;
         return set_err(c, FVC_E_NOMEM, "particle alloc");
@@ -534,10 +534,12 @@ extern "C" int fvc_particles_create(fvc_ctx *c, int64_t cap, fvc_particles **out
     for (int a = 0; a < 3; a++) p->V[a] = d + (3 + a) * cap;
     p->left_time = d + 6 * cap;
     p->scalar = d + 7 * cap;
-    int *iv = (int *)(d + 8 * cap);
+    for (int a = 0; a < 6; a++) p->H[a] = d + (8 + a) * cap;
+    int *iv = (int *)(d + 14 * cap);
     p->elem = iv;
     p->aux = iv + cap;
     p->left = iv + 2 * cap;
+    p->nhist = iv + 3 * cap;
     hipMemsetAsync(p->buf, 0, bytes, c->stream);
     hipStreamSynchronize(c->stream);
     *out = p;
@@ -575,6 +577,8 @@ extern "C" int fvc_particles_set(fvc_particles *p, int64_t begin, const fvc_poin
     int *idst[3] = {p->elem, p->aux, p->left};
     for (int a = 0; a < 3; a++)
         FVC_HIP(hipMemcpyAsync(idst[a] + begin, iv.data() + a * n, 4 * n, hipMemcpyHostToDevice, c->stream));
+    /* new states start without multistep history */
+    FVC_HIP(hipMemsetAsync(p->nhist + begin, 0, 4 * n, c->stream));
     FVC_HIP(hipStreamSynchronize(c->stream));
     return FVC_OK;
 }
@@ -612,10 +616,12 @@ static int particles_d2d(fvc_particles *dst, int64_t db, const fvc_particles *sr
     double *ss[8] = {src->X[0], src->X[1], src->X[2], src->V[0], src->V[1], src->V[2], src->left_time, src->scalar};
     for (int a = 0; a < 8; a++)
         FVC_HIP(hipMemcpyAsync(ds[a] + db, ss[a] + sb, 8 * n, hipMemcpyDeviceToDevice, c->stream));
-    int *di[3] = {dst->elem, dst->aux, dst->left};
-    int *si[3] = {src->elem, src->aux, src->left};
-    for (int a = 0; a < 3; a++)
+    int *di[4] = {dst->elem, dst->aux, dst->left, dst->nhist};
+    int *si[4] = {src->elem, src->aux, src->left, src->nhist};
+    for (int a = 0; a < 4; a++)
         FVC_HIP(hipMemcpyAsync(di[a] + db, si[a] + sb, 4 * n, hipMemcpyDeviceToDevice, c->stream));
+    for (int a = 0; a < 6; a++)
+        FVC_HIP(hipMemcpyAsync(dst->H[a] + db, src->H[a] + sb, 8 * n, hipMemcpyDeviceToDevice, c->stream));
     FVC_HIP(hipStreamSynchronize(c->stream));
     return FVC_OK;
 }
@@ -654,7 +660,8 @@ extern "C" int fvc_launch_velocity(fvc_ctx *c, fvc_particles *p, int64_t begin, 
 extern "C" int fvc_advect(fvc_ctx *c, fvc_particles *p, int64_t begin, int64_t count, double t1, double t2, int32_t lo,
                           int32_t hi, double tmin, double tmax, const fvc_int_params *ip) {
     if (!c || !p || !ip || begin < 0 || count < 0 || begin + count > p->cap) return FVC_E_INVALID;
-    if (ip->int_type != FVC_INT_RK4 && ip->int_type != FVC_INT_EULER)
+    if (ip->int_type != FVC_INT_RK4 && ip->int_type != FVC_INT_EULER && ip->int_type != FVC_INT_AB2 &&
+        ip->int_type != FVC_INT_AB3)
         return set_err(c, FVC_E_INVALID, "integrator not available in fvc_advect");
     if (!(ip->time_step > 1.0e-10)) return set_err(c, FVC_E_INVALID, "Int_TimeStep must be positive");
     int rc = check_slots(c, lo, hi, tmin, tmax);

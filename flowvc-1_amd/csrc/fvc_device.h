@@ -457,6 +457,9 @@ struct PState {
     int elem;
     int left;
     double left_time;
+    /* multistep history: velocities at the two previous nodes, count held */
+    double F1[3], F2[3];
+    int nhist;
 };
 
 __device__ __forceinline__ double sign_of(double a, double b) { return b >= 0.0 ? fabs(a) : -fabs(a); }
@@ -640,6 +643,102 @@ __device__ double euler(const DevMesh &m, const DevField &f, PState &p, double t
     return t1;
 }
 
+/* Adams-Bashforth of order 2 or 3 (no ompVC counterpart; cudaVC's "AB2",
+ * cudaVC/src/multistep.cu:249-304, is not followed).  Textbook fixed-step
+ *   AB2: x+ = x + h (3 f0 - f1) / 2
+ *   AB3: x+ = x + h (23 f0 - 16 f1 + 5 f2) / 12
+ * with f0 = u(x, t).  The history (f1, f2) persists across calls; steps
+ * without enough history, or clipped to the interval end by more than 1e-9
+ * of h, are RK4 steps (their first stage is f0), and a clipped step restarts
+ * the history.  Steps shorter than 1e-12 dt (floating-point remainders) are
+ * Euler steps that keep the history.  Exit handling as RK4. */
+template <int MODE, int DIM, int UNSTRUCT>
+__device__ double ab(const DevMesh &m, const DevField &f, PState &p, double t0, double t1, double dt, int order,
+                     int extrap, int &err, uint32_t &nsteps) {
+#pragma clang fp contract(off)
+    if (p.left) {
+        err = E_LEFT;
+        return t1;
+    }
+    if (UNSTRUCT && (p.elem < 0 || p.elem >= m.ne)) {
+        err = E_ELEM;
+        return t1;
+    }
+    Probe mp[4];
+    for (int q = 0; q < 4; q++) {
+        mp[q].elem = p.elem;
+        mp[q].left = 0;
+    }
+    const double hfull = sign_of(dt, t1 - t0);
+    const double hmin = 1e-12 * fabs(dt);
+    double tc = t0, h = hfull;
+    while ((tc - t1) * (t1 - t0) < 0.0) {
+        if ((tc + h - t1) * (tc + h - t0) > 0.0) h = sign_of(t1 - tc, t1 - t0);
+        const bool micro = fabs(h) <= hmin;
+        const bool full = fabs(h - hfull) <= 1e-9 * fabs(hfull); /* accumulation remainders count as full */
+        double f0[3], inc[3];
+        stage_velocity<MODE, DIM, UNSTRUCT>(m, f, tc, p.X, mp[0], f0, err);
+        if (err) return t1;
+        if (micro) {
+            for (int c = 0; c < 3; c++) inc[c] = h * f0[c];
+        } else if (full && p.nhist >= order - 1) {
+            if (order == 2)
+                for (int c = 0; c < 3; c++) inc[c] = h * (3 * f0[c] - p.F1[c]) / 2;
+            else
+                for (int c = 0; c < 3; c++) inc[c] = h * (23 * f0[c] - 16 * p.F1[c] + 5 * p.F2[c]) / 12;
+        } else {
+            double k2[3], k3[3], k4[3], Y[3];
+            for (int c = 0; c < 3; c++) Y[c] = p.X[c] + 0.5 * f0[c] * h;
+            stage_velocity<MODE, DIM, UNSTRUCT>(m, f, tc + h / 2, Y, mp[1], k2, err);
+            if (err) return t1;
+            for (int c = 0; c < 3; c++) Y[c] = p.X[c] + 0.5 * k2[c] * h;
+            stage_velocity<MODE, DIM, UNSTRUCT>(m, f, tc + h / 2, Y, mp[2], k3, err);
+            if (err) return t1;
+            for (int c = 0; c < 3; c++) Y[c] = p.X[c] + k3[c] * h;
+            stage_velocity<MODE, DIM, UNSTRUCT>(m, f, tc + h, Y, mp[3], k4, err);
+            if (err) return t1;
+            for (int c = 0; c < 3; c++) inc[c] = h * (f0[c] + 2 * k2[c] + 2 * k3[c] + k4[c]) / 6;
+        }
+        for (int c = 0; c < 3; c++) p.X[c] = p.X[c] + inc[c];
+        tc = tc + h;
+        if (!micro) nsteps++;
+        bool out;
+        if (UNSTRUCT) {
+            double r, s, t;
+            int e = walk<MODE, DIM>(m, p.X[0], p.X[1], p.X[2], p.elem, ACC_SEARCH, r, s, t, err);
+            if (e == -2) return t1;
+            p.elem = e;
+            out = e < 0;
+        } else {
+            out = outside_domain(m, p.X[0], p.X[1], p.X[2]);
+        }
+        if (out || mp[0].left || mp[1].left || mp[2].left || mp[3].left) {
+            p.left = 1;
+            p.left_time = tc - h;
+            for (int c = 0; c < 3; c++) p.X[c] = p.X[c] - inc[c];
+            return p.left_time;
+        }
+        if (!micro) {
+            if (full) {
+                for (int c = 0; c < 3; c++) {
+                    p.F2[c] = p.F1[c];
+                    p.F1[c] = f0[c];
+                }
+                p.nhist = p.nhist < 2 ? p.nhist + 1 : 2;
+            } else {
+                p.nhist = 0;
+            }
+        }
+        /* the next step's first probe starts from the located element */
+        if (UNSTRUCT) mp[0].elem = p.elem;
+        if (extrap) {
+            get_velocity<MODE, DIM>(m, f, tc, p, p.V, err);
+            if (err) return t1;
+        }
+    }
+    return t1;
+}
+
 /* Advect (ompVC/integration.c:22-45) */
 template <int MODE, int DIM, int UNSTRUCT>
 __device__ __forceinline__ double advect(const DevMesh &m, const DevField &f, PState &p, double t1, double t2, int itype,
@@ -649,8 +748,13 @@ __device__ __forceinline__ double advect(const DevMesh &m, const DevField &f, PS
         for (int c = 0; c < 3; c++) p.X[c] = p.X[c] + p.V[c] * (t2 - t1);
         return t2;
     }
-    double stop = itype == 1 ? rk4<MODE, DIM, UNSTRUCT>(m, f, p, t1, t2, dt, extrap, err, nsteps)
-                             : euler<MODE, DIM, UNSTRUCT>(m, f, p, t1, t2, dt, extrap, err, nsteps);
+    double stop;
+    if (itype == 1)
+        stop = rk4<MODE, DIM, UNSTRUCT>(m, f, p, t1, t2, dt, extrap, err, nsteps);
+    else if (itype == 3 || itype == 4)
+        stop = ab<MODE, DIM, UNSTRUCT>(m, f, p, t1, t2, dt, itype == 3 ? 2 : 3, extrap, err, nsteps);
+    else
+        stop = euler<MODE, DIM, UNSTRUCT>(m, f, p, t1, t2, dt, extrap, err, nsteps);
     if ((fabs(stop - t2) > TINY) && extrap)
         for (int c = 0; c < 3; c++) p.X[c] = p.X[c] + p.V[c] * (t2 - stop);
     return stop;

@@ -55,6 +55,8 @@ struct fvc_particles {
     double *V[3];
     int *elem, *aux, *left;
     double *left_time, *scalar;
+    double *H[6]; /* multistep history F1 (x,y,z), F2 (x,y,z) */
+    int *nhist;
 };
 
 struct fvc_ftle {
@@ -93,6 +95,8 @@ struct PtrSet {
     int *elem;
     int *left;
     double *left_time;
+    double *H[6];
+    int *nhist;
 };
 
 inline PtrSet ptrs(fvc_particles *p) {
@@ -101,9 +105,11 @@ inline PtrSet ptrs(fvc_particles *p) {
         s.X[c] = p->X[c];
         s.V[c] = p->V[c];
     }
+    for (int c = 0; c < 6; c++) s.H[c] = p->H[c];
     s.elem = p->elem;
     s.left = p->left;
     s.left_time = p->left_time;
+    s.nhist = p->nhist;
     return s;
 }
 

@@ -44,7 +44,7 @@ __device__ inline void add_steps(unsigned long long *ctr, uint32_t ns) {
     }
 }
 
-__device__ inline void load_state(const PtrSet &p, int64_t i, PState &s) {
+__device__ inline void load_state(const PtrSet &p, int64_t i, PState &s, bool hist = false) {
     for (int c = 0; c < 3; c++) {
         s.X[c] = p.X[c][i];
         s.V[c] = p.V[c][i];
@@ -52,9 +52,17 @@ __device__ inline void load_state(const PtrSet &p, int64_t i, PState &s) {
     s.elem = p.elem[i];
     s.left = p.left[i];
     s.left_time = p.left_time[i];
+    s.nhist = 0;
+    if (hist) {
+        for (int c = 0; c < 3; c++) {
+            s.F1[c] = p.H[c][i];
+            s.F2[c] = p.H[3 + c][i];
+        }
+        s.nhist = p.nhist[i];
+    }
 }
 
-__device__ inline void store_state(const PtrSet &p, int64_t i, const PState &s) {
+__device__ inline void store_state(const PtrSet &p, int64_t i, const PState &s, bool hist = false) {
     for (int c = 0; c < 3; c++) {
         p.X[c][i] = s.X[c];
         p.V[c][i] = s.V[c];
@@ -62,6 +70,13 @@ __device__ inline void store_state(const PtrSet &p, int64_t i, const PState &s) 
     p.elem[i] = s.elem;
     p.left[i] = s.left;
     p.left_time[i] = s.left_time;
+    if (hist) {
+        for (int c = 0; c < 3; c++) {
+            p.H[c][i] = s.F1[c];
+            p.H[3 + c][i] = s.F2[c];
+        }
+        p.nhist[i] = s.nhist;
+    }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -78,14 +93,15 @@ __global__ __launch_bounds__(BLOCK) void k_advect(DevMesh m, DevField f, PtrSet 
         double lt = p.left_time[i];
         int lf = p.left[i];
         if (lt > -TINY && (!lf || extrap)) {
+            const bool hist = itype >= 3;
             PState s;
-            load_state(p, i, s);
+            load_state(p, i, s, hist);
             int err = 0;
             advect<MODE, DIM, UNS>(m, f, s, t1, t2, itype, dt, extrap, err, ns);
             if (err)
                 raise_err(eb, err, i);
             else
-                store_state(p, i, s);
+                store_state(p, i, s, hist);
         }
     }
     add_steps(steps, ns);
@@ -169,7 +185,7 @@ int launch_advect(fvc_ctx *c, PtrSet p, const int *index, int64_t begin, int64_t
         timing_begin(c);
         FVC_DISPATCH(c->precision, c->mesh.dims, c->mesh.mesh_type, k_advect,
                      <<<grid_for(count, block), block, 0, c->stream>>>(c->mesh, f, p, index, begin, count, t1, t2,
-                                                                       ip.int_type == FVC_INT_RK4 ? 1 : 0, ip.time_step,
+                                                                       ip.int_type, ip.time_step,
                                                                        ip.extrapolate, errbuf(c), c->d_steps));
         timing_end(c);
         FVC_HIP(hipGetLastError());

@@ -100,6 +100,17 @@ double oc_advect_counted(const oc_mesh *m, const oc_int *ip, oc_point *p, double
                          int64_t *steps);
 double oc_max_eig_sym3(const double a[3][3], int *err);
 
+/* Adams-Bashforth 2/3 with history carried across calls (no reference counterpart). */
+typedef struct oc_hist {
+    double F1[3], F2[3];
+    int32_t nhist;
+    int32_t pad_;
+} oc_hist;
+double oc_ab_counted(const oc_mesh *m, oc_point *p, oc_hist *h, double t0, double t1, double dt, int order, int *err,
+                     int64_t *ns);
+int oc_ab_batch(const oc_mesh *m, int64_t n, oc_point *pts, oc_hist *hist, double t1, double t2, double dt, int order,
+                int nthreads, int64_t *steps);
+
 /* batches (OpenMP over points; nthreads <= 0 leaves the runtime default) */
 int oc_locate_local_batch(const oc_mesh *m, int64_t n, const double *xyz, const int32_t *guess, int32_t *out);
 int oc_locate_global_batch(const oc_mesh *m, int64_t n, const double *xyz, int32_t *out, int nthreads);

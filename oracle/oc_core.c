@@ -464,6 +464,112 @@ fail:
     return t1;
 }
 
+/* Adams-Bashforth 2/3 (no ompVC counterpart; this project's definition, the
+ * same as the GPU path's): fixed step x+ = x + h (3 f0 - f1)/2 (AB2) or
+ * x+ = x + h (23 f0 - 16 f1 + 5 f2)/12 (AB3), RK4 steps while history is short
+ * or when the step is clipped (history restarts), Euler for remainders below
+ * 1e-12 dt (history kept); exits as RK4. */
+double oc_ab_counted(const oc_mesh *m, oc_point *p, oc_hist *hs, double t0, double t1, double dt, int order, int *err,
+                     int64_t *ns) {
+    oc_point mp[4];
+    *err = OC_OK;
+    if (p->LeftDomain) {
+        *err = OC_E_LEFT;
+        return t1;
+    }
+    memset(mp, 0, sizeof mp);
+    if (m->mesh_type == 1) {
+        if (p->ElementIndex < 0) {
+            *err = OC_E_ELEM_NEG1;
+            return t1;
+        }
+        for (int q = 0; q < 4; q++) mp[q].ElementIndex = p->ElementIndex;
+    }
+    const double hfull = sign_of(dt, t1 - t0), hmin = 1e-12 * fabs(dt);
+    double tc = t0, h = hfull;
+    int rc;
+    while ((tc - t1) * (t1 - t0) < 0.0) {
+        if ((tc + h - t1) * (tc + h - t0) > 0.0) h = sign_of(t1 - tc, t1 - t0);
+        int micro = fabs(h) <= hmin, full = fabs(h - hfull) <= 1e-9 * fabs(hfull);
+        double f0[3], inc[3];
+        memcpy(mp[0].X, p->X, sizeof p->X);
+        if ((rc = oc_velocity(m, tc, &mp[0], f0))) goto fail;
+        if (micro) {
+            for (int c = 0; c < 3; c++) inc[c] = h * f0[c];
+        } else if (full && hs->nhist >= order - 1) {
+            if (order == 2)
+                for (int c = 0; c < 3; c++) inc[c] = h * (3 * f0[c] - hs->F1[c]) / 2;
+            else
+                for (int c = 0; c < 3; c++) inc[c] = h * (23 * f0[c] - 16 * hs->F1[c] + 5 * hs->F2[c]) / 12;
+        } else {
+            double k2[3], k3[3], k4[3];
+            for (int c = 0; c < 3; c++) mp[1].X[c] = p->X[c] + 0.5 * f0[c] * h;
+            if ((rc = oc_velocity(m, tc + h / 2, &mp[1], k2))) goto fail;
+            for (int c = 0; c < 3; c++) mp[2].X[c] = p->X[c] + 0.5 * k2[c] * h;
+            if ((rc = oc_velocity(m, tc + h / 2, &mp[2], k3))) goto fail;
+            for (int c = 0; c < 3; c++) mp[3].X[c] = p->X[c] + k3[c] * h;
+            if ((rc = oc_velocity(m, tc + h, &mp[3], k4))) goto fail;
+            for (int c = 0; c < 3; c++) inc[c] = h * (f0[c] + 2 * k2[c] + 2 * k3[c] + k4[c]) / 6;
+        }
+        for (int c = 0; c < 3; c++) p->X[c] = p->X[c] + inc[c];
+        tc = tc + h;
+        if (!micro) (*ns)++;
+        if (m->mesh_type == 1) {
+            p->ElementIndex = oc_local_search(m, p->X, p->ElementIndex, err);
+            if (*err) return t1;
+        }
+        if ((m->mesh_type == 0 && oc_test_outside_domain(m, p->X)) || (m->mesh_type == 1 && p->ElementIndex < 0) ||
+            mp[0].LeftDomain || mp[1].LeftDomain || mp[2].LeftDomain || mp[3].LeftDomain) {
+            p->LeftDomain = 1;
+            p->LeftDomainTime = tc - h;
+            for (int c = 0; c < 3; c++) p->X[c] = p->X[c] - inc[c];
+            return p->LeftDomainTime;
+        }
+        if (!micro) {
+            if (full) {
+                memcpy(hs->F2, hs->F1, sizeof hs->F1);
+                memcpy(hs->F1, f0, sizeof f0);
+                hs->nhist = hs->nhist < 2 ? hs->nhist + 1 : 2;
+            } else {
+                hs->nhist = 0;
+            }
+        }
+        if (m->mesh_type == 1) mp[0].ElementIndex = p->ElementIndex;
+    }
+    return t1;
+fail:
+    *err = rc;
+    return t1;
+}
+
+int oc_ab_batch(const oc_mesh *m, int64_t n, oc_point *pts, oc_hist *hist, double t1, double t2, double dt, int order,
+                int nthreads, int64_t *steps) {
+    int rc = OC_OK;
+    int64_t total = 0;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : total)
+#endif
+    for (int64_t q = 0; q < n; q++) {
+        oc_point *p = &pts[q];
+        if (p->LeftDomainTime > -TINY && !p->LeftDomain) {
+            int err = OC_OK;
+            int64_t ns = 0;
+            oc_ab_counted(m, p, &hist[q], t1, t2, dt, order, &err, &ns);
+            total += ns;
+            if (err) {
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+                if (!rc) rc = err;
+            }
+        }
+    }
+    (void)nthreads;
+    if (steps) *steps = total;
+    return rc;
+}
+
 /* integration.c:22-45 (inertial particles and RKF are outside this path) */
 double oc_advect(const oc_mesh *m, const oc_int *ip, oc_point *p, double t1, double t2, int *err) {
     int64_t ns = 0;

@@ -123,3 +123,25 @@ def max_eig(mats):
     out = np.empty(mats.shape[0])
     lib().oc_eig_batch(mats.shape[0], _p(mats), _p(out))
     return out
+
+
+HIST_DTYPE = np.dtype([("F1", "<f8", 3), ("F2", "<f8", 3), ("nhist", "<i4"), ("pad", "<i4")])
+
+
+def _ab_lib():
+    L = lib()
+    if not getattr(L, "_ab_ready", False):
+        L.oc_ab_batch.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                                  ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                  ctypes.POINTER(ctypes.c_int64)]
+        L._ab_ready = True
+    return L
+
+
+def ab_advect(mesh, pts, hist, t1, t2, dt, order=3, nthreads=1):
+    """Adams-Bashforth (order 2/3) over [t1, t2]; pts and hist updated in place."""
+    assert pts.dtype == POINT_DTYPE and hist.dtype == HIST_DTYPE
+    steps = ctypes.c_int64(0)
+    rc = _ab_lib().oc_ab_batch(mesh.h, pts.shape[0], _p(pts), _p(hist), t1, t2, dt, order, nthreads,
+                               ctypes.byref(steps))
+    return rc, steps.value
