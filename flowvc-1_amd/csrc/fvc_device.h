@@ -531,216 +531,148 @@ __device__ __forceinline__ void stage_velocity(const DevMesh &m, const DevField 
     }
 }
 
-/* RK4 (ompVC/integration.c:317-462).  Returns the stop time; counts steps. */
-template <int MODE, int DIM, int UNSTRUCT>
-__device__ double rk4(const DevMesh &m, const DevField &f, PState &p, double t0, double t1, double dt, int extrap,
-                      int &err, uint32_t &nsteps) {
-#pragma clang fp contract(off)
-    if (p.left) {
-        err = E_LEFT;
-        return t1;
-    }
-    if (UNSTRUCT && (p.elem < 0 || p.elem >= m.ne)) {
-        err = E_ELEM;
-        return t1;
-    }
+/* ------------------------------------------------------------------------ */
+/* Resumable integration of one interval [a, b]: one Advect call of the      */
+/* reference.  IState keeps what the reference keeps in locals across its    */
+/* step loop: the current time, the step size (which stays clipped once      */
+/* clipped, integration.c:346-352), and the four RK4 stage probes, each with */
+/* its own walk element and left flag (integration.c:337-341).  With         */
+/* max_steps unlimited this is the reference's loop; with a budget, a        */
+/* persistent kernel can stop and resume it with identical arithmetic.       */
+/* ------------------------------------------------------------------------ */
+struct IState {
+    double tc, h;
     Probe mp[4];
-    for (int q = 0; q < 4; q++) {
-        mp[q].elem = p.elem;
-        mp[q].left = 0;
-    }
-    double tc = t0, h = sign_of(dt, t1 - t0);
-    const double hmin = 1e-12 * fabs(dt);
-    while ((tc - t1) * (t1 - t0) < 0.0) {
-        if ((tc + h - t1) * (tc + h - t0) > 0.0) h = sign_of(t1 - tc, t1 - t0);
-        double k1[3], k2[3], k3[3], k4[3], Y[3];
-        stage_velocity<MODE, DIM, UNSTRUCT>(m, f, tc, p.X, mp[0], k1, err);
-        if (err) return t1;
-        for (int c = 0; c < 3; c++) Y[c] = p.X[c] + 0.5 * k1[c] * h;
-        stage_velocity<MODE, DIM, UNSTRUCT>(m, f, tc + h / 2, Y, mp[1], k2, err);
-        if (err) return t1;
-        for (int c = 0; c < 3; c++) Y[c] = p.X[c] + 0.5 * k2[c] * h;
-        stage_velocity<MODE, DIM, UNSTRUCT>(m, f, tc + h / 2, Y, mp[2], k3, err);
-        if (err) return t1;
-        for (int c = 0; c < 3; c++) Y[c] = p.X[c] + k3[c] * h;
-        stage_velocity<MODE, DIM, UNSTRUCT>(m, f, tc + h, Y, mp[3], k4, err);
-        if (err) return t1;
-        double inc[3];
-        for (int c = 0; c < 3; c++) {
-            inc[c] = h * (k1[c] + 2 * k2[c] + 2 * k3[c] + k4[c]) / 6;
-            p.X[c] = p.X[c] + inc[c];
-        }
-        tc = tc + h;
-        if (fabs(h) > hmin) nsteps++;
-        bool out;
-        if (UNSTRUCT) {
-            double r, s, t;
-            int e = walk<MODE, DIM>(m, p.X[0], p.X[1], p.X[2], p.elem, ACC_SEARCH, r, s, t, err);
-            if (e == -2) return t1;
-            p.elem = e;
-            out = e < 0;
-        } else {
-            out = outside_domain(m, p.X[0], p.X[1], p.X[2]);
-        }
-        if (out || mp[0].left || mp[1].left || mp[2].left || mp[3].left) {
-            p.left = 1;
-            p.left_time = tc - h;
-            for (int c = 0; c < 3; c++) p.X[c] = p.X[c] - inc[c];
-            return p.left_time;
-        }
-        if (extrap) {
-            get_velocity<MODE, DIM>(m, f, tc, p, p.V, err);
-            if (err) return t1;
-        }
-    }
-    return t1;
-}
+    int started;
+};
 
-/* Euler (ompVC/integration.c:222-311) */
-template <int MODE, int DIM, int UNSTRUCT>
-__device__ double euler(const DevMesh &m, const DevField &f, PState &p, double t0, double t1, double dt, int extrap,
-                        int &err, uint32_t &nsteps) {
+enum { INT_EULER = 0, INT_RK4 = 1, INT_AB2 = 3, INT_AB3 = 4 };
+
+/* Returns 1 when the interval is finished (end reached, point exited or an
+ * error), with *stop the stop time; 0 when max_steps were taken first.
+ *   RK4   ompVC/integration.c:317-462
+ *   Euler ompVC/integration.c:222-311 (GetVelocity on the point itself)
+ *   AB2/AB3 (no ompVC counterpart; cudaVC/src/multistep.cu:249-304 is not
+ *   followed): x+ = x + h (3 f0 - f1)/2, x+ = x + h (23 f0 - 16 f1 + 5 f2)/12
+ *   with f0 = u(x, t); RK4 steps while the history is short or the step is
+ *   clipped by more than 1e-9 h (history restarts); steps below 1e-12 dt
+ *   (accumulation remainders) are Euler steps that keep the history. */
+template <int MODE, int DIM, int UNSTRUCT, int GEN>
+__device__ int interval_run(const DevMesh &m, const DevField &f, PState &p, IState &is, double a, double b, double dt,
+                            int itype_rt, int extrap, int max_steps, int &err, uint32_t &nsteps, double &stop) {
 #pragma clang fp contract(off)
-    if (p.left) {
-        err = E_LEFT;
-        return t1;
+    /* GEN = 0 compiles RK4 only (no multistep state in registers) */
+    const int itype = GEN ? itype_rt : INT_RK4;
+    if (!is.started) {
+        if (p.left) {
+            err = E_LEFT;
+            stop = b;
+            return 1;
+        }
+        if (UNSTRUCT && (p.elem < 0 || p.elem >= m.ne)) {
+            err = E_ELEM;
+            stop = b;
+            return 1;
+        }
+        is.tc = a;
+        is.h = sign_of(dt, b - a);
+        for (int q = 0; q < 4; q++) {
+            is.mp[q].elem = p.elem;
+            is.mp[q].left = 0;
+        }
+        is.started = 1;
     }
-    if (UNSTRUCT && (p.elem < 0 || p.elem >= m.ne)) {
-        err = E_ELEM;
-        return t1;
-    }
-    double tc = t0, h = sign_of(dt, t1 - t0);
+    const double hfull = sign_of(dt, b - a);
     const double hmin = 1e-12 * fabs(dt);
-    while ((tc - t1) * (t1 - t0) < 0.0) {
-        if ((tc + h - t1) * (tc + h - t0) > 0.0) h = sign_of(t1 - tc, t1 - t0);
-        double ve[3];
-        get_velocity<MODE, DIM>(m, f, tc, p, ve, err);
-        if (err) return t1;
-        for (int c = 0; c < 3; c++) p.X[c] = p.X[c] + h * ve[c];
-        tc = tc + h;
-        if (fabs(h) > hmin) nsteps++;
+    const int order = itype == INT_AB2 ? 2 : 3;
+    int taken = 0;
+    while ((is.tc - b) * (b - a) < 0.0) {
+        if (taken == max_steps) return 0;
+        if ((is.tc + is.h - b) * (is.tc + is.h - a) > 0.0) is.h = sign_of(b - is.tc, b - a);
+        const double h = is.h, tc = is.tc;
+        const bool micro = fabs(h) <= hmin;
+        double inc[3], f0[3];
+        bool ab_step = false;
+        if (itype == INT_EULER) {
+            double ve[3];
+            get_velocity<MODE, DIM>(m, f, tc, p, ve, err);
+            if (err) break;
+            for (int c = 0; c < 3; c++) inc[c] = h * ve[c];
+        } else {
+            stage_velocity<MODE, DIM, UNSTRUCT>(m, f, tc, p.X, is.mp[0], f0, err);
+            if (err) break;
+            const bool full = fabs(h - hfull) <= 1e-9 * fabs(hfull);
+            if (itype != INT_RK4 && micro) {
+                for (int c = 0; c < 3; c++) inc[c] = h * f0[c];
+            } else if (itype != INT_RK4 && full && p.nhist >= order - 1) {
+                ab_step = true;
+                if (order == 2)
+                    for (int c = 0; c < 3; c++) inc[c] = h * (3 * f0[c] - p.F1[c]) / 2;
+                else
+                    for (int c = 0; c < 3; c++) inc[c] = h * (23 * f0[c] - 16 * p.F1[c] + 5 * p.F2[c]) / 12;
+            } else {
+                double k2[3], k3[3], k4[3], Y[3];
+                for (int c = 0; c < 3; c++) Y[c] = p.X[c] + 0.5 * f0[c] * h;
+                stage_velocity<MODE, DIM, UNSTRUCT>(m, f, tc + h / 2, Y, is.mp[1], k2, err);
+                if (err) break;
+                for (int c = 0; c < 3; c++) Y[c] = p.X[c] + 0.5 * k2[c] * h;
+                stage_velocity<MODE, DIM, UNSTRUCT>(m, f, tc + h / 2, Y, is.mp[2], k3, err);
+                if (err) break;
+                for (int c = 0; c < 3; c++) Y[c] = p.X[c] + k3[c] * h;
+                stage_velocity<MODE, DIM, UNSTRUCT>(m, f, tc + h, Y, is.mp[3], k4, err);
+                if (err) break;
+                for (int c = 0; c < 3; c++) inc[c] = h * (f0[c] + 2 * k2[c] + 2 * k3[c] + k4[c]) / 6;
+            }
+            (void)ab_step;
+        }
+        for (int c = 0; c < 3; c++) p.X[c] = p.X[c] + inc[c];
+        is.tc = tc + h;
+        if (!micro) nsteps++;
+        taken++;
         bool out;
         if (UNSTRUCT) {
             double r, s, t;
             int e = p.elem < 0 ? -1 : walk<MODE, DIM>(m, p.X[0], p.X[1], p.X[2], p.elem, ACC_SEARCH, r, s, t, err);
-            if (e == -2) return t1;
+            if (e == -2) break;
             p.elem = e;
             out = e < 0;
         } else {
             out = outside_domain(m, p.X[0], p.X[1], p.X[2]);
         }
+        if (itype != INT_EULER)
+            out = out || is.mp[0].left || is.mp[1].left || is.mp[2].left || is.mp[3].left;
         if (out) {
             p.left = 1;
-            p.left_time = tc - h;
-            for (int c = 0; c < 3; c++) p.X[c] = p.X[c] - h * ve[c];
-            return p.left_time;
-        }
-        if (extrap) {
-            get_velocity<MODE, DIM>(m, f, tc, p, p.V, err);
-            if (err) return t1;
-        }
-    }
-    return t1;
-}
-
-/* Adams-Bashforth of order 2 or 3 (no ompVC counterpart; cudaVC's "AB2",
- * cudaVC/src/multistep.cu:249-304, is not followed).  Textbook fixed-step
- *   AB2: x+ = x + h (3 f0 - f1) / 2
- *   AB3: x+ = x + h (23 f0 - 16 f1 + 5 f2) / 12
- * with f0 = u(x, t).  The history (f1, f2) persists across calls; steps
- * without enough history, or clipped to the interval end by more than 1e-9
- * of h, are RK4 steps (their first stage is f0), and a clipped step restarts
- * the history.  Steps shorter than 1e-12 dt (floating-point remainders) are
- * Euler steps that keep the history.  Exit handling as RK4. */
-template <int MODE, int DIM, int UNSTRUCT>
-__device__ double ab(const DevMesh &m, const DevField &f, PState &p, double t0, double t1, double dt, int order,
-                     int extrap, int &err, uint32_t &nsteps) {
-#pragma clang fp contract(off)
-    if (p.left) {
-        err = E_LEFT;
-        return t1;
-    }
-    if (UNSTRUCT && (p.elem < 0 || p.elem >= m.ne)) {
-        err = E_ELEM;
-        return t1;
-    }
-    Probe mp[4];
-    for (int q = 0; q < 4; q++) {
-        mp[q].elem = p.elem;
-        mp[q].left = 0;
-    }
-    const double hfull = sign_of(dt, t1 - t0);
-    const double hmin = 1e-12 * fabs(dt);
-    double tc = t0, h = hfull;
-    while ((tc - t1) * (t1 - t0) < 0.0) {
-        if ((tc + h - t1) * (tc + h - t0) > 0.0) h = sign_of(t1 - tc, t1 - t0);
-        const bool micro = fabs(h) <= hmin;
-        const bool full = fabs(h - hfull) <= 1e-9 * fabs(hfull); /* accumulation remainders count as full */
-        double f0[3], inc[3];
-        stage_velocity<MODE, DIM, UNSTRUCT>(m, f, tc, p.X, mp[0], f0, err);
-        if (err) return t1;
-        if (micro) {
-            for (int c = 0; c < 3; c++) inc[c] = h * f0[c];
-        } else if (full && p.nhist >= order - 1) {
-            if (order == 2)
-                for (int c = 0; c < 3; c++) inc[c] = h * (3 * f0[c] - p.F1[c]) / 2;
-            else
-                for (int c = 0; c < 3; c++) inc[c] = h * (23 * f0[c] - 16 * p.F1[c] + 5 * p.F2[c]) / 12;
-        } else {
-            double k2[3], k3[3], k4[3], Y[3];
-            for (int c = 0; c < 3; c++) Y[c] = p.X[c] + 0.5 * f0[c] * h;
-            stage_velocity<MODE, DIM, UNSTRUCT>(m, f, tc + h / 2, Y, mp[1], k2, err);
-            if (err) return t1;
-            for (int c = 0; c < 3; c++) Y[c] = p.X[c] + 0.5 * k2[c] * h;
-            stage_velocity<MODE, DIM, UNSTRUCT>(m, f, tc + h / 2, Y, mp[2], k3, err);
-            if (err) return t1;
-            for (int c = 0; c < 3; c++) Y[c] = p.X[c] + k3[c] * h;
-            stage_velocity<MODE, DIM, UNSTRUCT>(m, f, tc + h, Y, mp[3], k4, err);
-            if (err) return t1;
-            for (int c = 0; c < 3; c++) inc[c] = h * (f0[c] + 2 * k2[c] + 2 * k3[c] + k4[c]) / 6;
-        }
-        for (int c = 0; c < 3; c++) p.X[c] = p.X[c] + inc[c];
-        tc = tc + h;
-        if (!micro) nsteps++;
-        bool out;
-        if (UNSTRUCT) {
-            double r, s, t;
-            int e = walk<MODE, DIM>(m, p.X[0], p.X[1], p.X[2], p.elem, ACC_SEARCH, r, s, t, err);
-            if (e == -2) return t1;
-            p.elem = e;
-            out = e < 0;
-        } else {
-            out = outside_domain(m, p.X[0], p.X[1], p.X[2]);
-        }
-        if (out || mp[0].left || mp[1].left || mp[2].left || mp[3].left) {
-            p.left = 1;
-            p.left_time = tc - h;
+            p.left_time = is.tc - h;
             for (int c = 0; c < 3; c++) p.X[c] = p.X[c] - inc[c];
-            return p.left_time;
+            stop = p.left_time;
+            return 1;
         }
-        if (!micro) {
-            if (full) {
-                for (int c = 0; c < 3; c++) {
-                    p.F2[c] = p.F1[c];
-                    p.F1[c] = f0[c];
+        if (itype == INT_AB2 || itype == INT_AB3) {
+            if (!micro) {
+                if (fabs(h - hfull) <= 1e-9 * fabs(hfull)) {
+                    for (int c = 0; c < 3; c++) {
+                        p.F2[c] = p.F1[c];
+                        p.F1[c] = f0[c];
+                    }
+                    p.nhist = p.nhist < 2 ? p.nhist + 1 : 2;
+                } else {
+                    p.nhist = 0;
                 }
-                p.nhist = p.nhist < 2 ? p.nhist + 1 : 2;
-            } else {
-                p.nhist = 0;
             }
+            if (UNSTRUCT) is.mp[0].elem = p.elem; /* next f0 walks from the located element */
         }
-        /* the next step's first probe starts from the located element */
-        if (UNSTRUCT) mp[0].elem = p.elem;
         if (extrap) {
-            get_velocity<MODE, DIM>(m, f, tc, p, p.V, err);
-            if (err) return t1;
+            get_velocity<MODE, DIM>(m, f, is.tc, p, p.V, err);
+            if (err) break;
         }
     }
-    return t1;
+    stop = b;
+    return 1;
 }
 
-/* Advect (ompVC/integration.c:22-45) */
-template <int MODE, int DIM, int UNSTRUCT>
+/* Advect (ompVC/integration.c:22-45) over [t1, t2] in one go */
+template <int MODE, int DIM, int UNSTRUCT, int GEN = 1>
 __device__ __forceinline__ double advect(const DevMesh &m, const DevField &f, PState &p, double t1, double t2, int itype,
                                          double dt, int extrap, int &err, uint32_t &nsteps) {
 #pragma clang fp contract(off)
@@ -748,17 +680,15 @@ __device__ __forceinline__ double advect(const DevMesh &m, const DevField &f, PS
         for (int c = 0; c < 3; c++) p.X[c] = p.X[c] + p.V[c] * (t2 - t1);
         return t2;
     }
-    double stop;
-    if (itype == 1)
-        stop = rk4<MODE, DIM, UNSTRUCT>(m, f, p, t1, t2, dt, extrap, err, nsteps);
-    else if (itype == 3 || itype == 4)
-        stop = ab<MODE, DIM, UNSTRUCT>(m, f, p, t1, t2, dt, itype == 3 ? 2 : 3, extrap, err, nsteps);
-    else
-        stop = euler<MODE, DIM, UNSTRUCT>(m, f, p, t1, t2, dt, extrap, err, nsteps);
+    IState is;
+    is.started = 0;
+    double stop = t2;
+    interval_run<MODE, DIM, UNSTRUCT, GEN>(m, f, p, is, t1, t2, dt, itype, extrap, 0x7fffffff, err, nsteps, stop);
     if ((fabs(stop - t2) > TINY) && extrap)
         for (int c = 0; c < 3; c++) p.X[c] = p.X[c] + p.V[c] * (t2 - stop);
     return stop;
 }
+
 
 /* ------------------------------------------------------------------------ */
 /* largest eigenvalue of a symmetric 3x3, cyclic Jacobi rotations (FP64)     */

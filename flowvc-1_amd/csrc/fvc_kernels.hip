@@ -44,10 +44,10 @@ __device__ inline void add_steps(unsigned long long *ctr, uint32_t ns) {
     }
 }
 
-__device__ inline void load_state(const PtrSet &p, int64_t i, PState &s, bool hist = false) {
+__device__ inline void load_state(const PtrSet &p, int64_t i, PState &s, bool hist = false, bool vel = true) {
     for (int c = 0; c < 3; c++) {
         s.X[c] = p.X[c][i];
-        s.V[c] = p.V[c][i];
+        s.V[c] = vel ? p.V[c][i] : 0.0;
     }
     s.elem = p.elem[i];
     s.left = p.left[i];
@@ -62,10 +62,10 @@ __device__ inline void load_state(const PtrSet &p, int64_t i, PState &s, bool hi
     }
 }
 
-__device__ inline void store_state(const PtrSet &p, int64_t i, const PState &s, bool hist = false) {
+__device__ inline void store_state(const PtrSet &p, int64_t i, const PState &s, bool hist = false, bool vel = true) {
     for (int c = 0; c < 3; c++) {
         p.X[c][i] = s.X[c];
-        p.V[c][i] = s.V[c];
+        if (vel) p.V[c][i] = s.V[c];
     }
     p.elem[i] = s.elem;
     p.left[i] = s.left;
@@ -82,10 +82,11 @@ __device__ inline void store_state(const PtrSet &p, int64_t i, const PState &s, 
 /* ------------------------------------------------------------------------ */
 /* advect / launch velocity                                                   */
 /* ------------------------------------------------------------------------ */
-template <int MODE, int DIM, int UNS>
-__global__ __launch_bounds__(BLOCK) void k_advect(DevMesh m, DevField f, PtrSet p, const int *__restrict__ index,
-                                                 int64_t begin, int64_t count, double t1, double t2, int itype,
-                                                 double dt, int extrap, ErrBuf eb, unsigned long long *steps) {
+template <int MODE, int DIM, int UNS, int GEN>
+__device__ __forceinline__ void advect_body(const DevMesh &m, const DevField &f, const PtrSet &p,
+                                            const int *__restrict__ index, int64_t begin, int64_t count, double t1,
+                                            double t2, int itype, double dt, int extrap, ErrBuf eb,
+                                            unsigned long long *steps) {
     int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t ns = 0;
     if (q < count) {
@@ -93,18 +94,34 @@ __global__ __launch_bounds__(BLOCK) void k_advect(DevMesh m, DevField f, PtrSet 
         double lt = p.left_time[i];
         int lf = p.left[i];
         if (lt > -TINY && (!lf || extrap)) {
-            const bool hist = itype >= 3;
+            const bool hist = GEN && itype >= 3;
             PState s;
-            load_state(p, i, s, hist);
+            load_state(p, i, s, hist, extrap);
             int err = 0;
-            advect<MODE, DIM, UNS>(m, f, s, t1, t2, itype, dt, extrap, err, ns);
+            advect<MODE, DIM, UNS, GEN>(m, f, s, t1, t2, itype, dt, extrap, err, ns);
             if (err)
                 raise_err(eb, err, i);
             else
-                store_state(p, i, s, hist);
+                store_state(p, i, s, hist, extrap);
         }
     }
     add_steps(steps, ns);
+}
+
+/* RK4 only: no multistep state in registers */
+template <int MODE, int DIM, int UNS>
+__global__ __launch_bounds__(BLOCK) void k_advect(DevMesh m, DevField f, PtrSet p, const int *__restrict__ index,
+                                                 int64_t begin, int64_t count, double t1, double t2, int itype,
+                                                 double dt, int extrap, ErrBuf eb, unsigned long long *steps) {
+    advect_body<MODE, DIM, UNS, 0>(m, f, p, index, begin, count, t1, t2, itype, dt, extrap, eb, steps);
+}
+
+/* Euler, AB2, AB3 */
+template <int MODE, int DIM, int UNS>
+__global__ __launch_bounds__(BLOCK) void k_advect_gen(DevMesh m, DevField f, PtrSet p, const int *__restrict__ index,
+                                                     int64_t begin, int64_t count, double t1, double t2, int itype,
+                                                     double dt, int extrap, ErrBuf eb, unsigned long long *steps) {
+    advect_body<MODE, DIM, UNS, 1>(m, f, p, index, begin, count, t1, t2, itype, dt, extrap, eb, steps);
 }
 
 template <int MODE, int DIM, int UNS>
@@ -183,10 +200,16 @@ int launch_advect(fvc_ctx *c, PtrSet p, const int *index, int64_t begin, int64_t
         int block = ip.block > 0 ? ip.block : BLOCK;
         if (block != BLOCK) block = BLOCK; /* kernels are compiled for 256-lane blocks */
         timing_begin(c);
-        FVC_DISPATCH(c->precision, c->mesh.dims, c->mesh.mesh_type, k_advect,
-                     <<<grid_for(count, block), block, 0, c->stream>>>(c->mesh, f, p, index, begin, count, t1, t2,
-                                                                       ip.int_type, ip.time_step,
-                                                                       ip.extrapolate, errbuf(c), c->d_steps));
+        if (ip.int_type == FVC_INT_RK4)
+            FVC_DISPATCH(c->precision, c->mesh.dims, c->mesh.mesh_type, k_advect,
+                         <<<grid_for(count, block), block, 0, c->stream>>>(c->mesh, f, p, index, begin, count, t1, t2,
+                                                                           ip.int_type, ip.time_step, ip.extrapolate,
+                                                                           errbuf(c), c->d_steps));
+        else
+            FVC_DISPATCH(c->precision, c->mesh.dims, c->mesh.mesh_type, k_advect_gen,
+                         <<<grid_for(count, block), block, 0, c->stream>>>(c->mesh, f, p, index, begin, count, t1, t2,
+                                                                           ip.int_type, ip.time_step, ip.extrapolate,
+                                                                           errbuf(c), c->d_steps));
         timing_end(c);
         FVC_HIP(hipGetLastError());
     }
@@ -507,19 +530,22 @@ static FtleDev ftle_dev(fvc_ftle *f) {
     return d;
 }
 
-/* staged advection of the active points (ftle.c:109-169 without the early part) */
-template <int MODE, int DIM, int UNS>
-__global__ __launch_bounds__(BLOCK) void k_ftle_advect(DevMesh m, DevField f, FtleDev d, const int *__restrict__ active,
-                                                      int64_t nactive, double t1, double t2, int itype, double dt,
-                                                      int extrap, ErrBuf eb, unsigned long long *steps) {
+/* staged advection of the active points (ftle.c:109-169 without the early
+ * part).  The particle velocity V only changes with extrapolation
+ * (integration.c:419-425), so it moves only then. */
+template <int MODE, int DIM, int UNS, int GEN>
+__device__ __forceinline__ void ftle_advect_body(const DevMesh &m, const DevField &f, const FtleDev &d,
+                                                 const int *__restrict__ active, int64_t nactive, double t1, double t2,
+                                                 int itype, double dt, int extrap, ErrBuf eb,
+                                                 unsigned long long *steps) {
     int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t ns = 0;
     if (q < nactive) {
         int64_t i = active[q];
         PState s;
-        load_state(d.p, i, s);
+        load_state(d.p, i, s, false, extrap);
         int err = 0;
-        advect<MODE, DIM, UNS>(m, f, s, t1, t2, itype, dt, extrap, err, ns);
+        advect<MODE, DIM, UNS, GEN>(m, f, s, t1, t2, itype, dt, extrap, err, ns);
         if (err) raise_err(eb, err, i);
         d.advected[i] = 1;
         int ex = s.left && !extrap;
@@ -527,13 +553,27 @@ __global__ __launch_bounds__(BLOCK) void k_ftle_advect(DevMesh m, DevField f, Ft
         d.exit_time[i] = s.left_time;
         for (int c = 0; c < 3; c++) {
             d.Xn[c][i] = s.X[c];
-            d.Vn[c][i] = s.V[c];
+            if (extrap) d.Vn[c][i] = s.V[c];
         }
-        /* with extrapolation the left flag travels through elem_new's sign bit slot */
         d.elem_new[i] = s.elem;
         if (extrap) d.p.left[i] = s.left;
     }
     add_steps(steps, ns);
+}
+
+template <int MODE, int DIM, int UNS>
+__global__ __launch_bounds__(BLOCK) void k_ftle_advect(DevMesh m, DevField f, FtleDev d, const int *__restrict__ active,
+                                                      int64_t nactive, double t1, double t2, int itype, double dt,
+                                                      int extrap, ErrBuf eb, unsigned long long *steps) {
+    ftle_advect_body<MODE, DIM, UNS, 0>(m, f, d, active, nactive, t1, t2, itype, dt, extrap, eb, steps);
+}
+
+template <int MODE, int DIM, int UNS>
+__global__ __launch_bounds__(BLOCK) void k_ftle_advect_gen(DevMesh m, DevField f, FtleDev d,
+                                                          const int *__restrict__ active, int64_t nactive, double t1,
+                                                          double t2, int itype, double dt, int extrap, ErrBuf eb,
+                                                          unsigned long long *steps) {
+    ftle_advect_body<MODE, DIM, UNS, 1>(m, f, d, active, nactive, t1, t2, itype, dt, extrap, eb, steps);
 }
 
 /* Advect_FTLEPoint (ftle.c:1205-1227) from the frame-start state; the element
@@ -673,7 +713,7 @@ __global__ __launch_bounds__(BLOCK) void k_ftle_update(FtleDev d, const int *__r
     } else {
         for (int c = 0; c < 3; c++) {
             d.p.X[c][i] = d.Xn[c][i];
-            d.p.V[c][i] = d.Vn[c][i];
+            if (extrap) d.p.V[c][i] = d.Vn[c][i];
         }
         if (uns) d.p.elem[i] = d.elem_new[i];
     }
@@ -770,10 +810,18 @@ int launch_ftle_frame(fvc_ftle *f, int lo, int hi, double tmin, double tmax, dou
     FVC_HIP(hipMemsetAsync(c->d_steps, 0, sizeof(unsigned long long), c->stream));
     if (f->nactive > 0) {
         timing_begin(c);
-        FVC_DISPATCH(c->precision, c->mesh.dims, c->mesh.mesh_type, k_ftle_advect,
-                     <<<grid_for(f->nactive, BLOCK), BLOCK, 0, c->stream>>>(c->mesh, fld, d, f->active, f->nactive,
-                                                                            t1, t2, itype, ip.time_step,
-                                                                            ip.extrapolate, errbuf(c), c->d_steps));
+        if (itype == 1)
+            FVC_DISPATCH(c->precision, c->mesh.dims, c->mesh.mesh_type, k_ftle_advect,
+                         <<<grid_for(f->nactive, BLOCK), BLOCK, 0, c->stream>>>(c->mesh, fld, d, f->active,
+                                                                                f->nactive, t1, t2, itype,
+                                                                                ip.time_step, ip.extrapolate,
+                                                                                errbuf(c), c->d_steps));
+        else
+            FVC_DISPATCH(c->precision, c->mesh.dims, c->mesh.mesh_type, k_ftle_advect_gen,
+                         <<<grid_for(f->nactive, BLOCK), BLOCK, 0, c->stream>>>(c->mesh, fld, d, f->active,
+                                                                                f->nactive, t1, t2, itype,
+                                                                                ip.time_step, ip.extrapolate,
+                                                                                errbuf(c), c->d_steps));
         timing_end(c);
         FVC_HIP(hipGetLastError());
         if (!ip.extrapolate) {
