@@ -33,14 +33,14 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_inputs(args, host, np):
+def build_inputs(args, host, np, with_seeds=True):
     t0 = time.time()
     tube = host.pipe(args.nx, args.nx, args.nz, 1.0, 10.0)
     xyz, conn, adj = host.tube_mesh(tube)
     slices = [host.tube_velocity(tube, xyz, s * 0.02, 0.01, 1234, s) for s in range(50)]
-    seeds = host.tube_seeds(tube, args.tracers, 0.0, 10.0 / 3.0, 42)
-    log("inputs: %d nodes, %d tets, %d tracers (%.1f s)" % (xyz.shape[0], conn.shape[0], seeds.shape[0],
-                                                           time.time() - t0))
+    seeds = host.tube_seeds(tube, args.tracers, 0.0, 10.0 / 3.0, 42) if with_seeds else None
+    log("inputs: %d nodes, %d tets, %d tracers (%.1f s)" % (xyz.shape[0], conn.shape[0],
+                                                           0 if seeds is None else seeds.shape[0], time.time() - t0))
     return tube, xyz, conn, adj, slices, seeds
 
 
@@ -111,6 +111,184 @@ def cpu_baseline(args, np, xyz, conn, adj, slices, recs_sample):
     return dict(value=steps / t_int, seconds=t_int, cores=cores, steps=steps, pts=pts)
 
 
+def make_releases(np, fvc, n_rel, lo_seeds, hi_seeds, seed):
+    """64 releases: seed counts log-uniform in [lo, hi], start times uniform in
+    the period, horizons uniform in [0.1, 1.0] periods, RK4 or AB3."""
+    rng = np.random.default_rng(seed)
+    counts = np.exp(rng.uniform(np.log(lo_seeds), np.log(hi_seeds), n_rel)).astype(np.int64)
+    starts = rng.uniform(0.0, 1.0, n_rel)
+    horizons = rng.uniform(0.1, 1.0, n_rel)
+    types = rng.choice([fvc.INT_RK4, fvc.INT_AB3], n_rel)
+    return counts, starts, horizons, types
+
+
+def release_table(counts, starts, horizons, types, scale=1):
+    rel, b = [], 0
+    for c, ts, h, it in zip(counts, starts, horizons, types):
+        c = max(1, int(c) // scale)
+        rel.append((b, c, float(ts), float(ts + h), int(it)))
+        b += c
+    return rel
+
+
+def bench_batch(args, np, fvc, host):
+    """Uneven releases fused into one persistent kernel (fvc_batch_*): the
+    autotuner picks block size and steps per launch on a 1/8-size copy of the
+    batch, then K timed runs of the full batch (reset from the device-resident
+    initial state + run, launches replayed from a captured hipGraph)."""
+    tube, xyz, conn, adj, slices, _ = build_inputs(args, host, np, with_seeds=False)
+    counts, starts, horizons, types = make_releases(np, fvc, args.releases, 1e4, 2e6, 7)
+    ctx = make_context(fvc, args.precision, xyz, conn, adj, slices)
+    lo, hi = np.array([-1.0, -1.0, 0.0]), np.array([1.0, 1.0, 10.0])
+    t0 = time.time()
+    recs = []
+    for r, c in enumerate(counts):
+        sd = host.tube_seeds(tube, int(c), 0.0, 10.0 / 3.0, 1000 + r)
+        sd = sd[ctx.zsort(sd, lo, hi)]
+        recs.append(ctx.seed_list(sd, 1))
+    recs = np.concatenate(recs)
+    rel = release_table(counts, starts, horizons, types)
+    n = recs.shape[0]
+    log("batch: %d releases, %d seeds (%d RK4 / %d AB3 releases), setup %.1f s" %
+        (len(rel), n, int((types == fvc.INT_RK4).sum()), int((types == fvc.INT_AB3).sum()), time.time() - t0))
+    # autotune on a 1/8 copy (same releases, an eighth of the seeds each)
+    rel_t = release_table(counts, starts, horizons, types, scale=8)
+    idx_t = np.concatenate([np.arange(b0, b0 + rt[1]) for (b0, _, _, _, _), rt in zip(rel, rel_t)])
+    Pt = fvc.Particles(ctx, idx_t.size)
+    Pt.set(recs[idx_t])
+    Bt = fvc.Batch(ctx, Pt, rel_t, 0.0, 0.02, 50, 1e-3)
+    t0 = time.time()
+    blocks = [64, 128, 256, 512, 1024] if args.precision == "f32" else [256]
+    budgets = [1, 4, 16, 64, 256, 0]
+    best, best_ms = Bt.tune(blocks, budgets, 1)
+    Bt.close()
+    del Pt
+    log("autotune %.1f s over blocks %s x budgets %s: best %s (%.1f ms on the 1/8 batch)" %
+        (time.time() - t0, blocks, budgets, best, best_ms))
+    P = fvc.Particles(ctx, n)
+    P.set(recs)
+    B = fvc.Batch(ctx, P, rel, 0.0, 0.02, 50, 1e-3)
+    for w in range(args.warmup):
+        B.reset()
+        st, nl = B.run(**best)
+        log("warmup %d: %d particle-steps, %d launches" % (w, st, nl))
+    fvc.lib().fvc_sync(ctx.h)
+    tot_steps = 0
+    launches = 0
+    wall0 = time.perf_counter()
+    for k in range(args.steps):
+        B.reset()
+        st, nl = B.run(**best)
+        tot_steps += st
+        launches = nl
+    fvc.lib().fvc_sync(ctx.h)
+    wall = time.perf_counter() - wall0
+    # device time of one more run (events around all of its kernels)
+    B.reset()
+    ctx.kernel_time(reset=True)
+    ctx.set_timing(True)
+    B.run(**best)
+    k_ms, _, k_steps = ctx.kernel_time(reset=True)
+    ctx.set_timing(False)
+    got = P.get()
+    K = args.steps
+    value = tot_steps / wall
+    bpp = BYTES_PER_STEP_F32 if args.precision == "f32" else (512 if args.precision in ("bf16", "f16") else 1328)
+    achieved = k_steps * bpp / (k_ms * 1e-3) if k_ms > 0 else 0.0
+    result = {
+        "metric": "particle-steps/s + FTLE seeds/s at stated FP64 error; p50/p99 request latency",
+        "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": K, "warmup": args.warmup,
+        "ms_per_step": 1000.0 * wall / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": {"f32": "f32 (fp64 position/time state)", "bf16": "bf16 storage, f32 math",
+                  "f16": "f16 storage, f32 math", "f64": "f64"}[args.precision],
+        "data": "synthetic (deterministic generators, seeded)",
+        "config": {
+            "workload": "batching on the pipe mesh (%d tets): %d releases in one persistent kernel, seed counts "
+                        "log-uniform [10k, 2M] (%d seeds), start uniform in the period, horizon uniform [0.1, 1] "
+                        "periods, RK4/AB3 mix, dt=1e-3, autotuned block/steps-per-launch, hipGraph replay" %
+                        (conn.shape[0], len(rel), n),
+            "releases": len(rel), "seeds": n, "precision": args.precision,
+        },
+        "batch": {"tuned": best, "tune_ms_eighth_batch": best_ms, "launch_groups": launches,
+                  "particle_steps_per_run": tot_steps // K},
+        "roofline": {
+            "kernel": "k_batch (whole run: batch kernels + compaction, device time)",
+            "bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK, "traffic": None, "bytes_per_particle_step": bpp,
+            "avg_launch_ms": k_ms,
+        },
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(HERE, "oracle"))
+        import oracle
+        cores = min(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), os.cpu_count() or 1, 16)
+        om = oracle.Mesh(3, [-1.0, 1.0, -1.0, 1.0, 0.0, 10.0], coords=xyz, conn=conn, adj=adj)
+        stride = max(1, args.batch_cpu_stride)
+
+        def frame_of(t):
+            f = int(np.floor(t / 0.02))
+            while f * 0.02 > t:
+                f -= 1
+            while (f + 1) * 0.02 <= t:
+                f += 1
+            return f
+
+        # frame-major sweep (slots set once per frame), every release advanced
+        # over its part of each frame exactly as the batch does
+        st = []
+        for (b0, c, ts, te, it) in rel:
+            idx = np.arange(b0, b0 + c, stride)
+            f0 = frame_of(ts)
+            f1 = f0
+            while (f1 + 1) * 0.02 < te:
+                f1 += 1
+            st.append(dict(idx=idx, pts=np.ascontiguousarray(recs[idx].copy()), f0=f0, f1=f1, ts=ts, te=te, it=it,
+                           hist=np.zeros(idx.size, oracle.HIST_DTYPE)))
+        t_int, steps = 0.0, 0
+        for f in range(min(r["f0"] for r in st), max(r["f1"] for r in st) + 1):
+            tmin = f * 0.02
+            tmax = tmin + 0.02
+            om.set_slot(0, slices[f % 50])
+            om.set_slot(1, slices[(f + 1) % 50])
+            om.set_interval(tmin, tmax)
+            for r in st:
+                if not (r["f0"] <= f <= r["f1"]):
+                    continue
+                pts = r["pts"]
+                if f == r["f0"]:
+                    live = np.nonzero(pts["LeftDomain"] == 0)[0]
+                    sub = pts[live].copy()
+                    v, _ = om.velocity(sub, r["ts"])
+                    sub["V"] = v
+                    pts[live] = sub
+                a, bb = max(r["ts"], tmin), min(r["te"], tmax)
+                t1 = time.perf_counter()
+                if r["it"] == fvc.INT_AB3:
+                    rc, ns = oracle.ab_advect(om, pts, r["hist"], a, bb, 1e-3, 3, nthreads=cores)
+                else:
+                    rc = om.advect(pts, a, bb, 1e-3, nthreads=cores, chunk=max(1, pts.shape[0] // (4 * cores)))
+                    ns = om.last_steps.value
+                t_int += time.perf_counter() - t1
+                steps += ns
+                if rc:
+                    raise RuntimeError("oracle failed (%d)" % rc)
+        errs = []
+        for r in st:
+            g, pts = got[r["idx"]], r["pts"]
+            ok = (g["LeftDomain"] == 0) & (pts["LeftDomain"] == 0) & (pts["LeftDomainTime"] > -1e-10)
+            if ok.any():
+                errs.append(np.abs(g["X"][ok] - pts["X"][ok]).max(axis=1))
+        err = np.concatenate(errs) if errs else np.zeros(1)
+        result["cpu_baseline"] = {
+            "value": steps / t_int, "unit": "particle-steps/s", "cores": cores, "kind": "port",
+            "sample": "every %d-th seed of each release, same releases/horizons/integrators, integrate-only "
+                      "time %.1f s, oracle FP64 restatement (OpenMP)" % (stride, t_int)}
+        result["pos_err_vs_fp64"] = {"max": float(err.max()), "p99": float(np.percentile(err, 99)),
+                                     "tolerance": 1e-4, "domain_size": 10.0, "points": int(err.size)}
+        result["speedup_vs_cpu"] = value / (steps / t_int)
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,6 +302,10 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=1 << 18)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-f64-check", action="store_true")
+    ap.add_argument("--workload", default="pipe", choices=["pipe", "batch"],
+                    help="pipe: tracers + FTLE grid over one period; batch: uneven releases in one persistent kernel")
+    ap.add_argument("--releases", type=int, default=64)
+    ap.add_argument("--batch-cpu-stride", type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one GPU and this build has no multi-GPU path "
@@ -133,6 +315,9 @@ def main():
     import numpy as np
     from fvcpy import fvc, host
 
+    if args.workload == "batch":
+        bench_batch(args, np, fvc, host)
+        return
     res = [int(v) for v in args.ftle.split(",")]
     tube, xyz, conn, adj, slices, seeds = build_inputs(args, host, np)
     t0 = time.time()

@@ -35,6 +35,16 @@ class IntParams(ctypes.Structure):
                 ("block", ctypes.c_int32), ("steps_per_launch", ctypes.c_int32)]
 
 
+class Release(ctypes.Structure):
+    _fields_ = [("begin", ctypes.c_int64), ("count", ctypes.c_int64), ("t_start", ctypes.c_double),
+                ("t_stop", ctypes.c_double), ("int_type", ctypes.c_int32), ("pad_", ctypes.c_int32)]
+
+
+class BatchParams(ctypes.Structure):
+    _fields_ = [("block", ctypes.c_int32), ("steps_per_launch", ctypes.c_int32), ("use_graph", ctypes.c_int32),
+                ("waves_per_cu", ctypes.c_int32)]
+
+
 class FvcError(RuntimeError):
     def __init__(self, code, what, detail=""):
         self.code = code
@@ -86,6 +96,11 @@ def lib():
         L.fvc_set_timing.argtypes = [vp, i32]
         L.fvc_kernel_time.argtypes = [vp, ctypes.POINTER(d), ctypes.POINTER(i64), ctypes.POINTER(i64), i32]
         L.fvc_copy_probe.argtypes = [vp, i64]
+        L.fvc_batch_create.argtypes = [vp, vp, ctypes.POINTER(Release), i32, d, d, i32, d, ctypes.POINTER(vp)]
+        L.fvc_batch_destroy.argtypes = [vp]
+        L.fvc_batch_reset.argtypes = [vp]
+        L.fvc_batch_run.argtypes = [vp, ctypes.POINTER(BatchParams), ctypes.POINTER(i64), ctypes.POINTER(i64)]
+        L.fvc_batch_tune.argtypes = [vp, vp, i32, vp, i32, i32, ctypes.POINTER(BatchParams), ctypes.POINTER(d)]
         _LIB = L
     return _LIB
 
@@ -286,6 +301,55 @@ class FtleSlide:
     def close(self):
         if self.h:
             lib().fvc_ftle_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Releases fused into one persistent kernel (fvc_batch_*).
+
+    releases: sequence of (begin, count, t_start, t_stop, int_type).  Data
+    frame f spans [t0 + f*frame_dt, t0 + (f+1)*frame_dt] and interpolates
+    slots f % nslots and (f+1) % nslots."""
+
+    def __init__(self, ctx, particles, releases, t0, frame_dt, nslots, dt):
+        self.ctx = ctx
+        arr = (Release * len(releases))()
+        for k, (b, c, ts, te, it) in enumerate(releases):
+            arr[k] = Release(int(b), int(c), float(ts), float(te), int(it), 0)
+        h = ctypes.c_void_p()
+        _chk(lib().fvc_batch_create(ctx.h, particles.h, arr, len(releases), t0, frame_dt, nslots, dt,
+                                    ctypes.byref(h)), "fvc_batch_create", ctx.h)
+        self.h = h
+
+    def reset(self):
+        _chk(lib().fvc_batch_reset(self.h), "fvc_batch_reset", self.ctx.h)
+
+    def run(self, block=256, steps_per_launch=64, use_graph=1, waves_per_cu=0):
+        bp = BatchParams(block, steps_per_launch, use_graph, waves_per_cu)
+        st, nl = ctypes.c_int64(), ctypes.c_int64()
+        _chk(lib().fvc_batch_run(self.h, ctypes.byref(bp), ctypes.byref(st), ctypes.byref(nl)), "fvc_batch_run",
+             self.ctx.h)
+        return st.value, nl.value
+
+    def tune(self, blocks=(64, 128, 256, 512, 1024), budgets=(1, 4, 16, 64, 256), use_graph=1):
+        bl = np.asarray(blocks, np.int32)
+        bu = np.asarray(budgets, np.int32)
+        best = BatchParams()
+        ms = ctypes.c_double()
+        _chk(lib().fvc_batch_tune(self.h, _p(bl), len(bl), _p(bu), len(bu), use_graph, ctypes.byref(best),
+                                  ctypes.byref(ms)), "fvc_batch_tune", self.ctx.h)
+        return {"block": best.block, "steps_per_launch": best.steps_per_launch, "use_graph": best.use_graph,
+                "waves_per_cu": best.waves_per_cu}, ms.value
+
+    def close(self):
+        if self.h:
+            lib().fvc_batch_destroy(self.h)
             self.h = None
 
     def __del__(self):

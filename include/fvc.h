@@ -189,12 +189,53 @@ int fvc_ftle_reset(fvc_ftle *f);
 /* ---- measurement ------------------------------------------------------------ */
 /* When enabled, every launch of the integration kernels (fvc_advect's and the
  * FTLE frame's advection kernel) is bracketed by HIP events on the context's
- * stream; fvc_kernel_time returns the summed milliseconds, launch count and
+ * stream, and so is each fvc_batch_run as a whole (all of its kernels);
+ * fvc_kernel_time returns the summed milliseconds, launch (run) count and
  * particle-steps since the last reset. */
 int fvc_set_timing(fvc_ctx *ctx, int32_t enable);
 int fvc_kernel_time(fvc_ctx *ctx, double *ms, int64_t *launches, int64_t *particle_steps, int32_t reset);
 /* A plain device copy of `bytes` bytes (HBM traffic calibration for counters). */
 int fvc_copy_probe(fvc_ctx *ctx, int64_t bytes);
+
+/* ---- batched releases (no reference counterpart: Compute_Tracer,
+ *      tracers.c:25-289, advances releases one after another) ---------------- */
+/* Many releases of different sizes, start times, horizons and integrators in
+ * one persistent kernel.  Release r covers particles [begin, begin+count) of
+ * the particle set and integrates them from t_start to t_stop across data
+ * frames f = [t0 + f dt, t0 + (f+1) dt] interpolating slots (f mod nslots) and
+ * ((f+1) mod nslots) - per frame exactly as fvc_advect would (same stepping,
+ * same arithmetic), so a release's result does not depend on the batch it
+ * runs in.  Releases begin with the launch-time velocity lookup; points that
+ * have left, or carry LeftDomainTime <= -1e-10, are not integrated.  Needs a
+ * 3-D unstructured mesh, t_stop >= t_start, t_start >= t0; releases must not
+ * overlap.  FP32 contexts accept every block size, others 256 only. */
+typedef struct fvc_release {
+    int64_t begin, count;
+    double t_start, t_stop;
+    int32_t int_type; /* enum fvc_int_type */
+    int32_t pad_;
+} fvc_release;
+
+typedef struct fvc_batch_params {
+    int32_t block;            /* threads per block: 64, 128, 256, 512 or 1024 */
+    int32_t steps_per_launch; /* step budget of one particle per launch; 0 = unlimited */
+    int32_t use_graph;        /* capture the launch sequence in a hipGraph and replay it */
+    int32_t waves_per_cu;     /* persistent grid size: 0 = occupancy limit */
+} fvc_batch_params;
+
+typedef struct fvc_batch fvc_batch;
+int fvc_batch_create(fvc_ctx *ctx, fvc_particles *p, const fvc_release *rel, int32_t nrel, double frame_t0,
+                     double frame_dt, int32_t nslots, double time_step, fvc_batch **out);
+void fvc_batch_destroy(fvc_batch *b);
+/* Restore the particles of every release to their state at fvc_batch_create. */
+int fvc_batch_reset(fvc_batch *b);
+/* Run every release to its stop time (or exit).  Returns particle-steps and
+ * the number of kernel launches. */
+int fvc_batch_run(fvc_batch *b, const fvc_batch_params *bp, int64_t *particle_steps, int64_t *launches);
+/* Autotuner: times fvc_batch_run over block sizes and step budgets (resetting
+ * the batch between trials) and returns the fastest parameters. */
+int fvc_batch_tune(fvc_batch *b, const int32_t *blocks, int32_t nblocks, const int32_t *budgets, int32_t nbudgets,
+                   int32_t use_graph, fvc_batch_params *best, double *best_ms);
 
 /* ---- Z-order seed sort (cudaVC/ZOrderSort/main.cpp:86-101 ordering) ------- */
 /* Sort n points by the interleaved-bit (x most significant) order of their
