@@ -838,6 +838,11 @@ extern "C" int fvc_ftle_create(fvc_ctx *c, const double mn[3], const double mx[3
     f->exited = iv + 2 * n;
     f->advected = iv + 3 * n;
     f->active = iv + 4 * n;
+    f->zperm = iv + 5 * n;
+    if ((rc = launch_grid_zorder(c, f->res, f->n, f->zperm))) {
+        fvc_ftle_destroy(f);
+        return rc;
+    }
     /* location chain of InitializeFTLEArray, then the grid records */
     int *ce = f->elem_new, *cl = f->exited; /* reuse staging as chain output */
     double *clt = f->exit_time;

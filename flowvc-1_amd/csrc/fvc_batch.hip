@@ -5,8 +5,8 @@
  * time and integrator.  The work per release is what the tracer driver does
  * for it (tracers.c:202-229 through main.c:170-227): GetVelocity at the
  * release time on the data frame holding it, then one Advect per data frame
- * over [max(t_start, tmin), min(t_stop, tmax)] until the frame that reaches
- * t_stop, skipping points that left (LeftDomain) or carry LeftDomainTime <=
+ * over [max(t_start, tmin), min(t_stop, tmax)] until the frame whose end
+ * is within TINY of t_stop or beyond it, skipping points that left (LeftDomain) or carry LeftDomainTime <=
  * -TINY.  Frame f interpolates between slots f mod nslots and (f+1) mod
  * nslots over [t0 + f*dt_frame, t0 + (f+1)*dt_frame].
  *
@@ -152,7 +152,7 @@ __device__ void batch_particle(const DevMesh &m, const BatchDev &b, int i, int b
         is.started = 0;
         /* every pass consumes at least one unit, so a launch always ends */
         left_budget -= ns - before > 0 ? (int)(ns - before) : 1;
-        if (s.left || !(fld.tmax < R.t_stop)) {
+        if (s.left || R.t_stop - fld.tmax < TINY) { /* status 2 rule, tracers.c:283-284 */
             fl |= FL_DONE;
             break;
         }

@@ -77,7 +77,8 @@ struct fvc_ftle {
     int *advected; /* 1 when the point was integrated this frame */
     double *exit_time;
     double *Vn[3];
-    int *active;    /* compacted indices of points to integrate */
+    int *active;    /* compacted indices of points to integrate (Z-ordered) */
+    int *zperm;     /* grid points in Z-order */
     int64_t nactive;
     void *buf;
     int64_t inside;
@@ -113,6 +114,20 @@ inline PtrSet ptrs(fvc_particles *p) {
     return s;
 }
 
+/* one FTLE grid's device arrays (grid index P = (i*res1 + j)*res2 + k) */
+struct FtleDev {
+    int res[3];
+    double del[3];
+    double start_time;
+    PtrSet p;
+    int *have;
+    double *wt, *wot;
+    double *Xn[3], *Vn[3];
+    int *elem_new, *exited, *advected;
+    double *exit_time;
+};
+FtleDev ftle_dev(fvc_ftle *f);
+
 DevField bind_field(fvc_ctx *c, int lo, int hi, double tmin, double tmax);
 
 int launch_velocity(fvc_ctx *c, PtrSet p, int64_t begin, int64_t count, double t, const DevField &f);
@@ -126,7 +141,15 @@ int launch_ftle_frame(fvc_ftle *f, int lo, int hi, double tmin, double tmax, dou
                       const fvc_int_params &ip);
 int launch_ftle_fill(fvc_ftle *f, double int_t);
 int launch_ftle_mask(fvc_ftle *f);
-int launch_compact(fvc_ctx *c, const int *flags, int64_t n, int *out_idx, int64_t *count);
+/* the same on any FTLE grid view (enqueue only, no synchronisation) */
+int launch_ftle_early(fvc_ctx *c, const FtleDev &d, int64_t n, const DevField &fld, double t1, int itype, double dt);
+int launch_ftle_update_all(fvc_ctx *c, const FtleDev &d, int64_t n);
+int launch_ftle_fill_dev(fvc_ctx *c, const FtleDev &d, int64_t n, double int_t);
+int launch_ftle_mask_dev(fvc_ctx *c, const FtleDev &d, int64_t n);
+/* indices i (or map[i]) of the nonzero flags, in order */
+int launch_compact(fvc_ctx *c, const int *flags, int64_t n, int *out_idx, int64_t *count, const int *map = nullptr);
+/* Z-order (Morton, i most significant) permutation of the points of a grid */
+int launch_grid_zorder(fvc_ctx *c, const int32_t res[3], int64_t n, int *dperm);
 int launch_zsort(fvc_ctx *c, const double *dxyz, int64_t n, const double lo[3], const double hi[3], int *dperm);
 
 int launch_copy_probe(fvc_ctx *c, void *dst, const void *src, int64_t bytes);

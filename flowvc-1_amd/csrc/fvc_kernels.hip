@@ -30,6 +30,18 @@ constexpr int BLOCK = 256;
 /* ------------------------------------------------------------------------ */
 /* helpers                                                                   */
 /* ------------------------------------------------------------------------ */
+/* The dispatcher deals workgroups to the eight XCDs round-robin (block b runs
+ * on XCD b % 8).  Renumbering so that XCD x takes the x-th contiguous eighth
+ * of the grid keeps the blocks in flight on one XCD on neighbouring,
+ * Z-ordered particles, so one L2 holds their mesh and velocity gathers
+ * instead of all eight L2s holding the whole in-flight region.  Bijective on
+ * [0, gridDim.x). */
+__device__ __forceinline__ int64_t xcd_block() {
+    const int64_t G = gridDim.x, b = blockIdx.x;
+    const int64_t per = G >> 3, rem = G & 7, x = b & 7, k = b >> 3;
+    return x * per + (x < rem ? x : rem) + k;
+}
+
 __device__ inline void add_steps(unsigned long long *ctr, uint32_t ns) {
     __shared__ unsigned int red[BLOCK / 64];
     unsigned long long v = ns;
@@ -87,7 +99,7 @@ __device__ __forceinline__ void advect_body(const DevMesh &m, const DevField &f,
                                             const int *__restrict__ index, int64_t begin, int64_t count, double t1,
                                             double t2, int itype, double dt, int extrap, ErrBuf eb,
                                             unsigned long long *steps) {
-    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t q = xcd_block() * blockDim.x + threadIdx.x;
     uint32_t ns = 0;
     if (q < count) {
         int64_t i = index ? (int64_t)index[q] : begin + q;
@@ -301,7 +313,9 @@ int launch_locate(fvc_ctx *c, const double *dxyz, const int *dguess, int64_t n, 
 /* ------------------------------------------------------------------------ */
 /* seed chains                                                                */
 /* ------------------------------------------------------------------------ */
-enum { SF_OUTB = 1, SF_MULTI = 2 };
+/* per-point chain flags: outside the bounding box, inside more than one
+ * element, and (FTLE chain) the first / last point of a k-row or j-plane */
+enum { SF_OUTB = 1, SF_MULTI = 2, SF_ROWSTART = 4, SF_PLANESTART = 8, SF_ROWEND = 16, SF_PLANEEND = 32 };
 
 struct ChainGeom {
     int chain;
@@ -341,6 +355,11 @@ __global__ __launch_bounds__(BLOCK) void k_seed_prepare(DevMesh m, ChainGeom g, 
     double X[3];
     chain_point(g, i, X);
     int flags = 0, emin = -1;
+    if (g.chain == 2) {
+        const int kk = (int)(i % g.res[2]), jj = (int)((i / g.res[2]) % g.res[1]);
+        if (kk == 0) flags |= SF_ROWSTART | (jj == 0 ? SF_PLANESTART : 0);
+        if (kk == g.res[2] - 1) flags |= SF_ROWEND | (jj == g.res[1] - 1 ? SF_PLANEEND : 0);
+    }
     if (outside_domain(m, X[0], X[1], X[2]))
         flags |= SF_OUTB;
     else if (m.mesh_type == 1) {
@@ -351,8 +370,12 @@ __global__ __launch_bounds__(BLOCK) void k_seed_prepare(DevMesh m, ChainGeom g, 
     info[i] = make_int2(emin, flags);
 }
 
-/* One wave replays the chain; lanes hold 64 consecutive points' data and the
- * per-point step is executed uniformly by the whole wave. */
+/* One wave replays the chain over chunks of 64 consecutive points.  A chunk
+ * whose results cannot depend on the chain state (every point outside, or
+ * inside exactly one element with LocalSearchChecking, after the first found
+ * point, and no FTLE row boundary) is resolved by all lanes at once and the
+ * state (last found element, first found element of the row and plane) is
+ * advanced with ballots; any other chunk is replayed point by point. */
 template <int DIM>
 __global__ __launch_bounds__(64) void k_seed_chain(DevMesh m, ChainGeom g, int64_t n, const int2 *__restrict__ info,
                                                   int lsc, int seed0, int *__restrict__ out_elem,
@@ -363,93 +386,149 @@ __global__ __launch_bounds__(64) void k_seed_chain(DevMesh m, ChainGeom g, int64
     int seed = seed0, guess = seed0 >= 0 ? seed0 : -1;
     int iguess = -1, jguess = -1;  /* FTLE chain */
     for (int64_t base = 0; base < n; base += 64) {
-        int2 my = base + lane < n ? info[base + lane] : make_int2(-1, SF_OUTB);
+        const bool active = base + lane < n;
+        const int2 my = active ? info[base + lane] : make_int2(-1, SF_OUTB);
         int r_elem = -1, r_left = 0;
         double r_lt = 0.0;
-        int lim = (int)min((int64_t)64, n - base);
-        for (int l = 0; l < lim; l++) {
-            int64_t i = base + l;
-            int emin = __shfl(my.x, l, 64), flags = __shfl(my.y, l, 64);
-            int elem = -1, left = (flags & SF_OUTB) ? 1 : 0;
+        /* fast path test */
+        const bool outb = (my.y & SF_OUTB) != 0;
+        const bool indep = outb || !uns || (lsc && !(my.y & SF_MULTI));
+        const bool state_ok = g.chain == 0 ? found > 0 : seed >= 0;
+        const bool all_outb = __ballot(active && !(outb || !uns)) == 0ull;
+        const bool fast = __ballot(active && !indep) == 0ull && (state_ok || all_outb);
+        if (fast) {
+            int left = outb ? 1 : 0, elem = -1;
             double lt = 0.0;
-            int kk = 0, jj = 0;
-            if (g.chain == 2) {
-                kk = (int)(i % g.res[2]);
-                jj = (int)((i / g.res[2]) % g.res[1]);
-                if (kk == 0) {
-                    jguess = -1;
-                    if (jj == 0) iguess = -1;
+            bool hit = false;
+            if (active && !left && uns) {
+                if (my.x < 0) {
+                    left = 1;
+                    lt = -1.0;
+                } else {
+                    elem = my.x;
+                    hit = true;
                 }
             }
-            if (!left && uns) {
-                double X[3];
-                chain_point(g, i, X);
-                const bool direct = lsc && !(flags & SF_MULTI); /* result cannot depend on the guess */
-                int err = 0, index = -1;
-                if (g.chain == 0) {
-                    if (!found) {
-                        index = emin;
-                    } else {
-                        index = direct ? emin : local_search<DIM>(m, X[0], X[1], X[2], guess, err);
-                        if (!direct && index < 0 && lsc) index = emin;
+            if (g.chain == 0) lt = left ? -1.0 : 0.0;
+            if (g.chain == 2 && left) lt = -1.0;
+            r_elem = elem;
+            r_left = left;
+            r_lt = lt;
+            /* advance the state segment by segment: a segment ends at a row end
+             * (FTLE chain), so a row start can only open a segment */
+            const unsigned long long hits = __ballot(hit);
+            const unsigned long long rowend = __ballot(active && (my.y & SF_ROWEND));
+            const unsigned long long rowstart = __ballot(active && (my.y & SF_ROWSTART));
+            const unsigned long long planestart = __ballot(active && (my.y & SF_PLANESTART));
+            const unsigned long long planeend = __ballot(active && (my.y & SF_PLANEEND));
+            int a = 0;
+            while (a < 64) {
+                const unsigned long long from_a = ~0ull << a;
+                const unsigned long long re = rowend & from_a;
+                const int b = re ? __ffsll((long long)re) - 1 : 63;
+                const unsigned long long seg = (b == 63 ? ~0ull : ((1ull << (b + 1)) - 1ull)) & from_a;
+                if ((rowstart >> a) & 1ull) {
+                    jguess = -1;
+                    if ((planestart >> a) & 1ull) iguess = -1;
+                }
+                const unsigned long long sh = hits & seg;
+                if (sh) {
+                    const int last = 63 - __clzll(sh), first = __ffsll((long long)sh) - 1;
+                    guess = __shfl(elem, last, 64);
+                    if (g.chain == 0) found += __popcll(sh);
+                    if (g.chain == 2) {
+                        const int e1 = __shfl(elem, first, 64);
+                        if (jguess < 0) jguess = e1;
+                        if (iguess < 0) iguess = e1;
                     }
-                    if (index < 0) {
-                        left = 1;
-                        elem = -1;
-                    } else {
-                        elem = index;
-                        guess = index;
-                        found++;
-                    }
-                } else {
-                    if (seed < 0) {
-                        seed = emin;
-                        if (seed < 0) {
-                            left = 1;
-                            elem = -1;
-                            lt = -1.0;
-                        } else {
-                            elem = seed;
-                            guess = seed;
-                            if (g.chain == 2) jguess = iguess = seed;
-                        }
-                    } else {
-                        if (direct)
+                }
+                if ((rowend >> b) & 1ull) {
+                    if (jguess >= 0) guess = jguess;
+                    if (((planeend >> b) & 1ull) && iguess >= 0) guess = iguess;
+                }
+                a = b + 1;
+            }
+        } else {
+            const int lim = (int)min((int64_t)64, n - base);
+            for (int l = 0; l < lim; l++) {
+                const int64_t i = base + l;
+                const int emin = __shfl(my.x, l, 64), flags = __shfl(my.y, l, 64);
+                int elem = -1, left = (flags & SF_OUTB) ? 1 : 0;
+                double lt = 0.0;
+                if (g.chain == 2 && (flags & SF_ROWSTART)) {
+                    jguess = -1;
+                    if (flags & SF_PLANESTART) iguess = -1;
+                }
+                if (!left && uns) {
+                    const bool direct = lsc && !(flags & SF_MULTI); /* result cannot depend on the guess */
+                    double X[3] = {0.0, 0.0, 0.0};
+                    if (!direct) chain_point(g, i, X);
+                    int err = 0, index = -1;
+                    if (g.chain == 0) {
+                        if (!found) {
                             index = emin;
-                        else {
-                            index = local_search<DIM>(m, X[0], X[1], X[2], guess, err);
-                            if (index < 0 && !err) index = local_search<DIM>(m, X[0], X[1], X[2], seed, err);
-                            if (index < 0 && !err && lsc) index = emin;
+                        } else {
+                            index = direct ? emin : local_search<DIM>(m, X[0], X[1], X[2], guess, err);
+                            if (!direct && index < 0 && lsc) index = emin;
                         }
                         if (index < 0) {
                             left = 1;
                             elem = -1;
-                            lt = -1.0;
                         } else {
                             elem = index;
                             guess = index;
-                            if (g.chain == 2) {
-                                if (jguess < 0) jguess = index;
-                                if (iguess < 0) iguess = index;
+                            found++;
+                        }
+                    } else {
+                        if (seed < 0) {
+                            seed = emin;
+                            if (seed < 0) {
+                                left = 1;
+                                elem = -1;
+                                lt = -1.0;
+                            } else {
+                                elem = seed;
+                                guess = seed;
+                                if (g.chain == 2) jguess = iguess = seed;
+                            }
+                        } else {
+                            if (direct)
+                                index = emin;
+                            else {
+                                index = local_search<DIM>(m, X[0], X[1], X[2], guess, err);
+                                if (index < 0 && !err) index = local_search<DIM>(m, X[0], X[1], X[2], seed, err);
+                                if (index < 0 && !err && lsc) index = emin;
+                            }
+                            if (index < 0) {
+                                left = 1;
+                                elem = -1;
+                                lt = -1.0;
+                            } else {
+                                elem = index;
+                                guess = index;
+                                if (g.chain == 2) {
+                                    if (jguess < 0) jguess = index;
+                                    if (iguess < 0) iguess = index;
+                                }
                             }
                         }
                     }
+                    if (err && lane == 0) raise_err(eb, err, i);
                 }
-                if (err && lane == 0) raise_err(eb, err, i);
-            }
-            if (g.chain == 0) lt = left ? -1.0 : 0.0;
-            if (g.chain == 2 && left) lt = -1.0;
-            if (g.chain == 2 && kk == g.res[2] - 1) {
-                if (jguess >= 0) guess = jguess;
-                if (jj == g.res[1] - 1 && iguess >= 0) guess = iguess;
-            }
-            if (lane == l) {
-                r_elem = elem;
-                r_left = left;
-                r_lt = lt;
+                if (g.chain == 0) lt = left ? -1.0 : 0.0;
+                if (g.chain == 2 && left) lt = -1.0;
+                if (g.chain == 2 && (flags & SF_ROWEND)) {
+                    if (jguess >= 0) guess = jguess;
+                    if ((flags & SF_PLANEEND) && iguess >= 0) guess = iguess;
+                }
+                if (lane == l) {
+                    r_elem = elem;
+                    r_left = left;
+                    r_lt = lt;
+                }
             }
         }
-        if (base + lane < n) {
+        if (active) {
             out_elem[base + lane] = r_elem;
             out_left[base + lane] = r_left;
             out_lt[base + lane] = r_lt;
@@ -500,19 +579,7 @@ int launch_seed_chain(fvc_ctx *c, int chain, int64_t n, const double *dxyz, cons
 /* ------------------------------------------------------------------------ */
 /* FTLE                                                                       */
 /* ------------------------------------------------------------------------ */
-struct FtleDev {
-    int res[3];
-    double del[3];
-    double start_time;
-    PtrSet p;
-    int *have;
-    double *wt, *wot;
-    double *Xn[3], *Vn[3];
-    int *elem_new, *exited, *advected;
-    double *exit_time;
-};
-
-static FtleDev ftle_dev(fvc_ftle *f) {
+FtleDev ftle_dev(fvc_ftle *f) {
     FtleDev d;
     for (int a = 0; a < 3; a++) {
         d.res[a] = f->res[a];
@@ -540,7 +607,7 @@ __device__ __forceinline__ void ftle_advect_body(const DevMesh &m, const DevFiel
                                                  const int *__restrict__ active, int64_t nactive, double t1, double t2,
                                                  int itype, double dt, int extrap, ErrBuf eb,
                                                  unsigned long long *steps) {
-    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t q = xcd_block() * blockDim.x + threadIdx.x;
     uint32_t ns = 0;
     if (q < nactive) {
         int64_t i = active[q];
@@ -639,63 +706,157 @@ __device__ inline void gradient(const double lo[3][3], const double hi[3][3], co
 
 /* GetFTLEForPointEarly (ftle.c:788-1048).  The trigger of point P is the
  * first exiting point, in the reference's loop order, among P and its six
- * neighbours; te is the trigger's exit time. */
-template <int MODE, int DIM, int UNS>
-__global__ __launch_bounds__(BLOCK) void k_ftle_early(DevMesh m, DevField f, FtleDev d, int64_t n, double t1, int itype,
-                                                     double dt, ErrBuf eb) {
-#pragma clang fp contract(off)
-    int64_t P = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (P >= n || d.have[P]) return;
-    const int64_t sk = 1, sj = d.res[2], si = (int64_t)d.res[2] * d.res[1];
-    int k = (int)(P % d.res[2]), j = (int)((P / d.res[2]) % d.res[1]), i = (int)(P / si);
-    int64_t nb[6] = {P - si, P + si, P - sj, P + sj, P - sk, P + sk};
-    bool has[6] = {i > 0, i < d.res[0] - 1, j > 0, j < d.res[1] - 1, k > 0, k < d.res[2] - 1};
+ * neighbours; te is the trigger's exit time.  The re-advections of a
+ * triggered point are independent, so they run as separate lanes: pass A
+ * advects the (up to) six neighbours to te, pass B the point itself and the
+ * gradient neighbours to MinExitTime, pass C forms the gradient and the FTLE.
+ * Each lane does exactly the reference's computation for its advection. */
+struct EarlyBuf {
+    const int *list; /* triggered points, ascending */
+    int64_t nt;
+    double *lt;  /* [nt][6] exit times of the neighbours (te if they stay) */
+    double *pos; /* [nt][7][3] self, then lo/hi neighbour per axis, at MinExitTime */
+};
+
+__device__ inline void grid_nbrs(const FtleDev &d, int64_t P, int cc[3], int64_t nb[6], bool has[6]) {
+    const int64_t sj = d.res[2], si = (int64_t)d.res[2] * d.res[1];
+    cc[0] = (int)(P / si);
+    cc[1] = (int)((P / d.res[2]) % d.res[1]);
+    cc[2] = (int)(P % d.res[2]);
+    nb[0] = P - si;
+    nb[1] = P + si;
+    nb[2] = P - sj;
+    nb[3] = P + sj;
+    nb[4] = P - 1;
+    nb[5] = P + 1;
+    for (int ax = 0; ax < 3; ax++) {
+        has[2 * ax] = cc[ax] > 0;
+        has[2 * ax + 1] = cc[ax] < d.res[ax] - 1;
+    }
+}
+
+__device__ inline int64_t early_trigger(const FtleDev &d, int64_t P, const int64_t nb[6], const bool has[6]) {
     int64_t trig = d.exited[P] ? P : INT64_MAX;
     for (int q = 0; q < 6; q++)
         if (has[q] && d.exited[nb[q]] && nb[q] < trig) trig = nb[q];
-    if (trig == INT64_MAX) return;
+    return trig;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_early_flag(FtleDev d, int64_t n, int *__restrict__ flags) {
+    const int64_t P = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (P >= n) return;
+    int fl = 0;
+    if (!d.have[P]) {
+        int cc[3];
+        int64_t nb[6];
+        bool has[6];
+        grid_nbrs(d, P, cc, nb, has);
+        fl = early_trigger(d, P, nb, has) != INT64_MAX;
+    }
+    flags[P] = fl;
+}
+
+__device__ inline double min_exit_time(const EarlyBuf &e, int64_t t, double te) {
+    const double *lt = e.lt + 6 * t;
+    return dmin(te, dmin(lt[5], dmin(lt[4], dmin(lt[3], dmin(lt[2], dmin(lt[1], lt[0]))))));
+}
+
+/* pass A: lane = (triggered point, neighbour q) */
+template <int MODE, int DIM, int UNS>
+__global__ __launch_bounds__(BLOCK) void k_early_a(DevMesh m, DevField f, FtleDev d, EarlyBuf e, double t1, int itype,
+                                                  double dt, ErrBuf eb) {
+#pragma clang fp contract(off)
+    const int64_t task = xcd_block() * blockDim.x + threadIdx.x;
+    if (task >= 6 * e.nt) return;
+    const int64_t t = task / 6;
+    const int q = (int)(task % 6);
+    const int64_t P = e.list[t];
+    int cc[3];
+    int64_t nb[6];
+    bool has[6];
+    grid_nbrs(d, P, cc, nb, has);
+    const int64_t trig = early_trigger(d, P, nb, has);
     const double te = d.exit_time[trig];
-    int err = 0;
-    double lt[6];
-    for (int q = 0; q < 6; q++) {
-        lt[q] = te;
-        if (!has[q]) continue;
+    double lt = te;
+    if (has[q]) {
+        int err = 0;
         PState s;
         advect_copy<MODE, DIM, UNS>(m, f, d, nb[q], trig, t1, te, itype, dt, s, err);
-        if (s.left) lt[q] = s.left_time;
+        if (err) raise_err(eb, err, P);
+        if (s.left) lt = s.left_time;
     }
-    double mt = dmin(te, dmin(lt[5], dmin(lt[4], dmin(lt[3], dmin(lt[2], dmin(lt[1], lt[0]))))));
-    PState self;
-    advect_copy<MODE, DIM, UNS>(m, f, d, P, trig, t1, mt, itype, dt, self, err);
+    e.lt[task] = lt;
+}
+
+/* pass B: lane = (triggered point, r): r = 0 the point itself, r = 1 + 2 ax
+ * + side its lower (side 0) or upper (side 1) neighbour along axis ax, when
+ * the difference formula for that axis uses it */
+template <int MODE, int DIM, int UNS>
+__global__ __launch_bounds__(BLOCK) void k_early_b(DevMesh m, DevField f, FtleDev d, EarlyBuf e, double t1, int itype,
+                                                  double dt, ErrBuf eb) {
+#pragma clang fp contract(off)
+    const int64_t task = xcd_block() * blockDim.x + threadIdx.x;
+    if (task >= 7 * e.nt) return;
+    const int64_t t = task / 7;
+    const int r = (int)(task % 7);
+    const int64_t P = e.list[t];
+    int cc[3];
+    int64_t nb[6];
+    bool has[6];
+    grid_nbrs(d, P, cc, nb, has);
+    int64_t who = P;
+    if (r > 0) {
+        const int ax = (r - 1) >> 1, side = (r - 1) & 1;
+        if (ax >= DIM) return;
+        const int last = d.res[ax] - 1;
+        /* one-sided at the grid edges (ftle.c:994-1020) */
+        if (side == 0 && cc[ax] == 0) return;
+        if (side == 1 && cc[ax] != 0 && cc[ax] == last) return;
+        who = nb[2 * ax + side];
+    }
+    const int64_t trig = early_trigger(d, P, nb, has);
+    const double mt = min_exit_time(e, t, d.exit_time[trig]);
+    int err = 0;
+    PState s;
+    advect_copy<MODE, DIM, UNS>(m, f, d, who, trig, t1, mt, itype, dt, s, err);
+    if (err) raise_err(eb, err, P);
+    for (int c = 0; c < 3; c++) e.pos[3 * task + c] = s.X[c];
+}
+
+/* pass C: gradient and FTLE of each triggered point */
+__global__ __launch_bounds__(BLOCK) void k_early_c(FtleDev d, EarlyBuf e, int dims, ErrBuf eb) {
+#pragma clang fp contract(off)
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= e.nt) return;
+    const int64_t P = e.list[t];
+    int cc[3];
+    int64_t nb[6];
+    bool has[6];
+    grid_nbrs(d, P, cc, nb, has);
+    const int64_t trig = early_trigger(d, P, nb, has);
+    const double mt = min_exit_time(e, t, d.exit_time[trig]);
+    const double *self = e.pos + 21 * t;
     double lo[3][3], hi[3][3], fac[3] = {1.0, 1.0, 1.0};
-    int cc[3] = {i, j, k};
-    for (int ax = 0; ax < DIM; ax++) {
-        int last = d.res[ax] - 1;
-        PState a, b;
-        if (cc[ax] == 0) {
-            advect_copy<MODE, DIM, UNS>(m, f, d, nb[2 * ax + 1], trig, t1, mt, itype, dt, b, err);
-            for (int c = 0; c < 3; c++) {
-                lo[ax][c] = self.X[c];
-                hi[ax][c] = b.X[c];
+    for (int ax = 0; ax < dims; ax++) {
+        const double *a = e.pos + 21 * t + 3 * (1 + 2 * ax), *b = a + 3;
+        const int last = d.res[ax] - 1;
+        for (int c = 0; c < 3; c++) {
+            if (cc[ax] == 0) {
+                lo[ax][c] = self[c];
+                hi[ax][c] = b[c];
+            } else if (cc[ax] == last) {
+                lo[ax][c] = a[c];
+                hi[ax][c] = self[c];
+            } else {
+                lo[ax][c] = a[c];
+                hi[ax][c] = b[c];
             }
-        } else if (cc[ax] == last) {
-            advect_copy<MODE, DIM, UNS>(m, f, d, nb[2 * ax], trig, t1, mt, itype, dt, a, err);
-            for (int c = 0; c < 3; c++) {
-                lo[ax][c] = a.X[c];
-                hi[ax][c] = self.X[c];
-            }
-        } else {
-            advect_copy<MODE, DIM, UNS>(m, f, d, nb[2 * ax], trig, t1, mt, itype, dt, a, err);
-            advect_copy<MODE, DIM, UNS>(m, f, d, nb[2 * ax + 1], trig, t1, mt, itype, dt, b, err);
-            for (int c = 0; c < 3; c++) {
-                lo[ax][c] = a.X[c];
-                hi[ax][c] = b.X[c];
-            }
-            fac[ax] = 2.0;
         }
+        if (cc[ax] != 0 && cc[ax] != last) fac[ax] = 2.0;
     }
     double D[3][3];
-    gradient(lo, hi, fac, d.del, DIM, D);
+    gradient(lo, hi, fac, d.del, dims, D);
+    int err = 0;
     double wt, wot;
     ftle_value(D, fabs(mt - d.start_time), wt, wot, err, fabs(mt - d.start_time) < TINY);
     if (err) raise_err(eb, err, P);
@@ -783,16 +944,69 @@ __global__ __launch_bounds__(BLOCK) void k_ftle_mask(FtleDev d, int64_t n) {
     d.advected[P] = 0;
 }
 
-/* active list: points integrated this frame (ftle.c:113) */
-__global__ __launch_bounds__(BLOCK) void k_ftle_flags(FtleDev d, int64_t n, int extrap, int *__restrict__ flags) {
-    int64_t P = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (P >= n) return;
-    flags[P] = d.p.left_time[P] > -TINY && (!d.p.left[P] || extrap);
+/* active list: points integrated this frame (ftle.c:113), flagged in the
+ * grid's Z-order so that the compacted list is Z-ordered too */
+__global__ __launch_bounds__(BLOCK) void k_ftle_flags(FtleDev d, int64_t n, int extrap, const int *__restrict__ zperm,
+                                                     int *__restrict__ flags) {
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int64_t P = zperm[q];
+    flags[q] = d.p.left_time[P] > -TINY && (!d.p.left[P] || extrap);
 }
 
-int launch_ftle_mask(fvc_ftle *f) {
-    fvc_ctx *c = f->ctx;
-    k_ftle_mask<<<grid_for(f->n, BLOCK), BLOCK, 0, c->stream>>>(ftle_dev(f), f->n);
+int launch_ftle_mask_dev(fvc_ctx *c, const FtleDev &d, int64_t n) {
+    if (n <= 0) return FVC_OK;
+    k_ftle_mask<<<grid_for(n, BLOCK), BLOCK, 0, c->stream>>>(d, n);
+    FVC_HIP(hipGetLastError());
+    return FVC_OK;
+}
+
+int launch_ftle_mask(fvc_ftle *f) { return launch_ftle_mask_dev(f->ctx, ftle_dev(f), f->n); }
+
+/* early FTLE of one frame for a grid of n points (after its advection, before
+ * the position update) */
+int launch_ftle_early(fvc_ctx *c, const FtleDev &d, int64_t n, const DevField &fld, double t1, int itype, double dt) {
+    /* triggered points: flags in the first half of the scratch, the list in the second */
+    int *eflags = (int *)scratch(c, sizeof(int) * 2 * (size_t)n);
+    if (!eflags) return FVC_E_NOMEM;
+    int *elist = eflags + n;
+    k_early_flag<<<grid_for(n, BLOCK), BLOCK, 0, c->stream>>>(d, n, eflags);
+    FVC_HIP(hipGetLastError());
+    int64_t nt = 0;
+    int rc = launch_compact(c, eflags, n, elist, &nt);
+    if (rc || nt == 0) return rc;
+    EarlyBuf e;
+    e.list = elist;
+    e.nt = nt;
+    FVC_HIP(hipMallocAsync((void **)&e.lt, sizeof(double) * 27 * (size_t)nt, c->stream));
+    e.pos = e.lt + 6 * nt;
+    FVC_DISPATCH(c->precision, c->mesh.dims, c->mesh.mesh_type, k_early_a,
+                 <<<grid_for(6 * nt, BLOCK), BLOCK, 0, c->stream>>>(c->mesh, fld, d, e, t1, itype, dt, errbuf(c)));
+    FVC_DISPATCH(c->precision, c->mesh.dims, c->mesh.mesh_type, k_early_b,
+                 <<<grid_for(7 * nt, BLOCK), BLOCK, 0, c->stream>>>(c->mesh, fld, d, e, t1, itype, dt, errbuf(c)));
+    k_early_c<<<grid_for(nt, BLOCK), BLOCK, 0, c->stream>>>(d, e, c->mesh.dims, errbuf(c));
+    FVC_HIP(hipGetLastError());
+    FVC_HIP(hipFreeAsync(e.lt, c->stream));
+    return FVC_OK;
+}
+
+/* position update over a whole grid (points advected this frame) */
+__global__ __launch_bounds__(BLOCK) void k_ftle_update_all(FtleDev d, int64_t n, int uns) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !d.advected[i]) return;
+    if (d.exited[i]) {
+        d.p.left[i] = 1;
+    } else {
+        for (int c = 0; c < 3; c++) d.p.X[c][i] = d.Xn[c][i];
+        if (uns) d.p.elem[i] = d.elem_new[i];
+    }
+    d.exited[i] = 0;
+    d.advected[i] = 0;
+}
+
+int launch_ftle_update_all(fvc_ctx *c, const FtleDev &d, int64_t n) {
+    if (n <= 0) return FVC_OK;
+    k_ftle_update_all<<<grid_for(n, BLOCK), BLOCK, 0, c->stream>>>(d, n, c->mesh.mesh_type);
     FVC_HIP(hipGetLastError());
     return FVC_OK;
 }
@@ -805,9 +1019,9 @@ int launch_ftle_frame(fvc_ftle *f, int lo, int hi, double tmin, double tmax, dou
     int itype = ip.int_type == FVC_INT_RK4 ? 1 : 0;
     int *flags = (int *)scratch(c, sizeof(int) * (size_t)f->n);
     if (!flags) return FVC_E_NOMEM;
-    k_ftle_flags<<<grid_for(f->n, BLOCK), BLOCK, 0, c->stream>>>(d, f->n, ip.extrapolate, flags);
+    k_ftle_flags<<<grid_for(f->n, BLOCK), BLOCK, 0, c->stream>>>(d, f->n, ip.extrapolate, f->zperm, flags);
     FVC_HIP(hipGetLastError());
-    int rc = launch_compact(c, flags, f->n, f->active, &f->nactive);
+    int rc = launch_compact(c, flags, f->n, f->active, &f->nactive, f->zperm);
     if (rc) return rc;
     FVC_HIP(hipMemsetAsync(c->d_steps, 0, sizeof(unsigned long long), c->stream));
     if (f->nactive > 0) {
@@ -826,12 +1040,7 @@ int launch_ftle_frame(fvc_ftle *f, int lo, int hi, double tmin, double tmax, dou
                                                                                 errbuf(c), c->d_steps));
         timing_end(c);
         FVC_HIP(hipGetLastError());
-        if (!ip.extrapolate) {
-            FVC_DISPATCH(c->precision, c->mesh.dims, c->mesh.mesh_type, k_ftle_early,
-                         <<<grid_for(f->n, BLOCK), BLOCK, 0, c->stream>>>(c->mesh, fld, d, f->n, t1, itype,
-                                                                          ip.time_step, errbuf(c)));
-            FVC_HIP(hipGetLastError());
-        }
+        if (!ip.extrapolate && (rc = launch_ftle_early(c, d, f->n, fld, t1, itype, ip.time_step))) return rc;
         k_ftle_update<<<grid_for(f->nactive, BLOCK), BLOCK, 0, c->stream>>>(d, f->active, f->nactive,
                                                                             c->mesh.mesh_type, ip.extrapolate);
         FVC_HIP(hipGetLastError());
@@ -862,11 +1071,17 @@ int launch_copy_probe(fvc_ctx *c, void *dst, const void *src, int64_t bytes) {
     return FVC_OK;
 }
 
+int launch_ftle_fill_dev(fvc_ctx *c, const FtleDev &d, int64_t n, double int_t) {
+    if (n <= 0) return FVC_OK;
+    k_ftle_fill<<<grid_for(n, BLOCK), BLOCK, 0, c->stream>>>(d, n, c->mesh.dims, int_t, errbuf(c));
+    FVC_HIP(hipGetLastError());
+    return FVC_OK;
+}
+
 int launch_ftle_fill(fvc_ftle *f, double int_t) {
     fvc_ctx *c = f->ctx;
-    k_ftle_fill<<<grid_for(f->n, BLOCK), BLOCK, 0, c->stream>>>(ftle_dev(f), f->n, c->mesh.dims, int_t, errbuf(c));
-    FVC_HIP(hipGetLastError());
-    return check_err(c, "fvc_ftle_output");
+    int rc = launch_ftle_fill_dev(c, ftle_dev(f), f->n, int_t);
+    return rc ? rc : check_err(c, "fvc_ftle_output");
 }
 
 /* ------------------------------------------------------------------------ */
@@ -914,7 +1129,8 @@ __global__ __launch_bounds__(1024) void k_scan_small(int *__restrict__ a, int n)
 }
 
 __global__ __launch_bounds__(BLOCK) void k_cmp_scatter(const int *__restrict__ flags, int64_t n,
-                                                      const int *__restrict__ offs, int *__restrict__ out) {
+                                                      const int *__restrict__ offs, const int *__restrict__ map,
+                                                      int *__restrict__ out) {
     __shared__ int wsum[BLOCK / 64];
     __shared__ int run;
     if (threadIdx.x == 0) run = offs[blockIdx.x];
@@ -930,14 +1146,14 @@ __global__ __launch_bounds__(BLOCK) void k_cmp_scatter(const int *__restrict__ f
         __syncthreads();
         int pre = 0;
         for (int q = 0; q < w; q++) pre += wsum[q];
-        if (fl) out[run + pre + before] = (int)i;
+        if (fl) out[run + pre + before] = map ? map[i] : (int)i;
         __syncthreads();
         if (threadIdx.x == 0) run += wsum[0] + wsum[1] + wsum[2] + wsum[3];
         __syncthreads();
     }
 }
 
-int launch_compact(fvc_ctx *c, const int *flags, int64_t n, int *out_idx, int64_t *count) {
+int launch_compact(fvc_ctx *c, const int *flags, int64_t n, int *out_idx, int64_t *count, const int *map) {
     int nb = (int)((n + CMP_TILE - 1) / CMP_TILE);
     if (nb == 0) {
         *count = 0;
@@ -947,7 +1163,7 @@ int launch_compact(fvc_ctx *c, const int *flags, int64_t n, int *out_idx, int64_
     FVC_HIP(hipMallocAsync((void **)&offs, sizeof(int) * (nb + 1), c->stream));
     k_cmp_count<<<nb, BLOCK, 0, c->stream>>>(flags, n, offs);
     k_scan_small<<<1, 1024, 0, c->stream>>>(offs, nb);
-    k_cmp_scatter<<<nb, BLOCK, 0, c->stream>>>(flags, n, offs, out_idx);
+    k_cmp_scatter<<<nb, BLOCK, 0, c->stream>>>(flags, n, offs, map, out_idx);
     int total = 0;
     FVC_HIP(hipMemcpyAsync(&total, offs + nb, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     FVC_HIP(hipFreeAsync(offs, c->stream));
@@ -985,6 +1201,16 @@ __global__ __launch_bounds__(BLOCK) void k_morton(const double *__restrict__ xyz
     /* x bit most significant within each level, then y, then z */
     keys[i] = (spread3(q[0]) << 2) | (spread3(q[1]) << 1) | spread3(q[2]);
     vals[i] = (int)i;
+}
+
+/* grid point P = (i*r1 + j)*r2 + k (k fastest) keyed by its integer (i,j,k) */
+__global__ __launch_bounds__(BLOCK) void k_grid_morton(int r1, int r2, int64_t n, uint64_t *__restrict__ keys,
+                                                      int *__restrict__ vals) {
+    int64_t P = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (P >= n) return;
+    const uint64_t k = (uint64_t)(P % r2), j = (uint64_t)((P / r2) % r1), i = (uint64_t)(P / ((int64_t)r2 * r1));
+    keys[P] = (spread3(i) << 2) | (spread3(j) << 1) | spread3(k);
+    vals[P] = (int)P;
 }
 
 constexpr int RS_TILE = 4096;
@@ -1066,20 +1292,11 @@ __global__ __launch_bounds__(1024) void k_rs_scan(int *__restrict__ a, int total
     }
 }
 
-int launch_zsort(fvc_ctx *c, const double *dxyz, int64_t n, const double lo[3], const double hi[3], int *dperm) {
-    if (n <= 0) return FVC_OK;
-    if (n > INT32_MAX / 2) return FVC_E_INVALID;
-    double sc[3];
-    for (int a = 0; a < 3; a++) sc[a] = hi[a] > lo[a] ? 2097152.0 / (hi[a] - lo[a]) : 0.0;
-    int nb = (int)((n + RS_TILE - 1) / RS_TILE);
-    uint64_t *k0 = nullptr, *k1 = nullptr;
-    int *v0 = nullptr, *v1 = nullptr, *hist = nullptr;
-    FVC_HIP(hipMallocAsync((void **)&k0, sizeof(uint64_t) * n, c->stream));
-    FVC_HIP(hipMallocAsync((void **)&k1, sizeof(uint64_t) * n, c->stream));
-    FVC_HIP(hipMallocAsync((void **)&v0, sizeof(int) * n, c->stream));
-    FVC_HIP(hipMallocAsync((void **)&v1, sizeof(int) * n, c->stream));
+/* stable sort of (key, value) pairs by key; dperm receives the values */
+static int sort_pairs(fvc_ctx *c, uint64_t *k0, int *v0, int64_t n, int *dperm, uint64_t *k1, int *v1) {
+    const int nb = (int)((n + RS_TILE - 1) / RS_TILE);
+    int *hist = nullptr;
     FVC_HIP(hipMallocAsync((void **)&hist, sizeof(int) * 256 * (size_t)nb, c->stream));
-    k_morton<<<grid_for(n, BLOCK), BLOCK, 0, c->stream>>>(dxyz, n, lo[0], lo[1], lo[2], sc[0], sc[1], sc[2], k0, v0);
     for (int shift = 0; shift < 64; shift += 8) {
         k_rs_hist<<<nb, BLOCK, 0, c->stream>>>(k0, n, shift, nb, hist);
         k_rs_scan<<<1, 1024, 0, c->stream>>>(hist, 256 * nb);
@@ -1089,13 +1306,46 @@ int launch_zsort(fvc_ctx *c, const double *dxyz, int64_t n, const double lo[3], 
     }
     FVC_HIP(hipGetLastError());
     FVC_HIP(hipMemcpyAsync(dperm, v0, sizeof(int) * n, hipMemcpyDeviceToDevice, c->stream));
-    FVC_HIP(hipFreeAsync(k0, c->stream));
-    FVC_HIP(hipFreeAsync(k1, c->stream));
-    FVC_HIP(hipFreeAsync(v0, c->stream));
-    FVC_HIP(hipFreeAsync(v1, c->stream));
     FVC_HIP(hipFreeAsync(hist, c->stream));
+    return FVC_OK;
+}
+
+static int zsort_keys(fvc_ctx *c, int64_t n, int *dperm, const double *dxyz, const double lo[3], const double hi[3],
+                      const int32_t *res) {
+    uint64_t *k[2] = {nullptr, nullptr};
+    int *v[2] = {nullptr, nullptr};
+    FVC_HIP(hipMallocAsync((void **)&k[0], sizeof(uint64_t) * n, c->stream));
+    FVC_HIP(hipMallocAsync((void **)&k[1], sizeof(uint64_t) * n, c->stream));
+    FVC_HIP(hipMallocAsync((void **)&v[0], sizeof(int) * n, c->stream));
+    FVC_HIP(hipMallocAsync((void **)&v[1], sizeof(int) * n, c->stream));
+    if (dxyz) {
+        double sc[3];
+        for (int a = 0; a < 3; a++) sc[a] = hi[a] > lo[a] ? 2097152.0 / (hi[a] - lo[a]) : 0.0;
+        k_morton<<<grid_for(n, BLOCK), BLOCK, 0, c->stream>>>(dxyz, n, lo[0], lo[1], lo[2], sc[0], sc[1], sc[2], k[0],
+                                                              v[0]);
+    } else {
+        k_grid_morton<<<grid_for(n, BLOCK), BLOCK, 0, c->stream>>>(res[1], res[2], n, k[0], v[0]);
+    }
+    int rc = sort_pairs(c, k[0], v[0], n, dperm, k[1], v[1]);
+    for (int a = 0; a < 2; a++) {
+        (void)hipFreeAsync(k[a], c->stream);
+        (void)hipFreeAsync(v[a], c->stream);
+    }
+    if (rc) return rc;
     FVC_HIP(hipStreamSynchronize(c->stream));
     return FVC_OK;
+}
+
+int launch_zsort(fvc_ctx *c, const double *dxyz, int64_t n, const double lo[3], const double hi[3], int *dperm) {
+    if (n <= 0) return FVC_OK;
+    if (n > INT32_MAX / 2) return FVC_E_INVALID;
+    return zsort_keys(c, n, dperm, dxyz, lo, hi, nullptr);
+}
+
+int launch_grid_zorder(fvc_ctx *c, const int32_t res[3], int64_t n, int *dperm) {
+    if (n <= 0) return FVC_OK;
+    if (n > INT32_MAX / 2 || res[0] > (1 << 21) || res[1] > (1 << 21) || res[2] > (1 << 21)) return FVC_E_INVALID;
+    return zsort_keys(c, n, dperm, nullptr, nullptr, nullptr, res);
 }
 
 } // namespace fvc

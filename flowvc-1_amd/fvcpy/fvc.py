@@ -45,6 +45,30 @@ class BatchParams(ctypes.Structure):
                 ("waves_per_cu", ctypes.c_int32)]
 
 
+class ServerParams(ctypes.Structure):
+    _fields_ = [("frame_t0", ctypes.c_double), ("frame_dt", ctypes.c_double), ("nslots", ctypes.c_int32),
+                ("time_step", ctypes.c_double), ("pool_particles", ctypes.c_int64), ("hbm_fraction", ctypes.c_double),
+                ("max_requests", ctypes.c_int32), ("block", ctypes.c_int32), ("steps_per_launch", ctypes.c_int32),
+                ("lsc", ctypes.c_int32)]
+
+
+class RequestDesc(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("int_type", ctypes.c_int32), ("t_start", ctypes.c_double),
+                ("t_stop", ctypes.c_double), ("n", ctypes.c_int64), ("xyz", ctypes.c_void_p),
+                ("ftle_min", ctypes.c_double * 3), ("ftle_max", ctypes.c_double * 3), ("ftle_res", ctypes.c_int32 * 3),
+                ("pad_", ctypes.c_int32)]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_int32), ("kind", ctypes.c_int32), ("n", ctypes.c_int64),
+                ("particle_steps", ctypes.c_int64), ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double),
+                ("t_done", ctypes.c_double)]
+
+
+REQ_TRACER, REQ_FTLE = 0, 1
+RS_QUEUED, RS_RUNNING, RS_DONE, RS_FAILED = 0, 1, 2, 3
+
+
 class FvcError(RuntimeError):
     def __init__(self, code, what, detail=""):
         self.code = code
@@ -101,6 +125,15 @@ def lib():
         L.fvc_batch_reset.argtypes = [vp]
         L.fvc_batch_run.argtypes = [vp, ctypes.POINTER(BatchParams), ctypes.POINTER(i64), ctypes.POINTER(i64)]
         L.fvc_batch_tune.argtypes = [vp, vp, i32, vp, i32, i32, ctypes.POINTER(BatchParams), ctypes.POINTER(d)]
+        L.fvc_server_create.argtypes = [vp, ctypes.POINTER(ServerParams), ctypes.POINTER(vp)]
+        L.fvc_server_destroy.argtypes = [vp]
+        L.fvc_server_capacity.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]
+        L.fvc_server_submit.argtypes = [vp, ctypes.POINTER(RequestDesc), ctypes.POINTER(i64)]
+        L.fvc_server_poll.argtypes = [vp, ctypes.POINTER(i32), ctypes.POINTER(i32)]
+        L.fvc_server_clock.restype = d
+        L.fvc_server_clock.argtypes = [vp]
+        L.fvc_server_info.argtypes = [vp, i64, ctypes.POINTER(RequestInfo)]
+        L.fvc_server_take.argtypes = [vp, i64, vp, vp, vp]
         _LIB = L
     return _LIB
 
@@ -350,6 +383,84 @@ class Batch:
     def close(self):
         if self.h:
             lib().fvc_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching request server over a context's resident slices
+    (fvc_server_*)."""
+
+    def __init__(self, ctx, frame_t0, frame_dt, nslots, dt, pool_particles=0, hbm_fraction=0.5, max_requests=256,
+                 block=256, steps_per_launch=32, lsc=1):
+        self.ctx = ctx
+        sp = ServerParams(frame_t0, frame_dt, nslots, dt, pool_particles, hbm_fraction, max_requests, block,
+                          steps_per_launch, lsc)
+        h = ctypes.c_void_p()
+        _chk(lib().fvc_server_create(ctx.h, ctypes.byref(sp), ctypes.byref(h)), "fvc_server_create", ctx.h)
+        self.h = h
+        self._keep = {}
+
+    def capacity(self):
+        n, b = ctypes.c_int64(), ctypes.c_int64()
+        _chk(lib().fvc_server_capacity(self.h, ctypes.byref(n), ctypes.byref(b)), "fvc_server_capacity", self.ctx.h)
+        return n.value, b.value
+
+    def submit_tracer(self, xyz, t_start, t_stop, int_type=INT_RK4):
+        xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+        rq = RequestDesc(REQ_TRACER, int_type, t_start, t_stop, xyz.shape[0], _p(xyz).value)
+        rid = ctypes.c_int64()
+        _chk(lib().fvc_server_submit(self.h, ctypes.byref(rq), ctypes.byref(rid)), "fvc_server_submit", self.ctx.h)
+        return rid.value
+
+    def submit_ftle(self, mn, mx, res, t_start, t_stop):
+        rq = RequestDesc(REQ_FTLE, INT_RK4, t_start, t_stop, 0, None)
+        for a in range(3):
+            rq.ftle_min[a], rq.ftle_max[a], rq.ftle_res[a] = float(mn[a]), float(mx[a]), int(res[a])
+        rid = ctypes.c_int64()
+        _chk(lib().fvc_server_submit(self.h, ctypes.byref(rq), ctypes.byref(rid)), "fvc_server_submit", self.ctx.h)
+        return rid.value
+
+    def poll(self):
+        r, q = ctypes.c_int32(), ctypes.c_int32()
+        _chk(lib().fvc_server_poll(self.h, ctypes.byref(r), ctypes.byref(q)), "fvc_server_poll", self.ctx.h)
+        return r.value, q.value
+
+    def clock(self):
+        return lib().fvc_server_clock(self.h)
+
+    def info(self, rid):
+        inf = RequestInfo()
+        _chk(lib().fvc_server_info(self.h, rid, ctypes.byref(inf)), "fvc_server_info", self.ctx.h)
+        return {"state": inf.state, "kind": inf.kind, "n": inf.n, "particle_steps": inf.particle_steps,
+                "t_submit": inf.t_submit, "t_admit": inf.t_admit, "t_done": inf.t_done}
+
+    def take(self, rid):
+        inf = self.info(rid)
+        if inf["kind"] == REQ_TRACER:
+            out = np.zeros(inf["n"], POINT_DTYPE)
+            _chk(lib().fvc_server_take(self.h, rid, _p(out), None, None), "fvc_server_take", self.ctx.h)
+            return out
+        wt = np.empty(inf["n"], np.float64)
+        wot = np.empty(inf["n"], np.float64)
+        _chk(lib().fvc_server_take(self.h, rid, None, _p(wt), _p(wot)), "fvc_server_take", self.ctx.h)
+        return wt, wot
+
+    def run_until_idle(self, max_ticks=1000000):
+        for _ in range(max_ticks):
+            r, q = self.poll()
+            if r == 0 and q == 0:
+                return
+        raise RuntimeError("server still busy after %d ticks" % max_ticks)
+
+    def close(self):
+        if self.h:
+            lib().fvc_server_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -237,6 +237,74 @@ int fvc_batch_run(fvc_batch *b, const fvc_batch_params *bp, int64_t *particle_st
 int fvc_batch_tune(fvc_batch *b, const int32_t *blocks, int32_t nblocks, const int32_t *budgets, int32_t nbudgets,
                    int32_t use_graph, fvc_batch_params *best, double *best_ms);
 
+/* ---- request serving (no reference counterpart: ompVC runs one FTLE slide
+ *      or tracer job per process, ftle.c:24-377 / tracers.c:25-289) -------- */
+/* A server keeps the context's velocity slices resident and runs tracer and
+ * FTLE requests with continuous batching: requests are admitted while others
+ * run, as soon as their particles fit in a device pool sized from free HBM,
+ * and the particles of every running request advance together in one
+ * persistent kernel (the fvc_batch_* kernel extended with FTLE frames).  Each
+ * request gives the result it gives when run alone:
+ *   tracer: seeds located as fvc_seed_list does (chain in input order), then
+ *           what fvc_batch_run does for one release; result = the records;
+ *   FTLE:   the grid of fvc_ftle_create (start = t_start, stop = t_stop), the
+ *           frames fvc_ftle_frame runs from the frame holding t_start, then
+ *           fvc_ftle_output(t_stop - t_start); RK4 only.
+ * Data frame f spans [t0 + f dt, t0 + (f+1) dt] over slots f mod nslots and
+ * (f+1) mod nslots, as for fvc_batch_create.  One fvc_server_poll call runs
+ * one scheduling tick: finish FTLE frames and retire completed requests,
+ * admit queued ones that fit, run one launch of the persistent kernel. */
+enum fvc_request_kind { FVC_REQ_TRACER = 0, FVC_REQ_FTLE = 1 };
+enum fvc_request_state { FVC_RS_QUEUED = 0, FVC_RS_RUNNING = 1, FVC_RS_DONE = 2, FVC_RS_FAILED = 3 };
+
+typedef struct fvc_server_params {
+    double frame_t0, frame_dt; /* data frames as for fvc_batch_create */
+    int32_t nslots;
+    double time_step;
+    int64_t pool_particles;    /* 0: from free HBM x hbm_fraction */
+    double hbm_fraction;       /* share of free HBM the pool may take (0 = 0.5) */
+    int32_t max_requests;      /* running at once (0 = 256) */
+    int32_t block;             /* as fvc_batch_params */
+    int32_t steps_per_launch;  /* step budget per particle per tick (0 = 32) */
+    int32_t lsc;               /* LocalSearchChecking for seed location */
+} fvc_server_params;
+
+typedef struct fvc_request_desc {
+    int32_t kind;     /* enum fvc_request_kind */
+    int32_t int_type; /* tracer: RK4, Euler, AB2, AB3; FTLE: RK4 */
+    double t_start, t_stop;
+    int64_t n;        /* tracer: seed count */
+    const double *xyz;/* tracer seeds (n x 3), copied by fvc_server_submit */
+    double ftle_min[3], ftle_max[3];
+    int32_t ftle_res[3];
+    int32_t pad_;
+} fvc_request_desc;
+
+typedef struct fvc_request_info {
+    int32_t state;       /* enum fvc_request_state */
+    int32_t kind;
+    int64_t n;           /* tracer: records; FTLE: grid points */
+    int64_t particle_steps;
+    double t_submit, t_admit, t_done; /* seconds on the server's steady clock */
+} fvc_request_info;
+
+typedef struct fvc_server fvc_server;
+int fvc_server_create(fvc_ctx *ctx, const fvc_server_params *sp, fvc_server **out);
+void fvc_server_destroy(fvc_server *s);
+/* Pool capacity in particles and its bytes. */
+int fvc_server_capacity(fvc_server *s, int64_t *particles, int64_t *bytes);
+/* Queue a request (inputs are copied); *id identifies it from then on. */
+int fvc_server_submit(fvc_server *s, const fvc_request_desc *rq, int64_t *id);
+/* One scheduling tick; returns counts of running and queued requests. */
+int fvc_server_poll(fvc_server *s, int32_t *running, int32_t *queued);
+/* Server clock (seconds), the clock of fvc_request_info's times. */
+double fvc_server_clock(fvc_server *s);
+int fvc_server_info(fvc_server *s, int64_t id, fvc_request_info *info);
+/* Results of a finished request (then forgotten): tracer records (info.n of
+ * them, input order) or FTLE values with and without the time factor (k, j,
+ * i order, as fvc_ftle_output). */
+int fvc_server_take(fvc_server *s, int64_t id, fvc_point *records, double *ftle_wt, double *ftle_wot);
+
 /* ---- Z-order seed sort (cudaVC/ZOrderSort/main.cpp:86-101 ordering) ------- */
 /* Sort n points by the interleaved-bit (x most significant) order of their
  * coordinates quantised to 21 bits on [lo, hi]; perm receives the stable order. */

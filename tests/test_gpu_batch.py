@@ -83,7 +83,7 @@ def run_one_at_a_time(ctx, recs, rel):
             tmax = tmin + FDT
             steps += P.advect(max(ts, tmin), min(te, tmax), f % NSLOTS, (f + 1) % NSLOTS, tmin, tmax, DT,
                               int_type=it, begin=b, count=c)
-            if not (tmax < te):
+            if te - tmax < 1e-10:
                 break
             f += 1
     return P.get(), steps
@@ -118,7 +118,7 @@ def run_oracle(om, slices, recs, rel):
             else:
                 rc = om.advect(pts, a, bb, DT, int_type=it)
             assert rc == 0
-            if not (tmax < te):
+            if te - tmax < 1e-10:
                 break
             f += 1
         out[b:b + c] = pts
