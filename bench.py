@@ -33,6 +33,22 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` on the FP32 pipe workload, from the
+    committed FETCH_SIZE / WRITE_SIZE counter passes (profiles/pmc_traffic_pipe.md,
+    scaled by the copy-kernel calibration there); None when not measured."""
+    path = os.path.join(HERE, "profiles", "pmc_traffic_pipe.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                w = line.split()
+                if len(w) == 3 and w[0] == "traffic_bytes_per_launch" and w[1] == kernel:
+                    return int(w[2])
+    except OSError:
+        pass
+    return None
+
+
 def build_inputs(args, host, np, with_seeds=True):
     t0 = time.time()
     tube = host.pipe(args.nx, args.nx, args.nz, 1.0, 10.0)
@@ -457,7 +473,8 @@ def bench_traffic(args, np, fvc, host):
     """Short run for hardware-counter passes: a plain device copy of 1 GiB
     (k_copy_probe: reads and writes 1 GiB each, the calibration of the byte
     counters) followed by `--frames` frames of the pipe tracer pass (k_advect)
-    on the full mesh and tracer set."""
+    and of the FTLE slide (k_ftle_advect) on the full mesh, tracer set and
+    grid."""
     tube, xyz, conn, adj, slices, seeds = build_inputs(args, host, np)
     ctx = make_context(fvc, args.precision, xyz, conn, adj, slices)
     lo, hi = np.array([-1.0, -1.0, 0.0]), np.array([1.0, 1.0, 10.0])
@@ -472,10 +489,19 @@ def bench_traffic(args, np, fvc, host):
         if k >= args.frames:
             break
         steps += P.advect(tmin, tmax, lo_, hi_, tmin, tmax, 1e-3)
+    res = [int(v) for v in args.ftle.split(",")]
+    F = fvc.FtleSlide(ctx, [-1.0, -1.0, 0.0], [1.0, 1.0, 10.0], res, 1, 0.0, 1.0)
+    fsteps = 0
+    for k, (tmin, tmax, lo_, hi_) in enumerate(frames()):
+        if k >= args.frames:
+            break
+        F.frame(1, lo_, hi_, tmin, tmax, 1e-3)
+        fsteps += ctx.last_steps()
     fvc.lib().fvc_sync(ctx.h)
     print(json.dumps({"workload": "traffic", "copy_bytes": 1 << 30, "frames": args.frames,
                       "tracers": int(recs.shape[0]), "particle_steps": int(steps),
-                      "particle_steps_per_launch": steps / max(1, args.frames)}), flush=True)
+                      "particle_steps_per_launch": steps / max(1, args.frames), "ftle_points": F.n,
+                      "ftle_particle_steps_per_launch": fsteps / max(1, args.frames)}), flush=True)
 
 
 def main():
@@ -580,7 +606,7 @@ def main():
     value = total_steps / (wall / K)
     ftle_seeds = F.inside
     # dominant kernel for the roofline
-    dom = ("k_advect (tracers)", k_tr) if k_tr[0] >= k_ft[0] else ("k_ftle_advect (FTLE grid)", k_ft)
+    dom = ("k_advect", k_tr) if k_tr[0] >= k_ft[0] else ("k_ftle_advect", k_ft)
     bpp = BYTES_PER_STEP_F32 if args.precision in ("f32",) else (
         512 if args.precision in ("bf16", "f16") else 1328)
     avg_ms = dom[1][0] / max(dom[1][1], 1)
@@ -618,7 +644,9 @@ def main():
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK,
-            "traffic": None,
+            "traffic": measured_traffic(dom[0]) if (args.precision == "f32" and args.tracers == 4_000_000 and
+                                                    res == [128, 128, 640] and args.nx == 60) else None,
+            "traffic_source": "profiles/pmc_traffic_pipe.md (FETCH_SIZE x2.000 + WRITE_SIZE x1.000 per launch)",
             "bytes_per_particle_step": bpp,
             "avg_launch_ms": avg_ms,
             "particle_steps_per_launch": steps_per_launch,

@@ -937,25 +937,41 @@ extern "C" int fvc_ftle_frame(fvc_ftle *f, int32_t dir, int32_t lo, int32_t hi, 
     return FVC_OK;
 }
 
+/* i-major storage (P = (i*r1 + j)*r2 + k) -> k, j, i file order (q = (k*r1 + j)*r0 + i) */
+__global__ void k_ftle_kji(const double *__restrict__ wt, const double *__restrict__ wot, int r0, int r1, int r2,
+                           double *__restrict__ out) {
+    const int64_t n = (int64_t)r0 * r1 * r2;
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int i = (int)(q % r0), j = (int)((q / r0) % r1), k = (int)(q / ((int64_t)r0 * r1));
+    const int64_t P = ((int64_t)i * r1 + j) * r2 + k;
+    out[q] = wt[P];
+    out[n + q] = wot[P];
+}
+
+/* FTLE values of a grid to host memory in the file order of ftle.c:1302-1315 */
+int fvc::copy_ftle_kji(fvc_ctx *c, const double *d_wt, const double *d_wot, const int32_t res[3], double *wt,
+                       double *wot) {
+    const size_t n = (size_t)res[0] * res[1] * res[2];
+    double *tmp = nullptr;
+    FVC_HIP(hipMallocAsync((void **)&tmp, sizeof(double) * 2 * n, c->stream));
+    k_ftle_kji<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(d_wt, d_wot, res[0], res[1], res[2], tmp);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && wt) e = hipMemcpyAsync(wt, tmp, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && wot) e = hipMemcpyAsync(wot, tmp + n, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream);
+    (void)hipFreeAsync(tmp, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        c->last_error = std::string("FTLE output copy: ") + hipGetErrorString(e);
+        return FVC_E_HIP;
+    }
+    return FVC_OK;
+}
+
 extern "C" int fvc_ftle_output(fvc_ftle *f, double int_t, double *wt, double *wot) {
     if (!f) return FVC_E_INVALID;
-    fvc_ctx *c = f->ctx;
     int rc = launch_ftle_fill(f, int_t);
-    if (rc) return rc;
-    size_t n = (size_t)f->n;
-    std::vector<double> a(n), b(n);
-    FVC_HIP(hipMemcpy(a.data(), f->wt, sizeof(double) * n, hipMemcpyDeviceToHost));
-    FVC_HIP(hipMemcpy(b.data(), f->wot, sizeof(double) * n, hipMemcpyDeviceToHost));
-    /* i-major storage -> k,j,i output order (ftle.c:1302-1315) */
-    int64_t q = 0;
-    for (int k = 0; k < f->res[2]; k++)
-        for (int j = 0; j < f->res[1]; j++)
-            for (int i = 0; i < f->res[0]; i++, q++) {
-                int64_t P = ((int64_t)i * f->res[1] + j) * f->res[2] + k;
-                if (wt) wt[q] = a[P];
-                if (wot) wot[q] = b[P];
-            }
-    return FVC_OK;
+    return rc ? rc : copy_ftle_kji(f->ctx, f->wt, f->wot, f->res, wt, wot);
 }
 
 extern "C" int fvc_ftle_points(fvc_ftle *f, int64_t *n, int64_t *inside) {

@@ -117,7 +117,7 @@ __device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : 
 
 /* exact decision on natural coordinates; returns 1 accept, 0 move (slot set), -1 indeterminate */
 template <int DIM>
-__device__ __forceinline__ int decide_exact(double r, double s, double t, int acc, int &slot) {
+__device__ __forceinline__ int decide_exact(double r, double s, double t, int acc, int &slot, int *tie = nullptr) {
 #pragma clang fp contract(off)
     double d, l0;
     if (DIM == 3) {
@@ -127,6 +127,7 @@ __device__ __forceinline__ int decide_exact(double r, double s, double t, int ac
         l0 = 1 - r - s;
         d = dmin(r, dmin(s, l0));
     }
+    if (tie && d == -TINY) *tie = 1; /* the one value on which the two acceptance rules differ */
     bool in = acc == ACC_SEARCH ? (d > -TINY) : ((d + TINY) >= 0);
     if (in) return 1;
     if (fabs(r - d) < TINY) slot = 0;
@@ -199,7 +200,7 @@ __device__ __forceinline__ int decide_fast(float r, float s, float t, float g, i
  * of the accepting element are returned in r,s,t (for interpolation). */
 template <int MODE, int DIM>
 __device__ __forceinline__ int walk(const DevMesh &m, double x, double y, double z, int e, int acc, double &r, double &s,
-                                    double &t, int &err) {
+                                    double &t, int &err, int *tie = nullptr) {
     for (int hop = 0; hop < m.walk_limit; hop++) {
         int slot = -1, dec;
         if constexpr (MODE == P_F64) {
@@ -208,7 +209,7 @@ __device__ __forceinline__ int walk(const DevMesh &m, double x, double y, double
                 nat2_exact(m, e, x, y, r, s);
                 t = 0.0;
             }
-            dec = decide_exact<DIM>(r, s, t, acc, slot);
+            dec = decide_exact<DIM>(r, s, t, acc, slot, tie);
         } else {
             int4 nd = m.conn[e];
             float fr, fs, ft = 0.0f;
@@ -221,7 +222,7 @@ __device__ __forceinline__ int walk(const DevMesh &m, double x, double y, double
                     nat2_exact(m, e, x, y, r, s);
                     t = 0.0;
                 }
-                dec = decide_exact<DIM>(r, s, t, acc, slot);
+                dec = decide_exact<DIM>(r, s, t, acc, slot, tie);
             } else {
                 r = fr;
                 s = fs;
@@ -299,6 +300,10 @@ struct Probe {
 };
 
 template <int MODE, int DIM>
+__device__ __forceinline__ void interp_unstruct(const DevMesh &m, const DevField &f, double tloc, int e, double r,
+                                                double s, double t, double out[3]);
+
+template <int MODE, int DIM>
 __device__ __forceinline__ void vel_unstruct(const DevMesh &m, const DevField &f, double tloc, double x, double y,
                                              double z, Probe &p, double out[3], int &err) {
     if (p.elem < 0 || p.elem >= m.ne) {
@@ -319,6 +324,13 @@ __device__ __forceinline__ void vel_unstruct(const DevMesh &m, const DevField &f
         return;
     }
     p.elem = e;
+    interp_unstruct<MODE, DIM>(m, f, tloc, e, r, s, t, out);
+}
+
+/* velocity at natural coordinates (r,s,t) of element e (velocity.c:603-619) */
+template <int MODE, int DIM>
+__device__ __forceinline__ void interp_unstruct(const DevMesh &m, const DevField &f, double tloc, int e, double r,
+                                                double s, double t, double out[3]) {
     int4 nd = m.conn[e];
     using VT = typename VelT<MODE>::T;
     if constexpr (MODE == P_F64) {
@@ -586,6 +598,17 @@ __device__ int interval_run(const DevMesh &m, const DevField &f, PState &p, ISta
     const double hmin = 1e-12 * fabs(dt);
     const int order = itype == INT_AB2 ? 2 : 3;
     int taken = 0;
+    /* The locate walk at the end of a step starts from the point's element
+     * and seeks the new position; the next step's first-stage walk seeks the
+     * same position from the first probe's element.  When the two start
+     * elements are equal (the usual case: both hold the previous position)
+     * the walks visit the same elements and take the same decisions (the
+     * acceptance rules d > -TINY and d + TINY >= 0 differ only at
+     * d == -TINY, which voids the reuse), so the first stage takes the
+     * locate's element and natural coordinates without walking again. */
+    int c_elem = -1;
+    using CT = typename VelT<MODE>::T; /* reduced-precision interpolation reads (float) r, s, t */
+    CT c_r = 0, c_s = 0, c_t = 0;
     while ((is.tc - b) * (b - a) < 0.0) {
         if (taken == max_steps) return 0;
         if ((is.tc + is.h - b) * (is.tc + is.h - a) > 0.0) is.h = sign_of(b - is.tc, b - a);
@@ -599,7 +622,18 @@ __device__ int interval_run(const DevMesh &m, const DevField &f, PState &p, ISta
             if (err) break;
             for (int c = 0; c < 3; c++) inc[c] = h * ve[c];
         } else {
-            stage_velocity<MODE, DIM, UNSTRUCT>(m, f, tc, p.X, is.mp[0], f0, err);
+            if (UNSTRUCT && c_elem >= 0) {
+                const double tloc = (tc - f.tmin) / (f.tmax - f.tmin);
+                if ((tloc > 1 + TINY) || (tloc < 0 - TINY)) {
+                    err = E_TLOC;
+                    break;
+                }
+                is.mp[0].elem = c_elem;
+                interp_unstruct<MODE, DIM>(m, f, tloc, c_elem, c_r, c_s, c_t, f0);
+            } else {
+                stage_velocity<MODE, DIM, UNSTRUCT>(m, f, tc, p.X, is.mp[0], f0, err);
+            }
+            c_elem = -1;
             if (err) break;
             const bool full = fabs(h - hfull) <= 1e-9 * fabs(hfull);
             if (itype != INT_RK4 && micro) {
@@ -632,10 +666,20 @@ __device__ int interval_run(const DevMesh &m, const DevField &f, PState &p, ISta
         bool out;
         if (UNSTRUCT) {
             double r, s, t;
-            int e = p.elem < 0 ? -1 : walk<MODE, DIM>(m, p.X[0], p.X[1], p.X[2], p.elem, ACC_SEARCH, r, s, t, err);
+            const int start = p.elem;
+            int tie = 0;
+            int e = p.elem < 0 ? -1
+                               : walk<MODE, DIM>(m, p.X[0], p.X[1], p.X[2], p.elem, ACC_SEARCH, r, s, t, err, &tie);
             if (e == -2) break;
             p.elem = e;
             out = e < 0;
+            /* RK4: the next first-stage walk starts at mp[0]; AB: at the located element */
+            if (e >= 0 && !tie && itype != INT_EULER && (itype != INT_RK4 || start == is.mp[0].elem)) {
+                c_elem = e;
+                c_r = (CT)r;
+                c_s = (CT)s;
+                c_t = (CT)t;
+            }
         } else {
             out = outside_domain(m, p.X[0], p.X[1], p.X[2]);
         }

@@ -153,6 +153,8 @@ int launch_grid_zorder(fvc_ctx *c, const int32_t res[3], int64_t n, int *dperm);
 int launch_zsort(fvc_ctx *c, const double *dxyz, int64_t n, const double lo[3], const double hi[3], int *dperm);
 
 int launch_copy_probe(fvc_ctx *c, void *dst, const void *src, int64_t bytes);
+/* a grid's FTLE values (i-major on the device) to host arrays in k, j, i order */
+int copy_ftle_kji(fvc_ctx *c, const double *d_wt, const double *d_wot, const int32_t res[3], double *wt, double *wot);
 
 int check_err(fvc_ctx *c, const char *what);
 void *scratch(fvc_ctx *c, size_t bytes);

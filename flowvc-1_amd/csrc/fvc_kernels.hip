@@ -376,163 +376,188 @@ __global__ __launch_bounds__(BLOCK) void k_seed_prepare(DevMesh m, ChainGeom g, 
  * point, and no FTLE row boundary) is resolved by all lanes at once and the
  * state (last found element, first found element of the row and plane) is
  * advanced with ballots; any other chunk is replayed point by point. */
+constexpr int CHAIN_STAGE = 1024; /* points staged through LDS per round */
+
 template <int DIM>
-__global__ __launch_bounds__(64) void k_seed_chain(DevMesh m, ChainGeom g, int64_t n, const int2 *__restrict__ info,
-                                                  int lsc, int seed0, int *__restrict__ out_elem,
-                                                  int *__restrict__ out_left, double *__restrict__ out_lt, ErrBuf eb) {
-    const int lane = threadIdx.x;
+__global__ __launch_bounds__(CHAIN_STAGE) void k_seed_chain(DevMesh m, ChainGeom g, int64_t n,
+                                                           const int2 *__restrict__ info, int lsc, int seed0,
+                                                           int *__restrict__ out_elem, int *__restrict__ out_left,
+                                                           double *__restrict__ out_lt, ErrBuf eb) {
+    /* the whole block stages CHAIN_STAGE points' data and results through LDS;
+     * wave 0 alone carries the chain state across them */
+    __shared__ int2 s_info[CHAIN_STAGE];
+    __shared__ int s_elem[CHAIN_STAGE], s_left[CHAIN_STAGE];
+    __shared__ double s_lt[CHAIN_STAGE];
+    const int lane = threadIdx.x & 63;
     const bool uns = m.mesh_type == 1;
     int found = 0;                 /* list chain */
     int seed = seed0, guess = seed0 >= 0 ? seed0 : -1;
     int iguess = -1, jguess = -1;  /* FTLE chain */
-    for (int64_t base = 0; base < n; base += 64) {
-        const bool active = base + lane < n;
-        const int2 my = active ? info[base + lane] : make_int2(-1, SF_OUTB);
-        int r_elem = -1, r_left = 0;
-        double r_lt = 0.0;
-        /* fast path test */
-        const bool outb = (my.y & SF_OUTB) != 0;
-        const bool indep = outb || !uns || (lsc && !(my.y & SF_MULTI));
-        const bool state_ok = g.chain == 0 ? found > 0 : seed >= 0;
-        const bool all_outb = __ballot(active && !(outb || !uns)) == 0ull;
-        const bool fast = __ballot(active && !indep) == 0ull && (state_ok || all_outb);
-        if (fast) {
-            int left = outb ? 1 : 0, elem = -1;
-            double lt = 0.0;
-            bool hit = false;
-            if (active && !left && uns) {
-                if (my.x < 0) {
-                    left = 1;
-                    lt = -1.0;
-                } else {
-                    elem = my.x;
-                    hit = true;
-                }
-            }
-            if (g.chain == 0) lt = left ? -1.0 : 0.0;
-            if (g.chain == 2 && left) lt = -1.0;
-            r_elem = elem;
-            r_left = left;
-            r_lt = lt;
-            /* advance the state segment by segment: a segment ends at a row end
-             * (FTLE chain), so a row start can only open a segment */
-            const unsigned long long hits = __ballot(hit);
-            const unsigned long long rowend = __ballot(active && (my.y & SF_ROWEND));
-            const unsigned long long rowstart = __ballot(active && (my.y & SF_ROWSTART));
-            const unsigned long long planestart = __ballot(active && (my.y & SF_PLANESTART));
-            const unsigned long long planeend = __ballot(active && (my.y & SF_PLANEEND));
-            int a = 0;
-            while (a < 64) {
-                const unsigned long long from_a = ~0ull << a;
-                const unsigned long long re = rowend & from_a;
-                const int b = re ? __ffsll((long long)re) - 1 : 63;
-                const unsigned long long seg = (b == 63 ? ~0ull : ((1ull << (b + 1)) - 1ull)) & from_a;
-                if ((rowstart >> a) & 1ull) {
-                    jguess = -1;
-                    if ((planestart >> a) & 1ull) iguess = -1;
-                }
-                const unsigned long long sh = hits & seg;
-                if (sh) {
-                    const int last = 63 - __clzll(sh), first = __ffsll((long long)sh) - 1;
-                    guess = __shfl(elem, last, 64);
-                    if (g.chain == 0) found += __popcll(sh);
-                    if (g.chain == 2) {
-                        const int e1 = __shfl(elem, first, 64);
-                        if (jguess < 0) jguess = e1;
-                        if (iguess < 0) iguess = e1;
-                    }
-                }
-                if ((rowend >> b) & 1ull) {
-                    if (jguess >= 0) guess = jguess;
-                    if (((planeend >> b) & 1ull) && iguess >= 0) guess = iguess;
-                }
-                a = b + 1;
-            }
-        } else {
-            const int lim = (int)min((int64_t)64, n - base);
-            for (int l = 0; l < lim; l++) {
-                const int64_t i = base + l;
-                const int emin = __shfl(my.x, l, 64), flags = __shfl(my.y, l, 64);
-                int elem = -1, left = (flags & SF_OUTB) ? 1 : 0;
-                double lt = 0.0;
-                if (g.chain == 2 && (flags & SF_ROWSTART)) {
-                    jguess = -1;
-                    if (flags & SF_PLANESTART) iguess = -1;
-                }
-                if (!left && uns) {
-                    const bool direct = lsc && !(flags & SF_MULTI); /* result cannot depend on the guess */
-                    double X[3] = {0.0, 0.0, 0.0};
-                    if (!direct) chain_point(g, i, X);
-                    int err = 0, index = -1;
-                    if (g.chain == 0) {
-                        if (!found) {
-                            index = emin;
-                        } else {
-                            index = direct ? emin : local_search<DIM>(m, X[0], X[1], X[2], guess, err);
-                            if (!direct && index < 0 && lsc) index = emin;
-                        }
-                        if (index < 0) {
+    for (int64_t stage = 0; stage < n; stage += CHAIN_STAGE) {
+        {
+            const int64_t i = stage + threadIdx.x;
+            s_info[threadIdx.x] = i < n ? info[i] : make_int2(-1, SF_OUTB);
+        }
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            for (int64_t base = stage; base < n && base < stage + CHAIN_STAGE; base += 64) {
+                const bool active = base + lane < n;
+                const int2 my = s_info[base - stage + lane];
+                int r_elem = -1, r_left = 0;
+                double r_lt = 0.0;
+                /* fast path test */
+                const bool outb = (my.y & SF_OUTB) != 0;
+                const bool indep = outb || !uns || (lsc && !(my.y & SF_MULTI));
+                const bool state_ok = g.chain == 0 ? found > 0 : seed >= 0;
+                const bool all_outb = __ballot(active && !(outb || !uns)) == 0ull;
+                const bool fast = __ballot(active && !indep) == 0ull && (state_ok || all_outb);
+                if (fast) {
+                    int left = outb ? 1 : 0, elem = -1;
+                    double lt = 0.0;
+                    bool hit = false;
+                    if (active && !left && uns) {
+                        if (my.x < 0) {
                             left = 1;
-                            elem = -1;
+                            lt = -1.0;
                         } else {
-                            elem = index;
-                            guess = index;
-                            found++;
-                        }
-                    } else {
-                        if (seed < 0) {
-                            seed = emin;
-                            if (seed < 0) {
-                                left = 1;
-                                elem = -1;
-                                lt = -1.0;
-                            } else {
-                                elem = seed;
-                                guess = seed;
-                                if (g.chain == 2) jguess = iguess = seed;
-                            }
-                        } else {
-                            if (direct)
-                                index = emin;
-                            else {
-                                index = local_search<DIM>(m, X[0], X[1], X[2], guess, err);
-                                if (index < 0 && !err) index = local_search<DIM>(m, X[0], X[1], X[2], seed, err);
-                                if (index < 0 && !err && lsc) index = emin;
-                            }
-                            if (index < 0) {
-                                left = 1;
-                                elem = -1;
-                                lt = -1.0;
-                            } else {
-                                elem = index;
-                                guess = index;
-                                if (g.chain == 2) {
-                                    if (jguess < 0) jguess = index;
-                                    if (iguess < 0) iguess = index;
-                                }
-                            }
+                            elem = my.x;
+                            hit = true;
                         }
                     }
-                    if (err && lane == 0) raise_err(eb, err, i);
-                }
-                if (g.chain == 0) lt = left ? -1.0 : 0.0;
-                if (g.chain == 2 && left) lt = -1.0;
-                if (g.chain == 2 && (flags & SF_ROWEND)) {
-                    if (jguess >= 0) guess = jguess;
-                    if ((flags & SF_PLANEEND) && iguess >= 0) guess = iguess;
-                }
-                if (lane == l) {
+                    if (g.chain == 0) lt = left ? -1.0 : 0.0;
+                    if (g.chain == 2 && left) lt = -1.0;
                     r_elem = elem;
                     r_left = left;
                     r_lt = lt;
+                    /* advance the state segment by segment: a segment ends at a row end
+                     * (FTLE chain), so a row start can only open a segment */
+                    const unsigned long long hits = __ballot(hit);
+                    const unsigned long long rowend = __ballot(active && (my.y & SF_ROWEND));
+                    const unsigned long long rowstart = __ballot(active && (my.y & SF_ROWSTART));
+                    const unsigned long long planestart = __ballot(active && (my.y & SF_PLANESTART));
+                    const unsigned long long planeend = __ballot(active && (my.y & SF_PLANEEND));
+                    int a = 0;
+                    while (a < 64) {
+                        const unsigned long long from_a = ~0ull << a;
+                        const unsigned long long re = rowend & from_a;
+                        const int b = re ? __ffsll((long long)re) - 1 : 63;
+                        const unsigned long long seg = (b == 63 ? ~0ull : ((1ull << (b + 1)) - 1ull)) & from_a;
+                        if ((rowstart >> a) & 1ull) {
+                            jguess = -1;
+                            if ((planestart >> a) & 1ull) iguess = -1;
+                        }
+                        const unsigned long long sh = hits & seg;
+                        if (sh) {
+                            const int last = 63 - __clzll(sh), first = __ffsll((long long)sh) - 1;
+                            guess = __shfl(elem, last, 64);
+                            if (g.chain == 0) found += __popcll(sh);
+                            if (g.chain == 2) {
+                                const int e1 = __shfl(elem, first, 64);
+                                if (jguess < 0) jguess = e1;
+                                if (iguess < 0) iguess = e1;
+                            }
+                        }
+                        if ((rowend >> b) & 1ull) {
+                            if (jguess >= 0) guess = jguess;
+                            if (((planeend >> b) & 1ull) && iguess >= 0) guess = iguess;
+                        }
+                        a = b + 1;
+                    }
+                } else {
+                    const int lim = (int)min((int64_t)64, n - base);
+                    for (int l = 0; l < lim; l++) {
+                        const int64_t i = base + l;
+                        const int emin = __shfl(my.x, l, 64), flags = __shfl(my.y, l, 64);
+                        int elem = -1, left = (flags & SF_OUTB) ? 1 : 0;
+                        double lt = 0.0;
+                        if (g.chain == 2 && (flags & SF_ROWSTART)) {
+                            jguess = -1;
+                            if (flags & SF_PLANESTART) iguess = -1;
+                        }
+                        if (!left && uns) {
+                            const bool direct = lsc && !(flags & SF_MULTI); /* result cannot depend on the guess */
+                            double X[3] = {0.0, 0.0, 0.0};
+                            if (!direct) chain_point(g, i, X);
+                            int err = 0, index = -1;
+                            if (g.chain == 0) {
+                                if (!found) {
+                                    index = emin;
+                                } else {
+                                    index = direct ? emin : local_search<DIM>(m, X[0], X[1], X[2], guess, err);
+                                    if (!direct && index < 0 && lsc) index = emin;
+                                }
+                                if (index < 0) {
+                                    left = 1;
+                                    elem = -1;
+                                } else {
+                                    elem = index;
+                                    guess = index;
+                                    found++;
+                                }
+                            } else {
+                                if (seed < 0) {
+                                    seed = emin;
+                                    if (seed < 0) {
+                                        left = 1;
+                                        elem = -1;
+                                        lt = -1.0;
+                                    } else {
+                                        elem = seed;
+                                        guess = seed;
+                                        if (g.chain == 2) jguess = iguess = seed;
+                                    }
+                                } else {
+                                    if (direct)
+                                        index = emin;
+                                    else {
+                                        index = local_search<DIM>(m, X[0], X[1], X[2], guess, err);
+                                        if (index < 0 && !err) index = local_search<DIM>(m, X[0], X[1], X[2], seed, err);
+                                        if (index < 0 && !err && lsc) index = emin;
+                                    }
+                                    if (index < 0) {
+                                        left = 1;
+                                        elem = -1;
+                                        lt = -1.0;
+                                    } else {
+                                        elem = index;
+                                        guess = index;
+                                        if (g.chain == 2) {
+                                            if (jguess < 0) jguess = index;
+                                            if (iguess < 0) iguess = index;
+                                        }
+                                    }
+                                }
+                            }
+                            if (err && lane == 0) raise_err(eb, err, i);
+                        }
+                        if (g.chain == 0) lt = left ? -1.0 : 0.0;
+                        if (g.chain == 2 && left) lt = -1.0;
+                        if (g.chain == 2 && (flags & SF_ROWEND)) {
+                            if (jguess >= 0) guess = jguess;
+                            if ((flags & SF_PLANEEND) && iguess >= 0) guess = iguess;
+                        }
+                        if (lane == l) {
+                            r_elem = elem;
+                            r_left = left;
+                            r_lt = lt;
+                        }
+                    }
                 }
+                s_elem[base - stage + lane] = r_elem;
+                s_left[base - stage + lane] = r_left;
+                s_lt[base - stage + lane] = r_lt;
             }
         }
-        if (active) {
-            out_elem[base + lane] = r_elem;
-            out_left[base + lane] = r_left;
-            out_lt[base + lane] = r_lt;
+        __syncthreads();
+        {
+            const int64_t i = stage + threadIdx.x;
+            if (i < n) {
+                out_elem[i] = s_elem[threadIdx.x];
+                out_left[i] = s_left[threadIdx.x];
+                out_lt[i] = s_lt[threadIdx.x];
+            }
         }
+        __syncthreads();
     }
 }
 
@@ -565,12 +590,12 @@ int launch_seed_chain(fvc_ctx *c, int chain, int64_t n, const double *dxyz, cons
     if (!info) return FVC_E_NOMEM;
     if (c->mesh.dims == 3) {
         k_seed_prepare<3><<<grid_for(n, BLOCK), BLOCK, 0, c->stream>>>(c->mesh, g, n, info);
-        k_seed_chain<3><<<1, 64, 0, c->stream>>>(c->mesh, g, n, info, lsc, seed0, d_elem, d_left, d_left_time,
-                                                 errbuf(c));
+        k_seed_chain<3><<<1, CHAIN_STAGE, 0, c->stream>>>(c->mesh, g, n, info, lsc, seed0, d_elem, d_left,
+                                                          d_left_time, errbuf(c));
     } else {
         k_seed_prepare<2><<<grid_for(n, BLOCK), BLOCK, 0, c->stream>>>(c->mesh, g, n, info);
-        k_seed_chain<2><<<1, 64, 0, c->stream>>>(c->mesh, g, n, info, lsc, seed0, d_elem, d_left, d_left_time,
-                                                 errbuf(c));
+        k_seed_chain<2><<<1, CHAIN_STAGE, 0, c->stream>>>(c->mesh, g, n, info, lsc, seed0, d_elem, d_left,
+                                                          d_left_time, errbuf(c));
     }
     FVC_HIP(hipGetLastError());
     return check_err(c, "seed chain");
@@ -631,9 +656,9 @@ __device__ __forceinline__ void ftle_advect_body(const DevMesh &m, const DevFiel
 }
 
 template <int MODE, int DIM, int UNS>
-__global__ __launch_bounds__(BLOCK) void k_ftle_advect(DevMesh m, DevField f, FtleDev d, const int *__restrict__ active,
-                                                      int64_t nactive, double t1, double t2, int itype, double dt,
-                                                      int extrap, ErrBuf eb, unsigned long long *steps) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void k_ftle_advect(
+    DevMesh m, DevField f, FtleDev d, const int *__restrict__ active, int64_t nactive, double t1, double t2, int itype,
+    double dt, int extrap, ErrBuf eb, unsigned long long *steps) {
     ftle_advect_body<MODE, DIM, UNS, 0>(m, f, d, active, nactive, t1, t2, itype, dt, extrap, eb, steps);
 }
 

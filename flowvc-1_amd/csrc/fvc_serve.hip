@@ -753,23 +753,9 @@ static int finish_ftle_frame(fvc_server *S, HostReq &R, bool *complete) {
     *complete = R.d.t_stop - fld.tmax < TINY; /* StopTime reached (ftle.c:352-377) */
     if (*complete) {
         if ((rc = launch_ftle_fill_dev(c, v, R.n, R.d.t_stop - R.d.t_start))) return rc;
-        std::vector<double> a(R.n), b(R.n);
-        FVC_HIP(hipMemcpyAsync(a.data(), S->wt + R.base, sizeof(double) * R.n, hipMemcpyDeviceToHost, c->stream));
-        FVC_HIP(hipMemcpyAsync(b.data(), S->wot + R.base, sizeof(double) * R.n, hipMemcpyDeviceToHost, c->stream));
-        FVC_HIP(hipStreamSynchronize(c->stream));
-        /* i-major storage -> k, j, i order (ftle.c:1302-1315) */
-        const int32_t *res = R.d.ftle_res;
         R.wt.resize(R.n);
         R.wot.resize(R.n);
-        int64_t q = 0;
-        for (int k = 0; k < res[2]; k++)
-            for (int j = 0; j < res[1]; j++)
-                for (int i = 0; i < res[0]; i++, q++) {
-                    const int64_t P = ((int64_t)i * res[1] + j) * res[2] + k;
-                    R.wt[q] = a[P];
-                    R.wot[q] = b[P];
-                }
-        return FVC_OK;
+        return copy_ftle_kji(c, S->wt + R.base, S->wot + R.base, R.d.ftle_res, R.wt.data(), R.wot.data());
     }
     R.frame++;
     k_srv_release<<<grid256(R.n), 256, 0, c->stream>>>(S->flags, S->frame, R.base, R.n, R.frame);
