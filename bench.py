@@ -372,13 +372,17 @@ def bench_serve(args, np, fvc, host):
     ids = [None] * len(reqs)
     i = 0
     peak_running = 0
+    ticks, t_poll = 0, 0.0
     clock0 = S.clock()
     while True:
         now = S.clock() - clock0
         while i < len(reqs) and reqs[i]["t"] <= now:
             ids[i] = submit(reqs[i])
             i += 1
+        tp = time.perf_counter()
         r, q = S.poll()
+        t_poll += time.perf_counter() - tp
+        ticks += 1
         peak_running = max(peak_running, r)
         if r == 0 and q == 0:
             if i == len(reqs):
@@ -436,6 +440,7 @@ def bench_serve(args, np, fvc, host):
                             np.array_equal(a["LeftDomainTime"], out["LeftDomainTime"]))
             else:
                 same += int(np.array_equal(a[0], out[0]) and np.array_equal(a[1], out[1]))
+    svc_by_kind = {kd: float(sum(sv for r, sv in zip(reqs, service) if r["kind"] == kd)) for kd in ("tracer", "ftle")}
     fin, blat = 0.0, []
     for r, sv in zip(reqs, service):
         fin = max(fin, r["t"]) + sv
@@ -460,9 +465,13 @@ def bench_serve(args, np, fvc, host):
         "p99_latency_ms": 1e3 * float(np.percentile(lat, 99)),
         "batch_at_a_time": {"p50_latency_ms": 1e3 * float(np.median(blat)),
                             "p99_latency_ms": 1e3 * float(np.percentile(blat, 99)),
-                            "busy_s": float(sum(service)),
+                            "busy_s": float(sum(service)), "busy_s_by_kind": svc_by_kind,
                             "throughput_particle_steps_per_s": steps / max(fin - reqs[0]["t"], 1e-9)},
-        "served": {"makespan_s": makespan, "peak_running": peak_running, "pool_particles": cap,
+        "served": {"makespan_s": makespan, "peak_running": peak_running, "ticks": ticks, "poll_s": t_poll,
+                   "latency_p50_ms_by_kind": {kd: 1e3 * float(np.median([l for l, r in zip(lat, reqs)
+                                                                         if r["kind"] == kd] or [0]))
+                                              for kd in ("tracer", "ftle")},
+                   "pool_particles": cap,
                    "pool_gb": pool_bytes / 1e9, "results_identical_to_alone": "%d/%d" % (same, len(served))},
         "roofline": None,
     }

@@ -604,6 +604,75 @@ int launch_seed_chain(fvc_ctx *c, int chain, int64_t n, const double *dxyz, cons
 /* ------------------------------------------------------------------------ */
 /* FTLE                                                                       */
 /* ------------------------------------------------------------------------ */
+/* ------------------------------------------------------------------------ */
+/* stable compaction: indices of nonzero flags, in order                      */
+/* ------------------------------------------------------------------------ */
+constexpr int CMP_TILE = 4096; /* items per block */
+
+__global__ __launch_bounds__(BLOCK) void k_cmp_count(const int *__restrict__ flags, int64_t n, int *__restrict__ counts) {
+    int64_t base = (int64_t)blockIdx.x * CMP_TILE;
+    int cnt = 0;
+    for (int q = threadIdx.x; q < CMP_TILE; q += BLOCK) {
+        int64_t i = base + q;
+        if (i < n && flags[i]) cnt++;
+    }
+    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
+    __shared__ int red[BLOCK / 64];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) counts[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+/* exclusive scan of a small int array in one block (in place); total at a[n] */
+__global__ __launch_bounds__(1024) void k_scan_small(int *__restrict__ a, int n) {
+    __shared__ int carry;
+    __shared__ int tmp[1024];
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < n; base += 1024) {
+        int i = base + threadIdx.x;
+        int v = i < n ? a[i] : 0;
+        tmp[threadIdx.x] = v;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            int add = threadIdx.x >= off ? tmp[threadIdx.x - off] : 0;
+            __syncthreads();
+            tmp[threadIdx.x] += add;
+            __syncthreads();
+        }
+        if (i < n) a[i] = carry + tmp[threadIdx.x] - v;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry += tmp[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) a[n] = carry;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_cmp_scatter(const int *__restrict__ flags, int64_t n,
+                                                      const int *__restrict__ offs, const int *__restrict__ map,
+                                                      int *__restrict__ out) {
+    __shared__ int wsum[BLOCK / 64];
+    __shared__ int run;
+    if (threadIdx.x == 0) run = offs[blockIdx.x];
+    __syncthreads();
+    int64_t base = (int64_t)blockIdx.x * CMP_TILE;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int r = 0; r < CMP_TILE; r += BLOCK) {
+        int64_t i = base + r + threadIdx.x;
+        int fl = (i < n && flags[i]) ? 1 : 0;
+        unsigned long long b = __ballot(fl);
+        int before = __popcll(b & ((1ull << lane) - 1ull));
+        if (lane == 0) wsum[w] = __popcll(b);
+        __syncthreads();
+        int pre = 0;
+        for (int q = 0; q < w; q++) pre += wsum[q];
+        if (fl) out[run + pre + before] = map ? map[i] : (int)i;
+        __syncthreads();
+        if (threadIdx.x == 0) run += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        __syncthreads();
+    }
+}
+
 FtleDev ftle_dev(fvc_ftle *f) {
     FtleDev d;
     for (int a = 0; a < 3; a++) {
@@ -672,7 +741,7 @@ __global__ __launch_bounds__(BLOCK) void k_ftle_advect_gen(DevMesh m, DevField f
 
 /* Advect_FTLEPoint (ftle.c:1205-1227) from the frame-start state; the element
  * is the one the reference's array holds when the trigger is processed. */
-template <int MODE, int DIM, int UNS>
+template <int MODE, int DIM, int UNS, int GEN = 1>
 __device__ void advect_copy(const DevMesh &m, const DevField &f, const FtleDev &d, int64_t idx, int64_t trig, double t1,
                             double t2, int itype, double dt, PState &s, int &err) {
     for (int c = 0; c < 3; c++) {
@@ -684,7 +753,7 @@ __device__ void advect_copy(const DevMesh &m, const DevField &f, const FtleDev &
     s.elem = updated ? d.elem_new[idx] : d.p.elem[idx];
     s.left_time = 0.0;
     uint32_t ns = 0;
-    advect<MODE, DIM, UNS>(m, f, s, t1, t2, itype, dt, 0, err, ns);
+    advect<MODE, DIM, UNS, GEN>(m, f, s, t1, t2, itype, dt, 0, err, ns);
 }
 
 __device__ inline void ftle_value(double D[3][3], double dt_int, double &wt, double &wot, int &err, bool zero_time) {
@@ -736,13 +805,6 @@ __device__ inline void gradient(const double lo[3][3], const double hi[3][3], co
  * advects the (up to) six neighbours to te, pass B the point itself and the
  * gradient neighbours to MinExitTime, pass C forms the gradient and the FTLE.
  * Each lane does exactly the reference's computation for its advection. */
-struct EarlyBuf {
-    const int *list; /* triggered points, ascending */
-    int64_t nt;
-    double *lt;  /* [nt][6] exit times of the neighbours (te if they stay) */
-    double *pos; /* [nt][7][3] self, then lo/hi neighbour per axis, at MinExitTime */
-};
-
 __device__ inline void grid_nbrs(const FtleDev &d, int64_t P, int cc[3], int64_t nb[6], bool has[6]) {
     const int64_t sj = d.res[2], si = (int64_t)d.res[2] * d.res[1];
     cc[0] = (int)(P / si);
@@ -781,113 +843,115 @@ __global__ __launch_bounds__(BLOCK) void k_early_flag(FtleDev d, int64_t n, int 
     flags[P] = fl;
 }
 
-__device__ inline double min_exit_time(const EarlyBuf &e, int64_t t, double te) {
-    const double *lt = e.lt + 6 * t;
-    return dmin(te, dmin(lt[5], dmin(lt[4], dmin(lt[3], dmin(lt[2], dmin(lt[1], lt[0]))))));
-}
-
-/* pass A: lane = (triggered point, neighbour q) */
-template <int MODE, int DIM, int UNS>
-__global__ __launch_bounds__(BLOCK) void k_early_a(DevMesh m, DevField f, FtleDev d, EarlyBuf e, double t1, int itype,
-                                                  double dt, ErrBuf eb) {
+/* One triggered point per group of eight lanes (grid-stride over the list,
+ * whose length is read from device memory):
+ *   lanes 0-5 advect the six neighbours to te -> their exit times; every lane
+ *   of the group then forms MinExitTime from the same six values;
+ *   lane 0 advects the point itself and lanes 1-6 the lower (odd) or upper
+ *   (even) neighbour along each axis that the difference formula uses, to
+ *   MinExitTime; lane 0 gathers the positions, forms the gradient (one-sided
+ *   at the grid edges, ftle.c:994-1020) and the FTLE. */
+template <int MODE, int DIM, int UNS, int GEN>
+__device__ __forceinline__ void early_body(const DevMesh &m, const DevField &f, const FtleDev &d,
+                                           const int *__restrict__ list, const int *__restrict__ d_nt, double t1,
+                                           int itype, double dt, ErrBuf eb) {
 #pragma clang fp contract(off)
-    const int64_t task = xcd_block() * blockDim.x + threadIdx.x;
-    if (task >= 6 * e.nt) return;
-    const int64_t t = task / 6;
-    const int q = (int)(task % 6);
-    const int64_t P = e.list[t];
-    int cc[3];
-    int64_t nb[6];
-    bool has[6];
-    grid_nbrs(d, P, cc, nb, has);
-    const int64_t trig = early_trigger(d, P, nb, has);
-    const double te = d.exit_time[trig];
-    double lt = te;
-    if (has[q]) {
+    const int nt = *d_nt;
+    const int sub = threadIdx.x & 7;
+    const int64_t ngroups = ((int64_t)gridDim.x * blockDim.x) >> 3;
+    for (int64_t t = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 3; t < nt; t += ngroups) {
+        const int64_t P = list[t];
+        int cc[3];
+        int64_t nb[6];
+        bool has[6];
+        grid_nbrs(d, P, cc, nb, has);
+        const int64_t trig = early_trigger(d, P, nb, has);
+        const double te = d.exit_time[trig];
         int err = 0;
-        PState s;
-        advect_copy<MODE, DIM, UNS>(m, f, d, nb[q], trig, t1, te, itype, dt, s, err);
-        if (err) raise_err(eb, err, P);
-        if (s.left) lt = s.left_time;
-    }
-    e.lt[task] = lt;
-}
-
-/* pass B: lane = (triggered point, r): r = 0 the point itself, r = 1 + 2 ax
- * + side its lower (side 0) or upper (side 1) neighbour along axis ax, when
- * the difference formula for that axis uses it */
-template <int MODE, int DIM, int UNS>
-__global__ __launch_bounds__(BLOCK) void k_early_b(DevMesh m, DevField f, FtleDev d, EarlyBuf e, double t1, int itype,
-                                                  double dt, ErrBuf eb) {
-#pragma clang fp contract(off)
-    const int64_t task = xcd_block() * blockDim.x + threadIdx.x;
-    if (task >= 7 * e.nt) return;
-    const int64_t t = task / 7;
-    const int r = (int)(task % 7);
-    const int64_t P = e.list[t];
-    int cc[3];
-    int64_t nb[6];
-    bool has[6];
-    grid_nbrs(d, P, cc, nb, has);
-    int64_t who = P;
-    if (r > 0) {
-        const int ax = (r - 1) >> 1, side = (r - 1) & 1;
-        if (ax >= DIM) return;
-        const int last = d.res[ax] - 1;
-        /* one-sided at the grid edges (ftle.c:994-1020) */
-        if (side == 0 && cc[ax] == 0) return;
-        if (side == 1 && cc[ax] != 0 && cc[ax] == last) return;
-        who = nb[2 * ax + side];
-    }
-    const int64_t trig = early_trigger(d, P, nb, has);
-    const double mt = min_exit_time(e, t, d.exit_time[trig]);
-    int err = 0;
-    PState s;
-    advect_copy<MODE, DIM, UNS>(m, f, d, who, trig, t1, mt, itype, dt, s, err);
-    if (err) raise_err(eb, err, P);
-    for (int c = 0; c < 3; c++) e.pos[3 * task + c] = s.X[c];
-}
-
-/* pass C: gradient and FTLE of each triggered point */
-__global__ __launch_bounds__(BLOCK) void k_early_c(FtleDev d, EarlyBuf e, int dims, ErrBuf eb) {
-#pragma clang fp contract(off)
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= e.nt) return;
-    const int64_t P = e.list[t];
-    int cc[3];
-    int64_t nb[6];
-    bool has[6];
-    grid_nbrs(d, P, cc, nb, has);
-    const int64_t trig = early_trigger(d, P, nb, has);
-    const double mt = min_exit_time(e, t, d.exit_time[trig]);
-    const double *self = e.pos + 21 * t;
-    double lo[3][3], hi[3][3], fac[3] = {1.0, 1.0, 1.0};
-    for (int ax = 0; ax < dims; ax++) {
-        const double *a = e.pos + 21 * t + 3 * (1 + 2 * ax), *b = a + 3;
-        const int last = d.res[ax] - 1;
-        for (int c = 0; c < 3; c++) {
-            if (cc[ax] == 0) {
-                lo[ax][c] = self[c];
-                hi[ax][c] = b[c];
-            } else if (cc[ax] == last) {
-                lo[ax][c] = a[c];
-                hi[ax][c] = self[c];
+        double mt = te;
+        double X[3] = {0.0, 0.0, 0.0};
+        /* one advection call site for both phases keeps a single inlined copy */
+        for (int phase = 0; phase < 2; phase++) {
+            bool need;
+            int64_t who;
+            if (phase == 0) {
+                need = sub < 6 && has[sub];
+                who = sub < 6 ? nb[sub] : P;
             } else {
-                lo[ax][c] = a[c];
-                hi[ax][c] = b[c];
+                need = sub == 0;
+                who = P;
+                if (sub >= 1 && sub <= 6) {
+                    const int ax = (sub - 1) >> 1, side = (sub - 1) & 1;
+                    if (ax < DIM) {
+                        const int last = d.res[ax] - 1;
+                        need = !(side == 0 && cc[ax] == 0) && !(side == 1 && cc[ax] != 0 && cc[ax] == last);
+                        who = nb[2 * ax + side];
+                    }
+                }
+            }
+            PState s;
+            s.left = 0;
+            s.left_time = 0.0;
+            if (need) {
+                advect_copy<MODE, DIM, UNS, GEN>(m, f, d, who, trig, t1, phase == 0 ? te : mt, itype, dt, s, err);
+                for (int c = 0; c < 3; c++) X[c] = s.X[c];
+            }
+            if (phase == 0) {
+                const double lt = (need && s.left) ? s.left_time : te;
+                double L[6];
+                for (int q = 0; q < 6; q++) L[q] = __shfl(lt, q, 8);
+                mt = dmin(te, dmin(L[5], dmin(L[4], dmin(L[3], dmin(L[2], dmin(L[1], L[0]))))));
             }
         }
-        if (cc[ax] != 0 && cc[ax] != last) fac[ax] = 2.0;
+        if (err) raise_err(eb, err, P);
+        double pos[7][3];
+        for (int r = 0; r < 7; r++)
+            for (int c = 0; c < 3; c++) pos[r][c] = __shfl(X[c], r, 8);
+        if (sub == 0) {
+            double lo[3][3], hi[3][3], fac[3] = {1.0, 1.0, 1.0};
+            for (int ax = 0; ax < DIM; ax++) {
+                const int last = d.res[ax] - 1;
+                for (int c = 0; c < 3; c++) {
+                    if (cc[ax] == 0) {
+                        lo[ax][c] = pos[0][c];
+                        hi[ax][c] = pos[2 + 2 * ax][c];
+                    } else if (cc[ax] == last) {
+                        lo[ax][c] = pos[1 + 2 * ax][c];
+                        hi[ax][c] = pos[0][c];
+                    } else {
+                        lo[ax][c] = pos[1 + 2 * ax][c];
+                        hi[ax][c] = pos[2 + 2 * ax][c];
+                    }
+                }
+                if (cc[ax] != 0 && cc[ax] != last) fac[ax] = 2.0;
+            }
+            double D[3][3];
+            gradient(lo, hi, fac, d.del, DIM, D);
+            int e2 = 0;
+            double wt, wot;
+            ftle_value(D, fabs(mt - d.start_time), wt, wot, e2, fabs(mt - d.start_time) < TINY);
+            if (e2) raise_err(eb, e2, P);
+            d.wt[P] = wt;
+            d.wot[P] = wot;
+            d.have[P] = 1;
+        }
     }
-    double D[3][3];
-    gradient(lo, hi, fac, d.del, dims, D);
-    int err = 0;
-    double wt, wot;
-    ftle_value(D, fabs(mt - d.start_time), wt, wot, err, fabs(mt - d.start_time) < TINY);
-    if (err) raise_err(eb, err, P);
-    d.wt[P] = wt;
-    d.wot[P] = wot;
-    d.have[P] = 1;
+}
+
+/* RK4 frames */
+template <int MODE, int DIM, int UNS>
+__global__ __launch_bounds__(BLOCK) void k_early(DevMesh m, DevField f, FtleDev d, const int *__restrict__ list,
+                                                const int *__restrict__ d_nt, double t1, int itype, double dt,
+                                                ErrBuf eb) {
+    early_body<MODE, DIM, UNS, 0>(m, f, d, list, d_nt, t1, itype, dt, eb);
+}
+
+/* Euler frames */
+template <int MODE, int DIM, int UNS>
+__global__ __launch_bounds__(BLOCK) void k_early_gen(DevMesh m, DevField f, FtleDev d, const int *__restrict__ list,
+                                                    const int *__restrict__ d_nt, double t1, int itype, double dt,
+                                                    ErrBuf eb) {
+    early_body<MODE, DIM, UNS, 1>(m, f, d, list, d_nt, t1, itype, dt, eb);
 }
 
 /* position update (ftle.c:160-169, UpdateFTLELocations ftle.c:1229-1242) */
@@ -991,27 +1055,25 @@ int launch_ftle_mask(fvc_ftle *f) { return launch_ftle_mask_dev(f->ctx, ftle_dev
 /* early FTLE of one frame for a grid of n points (after its advection, before
  * the position update) */
 int launch_ftle_early(fvc_ctx *c, const FtleDev &d, int64_t n, const DevField &fld, double t1, int itype, double dt) {
-    /* triggered points: flags in the first half of the scratch, the list in the second */
-    int *eflags = (int *)scratch(c, sizeof(int) * 2 * (size_t)n);
+    /* scratch: flags [n], triggered list [n], per-tile offsets and the count
+     * [nb + 1]; the count stays on the device, so nothing here waits */
+    const int nb = (int)((n + CMP_TILE - 1) / CMP_TILE);
+    int *eflags = (int *)scratch(c, sizeof(int) * (2 * (size_t)n + nb + 1));
     if (!eflags) return FVC_E_NOMEM;
-    int *elist = eflags + n;
+    int *elist = eflags + n, *offs = elist + n;
     k_early_flag<<<grid_for(n, BLOCK), BLOCK, 0, c->stream>>>(d, n, eflags);
+    k_cmp_count<<<nb, BLOCK, 0, c->stream>>>(eflags, n, offs);
+    k_scan_small<<<1, 1024, 0, c->stream>>>(offs, nb);
+    k_cmp_scatter<<<nb, BLOCK, 0, c->stream>>>(eflags, n, offs, nullptr, elist);
+    const unsigned groups = (unsigned)std::min<int64_t>(std::max<int64_t>(n / 8, 1), 16384);
+    const unsigned grid = (unsigned)(((int64_t)groups * 8 + BLOCK - 1) / BLOCK);
+    if (itype == INT_RK4)
+        FVC_DISPATCH(c->precision, c->mesh.dims, c->mesh.mesh_type, k_early,
+                     <<<grid, BLOCK, 0, c->stream>>>(c->mesh, fld, d, elist, offs + nb, t1, itype, dt, errbuf(c)));
+    else
+        FVC_DISPATCH(c->precision, c->mesh.dims, c->mesh.mesh_type, k_early_gen,
+                     <<<grid, BLOCK, 0, c->stream>>>(c->mesh, fld, d, elist, offs + nb, t1, itype, dt, errbuf(c)));
     FVC_HIP(hipGetLastError());
-    int64_t nt = 0;
-    int rc = launch_compact(c, eflags, n, elist, &nt);
-    if (rc || nt == 0) return rc;
-    EarlyBuf e;
-    e.list = elist;
-    e.nt = nt;
-    FVC_HIP(hipMallocAsync((void **)&e.lt, sizeof(double) * 27 * (size_t)nt, c->stream));
-    e.pos = e.lt + 6 * nt;
-    FVC_DISPATCH(c->precision, c->mesh.dims, c->mesh.mesh_type, k_early_a,
-                 <<<grid_for(6 * nt, BLOCK), BLOCK, 0, c->stream>>>(c->mesh, fld, d, e, t1, itype, dt, errbuf(c)));
-    FVC_DISPATCH(c->precision, c->mesh.dims, c->mesh.mesh_type, k_early_b,
-                 <<<grid_for(7 * nt, BLOCK), BLOCK, 0, c->stream>>>(c->mesh, fld, d, e, t1, itype, dt, errbuf(c)));
-    k_early_c<<<grid_for(nt, BLOCK), BLOCK, 0, c->stream>>>(d, e, c->mesh.dims, errbuf(c));
-    FVC_HIP(hipGetLastError());
-    FVC_HIP(hipFreeAsync(e.lt, c->stream));
     return FVC_OK;
 }
 
@@ -1107,75 +1169,6 @@ int launch_ftle_fill(fvc_ftle *f, double int_t) {
     fvc_ctx *c = f->ctx;
     int rc = launch_ftle_fill_dev(c, ftle_dev(f), f->n, int_t);
     return rc ? rc : check_err(c, "fvc_ftle_output");
-}
-
-/* ------------------------------------------------------------------------ */
-/* stable compaction: indices of nonzero flags, in order                      */
-/* ------------------------------------------------------------------------ */
-constexpr int CMP_TILE = 4096; /* items per block */
-
-__global__ __launch_bounds__(BLOCK) void k_cmp_count(const int *__restrict__ flags, int64_t n, int *__restrict__ counts) {
-    int64_t base = (int64_t)blockIdx.x * CMP_TILE;
-    int cnt = 0;
-    for (int q = threadIdx.x; q < CMP_TILE; q += BLOCK) {
-        int64_t i = base + q;
-        if (i < n && flags[i]) cnt++;
-    }
-    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
-    __shared__ int red[BLOCK / 64];
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = cnt;
-    __syncthreads();
-    if (threadIdx.x == 0) counts[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
-}
-
-/* exclusive scan of a small int array in one block (in place); total at a[n] */
-__global__ __launch_bounds__(1024) void k_scan_small(int *__restrict__ a, int n) {
-    __shared__ int carry;
-    __shared__ int tmp[1024];
-    if (threadIdx.x == 0) carry = 0;
-    __syncthreads();
-    for (int base = 0; base < n; base += 1024) {
-        int i = base + threadIdx.x;
-        int v = i < n ? a[i] : 0;
-        tmp[threadIdx.x] = v;
-        __syncthreads();
-        for (int off = 1; off < 1024; off <<= 1) {
-            int add = threadIdx.x >= off ? tmp[threadIdx.x - off] : 0;
-            __syncthreads();
-            tmp[threadIdx.x] += add;
-            __syncthreads();
-        }
-        if (i < n) a[i] = carry + tmp[threadIdx.x] - v;
-        __syncthreads();
-        if (threadIdx.x == 1023) carry += tmp[1023];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) a[n] = carry;
-}
-
-__global__ __launch_bounds__(BLOCK) void k_cmp_scatter(const int *__restrict__ flags, int64_t n,
-                                                      const int *__restrict__ offs, const int *__restrict__ map,
-                                                      int *__restrict__ out) {
-    __shared__ int wsum[BLOCK / 64];
-    __shared__ int run;
-    if (threadIdx.x == 0) run = offs[blockIdx.x];
-    __syncthreads();
-    int64_t base = (int64_t)blockIdx.x * CMP_TILE;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (int r = 0; r < CMP_TILE; r += BLOCK) {
-        int64_t i = base + r + threadIdx.x;
-        int fl = (i < n && flags[i]) ? 1 : 0;
-        unsigned long long b = __ballot(fl);
-        int before = __popcll(b & ((1ull << lane) - 1ull));
-        if (lane == 0) wsum[w] = __popcll(b);
-        __syncthreads();
-        int pre = 0;
-        for (int q = 0; q < w; q++) pre += wsum[q];
-        if (fl) out[run + pre + before] = map ? map[i] : (int)i;
-        __syncthreads();
-        if (threadIdx.x == 0) run += wsum[0] + wsum[1] + wsum[2] + wsum[3];
-        __syncthreads();
-    }
 }
 
 int launch_compact(fvc_ctx *c, const int *flags, int64_t n, int *out_idx, int64_t *count, const int *map) {

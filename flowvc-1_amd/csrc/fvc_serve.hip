@@ -674,11 +674,14 @@ static int append_range(fvc_server *S, const HostReq &R) {
     return FVC_OK;
 }
 
+__global__ void k_srv_set(int *dst, int v) {
+    if (threadIdx.x == 0) *dst = v;
+}
+
 static int set_remaining(fvc_server *S, int slot, int64_t n) {
+    k_srv_set<<<1, 64, 0, S->ctx->stream>>>(S->remaining + slot, (int)n); /* ordered, no host wait */
     fvc_ctx *c = S->ctx;
-    int v = (int)n;
-    FVC_HIP(hipMemcpyAsync(S->remaining + slot, &v, sizeof v, hipMemcpyHostToDevice, c->stream));
-    FVC_HIP(hipStreamSynchronize(c->stream)); /* v lives on this stack frame */
+    FVC_HIP(hipGetLastError());
     return FVC_OK;
 }
 
