@@ -176,7 +176,7 @@ def bench_batch(args, np, fvc, host):
     t0 = time.time()
     blocks = [64, 128, 256, 512, 1024] if args.precision == "f32" else [256]
     budgets = [1, 4, 16, 64, 256, 0]
-    best, best_ms = Bt.tune(blocks, budgets, 1)
+    best, best_ms = Bt.tune(blocks, budgets, args.batch_graph)
     Bt.close()
     del Pt
     log("autotune %.1f s over blocks %s x budgets %s: best %s (%.1f ms on the 1/8 batch)" %
@@ -221,8 +221,8 @@ def bench_batch(args, np, fvc, host):
         "config": {
             "workload": "batching on the pipe mesh (%d tets): %d releases in one persistent kernel, seed counts "
                         "log-uniform [10k, 2M] (%d seeds), start uniform in the period, horizon uniform [0.1, 1] "
-                        "periods, RK4/AB3 mix, dt=1e-3, autotuned block/steps-per-launch, hipGraph replay" %
-                        (conn.shape[0], len(rel), n),
+                        "periods, RK4/AB3 mix, dt=1e-3, autotuned block/steps-per-launch, %s" %
+                        (conn.shape[0], len(rel), n, "hipGraph replay" if args.batch_graph else "direct launches"),
             "releases": len(rel), "seeds": n, "precision": args.precision,
         },
         "batch": {"tuned": best, "tune_ms_eighth_batch": best_ms, "launch_groups": launches,
@@ -537,6 +537,8 @@ def main():
     ap.add_argument("--serve-check", type=int, default=40)
     ap.add_argument("--releases", type=int, default=64)
     ap.add_argument("--batch-cpu-stride", type=int, default=10)
+    ap.add_argument("--batch-graph", type=int, default=1, choices=[0, 1],
+                    help="replay the launch sequence from a captured hipGraph (1) or launch it directly (0)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one GPU and this build has no multi-GPU path "

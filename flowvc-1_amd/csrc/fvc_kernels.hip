@@ -383,12 +383,18 @@ __global__ __launch_bounds__(CHAIN_STAGE) void k_seed_chain(DevMesh m, ChainGeom
                                                            const int2 *__restrict__ info, int lsc, int seed0,
                                                            int *__restrict__ out_elem, int *__restrict__ out_left,
                                                            double *__restrict__ out_lt, ErrBuf eb) {
-    /* the whole block stages CHAIN_STAGE points' data and results through LDS;
-     * wave 0 alone carries the chain state across them */
+    /* The block stages CHAIN_STAGE points through LDS.  Every wave resolves
+     * its own chunk of 64 as far as the state allows (results that cannot
+     * depend on the chain, and ballot masks of the events that move the
+     * state); wave 0 then walks the chunks in order, advancing the state from
+     * the masks alone where the chunk allowed it and replaying the chunk
+     * point by point where it did not. */
     __shared__ int2 s_info[CHAIN_STAGE];
     __shared__ int s_elem[CHAIN_STAGE], s_left[CHAIN_STAGE];
     __shared__ double s_lt[CHAIN_STAGE];
-    const int lane = threadIdx.x & 63;
+    __shared__ unsigned long long s_mask[CHAIN_STAGE / 64][5]; /* hits, row end, row start, plane start, plane end */
+    __shared__ int s_flag[CHAIN_STAGE / 64][2];                /* chunk independent of the state, all outside */
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const bool uns = m.mesh_type == 1;
     int found = 0;                 /* list chain */
     int seed = seed0, guess = seed0 >= 0 ? seed0 : -1;
@@ -396,46 +402,58 @@ __global__ __launch_bounds__(CHAIN_STAGE) void k_seed_chain(DevMesh m, ChainGeom
     for (int64_t stage = 0; stage < n; stage += CHAIN_STAGE) {
         {
             const int64_t i = stage + threadIdx.x;
-            s_info[threadIdx.x] = i < n ? info[i] : make_int2(-1, SF_OUTB);
+            const bool active = i < n;
+            const int2 my = active ? info[i] : make_int2(-1, SF_OUTB);
+            s_info[threadIdx.x] = my;
+            const bool outb = (my.y & SF_OUTB) != 0;
+            const bool indep = outb || !uns || (lsc && !(my.y & SF_MULTI));
+            int left = outb ? 1 : 0, elem = -1;
+            double lt = 0.0;
+            bool hit = false;
+            if (active && !left && uns) {
+                if (my.x < 0) {
+                    left = 1;
+                    lt = -1.0;
+                } else {
+                    elem = my.x;
+                    hit = true;
+                }
+            }
+            if (g.chain == 0) lt = left ? -1.0 : 0.0;
+            if (g.chain == 2 && left) lt = -1.0;
+            s_elem[threadIdx.x] = elem;
+            s_left[threadIdx.x] = left;
+            s_lt[threadIdx.x] = lt;
+            const unsigned long long hits = __ballot(hit);
+            const unsigned long long rowend = __ballot(active && (my.y & SF_ROWEND));
+            const unsigned long long rowstart = __ballot(active && (my.y & SF_ROWSTART));
+            const unsigned long long planestart = __ballot(active && (my.y & SF_PLANESTART));
+            const unsigned long long planeend = __ballot(active && (my.y & SF_PLANEEND));
+            const bool chunk_indep = __ballot(active && !indep) == 0ull;
+            const bool all_outb = __ballot(active && !(outb || !uns)) == 0ull;
+            if (lane == 0) {
+                s_mask[wv][0] = hits;
+                s_mask[wv][1] = rowend;
+                s_mask[wv][2] = rowstart;
+                s_mask[wv][3] = planestart;
+                s_mask[wv][4] = planeend;
+                s_flag[wv][0] = chunk_indep;
+                s_flag[wv][1] = all_outb;
+            }
         }
         __syncthreads();
         if (threadIdx.x < 64) {
             for (int64_t base = stage; base < n && base < stage + CHAIN_STAGE; base += 64) {
-                const bool active = base + lane < n;
+                const int ck = (int)((base - stage) >> 6);
                 const int2 my = s_info[base - stage + lane];
-                int r_elem = -1, r_left = 0;
-                double r_lt = 0.0;
-                /* fast path test */
-                const bool outb = (my.y & SF_OUTB) != 0;
-                const bool indep = outb || !uns || (lsc && !(my.y & SF_MULTI));
                 const bool state_ok = g.chain == 0 ? found > 0 : seed >= 0;
-                const bool all_outb = __ballot(active && !(outb || !uns)) == 0ull;
-                const bool fast = __ballot(active && !indep) == 0ull && (state_ok || all_outb);
-                if (fast) {
-                    int left = outb ? 1 : 0, elem = -1;
-                    double lt = 0.0;
-                    bool hit = false;
-                    if (active && !left && uns) {
-                        if (my.x < 0) {
-                            left = 1;
-                            lt = -1.0;
-                        } else {
-                            elem = my.x;
-                            hit = true;
-                        }
-                    }
-                    if (g.chain == 0) lt = left ? -1.0 : 0.0;
-                    if (g.chain == 2 && left) lt = -1.0;
-                    r_elem = elem;
-                    r_left = left;
-                    r_lt = lt;
-                    /* advance the state segment by segment: a segment ends at a row end
-                     * (FTLE chain), so a row start can only open a segment */
-                    const unsigned long long hits = __ballot(hit);
-                    const unsigned long long rowend = __ballot(active && (my.y & SF_ROWEND));
-                    const unsigned long long rowstart = __ballot(active && (my.y & SF_ROWSTART));
-                    const unsigned long long planestart = __ballot(active && (my.y & SF_PLANESTART));
-                    const unsigned long long planeend = __ballot(active && (my.y & SF_PLANEEND));
+                if (s_flag[ck][0] && (state_ok || s_flag[ck][1])) {
+                    /* advance the state segment by segment: a segment ends at a
+                     * row end (FTLE chain), so a row start can only open one */
+                    const unsigned long long hits = s_mask[ck][0], rowend = s_mask[ck][1],
+                                             rowstart = s_mask[ck][2], planestart = s_mask[ck][3],
+                                             planeend = s_mask[ck][4];
+                    const int *ce = s_elem + (base - stage);
                     int a = 0;
                     while (a < 64) {
                         const unsigned long long from_a = ~0ull << a;
@@ -449,10 +467,10 @@ __global__ __launch_bounds__(CHAIN_STAGE) void k_seed_chain(DevMesh m, ChainGeom
                         const unsigned long long sh = hits & seg;
                         if (sh) {
                             const int last = 63 - __clzll(sh), first = __ffsll((long long)sh) - 1;
-                            guess = __shfl(elem, last, 64);
+                            guess = ce[last];
                             if (g.chain == 0) found += __popcll(sh);
                             if (g.chain == 2) {
-                                const int e1 = __shfl(elem, first, 64);
+                                const int e1 = ce[first];
                                 if (jguess < 0) jguess = e1;
                                 if (iguess < 0) iguess = e1;
                             }
@@ -464,6 +482,8 @@ __global__ __launch_bounds__(CHAIN_STAGE) void k_seed_chain(DevMesh m, ChainGeom
                         a = b + 1;
                     }
                 } else {
+                    int r_elem = -1, r_left = 0;
+                    double r_lt = 0.0;
                     const int lim = (int)min((int64_t)64, n - base);
                     for (int l = 0; l < lim; l++) {
                         const int64_t i = base + l;
@@ -542,10 +562,10 @@ __global__ __launch_bounds__(CHAIN_STAGE) void k_seed_chain(DevMesh m, ChainGeom
                             r_lt = lt;
                         }
                     }
+                    s_elem[base - stage + lane] = r_elem;
+                    s_left[base - stage + lane] = r_left;
+                    s_lt[base - stage + lane] = r_lt;
                 }
-                s_elem[base - stage + lane] = r_elem;
-                s_left[base - stage + lane] = r_left;
-                s_lt[base - stage + lane] = r_lt;
             }
         }
         __syncthreads();
