@@ -353,7 +353,8 @@ def bench_serve(args, np, fvc, host):
     reqs = serve_requests(np, host, tube, args.serve_rate, args.serve_window, 11)
     log("serve setup %.1f s: %d nodes, %d tets, 100 %s slices, %d requests over %.0f s" %
         (time.time() - t0, xyz.shape[0], conn.shape[0], args.serve_precision, len(reqs), args.serve_window))
-    S = fvc.Server(ctx, 0.0, 0.01, 100, 1e-3, hbm_fraction=0.6, steps_per_launch=args.serve_budget)
+    S = fvc.Server(ctx, 0.0, 0.01, 100, 1e-3, hbm_fraction=0.6, steps_per_launch=args.serve_budget,
+                   use_graph=not args.serve_no_graph)
     cap, pool_bytes = S.capacity()
 
     def submit(r):
@@ -388,6 +389,7 @@ def bench_serve(args, np, fvc, host):
             if i == len(reqs):
                 break
             time.sleep(max(0.0, min(0.01, reqs[i]["t"] - (S.clock() - clock0))))
+    sstats = S.stats()
     lat, steps, t_end = [], 0, 0.0
     served = {}
     for k, rid in enumerate(ids):
@@ -408,7 +410,7 @@ def bench_serve(args, np, fvc, host):
     # batch-at-a-time: every request alone through the plain path, FIFO
     service = []
     same = 0
-    for k, r in enumerate(reqs):
+    for k, r in enumerate(reqs if args.serve_baseline else []):
         t1 = time.perf_counter()
         if r["kind"] == "tracer":
             recs = ctx.seed_list(r["spec"], 1)
@@ -445,7 +447,7 @@ def bench_serve(args, np, fvc, host):
     for r, sv in zip(reqs, service):
         fin = max(fin, r["t"]) + sv
         blat.append(fin - r["t"])
-    blat = np.array(blat)
+    blat = np.array(blat if blat else [float("nan")])
     value = steps / makespan
     result = {
         "metric": "particle-steps/s + FTLE seeds/s at stated FP64 error; p50/p99 request latency",
@@ -466,8 +468,10 @@ def bench_serve(args, np, fvc, host):
         "batch_at_a_time": {"p50_latency_ms": 1e3 * float(np.median(blat)),
                             "p99_latency_ms": 1e3 * float(np.percentile(blat, 99)),
                             "busy_s": float(sum(service)), "busy_s_by_kind": svc_by_kind,
-                            "throughput_particle_steps_per_s": steps / max(fin - reqs[0]["t"], 1e-9)},
+                            "throughput_particle_steps_per_s": steps / max(fin - reqs[0]["t"], 1e-9)}
+        if args.serve_baseline else None,
         "served": {"makespan_s": makespan, "peak_running": peak_running, "ticks": ticks, "poll_s": t_poll,
+                   "host_phases_s": sstats,
                    "latency_p50_ms_by_kind": {kd: 1e3 * float(np.median([l for l, r in zip(lat, reqs)
                                                                          if r["kind"] == kd] or [0]))
                                               for kd in ("tracer", "ftle")},
@@ -535,6 +539,9 @@ def main():
     ap.add_argument("--serve-budget", type=int, default=32)
     ap.add_argument("--serve-precision", default="bf16", choices=["f32", "bf16", "f16"])
     ap.add_argument("--serve-check", type=int, default=40)
+    ap.add_argument("--serve-no-graph", action="store_true", help="server launches each tick directly")
+    ap.add_argument("--serve-baseline", type=int, default=1, choices=[0, 1],
+                    help="also run every request alone (batch-at-a-time comparison)")
     ap.add_argument("--releases", type=int, default=64)
     ap.add_argument("--batch-cpu-stride", type=int, default=10)
     ap.add_argument("--batch-graph", type=int, default=1, choices=[0, 1],

@@ -981,6 +981,11 @@ extern "C" int fvc_ftle_points(fvc_ftle *f, int64_t *n, int64_t *inside) {
     return FVC_OK;
 }
 
+extern "C" int fvc_ftle_records(fvc_ftle *f, fvc_point *recs) {
+    if (!f || !recs) return FVC_E_INVALID;
+    return fvc_particles_get(f->pts, 0, recs, f->n);
+}
+
 /* ------------------------------------------------------------------------ */
 /* Z-order                                                                    */
 /* ------------------------------------------------------------------------ */

@@ -482,6 +482,74 @@ __global__ __launch_bounds__(CHAIN_STAGE) void k_seed_chain(DevMesh m, ChainGeom
                         a = b + 1;
                     }
                 } else {
+                    /* Points whose result depends on the guess: every lane walks at
+                     * once from the result of the nearest earlier lane that found an
+                     * element (or the chunk's entry guess), and walks again while
+                     * that guess changes.  Lane l is final once lanes < l are, so
+                     * this ends within 64 rounds, usually 2-3, with the sequential
+                     * results.  Chunks with row boundaries, before the chain is
+                     * seeded, or where a walk fails, are replayed point by point. */
+                    const bool state_ok2 = g.chain == 0 ? found > 0 : seed >= 0;
+                    bool done = false;
+                    if (uns && state_ok2 && ((s_mask[ck][1] | s_mask[ck][2]) == 0ull)) {
+                        const bool active = base + lane < n;
+                        const bool outb = !active || (my.y & SF_OUTB);
+                        const bool walker = !outb && !(lsc && !(my.y & SF_MULTI));
+                        double X[3] = {0.0, 0.0, 0.0};
+                        if (walker) chain_point(g, base + lane, X);
+                        int r = outb ? -1 : my.x; /* direct lanes: final; walkers: first estimate */
+                        int used = -3;
+                        bool bad = false;
+                        for (int round = 0;; round++) {
+                            const unsigned long long fm = __ballot(!outb && r >= 0);
+                            const unsigned long long below = fm & ((1ull << lane) - 1ull);
+                            const int src = below ? 63 - __clzll(below) : 0;
+                            int gl = __shfl(r, src, 64);
+                            if (!below) gl = guess;
+                            const bool need = walker && gl != used;
+                            if (__ballot(need) == 0ull) break;
+                            if (round == 65) {
+                                bad = true;
+                                break;
+                            }
+                            if (need) {
+                                int err = 0;
+                                int idx = local_search<DIM>(m, X[0], X[1], X[2], gl, err);
+                                if (g.chain != 0 && idx < 0 && !err)
+                                    idx = local_search<DIM>(m, X[0], X[1], X[2], seed, err);
+                                if (idx < 0 && !err && lsc) idx = my.x;
+                                if (err) bad = true;
+                                r = idx;
+                                used = gl;
+                            }
+                            if (__ballot(bad)) {
+                                bad = true;
+                                break;
+                            }
+                        }
+                        if (!bad) {
+                            const bool left = outb || r < 0;
+                            double lt = (!outb && r < 0) ? -1.0 : 0.0;
+                            if (g.chain == 0) lt = left ? -1.0 : 0.0;
+                            if (g.chain == 2 && left) lt = -1.0;
+                            s_elem[base - stage + lane] = left ? -1 : r;
+                            s_left[base - stage + lane] = left ? 1 : 0;
+                            s_lt[base - stage + lane] = lt;
+                            const unsigned long long fm = __ballot(!left);
+                            if (fm) {
+                                const int last = 63 - __clzll(fm), first = __ffsll((long long)fm) - 1;
+                                guess = __shfl(r, last, 64);
+                                if (g.chain == 0) found += __popcll(fm);
+                                if (g.chain == 2) {
+                                    const int e1 = __shfl(r, first, 64);
+                                    if (jguess < 0) jguess = e1;
+                                    if (iguess < 0) iguess = e1;
+                                }
+                            }
+                            done = true;
+                        }
+                    }
+                    if (!done) {
                     int r_elem = -1, r_left = 0;
                     double r_lt = 0.0;
                     const int lim = (int)min((int64_t)64, n - base);
@@ -565,6 +633,7 @@ __global__ __launch_bounds__(CHAIN_STAGE) void k_seed_chain(DevMesh m, ChainGeom
                     s_elem[base - stage + lane] = r_elem;
                     s_left[base - stage + lane] = r_left;
                     s_lt[base - stage + lane] = r_lt;
+                    }
                 }
             }
         }

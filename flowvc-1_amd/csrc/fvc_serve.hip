@@ -454,7 +454,6 @@ struct HostReq {
     int64_t base = -1;
     int f0 = 0, frame = 0;
     double del[3] = {0, 0, 0};
-    std::vector<int> perm;
     double t_submit = 0, t_admit = 0, t_done = 0;
     int64_t steps = 0;
     std::vector<fvc_point> recs;
@@ -476,7 +475,7 @@ struct fvc_server {
     void *buf;
     /* pool-wide per-slot arrays */
     double *Xn[3], *exit_time, *tc, *h, *wt, *wot;
-    int *elem_new, *exited, *advected, *have, *req_id, *frame, *flags, *mpl, *lists[2][2];
+    int *elem_new, *exited, *advected, *have, *req_id, *frame, *flags, *mpl, *perm, *lists[2][2];
     int4 *mp;
     /* small device tables */
     SrvReq *d_req;
@@ -500,6 +499,7 @@ struct fvc_server {
     int64_t next_id;
     bool pending;
     double t_origin;
+    double st_wait, st_admit, st_frame, st_retire, st_ticks;
 };
 
 static int srv_err(fvc_ctx *c, int code, const char *msg) {
@@ -696,12 +696,12 @@ static int admit(fvc_server *S, HostReq &R) {
     if (R.d.kind == FVC_REQ_TRACER) {
         const int64_t n = R.n;
         double *dx = nullptr, *dlt = nullptr;
-        int *de = nullptr, *dl = nullptr, *dperm = nullptr;
+        int *de = nullptr, *dl = nullptr;
+        int *dperm = S->perm + R.base; /* pool slot k holds seed perm[k] */
         FVC_HIP(hipMallocAsync((void **)&dx, sizeof(double) * 3 * n, c->stream));
         FVC_HIP(hipMallocAsync((void **)&dlt, sizeof(double) * n, c->stream));
         FVC_HIP(hipMallocAsync((void **)&de, sizeof(int) * n, c->stream));
         FVC_HIP(hipMallocAsync((void **)&dl, sizeof(int) * n, c->stream));
-        FVC_HIP(hipMallocAsync((void **)&dperm, sizeof(int) * n, c->stream));
         FVC_HIP(hipMemcpyAsync(dx, R.xyz.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
         /* location chain in input order (ReadInTraceMesh), then Z-order for the pool */
         rc = launch_seed_chain(c, 0, n, dx, nullptr, nullptr, nullptr, S->sp.lsc, de, dl, dlt);
@@ -712,15 +712,11 @@ static int admit(fvc_server *S, HostReq &R) {
             k_srv_tracer_init<<<grid256(n), 256, 0, c->stream>>>(ptrs(S->pool), R.base, n, dx, dperm, de, dl, dlt,
                                                                 S->pool->aux, S->pool->scalar, S->req_id, S->frame,
                                                                 S->flags, slot, R.f0);
-            R.perm.resize(n);
-            if (hipMemcpyAsync(R.perm.data(), dperm, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
-                rc = FVC_E_HIP;
         }
         (void)hipFreeAsync(dx, c->stream);
         (void)hipFreeAsync(dlt, c->stream);
         (void)hipFreeAsync(de, c->stream);
         (void)hipFreeAsync(dl, c->stream);
-        (void)hipFreeAsync(dperm, c->stream);
         if (rc) return rc;
         std::vector<double>().swap(R.xyz);
     } else {
@@ -742,6 +738,46 @@ static int admit(fvc_server *S, HostReq &R) {
     if ((rc = launch_release_velocity(S, R))) return rc;
     if ((rc = set_remaining(S, slot, R.n))) return rc;
     return append_range(S, R);
+}
+
+/* finished tracer request -> records in input order (slot k holds seed perm[k]) */
+__global__ void k_srv_records(PtrSet p, const int *aux, const double *scalar, const int *perm, int64_t base, int64_t n,
+                              fvc_point *out) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int64_t i = base + k;
+    fvc_point r;
+    for (int c = 0; c < 3; c++) {
+        r.X[c] = p.X[c][i];
+        r.V[c] = p.V[c][i];
+    }
+    r.ElementIndex = p.elem[i];
+    r.AuxElementIndex = aux[i];
+    r.LeftDomain = p.left[i];
+    r.pad_ = 0;
+    r.LeftDomainTime = p.left_time[i];
+    r.Scalar = scalar[i];
+    out[perm[i]] = r;
+}
+
+static int copy_records(fvc_server *S, HostReq &R) {
+    fvc_ctx *c = S->ctx;
+    R.recs.resize(R.n);
+    if (R.n == 0) return FVC_OK;
+    fvc_point *d = nullptr;
+    FVC_HIP(hipMallocAsync((void **)&d, sizeof(fvc_point) * R.n, c->stream));
+    k_srv_records<<<grid256(R.n), 256, 0, c->stream>>>(ptrs(S->pool), S->pool->aux, S->pool->scalar, S->perm, R.base,
+                                                        R.n, d);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(R.recs.data(), d, sizeof(fvc_point) * R.n, hipMemcpyDeviceToHost, c->stream);
+    (void)hipFreeAsync(d, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        c->last_error = std::string("served tracer records: ") + hipGetErrorString(e);
+        return FVC_E_HIP;
+    }
+    return FVC_OK;
 }
 
 static int finish_ftle_frame(fvc_server *S, HostReq &R, bool *complete) {
@@ -794,8 +830,8 @@ extern "C" int fvc_server_create(fvc_ctx *c, const fvc_server_params *sp_in, fvc
         return srv_err(c, FVC_E_INVALID, "server parameters out of range");
     for (int s = 0; s < sp.nslots; s++)
         if (!c->slices[s]) return srv_err(c, FVC_E_STATE, "serving needs slots 0..nslots-1 loaded");
-    /* per pool slot: particle record SoA 128 B, serving state 48 B, FTLE staging 64 B, lists 16 B */
-    const int64_t per = 128 + 48 + 64 + 16;
+    /* per pool slot: particle record SoA 128 B, serving state 52 B, FTLE staging 64 B, lists 16 B */
+    const int64_t per = 128 + 52 + 64 + 16;
     int64_t cap = sp.pool_particles;
     if (cap <= 0) {
         size_t fr = 0, tot = 0;
@@ -816,7 +852,8 @@ extern "C" int fvc_server_create(fvc_ctx *c, const fvc_server_params *sp_in, fvc
     }
     const size_t n = (size_t)cap;
     const size_t sizes[] = {8 * n, 8 * n, 8 * n, 8 * n, 8 * n, 8 * n, 8 * n, 8 * n, /* Xn[3] exit_time tc h wt wot */
-                            4 * n, 4 * n, 4 * n, 4 * n, 4 * n, 4 * n, 4 * n, 4 * n, /* elem_new exited advected have req_id frame flags mpl */
+                            /* elem_new exited advected have req_id frame flags mpl perm */
+                            4 * n, 4 * n, 4 * n, 4 * n, 4 * n, 4 * n, 4 * n, 4 * n, 4 * n,
                             16 * n, 4 * n, 4 * n, 4 * n, 4 * n,                     /* mp lists */
                             sizeof(SrvReq) * sp.max_requests, sizeof(SrvSlot) * sp.nslots, 64, 4 * 2 * NQ,
                             4 * (size_t)sp.max_requests, 8 * (size_t)sp.max_requests, 4 * 2 * ((size_t)S->ntiles + 1)};
@@ -848,6 +885,7 @@ extern "C" int fvc_server_create(fvc_ctx *c, const fvc_server_params *sp_in, fvc
     S->frame = (int *)take();
     S->flags = (int *)take();
     S->mpl = (int *)take();
+    S->perm = (int *)take();
     S->mp = (int4 *)take();
     S->lists[0][0] = (int *)take();
     S->lists[0][1] = (int *)take();
@@ -889,7 +927,7 @@ extern "C" int fvc_server_create(fvc_ctx *c, const fvc_server_params *sp_in, fvc
     S->pending = false;
     S->t_origin = now_s();
     /* launch groups for both list parities */
-    for (int par = 0; par < 2; par++) {
+    for (int par = 0; par < 2 && !sp.no_graph; par++) {
         if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
             fvc_server_destroy(S);
             return srv_err(c, FVC_E_HIP, "server graph capture");
@@ -971,10 +1009,12 @@ extern "C" int fvc_server_poll(fvc_server *S, int32_t *running, int32_t *queued)
     if (S->pending) {
         std::vector<int> rem(M);
         std::vector<unsigned long long> st(M);
+        double t0 = now_s();
         FVC_HIP(hipMemcpyAsync(rem.data(), S->remaining, sizeof(int) * M, hipMemcpyDeviceToHost, c->stream));
         FVC_HIP(hipMemcpyAsync(st.data(), S->req_steps, sizeof(unsigned long long) * M, hipMemcpyDeviceToHost,
                                c->stream));
         if ((rc = check_err(c, "fvc_server_poll"))) return rc;
+        S->st_wait += now_s() - t0;
         S->pending = false;
         for (int s = 0; s < M; s++) {
             const int64_t id = S->slot_owner[s];
@@ -982,20 +1022,20 @@ extern "C" int fvc_server_poll(fvc_server *S, int32_t *running, int32_t *queued)
             HostReq &R = S->reqs[id];
             R.steps = (int64_t)st[s];
             if (rem[s] != 0) continue;
+            t0 = now_s();
             if (R.d.kind == FVC_REQ_TRACER) {
-                std::vector<fvc_point> tmp(R.n);
-                if (R.n && (rc = fvc_particles_get(S->pool, R.base, tmp.data(), R.n))) return rc;
-                R.recs.resize(R.n);
-                for (int64_t k = 0; k < R.n; k++) R.recs[R.perm[k]] = tmp[k];
-                std::vector<int>().swap(R.perm);
+                if ((rc = copy_records(S, R))) return rc;
                 retire(S, R, FVC_RS_DONE);
+                S->st_retire += now_s() - t0;
             } else {
                 bool complete = false;
                 if ((rc = finish_ftle_frame(S, R, &complete))) return rc;
                 if (complete) retire(S, R, FVC_RS_DONE);
+                S->st_frame += now_s() - t0;
             }
         }
     }
+    const double ta = now_s();
     /* admission: arrival order, first fit (a request that does not fit waits;
      * smaller later ones may pass it unless it has waited more than 2 s) */
     const double tnow = now_s() - S->t_origin;
@@ -1022,16 +1062,33 @@ extern "C" int fvc_server_poll(fvc_server *S, int32_t *running, int32_t *queued)
         }
         it = S->queued.erase(it);
     }
+    S->st_admit += now_s() - ta;
     int nrun = 0;
     for (int s = 0; s < M; s++) nrun += S->slot_owner[s] >= 0;
     if (nrun > 0) {
-        FVC_HIP(hipGraphLaunch(S->gexec[S->cur], c->stream));
+        if (S->sp.no_graph) {
+            enqueue_tick(S, S->cur);
+            FVC_HIP(hipGetLastError());
+        } else {
+            FVC_HIP(hipGraphLaunch(S->gexec[S->cur], c->stream));
+        }
         S->cur ^= 1;
         S->pending = true;
+        S->st_ticks += 1;
     }
     if (running) *running = nrun;
     if (queued) *queued = (int32_t)S->queued.size();
     return rc;
+}
+
+extern "C" int fvc_server_stats(fvc_server *S, double out[5]) {
+    if (!S || !out) return FVC_E_INVALID;
+    out[0] = S->st_wait;
+    out[1] = S->st_admit;
+    out[2] = S->st_frame;
+    out[3] = S->st_retire;
+    out[4] = S->st_ticks;
+    return FVC_OK;
 }
 
 extern "C" int fvc_server_info(fvc_server *S, int64_t id, fvc_request_info *info) {

@@ -49,7 +49,7 @@ class ServerParams(ctypes.Structure):
     _fields_ = [("frame_t0", ctypes.c_double), ("frame_dt", ctypes.c_double), ("nslots", ctypes.c_int32),
                 ("time_step", ctypes.c_double), ("pool_particles", ctypes.c_int64), ("hbm_fraction", ctypes.c_double),
                 ("max_requests", ctypes.c_int32), ("block", ctypes.c_int32), ("steps_per_launch", ctypes.c_int32),
-                ("lsc", ctypes.c_int32)]
+                ("lsc", ctypes.c_int32), ("no_graph", ctypes.c_int32)]
 
 
 class RequestDesc(ctypes.Structure):
@@ -113,6 +113,7 @@ def lib():
         L.fvc_ftle_frame.argtypes = [vp, i32, i32, i32, d, d, ctypes.POINTER(IntParams), ctypes.POINTER(i32)]
         L.fvc_ftle_output.argtypes = [vp, d, vp, vp]
         L.fvc_ftle_points.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]
+        L.fvc_ftle_records.argtypes = [vp, vp]
         L.fvc_zsort.argtypes = [vp, vp, i64, vp, vp, vp]
         L.fvc_particles_copy.argtypes = [vp, i64, vp, i64, i64]
         L.fvc_particles_get_xyz.argtypes = [vp, i64, vp, vp, i64]
@@ -133,6 +134,7 @@ def lib():
         L.fvc_server_clock.restype = d
         L.fvc_server_clock.argtypes = [vp]
         L.fvc_server_info.argtypes = [vp, i64, ctypes.POINTER(RequestInfo)]
+        L.fvc_server_stats.argtypes = [vp, vp]
         L.fvc_server_take.argtypes = [vp, i64, vp, vp, vp]
         _LIB = L
     return _LIB
@@ -325,6 +327,11 @@ class FtleSlide:
     def reset(self):
         _chk(lib().fvc_ftle_reset(self.h), "fvc_ftle_reset", self.ctx.h)
 
+    def records(self):
+        out = np.zeros(self.n, POINT_DTYPE)
+        _chk(lib().fvc_ftle_records(self.h, _p(out)), "fvc_ftle_records", self.ctx.h)
+        return out
+
     def output(self, int_t):
         wt = np.empty(self.n, np.float64)
         wot = np.empty(self.n, np.float64)
@@ -397,10 +404,10 @@ class Server:
     (fvc_server_*)."""
 
     def __init__(self, ctx, frame_t0, frame_dt, nslots, dt, pool_particles=0, hbm_fraction=0.5, max_requests=256,
-                 block=256, steps_per_launch=32, lsc=1):
+                 block=256, steps_per_launch=32, lsc=1, use_graph=True):
         self.ctx = ctx
         sp = ServerParams(frame_t0, frame_dt, nslots, dt, pool_particles, hbm_fraction, max_requests, block,
-                          steps_per_launch, lsc)
+                          steps_per_launch, lsc, 0 if use_graph else 1)
         h = ctypes.c_void_p()
         _chk(lib().fvc_server_create(ctx.h, ctypes.byref(sp), ctypes.byref(h)), "fvc_server_create", ctx.h)
         self.h = h
@@ -433,6 +440,12 @@ class Server:
 
     def clock(self):
         return lib().fvc_server_clock(self.h)
+
+    def stats(self):
+        out = np.zeros(5, np.float64)
+        _chk(lib().fvc_server_stats(self.h, _p(out)), "fvc_server_stats", self.ctx.h)
+        return {"wait_s": out[0], "admit_s": out[1], "ftle_frame_s": out[2], "retire_s": out[3],
+                "ticks": int(out[4])}
 
     def info(self, rid):
         inf = RequestInfo()

@@ -182,6 +182,9 @@ int fvc_ftle_frame(fvc_ftle *f, int32_t direction, int32_t lo, int32_t hi, doubl
  * int_t, then FTLEwT / FTLEwoT in k,j,i order (i fastest) as OutputFTLE writes. */
 int fvc_ftle_output(fvc_ftle *f, double int_t, double *ftle_wt, double *ftle_wot);
 int fvc_ftle_points(fvc_ftle *f, int64_t *n, int64_t *inside);
+/* The slide's point records (n of them, grid index (i*res1 + j)*res2 + k), as
+ * the FTLE temp files hold them (ftle.c:616-636). */
+int fvc_ftle_records(fvc_ftle *f, fvc_point *recs);
 /* Restore the slide to its state right after fvc_ftle_create (unlaunched,
  * located grid, masks), from a device-resident copy. */
 int fvc_ftle_reset(fvc_ftle *f);
@@ -267,6 +270,7 @@ typedef struct fvc_server_params {
     int32_t block;             /* as fvc_batch_params */
     int32_t steps_per_launch;  /* step budget per particle per tick (0 = 32) */
     int32_t lsc;               /* LocalSearchChecking for seed location */
+    int32_t no_graph;          /* 1: launch each tick directly instead of replaying a captured hipGraph */
 } fvc_server_params;
 
 typedef struct fvc_request_desc {
@@ -300,6 +304,9 @@ int fvc_server_poll(fvc_server *s, int32_t *running, int32_t *queued);
 /* Server clock (seconds), the clock of fvc_request_info's times. */
 double fvc_server_clock(fvc_server *s);
 int fvc_server_info(fvc_server *s, int64_t id, fvc_request_info *info);
+/* Cumulative host seconds: [0] waiting for ticks to finish, [1] admitting,
+ * [2] FTLE frame ends, [3] retiring tracer requests; [4] ticks launched. */
+int fvc_server_stats(fvc_server *s, double out[5]);
 /* Results of a finished request (then forgotten): tracer records (info.n of
  * them, input order) or FTLE values with and without the time factor (k, j,
  * i order, as fvc_ftle_output). */

@@ -143,6 +143,7 @@ typedef struct oc_ftle {
 oc_ftle *oc_ftle_create(const oc_mesh *m, const double min[3], const double max[3], const int32_t res[3],
                         int local_search_checking, int *err);
 void oc_ftle_free(oc_ftle *f);
+const oc_ftle_pt *oc_ftle_points(const oc_ftle *f); /* n records, index (i*res1 + j)*res2 + k */
 /* One frame of ComputeFTLE for this slide (ftle.c:68-214) */
 int oc_ftle_frame(const oc_mesh *m, const oc_int *ip, oc_ftle *f, int direction, double loaded_tmin,
                   double loaded_tmax);

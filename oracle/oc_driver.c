@@ -135,6 +135,8 @@ oc_ftle *oc_ftle_create(const oc_mesh *m, const double min[3], const double max[
     return f;
 }
 
+const oc_ftle_pt *oc_ftle_points(const oc_ftle *f) { return f->pts; }
+
 void oc_ftle_free(oc_ftle *f) {
     if (!f) return;
     free(f->pts);

@@ -48,6 +48,7 @@ def lib():
         L.oc_velocity_batch.argtypes = [vp, d, i64, vp, vp]
         L.oc_eig_batch.argtypes = [i64, vp, vp]
         L.oc_seed_list.argtypes = [vp, i64, vp, ctypes.c_int, vp]
+        L.oc_seed_grid.argtypes = [vp, vp, vp, vp, ctypes.c_int, vp]
         _LIB = L
     return _LIB
 
@@ -111,6 +112,35 @@ class Mesh:
         rc = lib().oc_velocity_batch(self.h, tq, pts.shape[0], _p(pts), _p(out))
         return out, rc
 
+    def seed_grid(self, mn, dl, res, lsc=1):
+        """GenerateTracerMesh chain (tracers.c:1298-1532)."""
+        mn, dl = np.asarray(mn, np.float64), np.asarray(dl, np.float64)
+        res = np.asarray(res, np.int32)
+        out = np.zeros(int(np.prod(res)), POINT_DTYPE)
+        rc = lib().oc_seed_grid(self.h, _p(mn), _p(dl), _p(res), lsc, _p(out))
+        return out, rc
+
+    def ftle_grid(self, mn, mx, res, lsc=1):
+        """InitializeFTLEArray's grid records (ftle.c:383-609), grid index (i*res1 + j)*res2 + k."""
+        mn, mx = np.asarray(mn, np.float64), np.asarray(mx, np.float64)
+        res = np.asarray(res, np.int32)
+        err = ctypes.c_int(0)
+        L = lib()
+        L.oc_ftle_create.restype = ctypes.c_void_p
+        L.oc_ftle_create.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_int)]
+        L.oc_ftle_free.argtypes = [ctypes.c_void_p]
+        L.oc_ftle_points.restype = ctypes.c_void_p
+        L.oc_ftle_points.argtypes = [ctypes.c_void_p]
+        f = L.oc_ftle_create(self.h, _p(mn), _p(mx), _p(res), lsc, ctypes.byref(err))
+        if not f:
+            raise RuntimeError("oc_ftle_create failed")
+        n = int(np.prod(res))
+        buf = (ctypes.c_char * (104 * n)).from_address(L.oc_ftle_points(f))
+        rec = np.frombuffer(bytes(buf), dtype=FTLE_PT_DTYPE).copy()
+        L.oc_ftle_free(f)
+        return rec, err.value
+
     def seed_list(self, xyz, lsc=1):
         xyz = np.ascontiguousarray(xyz, np.float64)
         out = np.zeros(xyz.shape[0], POINT_DTYPE)
@@ -124,6 +154,9 @@ def max_eig(mats):
     lib().oc_eig_batch(mats.shape[0], _p(mats), _p(out))
     return out
 
+
+FTLE_PT_DTYPE = np.dtype([("Pt", POINT_DTYPE), ("HaveFTLE", "<i4"), ("pad", "<i4"), ("FTLEwT", "<f8"),
+                          ("FTLEwoT", "<f8")])
 
 HIST_DTYPE = np.dtype([("F1", "<f8", 3), ("F2", "<f8", 3), ("nhist", "<i4"), ("pad", "<i4")])
 

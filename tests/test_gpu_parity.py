@@ -198,3 +198,39 @@ def test_adams_bashforth_f64_bit_exact(pipe, order):
     np.testing.assert_array_equal(got["X"], ref["X"])
     np.testing.assert_array_equal(got["LeftDomain"], ref["LeftDomain"])
     np.testing.assert_array_equal(got["ElementIndex"], ref["ElementIndex"])
+
+
+def _face_grid(res):
+    """A grid whose points are the pipe mesh's voxel corners: every point lies on
+    faces of several elements, so each one's element depends on the chain."""
+    return [-1.0, -1.0, 0.0], [1.0, 1.0, 10.0], res
+
+
+@pytest.mark.parametrize("lsc", [1, 0])
+def test_grid_chains_on_faces_match_oracle(pipe, lsc):
+    """Tracer-grid (GenerateTracerMesh) and FTLE-grid (InitializeFTLEArray)
+    location chains where every point is on element faces, so the chain's
+    guesses decide: element indices equal the oracle's."""
+    t, xyz, conn, adj, _ = pipe
+    ctx = fvc.Context(3, PIPE_BOUNDS, "f32", 2, coords=xyz, conn=conn, adj=adj)
+    om = oracle.Mesh(3, PIPE_BOUNDS, coords=xyz, conn=conn, adj=adj)
+    mn, mx, res = _face_grid([13, 13, 41])
+    dl = [(mx[a] - mn[a]) / (res[a] - 1) for a in range(3)]
+    print("tracer grid chain lsc=%d" % lsc, flush=True)
+    a = ctx.seed_grid(mn, dl, res, lsc)
+    b, rc = om.seed_grid(mn, dl, res, lsc)
+    assert rc == 0
+    inside = b["LeftDomain"] == 0
+    np.testing.assert_array_equal(a["LeftDomain"], b["LeftDomain"])
+    np.testing.assert_array_equal(a["ElementIndex"][inside], b["ElementIndex"][inside])
+    np.testing.assert_array_equal(a["LeftDomainTime"], b["LeftDomainTime"])
+    print("FTLE grid chain lsc=%d" % lsc, flush=True)
+    F = fvc.FtleSlide(ctx, mn, mx, res, lsc, 0.0, 0.02)
+    g = F.records()
+    o, rc = om.ftle_grid(mn, mx, res, lsc)
+    assert rc == 0
+    inside = o["Pt"]["LeftDomain"] == 0
+    np.testing.assert_array_equal(g["LeftDomain"], o["Pt"]["LeftDomain"])
+    np.testing.assert_array_equal(g["ElementIndex"][inside], o["Pt"]["ElementIndex"][inside])
+    np.testing.assert_array_equal(g["LeftDomainTime"], o["Pt"]["LeftDomainTime"])
+    assert (g["LeftDomain"] == 0).sum() > 1000

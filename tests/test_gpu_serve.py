@@ -109,8 +109,8 @@ def same(a, b, kind):
         np.testing.assert_array_equal(a[1], b[1])
 
 
-@pytest.mark.parametrize("precision", ["f64", "f32"])
-def test_served_results_equal_alone_and_in_mix(pipe, precision):
+@pytest.mark.parametrize("precision,graph", [("f64", True), ("f32", True), ("f32", False)])
+def test_served_results_equal_alone_and_in_mix(pipe, precision, graph):
     t = pipe[0]
     ctx = _ctx(pipe, precision)
     reqs = make_requests(t, 4)
@@ -120,7 +120,7 @@ def test_served_results_equal_alone_and_in_mix(pipe, precision):
     sizes = [rq[1].shape[0] if rq[0] == "tracer" else int(np.prod(rq[1][2])) for rq in reqs]
     pool = 3000  # holds the largest request, not the first five together
     assert max(sizes) <= pool < sum(sizes[:5])
-    S = fvc.Server(ctx, 0.0, FDT, NSLOTS, DT, pool_particles=pool, steps_per_launch=7)
+    S = fvc.Server(ctx, 0.0, FDT, NSLOTS, DT, pool_particles=pool, steps_per_launch=7, use_graph=graph)
     for rq, r in zip(reqs, ref):
         rid = submit(S, rq)
         S.run_until_idle()
@@ -141,6 +141,8 @@ def test_served_results_equal_alone_and_in_mix(pipe, precision):
         if r == 0 and q == 0:
             break
     assert saw_queue
+    st = S.stats()
+    assert st["ticks"] > 0 and st["wait_s"] >= 0.0
     for rq, r, rid in zip(reqs, ref, ids):
         inf = S.info(rid)
         assert inf["state"] == fvc.RS_DONE
