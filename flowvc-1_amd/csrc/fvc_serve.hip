@@ -95,7 +95,7 @@ __device__ inline DevField srv_field(const SrvDev &S, int f) {
 
 /* one particle for at most `budget` steps; returns the steps it took */
 template <int MODE, int GEN>
-__device__ uint32_t serve_particle(const DevMesh &m, const SrvDev &S, int i, int budget, int &r_out) {
+__device__ uint32_t serve_particle(const DevMesh &m, const SrvDev &S, int i, int budget, int &r_out, int &fin) {
 #pragma clang fp contract(off)
     int fl = S.flags[i];
     if (fl & (SF_DONE | SF_WAIT)) return 0;
@@ -140,7 +140,7 @@ __device__ uint32_t serve_particle(const DevMesh &m, const SrvDev &S, int i, int
     }
     if (!(s.left_time > -TINY && !s.left)) { /* not integrated (tracers.c:224, ftle.c:113) */
         S.flags[i] = ftle ? SF_WAIT : SF_DONE;
-        atomicSub(&S.remaining[r], 1);
+        fin = 1;
         return 0;
     }
     int err = 0;
@@ -205,7 +205,7 @@ __device__ uint32_t serve_particle(const DevMesh &m, const SrvDev &S, int i, int
         S.mp[i] = make_int4(is.mp[0].elem, is.mp[1].elem, is.mp[2].elem, is.mp[3].elem);
         S.mpl[i] = is.mp[0].left | (is.mp[1].left << 1) | (is.mp[2].left << 2) | (is.mp[3].left << 3);
     }
-    if (fl & (SF_DONE | SF_WAIT)) atomicSub(&S.remaining[r], 1);
+    fin = (fl & (SF_DONE | SF_WAIT)) ? 1 : 0;
     return ns;
 }
 
@@ -226,19 +226,24 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(3))) void k_
             base = __shfl(base, 0, 64);
             if (seg_lo + base >= seg_hi) break;
             const int idx = seg_lo + base + lane;
-            int r = -1;
+            int r = -1, fin = 0;
             uint32_t ns = 0;
-            if (idx < seg_hi) ns = serve_particle<MODE, GEN>(m, S, S.list_in[idx], budget, r);
+            if (idx < seg_hi) ns = serve_particle<MODE, GEN>(m, S, S.list_in[idx], budget, r, fin);
             total += ns;
-            /* per-request step counts: one atomic per wave when the wave serves one request */
+            /* per-request step counts and phase counters: one atomic per wave
+             * when the wave serves one request (the usual case: requests own
+             * contiguous list ranges), per lane otherwise */
             const int r0 = __shfl(r, 0, 64);
             const bool uniform = __ballot(r != r0 && r >= 0) == 0ull;
+            const unsigned long long fb = __ballot(fin);
             unsigned long long v = ns;
             if (uniform) {
                 for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
                 if (lane == 0 && r0 >= 0 && v) atomicAdd(&S.req_steps[r0], v);
-            } else if (r >= 0 && ns) {
-                atomicAdd(&S.req_steps[r], (unsigned long long)ns);
+                if (lane == 0 && r0 >= 0 && fb) atomicSub(&S.remaining[r0], __popcll(fb));
+            } else {
+                if (r >= 0 && ns) atomicAdd(&S.req_steps[r], (unsigned long long)ns);
+                if (fin) atomicSub(&S.remaining[r], 1);
             }
         }
     }
