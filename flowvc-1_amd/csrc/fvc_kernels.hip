@@ -483,30 +483,55 @@ __global__ __launch_bounds__(CHAIN_STAGE) void k_seed_chain(DevMesh m, ChainGeom
                     }
                 } else {
                     /* Points whose result depends on the guess: every lane walks at
-                     * once from the result of the nearest earlier lane that found an
-                     * element (or the chunk's entry guess), and walks again while
-                     * that guess changes.  Lane l is final once lanes < l are, so
-                     * this ends within 64 rounds, usually 2-3, with the sequential
-                     * results.  Chunks with row boundaries, before the chain is
-                     * seeded, or where a walk fails, are replayed point by point. */
+                     * once from the guess the chain would hand it given the other
+                     * lanes' current results (a 64-step scan of the chain state,
+                     * row and plane resets included), and walks again while that
+                     * guess changes.  Lane l is final once lanes < l are, so this
+                     * ends within 64 rounds, usually 2-3, with the sequential
+                     * results.  Before the chain is seeded, or where a walk
+                     * fails, the chunk is replayed point by point. */
                     const bool state_ok2 = g.chain == 0 ? found > 0 : seed >= 0;
                     bool done = false;
-                    if (uns && state_ok2 && ((s_mask[ck][1] | s_mask[ck][2]) == 0ull)) {
+                    if (uns && state_ok2) {
                         const bool active = base + lane < n;
                         const bool outb = !active || (my.y & SF_OUTB);
                         const bool walker = !outb && !(lsc && !(my.y & SF_MULTI));
+                        const int lim = (int)min((int64_t)64, n - base);
                         double X[3] = {0.0, 0.0, 0.0};
                         if (walker) chain_point(g, base + lane, X);
                         int r = outb ? -1 : my.x; /* direct lanes: final; walkers: first estimate */
                         int used = -3;
                         bool bad = false;
+                        int gs = guess, jg = jguess, ig = iguess, fc = found;
                         for (int round = 0;; round++) {
-                            const unsigned long long fm = __ballot(!outb && r >= 0);
-                            const unsigned long long below = fm & ((1ull << lane) - 1ull);
-                            const int src = below ? 63 - __clzll(below) : 0;
-                            int gl = __shfl(r, src, 64);
-                            if (!below) gl = guess;
-                            const bool need = walker && gl != used;
+                            /* the chain state before each lane, from the current results */
+                            gs = guess;
+                            jg = jguess;
+                            ig = iguess;
+                            fc = found;
+                            int my_g = -1;
+                            for (int l = 0; l < lim; l++) {
+                                const int fl = __shfl(my.y, l, 64);
+                                const int rl = __shfl(r, l, 64);
+                                if (g.chain == 2 && (fl & SF_ROWSTART)) {
+                                    jg = -1;
+                                    if (fl & SF_PLANESTART) ig = -1;
+                                }
+                                if (lane == l) my_g = gs;
+                                if (!(fl & SF_OUTB) && rl >= 0) {
+                                    gs = rl;
+                                    fc++;
+                                    if (g.chain == 2) {
+                                        if (jg < 0) jg = rl;
+                                        if (ig < 0) ig = rl;
+                                    }
+                                }
+                                if (g.chain == 2 && (fl & SF_ROWEND)) {
+                                    if (jg >= 0) gs = jg;
+                                    if ((fl & SF_PLANEEND) && ig >= 0) gs = ig;
+                                }
+                            }
+                            const bool need = walker && my_g != used;
                             if (__ballot(need) == 0ull) break;
                             if (round == 65) {
                                 bad = true;
@@ -514,13 +539,13 @@ __global__ __launch_bounds__(CHAIN_STAGE) void k_seed_chain(DevMesh m, ChainGeom
                             }
                             if (need) {
                                 int err = 0;
-                                int idx = local_search<DIM>(m, X[0], X[1], X[2], gl, err);
+                                int idx = local_search<DIM>(m, X[0], X[1], X[2], my_g, err);
                                 if (g.chain != 0 && idx < 0 && !err)
                                     idx = local_search<DIM>(m, X[0], X[1], X[2], seed, err);
                                 if (idx < 0 && !err && lsc) idx = my.x;
                                 if (err) bad = true;
                                 r = idx;
-                                used = gl;
+                                used = my_g;
                             }
                             if (__ballot(bad)) {
                                 bad = true;
@@ -535,17 +560,10 @@ __global__ __launch_bounds__(CHAIN_STAGE) void k_seed_chain(DevMesh m, ChainGeom
                             s_elem[base - stage + lane] = left ? -1 : r;
                             s_left[base - stage + lane] = left ? 1 : 0;
                             s_lt[base - stage + lane] = lt;
-                            const unsigned long long fm = __ballot(!left);
-                            if (fm) {
-                                const int last = 63 - __clzll(fm), first = __ffsll((long long)fm) - 1;
-                                guess = __shfl(r, last, 64);
-                                if (g.chain == 0) found += __popcll(fm);
-                                if (g.chain == 2) {
-                                    const int e1 = __shfl(r, first, 64);
-                                    if (jguess < 0) jguess = e1;
-                                    if (iguess < 0) iguess = e1;
-                                }
-                            }
+                            guess = gs;
+                            jguess = jg;
+                            iguess = ig;
+                            found = fc;
                             done = true;
                         }
                     }
