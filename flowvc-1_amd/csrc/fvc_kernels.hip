@@ -268,6 +268,34 @@ __device__ int hash_global(const DevMesh &m, double x, double y, double z, int &
     return first;
 }
 
+/* every element with d > -TINY, ascending: the first `cap` are written to
+ * `list` (when given), the lowest is returned in `first`, the count is
+ * returned whole */
+template <int DIM>
+__device__ int hash_accepting(const DevMesh &m, double x, double y, double z, int *list, int cap, int &first) {
+    first = -1;
+    double fx = (x - m.hlo[0]) * m.hinv[0], fy = (y - m.hlo[1]) * m.hinv[1];
+    double fz = DIM == 3 ? (z - m.hlo[2]) * m.hinv[2] : 0.0;
+    if (!(fx >= 0.0 && fy >= 0.0 && fz >= 0.0)) return 0;
+    if (!(fx < (double)m.hres[0] && fy < (double)m.hres[1] && fz < (double)m.hres[2])) return 0;
+    int ix = (int)fx, iy = (int)fy, iz = (int)fz;
+    int64_t cell = ((int64_t)iz * m.hres[1] + iy) * m.hres[0] + ix;
+    int cnt = 0;
+    for (int q = m.cell_start[cell]; q < m.cell_start[cell + 1]; q++) {
+        int e = m.cell_elems[q];
+        double r, s, t = 0.0;
+        if (DIM == 3) nat3_exact(m, e, x, y, z, r, s, t);
+        else nat2_exact(m, e, x, y, r, s);
+        int slot;
+        if (decide_exact<DIM>(r, s, t, ACC_SEARCH, slot) == 1) {
+            if (cnt == 0) first = e;
+            if (list && cnt < cap) list[cnt] = e;
+            cnt++;
+        }
+    }
+    return cnt;
+}
+
 template <int DIM>
 __device__ int local_search(const DevMesh &m, double x, double y, double z, int guess, int &err) {
     double r, s, t;
@@ -348,26 +376,59 @@ __device__ inline void chain_point(const ChainGeom &g, int64_t i, double X[3]) {
     X[2] = g.mn[2] + kk * g.del[2];
 }
 
+/* bits 8-15 of the flags: number of elements holding the point (capped at 255) */
+constexpr int SF_CNT_SHIFT = 8;
+__device__ __forceinline__ int sf_count(int flags) { return (flags >> SF_CNT_SHIFT) & 255; }
+
+/* bookkeeping for the tabulated chain (all null for the replayed chain) */
+struct TabPrep {
+    int *mflag;      /* 1 where a found point lies in more than one element */
+    int *misc;       /* [0] first found point, [1] largest element count */
+    int *rowfirst;   /* FTLE chain: first found point of each k-row */
+    int *planefirst; /* FTLE chain: first found point of each j-plane */
+};
+
 template <int DIM>
-__global__ __launch_bounds__(BLOCK) void k_seed_prepare(DevMesh m, ChainGeom g, int64_t n, int2 *__restrict__ info) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double X[3];
-    chain_point(g, i, X);
-    int flags = 0, emin = -1;
-    if (g.chain == 2) {
-        const int kk = (int)(i % g.res[2]), jj = (int)((i / g.res[2]) % g.res[1]);
-        if (kk == 0) flags |= SF_ROWSTART | (jj == 0 ? SF_PLANESTART : 0);
-        if (kk == g.res[2] - 1) flags |= SF_ROWEND | (jj == g.res[1] - 1 ? SF_PLANEEND : 0);
+__global__ __launch_bounds__(BLOCK) void k_seed_prepare(DevMesh m, ChainGeom g, int64_t n, int2 *__restrict__ info,
+                                                       TabPrep tp) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = i < n; /* no early return: the bookkeeping below uses whole-wave ballots */
+    int flags = 0, emin = -1, cnt = 0;
+    if (active) {
+        double X[3];
+        chain_point(g, i, X);
+        if (g.chain == 2) {
+            const int kk = (int)(i % g.res[2]), jj = (int)((i / g.res[2]) % g.res[1]);
+            if (kk == 0) flags |= SF_ROWSTART | (jj == 0 ? SF_PLANESTART : 0);
+            if (kk == g.res[2] - 1) flags |= SF_ROWEND | (jj == g.res[1] - 1 ? SF_PLANEEND : 0);
+        }
+        if (outside_domain(m, X[0], X[1], X[2]))
+            flags |= SF_OUTB;
+        else if (m.mesh_type == 1) {
+            cnt = hash_accepting<DIM>(m, X[0], X[1], X[2], nullptr, 0, emin);
+            if (cnt > 1) flags |= SF_MULTI;
+        }
+        info[i] = make_int2(emin, flags | (min(cnt, 255) << SF_CNT_SHIFT));
     }
-    if (outside_domain(m, X[0], X[1], X[2]))
-        flags |= SF_OUTB;
-    else if (m.mesh_type == 1) {
-        int second;
-        emin = hash_global<DIM>(m, X[0], X[1], X[2], second);
-        if (second >= 0) flags |= SF_MULTI;
+    if (!tp.mflag) return;
+    const bool found = active && !(flags & SF_OUTB) && emin >= 0;
+    if (active) tp.mflag[i] = (found && cnt > 1) ? 1 : 0;
+    int wmax = cnt;
+    for (int off = 32; off; off >>= 1) wmax = max(wmax, __shfl_xor(wmax, off, 64));
+    const unsigned long long F = __ballot(found);
+    const int lane = threadIdx.x & 63;
+    if (F && lane == __ffsll((long long)F) - 1) {
+        atomicMin(tp.misc, (int)i);
+        atomicMax(tp.misc + 1, wmax);
     }
-    info[i] = make_int2(emin, flags);
+    if (g.chain == 2 && found) {
+        /* one atomic per row (plane) and wave: the wave's first found point in it */
+        const unsigned long long below = F & ((1ull << lane) - 1ull);
+        const int64_t prev = below ? i - lane + (63 - __clzll(below)) : -1;
+        const int64_t rowlen = g.res[2], planelen = rowlen * g.res[1];
+        if (prev < 0 || prev / rowlen != i / rowlen) atomicMin(tp.rowfirst + i / rowlen, (int)i);
+        if (prev < 0 || prev / planelen != i / planelen) atomicMin(tp.planefirst + i / planelen, (int)i);
+    }
 }
 
 /* One wave replays the chain over chunks of 64 consecutive points.  A chunk
@@ -668,6 +729,277 @@ __global__ __launch_bounds__(CHAIN_STAGE) void k_seed_chain(DevMesh m, ChainGeom
     }
 }
 
+/* ------------------------------------------------------------------------ */
+/* tabulated chains (LocalSearchChecking on, unstructured mesh)                */
+/* ------------------------------------------------------------------------ */
+/* With LocalSearchChecking a point inside the mesh always ends up found (its
+ * walks fall back to the lowest holding element), so which points are found,
+ * and hence which earlier point's element each guess comes from, is fixed by
+ * the flags alone.  A point held by one element gets that element whatever
+ * the guess.  A point held by several (a face, edge or vertex point) gets
+ * whichever of them its walk reaches, a function of the guess; the guess is
+ * the element of its source point, one of that point's few holding elements.
+ * So every such point walks once from each holding element of its source
+ * (all points at once), giving a small table, and the tables are composed
+ * along the source links by pointer jumping.  The results equal the
+ * point-by-point replay (tracers.c:1318-1346, ftle.c:456-516, mesh.c:496-611). */
+constexpr int TAB_K = 32;     /* holding elements kept per point; more -> replay */
+constexpr int TAB_SEED = -2;  /* source: the centre seed element */
+constexpr int TAB_NONE = -3;  /* source: none (the first found point) */
+
+/* Source of every point: the last event before it in chain order that set
+ * the guess, as a point index.  A found point sets it to itself; an FTLE row
+ * end sets it to the row's first found point, a plane end to the plane's.
+ * Within each 64-point chunk from ballots; -1 where no event precedes the
+ * point inside its chunk.  chunk_last[c]: the chunk's last event, or -1. */
+__global__ __launch_bounds__(BLOCK) void k_tab_src(ChainGeom g, int64_t n, const int2 *__restrict__ info,
+                                                   const int *__restrict__ rowfirst,
+                                                   const int *__restrict__ planefirst, int *__restrict__ src,
+                                                   int *__restrict__ chunk_last) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const bool active = i < n;
+    const int2 my = active ? info[i] : make_int2(-1, SF_OUTB);
+    const bool found = active && !(my.y & SF_OUTB) && my.x >= 0;
+    int rv = -1; /* row-end event value */
+    if (g.chain == 2 && active && (my.y & SF_ROWEND)) {
+        const int64_t rowlen = g.res[2], planelen = rowlen * g.res[1];
+        const int pf = planefirst[i / planelen], rf = rowfirst[i / rowlen];
+        if ((my.y & SF_PLANEEND) && pf < n) rv = pf;
+        else if (rf < n) rv = rf;
+    }
+    const unsigned long long F = __ballot(found), E = __ballot(rv >= 0);
+    const int64_t base = i - lane;
+    /* a found point's event comes before its own row-end event */
+    const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    const unsigned long long fb = F & below, eb = E & below;
+    const int jf = fb ? 63 - __clzll(fb) : -1, je = eb ? 63 - __clzll(eb) : -1;
+    const int rvj = __shfl(rv, je < 0 ? 0 : je, 64);
+    if (active) src[i] = (je >= 0 && je >= jf) ? rvj : (jf >= 0 ? (int)(base + jf) : -1);
+    const int cf = F ? 63 - __clzll(F) : -1, ce = E ? 63 - __clzll(E) : -1;
+    const int rvc = __shfl(rv, ce < 0 ? 0 : ce, 64);
+    if (lane == 0 && base < n) chunk_last[base >> 6] = (ce >= 0 && ce >= cf) ? rvc : (cf >= 0 ? (int)(base + cf) : -1);
+}
+
+/* carry[c]: the last event before chunk c (gs0 where there is none) */
+__global__ __launch_bounds__(1024) void k_tab_carry(int64_t nch, const int *__restrict__ chunk_last, int gs0,
+                                                    int *__restrict__ carry) {
+    __shared__ int s[1024];
+    __shared__ int s_run;
+    const int tid = threadIdx.x;
+    if (tid == 0) s_run = gs0;
+    __syncthreads();
+    for (int64_t b = 0; b < nch; b += 1024) {
+        const int64_t c = b + tid;
+        const int v = c < nch ? chunk_last[c] : -1;
+        s[tid] = v >= 0 ? tid : -1;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) { /* inclusive prefix max of positions with an event */
+            const int o = tid >= off ? s[tid - off] : -1;
+            __syncthreads();
+            s[tid] = max(s[tid], o);
+            __syncthreads();
+        }
+        const int ex = tid ? s[tid - 1] : -1;
+        if (c < nch) carry[c] = ex >= 0 ? chunk_last[b + ex] : s_run;
+        __syncthreads();
+        if (tid == 0 && s[1023] >= 0) s_run = chunk_last[b + s[1023]];
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_tab_slots(int64_t nm, const int *__restrict__ mlist, int *__restrict__ mslot) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nm) mslot[mlist[k]] = (int)k;
+}
+
+template <int DIM>
+__global__ __launch_bounds__(BLOCK) void k_tab_accept(DevMesh m, ChainGeom g, int64_t nm, const int *__restrict__ mlist,
+                                                      int *__restrict__ A) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nm) return;
+    double X[3];
+    chain_point(g, mlist[k], X);
+    int first;
+    hash_accepting<DIM>(m, X[0], X[1], X[2], A + k * TAB_K, TAB_K, first);
+}
+
+/* Walks of every multi-element point from each holding element of its
+ * source: T[k][q] = index (into the point's holding list) of the element
+ * reached from the source's q-th holding element, or an error code.  Points
+ * whose source has one possible element are resolved here (par = -1). */
+template <int DIM>
+__global__ __launch_bounds__(BLOCK) void k_tab_walk(DevMesh m, ChainGeom g, int64_t n, int64_t nm,
+                                                    const int2 *__restrict__ info, const int *__restrict__ mlist,
+                                                    const int *__restrict__ mslot, const int *__restrict__ A,
+                                                    const int *__restrict__ src, const int *__restrict__ carry,
+                                                    int seed0, const int *__restrict__ misc, int *__restrict__ par,
+                                                    int *__restrict__ val, signed char *__restrict__ T) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nm) return;
+    const int i = mlist[k];
+    int s = src[i];
+    if (s < 0) s = carry[i >> 6];
+    double X[3];
+    chain_point(g, i, X);
+    const int *Ai = A + k * TAB_K;
+    const int ci = sf_count(info[i].y);
+    int seedv = seed0; /* walk fallback of the grid chains: the centre seed, else the first found element */
+    if (seedv < 0) {
+        const int f = misc[0];
+        seedv = f < n ? info[f].x : -1;
+    }
+    auto tab = [&](int guess) -> int {
+        int err = 0;
+        int e = local_search<DIM>(m, X[0], X[1], X[2], guess, err);
+        if (e < 0 && !err && g.chain != 0) e = local_search<DIM>(m, X[0], X[1], X[2], seedv, err);
+        if (err) return err;
+        if (e < 0) return 0; /* LocalSearchChecking: the lowest holding element */
+        for (int q = 0; q < ci; q++)
+            if (Ai[q] == e) return q;
+        return E_INDET;
+    };
+    if (s == TAB_NONE) {
+        par[k] = -1;
+        val[k] = 0;
+        return;
+    }
+    if (s == TAB_SEED) {
+        par[k] = -1;
+        val[k] = tab(seed0);
+        return;
+    }
+    const int2 ps = info[s];
+    const int cs = sf_count(ps.y);
+    if (cs <= 1) {
+        par[k] = -1;
+        val[k] = tab(ps.x);
+        return;
+    }
+    const int *As = A + (int64_t)mslot[s] * TAB_K;
+    signed char *Tk = T + k * TAB_K;
+    for (int q = 0; q < cs; q++) Tk[q] = (signed char)tab(As[q]);
+    par[k] = s;
+    val[k] = 0;
+}
+
+/* one pointer-jumping round: compose each unresolved table with its parent's */
+__global__ __launch_bounds__(BLOCK) void k_tab_jump(int64_t nm, const int2 *__restrict__ info,
+                                                    const int *__restrict__ mslot, const int *__restrict__ par_in,
+                                                    const int *__restrict__ val_in, const signed char *__restrict__ T_in,
+                                                    int *__restrict__ par_out, int *__restrict__ val_out,
+                                                    signed char *__restrict__ T_out) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nm) return;
+    const int p = par_in[k];
+    if (p < 0) {
+        par_out[k] = -1;
+        val_out[k] = val_in[k];
+        return;
+    }
+    const int64_t q = mslot[p];
+    const signed char *Tk = T_in + k * TAB_K;
+    const int pq = par_in[q];
+    if (pq < 0) {
+        const int vq = val_in[q];
+        par_out[k] = -1;
+        val_out[k] = vq < 0 ? vq : (int)Tk[vq];
+        return;
+    }
+    const signed char *Tq = T_in + q * TAB_K;
+    signed char *To = T_out + k * TAB_K;
+    const int dom = sf_count(info[pq].y);
+    for (int j = 0; j < dom; j++) {
+        const int t = Tq[j];
+        To[j] = t < 0 ? (signed char)t : Tk[t];
+    }
+    par_out[k] = pq;
+    val_out[k] = 0;
+}
+
+/* final records (the replay's rules for LeftDomain and LeftDomainTime) */
+__global__ __launch_bounds__(BLOCK) void k_tab_out(ChainGeom g, int64_t n, const int2 *__restrict__ info,
+                                                   const int *__restrict__ mslot, const int *__restrict__ A,
+                                                   const int *__restrict__ par, const int *__restrict__ val,
+                                                   int *__restrict__ out_elem, int *__restrict__ out_left,
+                                                   double *__restrict__ out_lt, ErrBuf eb) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int2 my = info[i];
+    int elem = -1, left = 1;
+    double lt = -1.0;
+    if (my.y & SF_OUTB) {
+        if (g.chain == 1) lt = 0.0;
+    } else if (my.x >= 0) {
+        if (sf_count(my.y) <= 1) {
+            elem = my.x;
+            left = 0;
+            lt = 0.0;
+        } else {
+            const int64_t k = mslot[i];
+            const int v = par[k] >= 0 ? E_INDET : val[k];
+            if (v < 0) {
+                raise_err(eb, v, i);
+            } else {
+                elem = A[k * TAB_K + v];
+                left = 0;
+                lt = 0.0;
+            }
+        }
+    }
+    out_elem[i] = elem;
+    out_left[i] = left;
+    out_lt[i] = lt;
+}
+
+template <int DIM>
+static int seed_chain_tab(fvc_ctx *c, const ChainGeom &g, int64_t n, const int2 *info, const int *mlist, int64_t nm,
+                          int *mslot, int *src, int *chunk_last, int *carry, const TabPrep &tp, int seed0,
+                          int *d_elem, int *d_left, double *d_left_time) {
+    const int64_t nch = (n + 63) / 64;
+    k_tab_src<<<grid_for(n, BLOCK), BLOCK, 0, c->stream>>>(g, n, info, tp.rowfirst, tp.planefirst, src, chunk_last);
+    const int gs0 = (g.chain != 0 && seed0 >= 0) ? TAB_SEED : TAB_NONE;
+    k_tab_carry<<<1, 1024, 0, c->stream>>>(nch, chunk_last, gs0, carry);
+    int *pv = nullptr;
+    int *A = nullptr;
+    signed char *T = nullptr;
+    int fin = 0;
+    if (nm > 0) {
+        if (hipMallocAsync((void **)&A, sizeof(int) * TAB_K * (size_t)nm, c->stream) != hipSuccess ||
+            hipMallocAsync((void **)&T, 2 * TAB_K * (size_t)nm, c->stream) != hipSuccess ||
+            hipMallocAsync((void **)&pv, sizeof(int) * 4 * (size_t)nm, c->stream) != hipSuccess) {
+            if (A) hipFreeAsync(A, c->stream);
+            if (T) hipFreeAsync(T, c->stream);
+            return FVC_E_NOMEM;
+        }
+        int *par[2] = {pv, pv + nm}, *val[2] = {pv + 2 * nm, pv + 3 * nm};
+        signed char *TT[2] = {T, T + TAB_K * nm};
+        k_tab_slots<<<grid_for(nm, BLOCK), BLOCK, 0, c->stream>>>(nm, mlist, mslot);
+        k_tab_accept<DIM><<<grid_for(nm, BLOCK), BLOCK, 0, c->stream>>>(c->mesh, g, nm, mlist, A);
+        k_tab_walk<DIM><<<grid_for(nm, BLOCK), BLOCK, 0, c->stream>>>(c->mesh, g, n, nm, info, mlist, mslot, A, src,
+                                                                      carry, seed0, tp.misc, par[0], val[0], TT[0]);
+        /* a link chain of length L resolves in ceil(log2 L) + 1 rounds */
+        int rounds = 1;
+        while ((int64_t)1 << (rounds - 1) < nm) rounds++;
+        for (int r = 0; r < rounds; r++) {
+            const int a = r & 1;
+            k_tab_jump<<<grid_for(nm, BLOCK), BLOCK, 0, c->stream>>>(nm, info, mslot, par[a], val[a], TT[a],
+                                                                     par[a ^ 1], val[a ^ 1], TT[a ^ 1]);
+        }
+        fin = rounds & 1;
+        k_tab_out<<<grid_for(n, BLOCK), BLOCK, 0, c->stream>>>(g, n, info, mslot, A, par[fin], val[fin], d_elem,
+                                                               d_left, d_left_time, errbuf(c));
+    } else {
+        k_tab_out<<<grid_for(n, BLOCK), BLOCK, 0, c->stream>>>(g, n, info, mslot, A, nullptr, nullptr, d_elem, d_left,
+                                                               d_left_time, errbuf(c));
+    }
+    FVC_HIP(hipGetLastError());
+    if (A) FVC_HIP(hipFreeAsync(A, c->stream));
+    if (T) FVC_HIP(hipFreeAsync(T, c->stream));
+    if (pv) FVC_HIP(hipFreeAsync(pv, c->stream));
+    return FVC_OK;
+}
+
 int launch_seed_chain(fvc_ctx *c, int chain, int64_t n, const double *dxyz, const double mn[3], const double del[3],
                       const int32_t res[3], int lsc, int *d_elem, int *d_left, double *d_left_time) {
     if (n <= 0) return FVC_OK;
@@ -679,8 +1011,9 @@ int launch_seed_chain(fvc_ctx *c, int chain, int64_t n, const double *dxyz, cons
         g.del[a] = del ? del[a] : 0.0;
         g.res[a] = res ? res[a] : 1;
     }
+    const bool uns = c->mesh.mesh_type == 1;
     int seed0 = -1;
-    if (chain != 0 && c->mesh.mesh_type == 1) {
+    if (chain != 0 && uns) {
         /* centre seed: global search at the middle grid point (ftle.c:456-465, tracers.c:1318-1322) */
         double xs[3] = {g.mn[0] + ((int)(g.res[0] / 2)) * g.del[0], g.mn[1] + ((int)(g.res[1] / 2)) * g.del[1],
                         g.mn[2] + ((int)(g.res[2] / 2)) * g.del[2]};
@@ -693,16 +1026,60 @@ int launch_seed_chain(fvc_ctx *c, int chain, int64_t n, const double *dxyz, cons
         FVC_HIP(hipMemcpyAsync(&seed0, dres, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         FVC_HIP(hipStreamSynchronize(c->stream));
     }
-    int2 *info = (int2 *)scratch(c, sizeof(int2) * (size_t)n);
-    if (!info) return FVC_E_NOMEM;
-    if (c->mesh.dims == 3) {
-        k_seed_prepare<3><<<grid_for(n, BLOCK), BLOCK, 0, c->stream>>>(c->mesh, g, n, info);
-        k_seed_chain<3><<<1, CHAIN_STAGE, 0, c->stream>>>(c->mesh, g, n, info, lsc, seed0, d_elem, d_left,
-                                                          d_left_time, errbuf(c));
-    } else {
-        k_seed_prepare<2><<<grid_for(n, BLOCK), BLOCK, 0, c->stream>>>(c->mesh, g, n, info);
-        k_seed_chain<2><<<1, CHAIN_STAGE, 0, c->stream>>>(c->mesh, g, n, info, lsc, seed0, d_elem, d_left,
-                                                          d_left_time, errbuf(c));
+    /* point indices of the tabulated chain are ints; 0x7f7f7f7f marks "none" */
+    const bool tab = uns && lsc && n < 0x7f000000LL;
+    const int64_t nch = (n + 63) / 64;
+    const int64_t nrows = chain == 2 ? n / g.res[2] : 1, nplanes = chain == 2 ? n / ((int64_t)g.res[2] * g.res[1]) : 1;
+    size_t bytes = sizeof(int2) * (size_t)n;
+    if (tab) bytes += sizeof(int) * (4 * (size_t)n + 2 * (size_t)nch + (size_t)nrows + (size_t)nplanes + 2);
+    char *base = (char *)scratch(c, bytes);
+    if (!base) return FVC_E_NOMEM;
+    int2 *info = (int2 *)base;
+    int *ip = (int *)(info + n);
+    TabPrep tp{nullptr, nullptr, nullptr, nullptr};
+    int *mlist = nullptr, *mslot = nullptr, *src = nullptr, *chunk_last = nullptr, *carry = nullptr;
+    if (tab) {
+        tp.mflag = ip;
+        mlist = ip + n;
+        mslot = ip + 2 * n;
+        src = ip + 3 * n;
+        chunk_last = ip + 4 * n;
+        carry = chunk_last + nch;
+        tp.rowfirst = carry + nch;
+        tp.planefirst = tp.rowfirst + nrows;
+        tp.misc = tp.planefirst + nplanes;
+        FVC_HIP(hipMemsetAsync(tp.rowfirst, 0x7f, sizeof(int) * (size_t)(nrows + nplanes + 1), c->stream));
+        FVC_HIP(hipMemsetAsync(tp.misc + 1, 0, sizeof(int), c->stream));
+    }
+    if (c->mesh.dims == 3)
+        k_seed_prepare<3><<<grid_for(n, BLOCK), BLOCK, 0, c->stream>>>(c->mesh, g, n, info, tp);
+    else
+        k_seed_prepare<2><<<grid_for(n, BLOCK), BLOCK, 0, c->stream>>>(c->mesh, g, n, info, tp);
+    FVC_HIP(hipGetLastError());
+    bool replay = !tab;
+    if (tab) {
+        int misc[2] = {0, 0};
+        FVC_HIP(hipMemcpyAsync(misc, tp.misc, sizeof misc, hipMemcpyDeviceToHost, c->stream));
+        int64_t nm = 0;
+        int rc = launch_compact(c, tp.mflag, n, mlist, &nm, nullptr); /* synchronizes the stream */
+        if (rc) return rc;
+        if (misc[1] > TAB_K) {
+            replay = true; /* a point held by more elements than a table keeps */
+        } else {
+            rc = c->mesh.dims == 3 ? seed_chain_tab<3>(c, g, n, info, mlist, nm, mslot, src, chunk_last, carry, tp,
+                                                       seed0, d_elem, d_left, d_left_time)
+                                   : seed_chain_tab<2>(c, g, n, info, mlist, nm, mslot, src, chunk_last, carry, tp,
+                                                       seed0, d_elem, d_left, d_left_time);
+            if (rc) return rc;
+        }
+    }
+    if (replay) {
+        if (c->mesh.dims == 3)
+            k_seed_chain<3><<<1, CHAIN_STAGE, 0, c->stream>>>(c->mesh, g, n, info, lsc, seed0, d_elem, d_left,
+                                                              d_left_time, errbuf(c));
+        else
+            k_seed_chain<2><<<1, CHAIN_STAGE, 0, c->stream>>>(c->mesh, g, n, info, lsc, seed0, d_elem, d_left,
+                                                              d_left_time, errbuf(c));
     }
     FVC_HIP(hipGetLastError());
     return check_err(c, "seed chain");

@@ -234,3 +234,72 @@ def test_grid_chains_on_faces_match_oracle(pipe, lsc):
     np.testing.assert_array_equal(g["ElementIndex"][inside], o["Pt"]["ElementIndex"][inside])
     np.testing.assert_array_equal(g["LeftDomainTime"], o["Pt"]["LeftDomainTime"])
     assert (g["LeftDomain"] == 0).sum() > 1000
+
+
+def _fan(nt):
+    """2-D fan: nt triangles around a hub node, so the hub is held by all of
+    them and points on a spoke by two."""
+    ang = 2 * np.pi * np.arange(nt) / nt
+    xyz = np.zeros((nt + 1, 3))
+    xyz[1:, 0], xyz[1:, 1] = np.cos(ang), np.sin(ang)
+    k = np.arange(nt)
+    conn = np.stack([np.zeros(nt, np.int64), 1 + k, 1 + (k + 1) % nt, np.zeros(nt, np.int64)], 1).astype(np.int32)
+    # slot 0 crosses the edge opposite node 1, slot 1 opposite node 2, slot 2 opposite node 0
+    adj = np.stack([(k + 1) % nt, (k - 1) % nt, -np.ones(nt, np.int64), -np.ones(nt, np.int64)], 1).astype(np.int32)
+    return xyz, conn, adj
+
+
+@pytest.mark.parametrize("lsc", [1, 0])
+def test_fan_chains_match_oracle(lsc):
+    """List and grid chains on a 2-D fan: grids through the hub (held by 40
+    triangles, more than a chain table keeps, so the chain is replayed point by
+    point) and grids along spokes only (two triangles each, tabulated)."""
+    xyz, conn, adj = _fan(40)
+    bounds = [-1.0, 1.0, -1.0, 1.0, 0.0, 0.0]
+    ctx = fvc.Context(2, bounds, "f64", 2, coords=xyz, conn=conn, adj=adj)
+    om = oracle.Mesh(2, bounds, coords=xyz, conn=conn, adj=adj)
+    for mn, res in (([-1.0, -1.0, 0.0], [21, 21, 1]), ([-0.95, -1.0, 0.0], [20, 21, 1])):
+        dl = [0.1, 0.1, 0.0]
+        print("fan grid chain lsc=%d mn=%s" % (lsc, mn), flush=True)
+        a = ctx.seed_grid(mn, dl, res, lsc)
+        b, rc = om.seed_grid(mn, dl, res, lsc)
+        assert rc == 0
+        inside = b["LeftDomain"] == 0
+        assert inside.sum() > 100
+        np.testing.assert_array_equal(a["LeftDomain"], b["LeftDomain"])
+        np.testing.assert_array_equal(a["ElementIndex"][inside], b["ElementIndex"][inside])
+        np.testing.assert_array_equal(a["LeftDomainTime"], b["LeftDomainTime"])
+        q = np.stack(np.meshgrid(mn[0] + 0.1 * np.arange(res[0]), mn[1] + 0.1 * np.arange(res[1]), indexing="ij"), -1)
+        q = np.concatenate([q.reshape(-1, 2), np.zeros((res[0] * res[1], 1))], 1)
+        q = q[np.random.default_rng(5).permutation(q.shape[0])]
+        a = ctx.seed_list(q, lsc)
+        b, rc = om.seed_list(q, lsc)
+        assert rc == 0
+        inside = b["LeftDomain"] == 0
+        np.testing.assert_array_equal(a["LeftDomain"], b["LeftDomain"])
+        np.testing.assert_array_equal(a["ElementIndex"][inside], b["ElementIndex"][inside])
+        np.testing.assert_array_equal(a["LeftDomainTime"], b["LeftDomainTime"])
+
+
+def test_ftle_chain_on_voxel_edges_matches_oracle():
+    """FTLE grid over a Y-bifurcation's bounding box with a spacing of two
+    voxels in x and z, so every grid point inside lies on a voxel edge and its
+    element depends on the chain's guesses."""
+    t = host.ybif(24, 9, 32)
+    xyz, conn, adj = host.tube_mesh(t)
+    lo, hi = xyz.min(0), xyz.max(0)
+    bounds = [float(lo[0]), float(hi[0]), float(lo[1]), float(hi[1]), float(lo[2]), float(hi[2])]
+    ctx = fvc.Context(3, bounds, "f32", 2, coords=xyz, conn=conn, adj=adj)
+    om = oracle.Mesh(3, bounds, coords=xyz, conn=conn, adj=adj)
+    mn, mx = [t.lo[0], t.lo[1], t.lo[2]], [t.hi[0], t.hi[1], t.hi[2]]
+    for res in ([13, 7, 17], [25, 5, 33]):
+        print("FTLE grid chain on voxel edges res=%s" % res, flush=True)
+        F = fvc.FtleSlide(ctx, mn, mx, res, 1, 0.0, 0.02)
+        g = F.records()
+        o, rc = om.ftle_grid(mn, mx, res, 1)
+        assert rc == 0
+        inside = o["Pt"]["LeftDomain"] == 0
+        assert inside.sum() > 100
+        np.testing.assert_array_equal(g["LeftDomain"], o["Pt"]["LeftDomain"])
+        np.testing.assert_array_equal(g["ElementIndex"][inside], o["Pt"]["ElementIndex"][inside])
+        np.testing.assert_array_equal(g["LeftDomainTime"], o["Pt"]["LeftDomainTime"])
