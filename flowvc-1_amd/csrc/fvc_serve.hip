@@ -265,26 +265,35 @@ __global__ void k_srv_add(int *count, int64_t n) {
     if (threadIdx.x == 0) *count += (int)n;
 }
 
+/* The list compaction works on the live tiles only (the list length is read
+ * on the device): a fixed grid strides over them, so a tick costs the same
+ * whatever the pool's capacity. */
+constexpr int CMP_GRID = 1024;
+
 __global__ __launch_bounds__(256) void k_srv_count(const int *list_in, const int *n_in, const int *flags,
                                                    int *tile_counts) {
     const int n = *n_in;
-    const int64_t base = (int64_t)blockIdx.x * TILE;
-    int cnt = 0;
-    if (base < n)
+    const int nt = (n + TILE - 1) / TILE;
+    __shared__ int red[4];
+    for (int t = blockIdx.x; t < nt; t += gridDim.x) {
+        const int64_t base = (int64_t)t * TILE;
+        int cnt = 0;
         for (int q = threadIdx.x; q < TILE; q += 256) {
             const int64_t i = base + q;
             if (i < n && !(flags[list_in[i]] & (SF_DONE | SF_WAIT))) cnt++;
         }
-    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
-    __shared__ int red[4];
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = cnt;
-    __syncthreads();
-    if (threadIdx.x == 0) tile_counts[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+        for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = cnt;
+        __syncthreads();
+        if (threadIdx.x == 0) tile_counts[t] = red[0] + red[1] + red[2] + red[3];
+        __syncthreads();
+    }
 }
 
-__global__ __launch_bounds__(1024) void k_srv_scan(int *a, int ntiles, int *n_out) {
+__global__ __launch_bounds__(1024) void k_srv_scan(int *a, const int *n_in, int *n_out) {
     __shared__ int tmp[1024];
     __shared__ int carry;
+    const int ntiles = (*n_in + TILE - 1) / TILE;
     if (threadIdx.x == 0) carry = 0;
     __syncthreads();
     for (int base = 0; base < ntiles; base += 1024) {
@@ -309,30 +318,32 @@ __global__ __launch_bounds__(1024) void k_srv_scan(int *a, int ntiles, int *n_ou
 __global__ __launch_bounds__(256) void k_srv_scatter(const int *list_in, const int *n_in, const int *flags,
                                                      const int *offs, int *list_out) {
     const int n = *n_in;
-    const int64_t base = (int64_t)blockIdx.x * TILE;
-    if (base >= n) return;
+    const int nt = (n + TILE - 1) / TILE;
     __shared__ int wsum[4];
     __shared__ int run;
-    if (threadIdx.x == 0) run = offs[blockIdx.x];
-    __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (int r = 0; r < TILE; r += 256) {
-        const int64_t i = base + r + threadIdx.x;
-        int keep = 0, v = 0;
-        if (i < n) {
-            v = list_in[i];
-            keep = !(flags[v] & (SF_DONE | SF_WAIT));
+    for (int t = blockIdx.x; t < nt; t += gridDim.x) {
+        const int64_t base = (int64_t)t * TILE;
+        if (threadIdx.x == 0) run = offs[t];
+        __syncthreads();
+        for (int r = 0; r < TILE; r += 256) {
+            const int64_t i = base + r + threadIdx.x;
+            int keep = 0, v = 0;
+            if (i < n) {
+                v = list_in[i];
+                keep = !(flags[v] & (SF_DONE | SF_WAIT));
+            }
+            const unsigned long long bal = __ballot(keep);
+            const int before = __popcll(bal & ((1ull << lane) - 1ull));
+            if (lane == 0) wsum[w] = __popcll(bal);
+            __syncthreads();
+            int pre = 0;
+            for (int q = 0; q < w; q++) pre += wsum[q];
+            if (keep) list_out[run + pre + before] = v;
+            __syncthreads();
+            if (threadIdx.x == 0) run += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+            __syncthreads();
         }
-        const unsigned long long bal = __ballot(keep);
-        const int before = __popcll(bal & ((1ull << lane) - 1ull));
-        if (lane == 0) wsum[w] = __popcll(bal);
-        __syncthreads();
-        int pre = 0;
-        for (int q = 0; q < w; q++) pre += wsum[q];
-        if (keep) list_out[run + pre + before] = v;
-        __syncthreads();
-        if (threadIdx.x == 0) run += wsum[0] + wsum[1] + wsum[2] + wsum[3];
-        __syncthreads();
     }
 }
 
@@ -592,9 +603,10 @@ static void enqueue_tick(fvc_server *S, int par) {
                 launch_serve<P_F32, 256>(S, k, S->grid[k], d, budget);
         }
         int *tiles = S->tiles + k * (S->ntiles + 1);
-        k_srv_count<<<S->ntiles, 256, 0, c->stream>>>(d.list_in, d.n_in, S->flags, tiles);
-        k_srv_scan<<<1, 1024, 0, c->stream>>>(tiles, S->ntiles, d.n_out);
-        k_srv_scatter<<<S->ntiles, 256, 0, c->stream>>>(d.list_in, d.n_in, S->flags, tiles, d.list_out);
+        const int cg = std::min(S->ntiles, CMP_GRID);
+        k_srv_count<<<cg, 256, 0, c->stream>>>(d.list_in, d.n_in, S->flags, tiles);
+        k_srv_scan<<<1, 1024, 0, c->stream>>>(tiles, d.n_in, d.n_out);
+        k_srv_scatter<<<cg, 256, 0, c->stream>>>(d.list_in, d.n_in, S->flags, tiles, d.list_out);
     }
 }
 
