@@ -55,6 +55,12 @@ def build_inputs(args, host, np, with_seeds=True):
     xyz, conn, adj = host.tube_mesh(tube)
     slices = [host.tube_velocity(tube, xyz, s * 0.02, 0.01, 1234, s) for s in range(50)]
     seeds = host.tube_seeds(tube, args.tracers, 0.0, 10.0 / 3.0, 42) if with_seeds else None
+    if getattr(args, "mesh_order", "zorder") == "zorder":
+        from fvcpy import fvc
+        tz = time.time()
+        no, _, xyz, conn, adj = fvc.mesh_zorder(xyz, conn, adj)
+        slices = [fvc.gather_rows(no, sl) for sl in slices]
+        log("mesh renumbered in Z-order on the device (%.2f s with the 50 slices)" % (time.time() - tz))
     log("inputs: %d nodes, %d tets, %d tracers (%.1f s)" % (xyz.shape[0], conn.shape[0],
                                                            0 if seeds is None else seeds.shape[0], time.time() - t0))
     return tube, xyz, conn, adj, slices, seeds
@@ -529,6 +535,9 @@ def main():
     ap.add_argument("--nz", type=int, default=330)
     ap.add_argument("--cpu-sample", type=int, default=1 << 18)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mesh-order", default="zorder", choices=["native", "zorder"],
+                    help="element and node numbering of the pipe mesh: renumbered in Z-order by fvc_mesh_zorder "
+                         "before upload (cudaVC's ZOrderSort step; untimed setup), or as generated (voxel scan order)")
     ap.add_argument("--no-f64-check", action="store_true")
     ap.add_argument("--workload", default="pipe", choices=["pipe", "batch", "serve", "traffic"],
                     help="pipe: tracers + FTLE grid over one period; batch: uneven releases in one persistent "
@@ -645,10 +654,13 @@ def main():
                   "f16": "f16 storage, f32 math", "f64": "f64"}[args.precision],
         "data": "synthetic (deterministic generators, seeded)",
         "config": {
-            "workload": "pipe R=1 L=10, 60x60x330 voxels Kuhn-split (%d tets, %d nodes), 50 slices/period, "
+            "workload": "pipe R=1 L=10, 60x60x330 voxels Kuhn-split (%d tets, %d nodes%s), 50 slices/period, "
                         "%d tracers + %dx%dx%d FTLE grid, RK4 dt=1e-3 over one period" %
-                        (conn.shape[0], xyz.shape[0], n_tr, res[0], res[1], res[2]),
+                        (conn.shape[0], xyz.shape[0],
+                         ", renumbered in Z-order" if args.mesh_order == "zorder" else "",
+                         n_tr, res[0], res[1], res[2]),
             "tracers": n_tr, "ftle_grid": res, "frames": 50, "dt": 1e-3, "precision": args.precision,
+            "mesh_order": args.mesh_order,
         },
         "tracer_particle_steps_per_s": tr_steps / tr_t if tr_t > 0 else None,
         "ftle_seeds_per_s": ftle_seeds / (ft_t / K) if ft_t > 0 else None,

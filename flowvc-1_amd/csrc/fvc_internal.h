@@ -135,6 +135,9 @@ int launch_advect(fvc_ctx *c, PtrSet p, const int *index, int64_t begin, int64_t
                   const DevField &f, const fvc_int_params &ip);
 int launch_locate(fvc_ctx *c, const double *dxyz, const int *dguess, int64_t n, int mode, int *dout);
 /* chain: 0 = point list (ReadInTraceMesh), 1 = tracer grid (GenerateTracerMesh), 2 = FTLE grid */
+/* device sorting helpers (any stream) */
+int radix_sort_pairs(hipStream_t s, uint64_t *k0, int *v0, uint64_t *k1, int *v1, int64_t n, int bits, int *which);
+int scan_exclusive_small(hipStream_t s, int *a, int n);
 int launch_seed_chain(fvc_ctx *c, int chain, int64_t n, const double *dxyz, const double mn[3], const double del[3],
                       const int32_t res[3], int lsc, int *d_elem, int *d_left, double *d_left_time);
 int launch_ftle_frame(fvc_ftle *f, int lo, int hi, double tmin, double tmax, double t1, double t2,

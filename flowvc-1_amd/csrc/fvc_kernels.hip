@@ -1794,21 +1794,39 @@ __global__ __launch_bounds__(1024) void k_rs_scan(int *__restrict__ a, int total
     }
 }
 
-/* stable sort of (key, value) pairs by key; dperm receives the values */
-static int sort_pairs(fvc_ctx *c, uint64_t *k0, int *v0, int64_t n, int *dperm, uint64_t *k1, int *v1) {
+/* stable LSD radix sort of (key, value) pairs on the low `bits` bits of the
+ * keys, ping-ponging between (k0, v0) and (k1, v1); *which = 0 when the sorted
+ * pairs end in (k0, v0), 1 when in (k1, v1) */
+int radix_sort_pairs(hipStream_t s, uint64_t *k0, int *v0, uint64_t *k1, int *v1, int64_t n, int bits, int *which) {
+    *which = 0;
+    if (n <= 0) return FVC_OK;
     const int nb = (int)((n + RS_TILE - 1) / RS_TILE);
     int *hist = nullptr;
-    FVC_HIP(hipMallocAsync((void **)&hist, sizeof(int) * 256 * (size_t)nb, c->stream));
-    for (int shift = 0; shift < 64; shift += 8) {
-        k_rs_hist<<<nb, BLOCK, 0, c->stream>>>(k0, n, shift, nb, hist);
-        k_rs_scan<<<1, 1024, 0, c->stream>>>(hist, 256 * nb);
-        k_rs_scatter<<<nb, BLOCK, 0, c->stream>>>(k0, v0, n, shift, nb, hist, k1, v1);
+    if (hipMallocAsync((void **)&hist, sizeof(int) * 256 * (size_t)nb, s) != hipSuccess) return FVC_E_NOMEM;
+    for (int shift = 0; shift < bits && shift < 64; shift += 8) {
+        k_rs_hist<<<nb, BLOCK, 0, s>>>(k0, n, shift, nb, hist);
+        k_rs_scan<<<1, 1024, 0, s>>>(hist, 256 * nb);
+        k_rs_scatter<<<nb, BLOCK, 0, s>>>(k0, v0, n, shift, nb, hist, k1, v1);
         std::swap(k0, k1);
         std::swap(v0, v1);
+        *which ^= 1;
     }
-    FVC_HIP(hipGetLastError());
-    FVC_HIP(hipMemcpyAsync(dperm, v0, sizeof(int) * n, hipMemcpyDeviceToDevice, c->stream));
-    FVC_HIP(hipFreeAsync(hist, c->stream));
+    (void)hipFreeAsync(hist, s);
+    return hipGetLastError() == hipSuccess ? FVC_OK : FVC_E_HIP;
+}
+
+/* exclusive prefix sum of a[0..n) in place (one block) */
+int scan_exclusive_small(hipStream_t s, int *a, int n) {
+    if (n > 0) k_rs_scan<<<1, 1024, 0, s>>>(a, n);
+    return hipGetLastError() == hipSuccess ? FVC_OK : FVC_E_HIP;
+}
+
+/* stable sort of (key, value) pairs by key; dperm receives the values */
+static int sort_pairs(fvc_ctx *c, uint64_t *k0, int *v0, int64_t n, int *dperm, uint64_t *k1, int *v1) {
+    int which = 0;
+    int rc = radix_sort_pairs(c->stream, k0, v0, k1, v1, n, 64, &which);
+    if (rc) return rc;
+    FVC_HIP(hipMemcpyAsync(dperm, which ? v1 : v0, sizeof(int) * n, hipMemcpyDeviceToDevice, c->stream));
     return FVC_OK;
 }
 

@@ -136,6 +136,8 @@ def lib():
         L.fvc_server_info.argtypes = [vp, i64, ctypes.POINTER(RequestInfo)]
         L.fvc_server_stats.argtypes = [vp, vp]
         L.fvc_server_take.argtypes = [vp, i64, vp, vp, vp]
+        L.fvc_mesh_zorder.argtypes = [i64, vp, i64, vp, vp, vp, vp, vp, vp, vp]
+        L.fvc_gather_rows.argtypes = [i64, ctypes.c_int, vp, vp, vp]
         _LIB = L
     return _LIB
 
@@ -148,6 +150,34 @@ def _chk(rc, what, ctx=None):
     if rc != OK:
         detail = lib().fvc_last_error(ctx).decode() if ctx else ""
         raise FvcError(rc, what, detail)
+
+
+def mesh_zorder(xyz, conn, adj=None):
+    """Z-order renumbering of a tetrahedral mesh on the device
+    (fvc_mesh_zorder; cudaVC/ZOrderSort/main.cpp:103-259).  Returns
+    (node_order, elem_order, xyz_new, conn_new, adj_new)."""
+    xyz = np.ascontiguousarray(xyz, np.float64)
+    conn = np.ascontiguousarray(conn, np.int32)
+    nn, ne = xyz.shape[0], conn.shape[0]
+    no, eo = np.zeros(nn, np.int32), np.zeros(ne, np.int32)
+    xn, cn = np.zeros_like(xyz), np.zeros_like(conn)
+    an = None
+    if adj is not None:
+        adj = np.ascontiguousarray(adj, np.int32)
+        an = np.zeros_like(adj)
+    _chk(lib().fvc_mesh_zorder(nn, _p(xyz), ne, _p(conn), _p(adj) if adj is not None else None, _p(no), _p(eo),
+                               _p(xn), _p(cn), _p(an) if an is not None else None), "fvc_mesh_zorder")
+    return no, eo, xn, cn, an
+
+
+def gather_rows(order, data):
+    """Rows of `data` (n x width doubles) in the order `order` (fvc_gather_rows)."""
+    order = np.ascontiguousarray(order, np.int32)
+    data = np.ascontiguousarray(data, np.float64)
+    out = np.empty_like(data)
+    width = 1 if data.ndim == 1 else data.shape[1]
+    _chk(lib().fvc_gather_rows(data.shape[0], width, _p(order), _p(data), _p(out)), "fvc_gather_rows")
+    return out
 
 
 class Context:

@@ -317,6 +317,23 @@ int fvc_server_take(fvc_server *s, int64_t id, fvc_point *records, double *ftle_
  * coordinates quantised to 21 bits on [lo, hi]; perm receives the stable order. */
 int fvc_zsort(fvc_ctx *ctx, const double *xyz, int64_t n, const double lo[3], const double hi[3], int32_t *perm);
 
+/* ---- Z-order mesh renumbering (cudaVC/ZOrderSort/main.cpp:103-299) -------- */
+/* Renumbers a tetrahedral mesh (host arrays in the ompVC file layouts:
+ * coords[3*num_nodes], conn and adj [4*num_elems], adj -1 at boundaries) so
+ * that nodes and elements follow the Z-order of their coordinate ranks and
+ * centroid ranks; replaces the reference program's Work() (main.cpp:103-259).
+ * node_order[i] / elem_order[i] = old id of new node / element i (the
+ * reference's sortedPointIDs / sortedCellIDs).  Outputs may be NULL; adj may
+ * be NULL (then adj_out is not written).  Runs on the current device with no
+ * context.  Returns FVC_E_INVALID for out-of-range node or neighbour ids. */
+int fvc_mesh_zorder(int64_t num_nodes, const double *coords, int64_t num_elems, const int32_t *conn,
+                    const int32_t *adj, int32_t *node_order, int32_t *elem_order, double *coords_out,
+                    int32_t *conn_out, int32_t *adj_out);
+/* out row i = in row order[i], rows of `width` doubles (node data such as a
+ * velocity slice renumbered with node_order; main.cpp:262-291).  order must
+ * be a permutation of 0..n-1 (checked). */
+int fvc_gather_rows(int64_t n, int width, const int32_t *order, const double *in, double *out);
+
 #ifdef __cplusplus
 }
 #endif

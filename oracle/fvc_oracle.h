@@ -40,6 +40,8 @@ enum {
     OC_E_EIG = -8,
     OC_E_IO = -12,
     OC_E_PARAM = -13,
+    OC_E_NOMEM = -2,
+    OC_E_BADMESH = -3, /* connectivity or adjacency index out of range */
 };
 
 typedef struct oc_point { /* ompVC LagrangianPoint, 80 bytes */
@@ -157,6 +159,12 @@ int oc_seed_list(const oc_mesh *m, int64_t n, const double *xyz, int local_searc
 /* GenerateTracerMesh chain (tracers.c:1298-1532), k,j,i order, x fastest. */
 int oc_seed_grid(const oc_mesh *m, const double min[3], const double del[3], const int32_t res[3],
                  int local_search_checking, oc_point *pts);
+
+/* Z-order renumbering of a tetrahedral mesh (cudaVC/ZOrderSort/main.cpp:103-259):
+ * node_order[i] / elem_order[i] = old id of new node / element i; any of the
+ * renumbered outputs may be NULL. */
+int oc_mesh_zorder(int64_t nn, const double *xyz, int64_t ne, const int32_t *conn, const int32_t *adj,
+                   int32_t *node_order, int32_t *elem_order, double *xyz_out, int32_t *conn_out, int32_t *adj_out);
 
 /* Deck-driven run writing ompVC-format outputs.  nproc emulates ompVC's loop
  * choice (parameters.c:398-421); <= 0 uses the host count.  Returns 0 or an error. */

@@ -49,12 +49,26 @@ def lib():
         L.oc_eig_batch.argtypes = [i64, vp, vp]
         L.oc_seed_list.argtypes = [vp, i64, vp, ctypes.c_int, vp]
         L.oc_seed_grid.argtypes = [vp, vp, vp, vp, ctypes.c_int, vp]
+        L.oc_mesh_zorder.argtypes = [i64, vp, i64, vp, vp, vp, vp, vp, vp, vp]
         _LIB = L
     return _LIB
 
 
 def _p(a):
     return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def mesh_zorder(xyz, conn, adj):
+    """Z-order renumbering (cudaVC/ZOrderSort/main.cpp:103-259): returns
+    (node_order, elem_order, xyz_new, conn_new, adj_new, rc)."""
+    xyz = np.ascontiguousarray(xyz, np.float64)
+    conn = np.ascontiguousarray(conn, np.int32)
+    adj = np.ascontiguousarray(adj, np.int32)
+    nn, ne = xyz.shape[0], conn.shape[0]
+    no, eo = np.zeros(nn, np.int32), np.zeros(ne, np.int32)
+    xn, cn, an = np.zeros_like(xyz), np.zeros_like(conn), np.zeros_like(adj)
+    rc = lib().oc_mesh_zorder(nn, _p(xyz), ne, _p(conn), _p(adj), _p(no), _p(eo), _p(xn), _p(cn), _p(an))
+    return no, eo, xn, cn, an, rc
 
 
 class Mesh:
