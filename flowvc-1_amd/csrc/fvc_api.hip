@@ -669,6 +669,33 @@ extern "C" int fvc_advect(fvc_ctx *c, fvc_particles *p, int64_t begin, int64_t c
     return launch_advect(c, ptrs(p), nullptr, begin, count, t1, t2, bind_field(c, lo, hi, tmin, tmax), *ip);
 }
 
+extern "C" int fvc_advect_times(fvc_ctx *c, fvc_particles *p, int64_t begin, int64_t count, const double *t12,
+                                int32_t lo, int32_t hi, double tmin, double tmax, const fvc_int_params *ip) {
+    if (!c || !p || !ip || begin < 0 || count < 0 || begin + count > p->cap || (count > 0 && !t12))
+        return FVC_E_INVALID;
+    if (ip->int_type != FVC_INT_RK4 && ip->int_type != FVC_INT_EULER)
+        return set_err(c, FVC_E_INVALID, "integrator not available in fvc_advect_times");
+    if (!(ip->time_step > 1.0e-10)) return set_err(c, FVC_E_INVALID, "Int_TimeStep must be positive");
+    int rc = check_slots(c, lo, hi, tmin, tmax);
+    if (rc) return rc;
+    for (int64_t q = 0; q < count; q++) { /* each segment inside the loaded interval, t1 <= t2 */
+        const double a = t12[2 * q], b = t12[2 * q + 1];
+        if (a != a) continue;
+        if (!(b >= a) || a < tmin - TINY || b > tmax + TINY)
+            return set_err(c, FVC_E_INVALID, "fvc_advect_times: segment outside the loaded interval");
+    }
+    if (count == 0) {
+        c->last_steps = 0;
+        return FVC_OK;
+    }
+    double *d = nullptr;
+    FVC_HIP(hipMallocAsync((void **)&d, sizeof(double) * 2 * count, c->stream));
+    FVC_HIP(hipMemcpyAsync(d, t12, sizeof(double) * 2 * count, hipMemcpyHostToDevice, c->stream));
+    rc = launch_advect_times(c, ptrs(p), begin, count, d, bind_field(c, lo, hi, tmin, tmax), *ip);
+    (void)hipFreeAsync(d, c->stream);
+    return rc;
+}
+
 extern "C" int fvc_last_step_count(fvc_ctx *c, int64_t *n) {
     if (!c || !n) return FVC_E_INVALID;
     *n = c->last_steps;

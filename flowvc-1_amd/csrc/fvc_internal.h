@@ -131,6 +131,8 @@ FtleDev ftle_dev(fvc_ftle *f);
 DevField bind_field(fvc_ctx *c, int lo, int hi, double tmin, double tmax);
 
 int launch_velocity(fvc_ctx *c, PtrSet p, int64_t begin, int64_t count, double t, const DevField &f);
+int launch_advect_times(fvc_ctx *c, PtrSet p, int64_t begin, int64_t count, const double *dtt, const DevField &f,
+                        const fvc_int_params &ip);
 int launch_advect(fvc_ctx *c, PtrSet p, const int *index, int64_t begin, int64_t count, double t1, double t2,
                   const DevField &f, const fvc_int_params &ip);
 int launch_locate(fvc_ctx *c, const double *dxyz, const int *dguess, int64_t n, int mode, int *dout);

@@ -138,6 +138,32 @@ __global__ __launch_bounds__(BLOCK) void k_advect_gen(DevMesh m, DevField f, Ptr
     advect_body<MODE, DIM, UNS, 1>(m, f, p, index, begin, count, t1, t2, itype, dt, extrap, eb, steps);
 }
 
+/* Each particle over its own segment: tt[2q], tt[2q+1] = t1, t2 of particle
+ * begin+q, skipped where t1 is NaN (staggered release, tracers.c:336-397). */
+template <int MODE, int DIM, int UNS, int GEN>
+__device__ __forceinline__ void k_advect_times_body(const DevMesh &m, const DevField &f, const PtrSet &p, int64_t begin,
+                                                    int64_t count, const double *__restrict__ tt, int itype, double dt,
+                                                    int extrap, ErrBuf eb, unsigned long long *steps) {
+    const int64_t q = xcd_block() * blockDim.x + threadIdx.x;
+    uint32_t ns = 0;
+    if (q < count) {
+        const double t1 = tt[2 * q], t2 = tt[2 * q + 1];
+        const int64_t i = begin + q;
+        if (t1 == t1 && p.left_time[i] > -TINY && (!p.left[i] || extrap)) {
+            const bool hist = GEN && itype >= 3;
+            PState s;
+            load_state(p, i, s, hist, extrap);
+            int err = 0;
+            advect<MODE, DIM, UNS, GEN>(m, f, s, t1, t2, itype, dt, extrap, err, ns);
+            if (err)
+                raise_err(eb, err, i);
+            else
+                store_state(p, i, s, hist, extrap);
+        }
+    }
+    add_steps(steps, ns);
+}
+
 template <int MODE, int DIM, int UNS>
 __global__ __launch_bounds__(BLOCK) void k_launch_vel(DevMesh m, DevField f, PtrSet p, int64_t begin, int64_t count,
                                                      double t, ErrBuf eb) {
@@ -205,6 +231,51 @@ int launch_velocity(fvc_ctx *c, PtrSet p, int64_t begin, int64_t count, double t
                  <<<grid_for(count, BLOCK), BLOCK, 0, c->stream>>>(c->mesh, f, p, begin, count, t, errbuf(c)));
     FVC_HIP(hipGetLastError());
     return check_err(c, "fvc_launch_velocity");
+}
+
+template <int MODE, int DIM, int UNS>
+__global__ __launch_bounds__(BLOCK) void k_advect_times_rk4(DevMesh m, DevField f, PtrSet p, int64_t begin,
+                                                           int64_t count, const double *__restrict__ tt, int itype,
+                                                           double dt, int extrap, ErrBuf eb, unsigned long long *steps);
+template <int MODE, int DIM, int UNS>
+__global__ __launch_bounds__(BLOCK) void k_advect_times_gen(DevMesh m, DevField f, PtrSet p, int64_t begin,
+                                                           int64_t count, const double *__restrict__ tt, int itype,
+                                                           double dt, int extrap, ErrBuf eb, unsigned long long *steps);
+
+int launch_advect_times(fvc_ctx *c, PtrSet p, int64_t begin, int64_t count, const double *dtt, const DevField &f,
+                        const fvc_int_params &ip) {
+    FVC_HIP(hipMemsetAsync(c->d_steps, 0, sizeof(unsigned long long), c->stream));
+    if (count > 0) {
+        if (ip.int_type == FVC_INT_RK4)
+            FVC_DISPATCH(c->precision, c->mesh.dims, c->mesh.mesh_type, k_advect_times_rk4,
+                         <<<grid_for(count, BLOCK), BLOCK, 0, c->stream>>>(c->mesh, f, p, begin, count, dtt,
+                                                                           ip.int_type, ip.time_step, ip.extrapolate,
+                                                                           errbuf(c), c->d_steps));
+        else
+            FVC_DISPATCH(c->precision, c->mesh.dims, c->mesh.mesh_type, k_advect_times_gen,
+                         <<<grid_for(count, BLOCK), BLOCK, 0, c->stream>>>(c->mesh, f, p, begin, count, dtt,
+                                                                           ip.int_type, ip.time_step, ip.extrapolate,
+                                                                           errbuf(c), c->d_steps));
+        FVC_HIP(hipGetLastError());
+    }
+    unsigned long long steps = 0;
+    FVC_HIP(hipMemcpyAsync(&steps, c->d_steps, sizeof steps, hipMemcpyDeviceToHost, c->stream));
+    int rc = check_err(c, "fvc_advect_times");
+    c->last_steps = (int64_t)steps;
+    return rc;
+}
+
+template <int MODE, int DIM, int UNS>
+__global__ __launch_bounds__(BLOCK) void k_advect_times_rk4(DevMesh m, DevField f, PtrSet p, int64_t begin,
+                                                           int64_t count, const double *__restrict__ tt, int itype,
+                                                           double dt, int extrap, ErrBuf eb, unsigned long long *steps) {
+    k_advect_times_body<MODE, DIM, UNS, 0>(m, f, p, begin, count, tt, itype, dt, extrap, eb, steps);
+}
+template <int MODE, int DIM, int UNS>
+__global__ __launch_bounds__(BLOCK) void k_advect_times_gen(DevMesh m, DevField f, PtrSet p, int64_t begin,
+                                                           int64_t count, const double *__restrict__ tt, int itype,
+                                                           double dt, int extrap, ErrBuf eb, unsigned long long *steps) {
+    k_advect_times_body<MODE, DIM, UNS, 1>(m, f, p, begin, count, tt, itype, dt, extrap, eb, steps);
 }
 
 int launch_advect(fvc_ctx *c, PtrSet p, const int *index, int64_t begin, int64_t count, double t1, double t2,

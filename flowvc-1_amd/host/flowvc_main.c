@@ -11,9 +11,9 @@
  * file is uploaded once into its own slot (the reference re-reads two slices
  * per frame, velocity.c:61-234).  --nproc emulates the host processor count
  * that selects ompVC's tracer output loop (parameters.c:411-421).
- * Modes outside the advection path (inertial particles, RKF, staggered
- * release, exposure/residence time, strain rate, VelOut, normal flow) stop
- * with an error.
+ * Staggered tracer release follows tracers.c:293-1000.  Modes outside the
+ * advection path (inertial particles, RKF, exposure/residence time, strain
+ * rate, VelOut, normal flow) stop with an error.
  */
 #include <math.h>
 #include <stdio.h>
@@ -160,6 +160,413 @@ static double *read_seed_list(const fvc_deck *d, int64_t *n) {
     return xyz;
 }
 
+/* ------------------------------------------------------------------------ */
+/* staggered release (tracers.c:293-1000)                                    */
+/* ------------------------------------------------------------------------ */
+/* Release points from Trace_InFile (binary list), located by the list chain
+ * with the global fallback always on; the spacing voxdim is the smallest
+ * distance between consecutive points (LoadReleasePoints, tracers.c:798-990). */
+static fvc_point *stagger_points(const fvc_deck *d, fvc_ctx *c, int64_t *nloc, double *voxdim) {
+    char path[3 * FVC_STR];
+    snprintf(path, sizeof path, "%s%s", d->Path_Data, d->Trace_InFile);
+    FILE *f = fopen(path, "rb");
+    int32_t cnt = 0;
+    if (!f) fatal("Could not open %s", path);
+    if (fread(&cnt, 4, 1, f) != 1) fatal("Could not read Trace_NumReleaseLocations from %s", path);
+    double *xyz = (double *)calloc((size_t)3 * (cnt > 0 ? cnt : 1), sizeof(double));
+    for (int q = 0; q < cnt; q++)
+        if (fread(xyz + 3 * q, 8, 3, f) != 3) fatal("Could not read complete coordinate list from %s", path);
+    fclose(f);
+    if (cnt > 1) { /* dist() of mymath.c:29-35 */
+        double m = -1.0;
+        for (int q = 0; q + 1 < cnt; q++) {
+            double dv[3], s2 = 0;
+            for (int a = 0; a < 3; a++) dv[a] = xyz[3 * q + a] - xyz[3 * (q + 1) + a];
+            for (int a = 0; a < 3; a++) s2 = s2 + dv[a] * dv[a];
+            double dd = sqrt(s2);
+            m = q == 0 ? dd : fmin(m, dd);
+        }
+        *voxdim = m;
+        if (*voxdim < TINY) fatal("%s", "Spacing between release points becomes nonpositive");
+    } else if (cnt > 0) {
+        fprintf(stderr, "\nWarning: Only 1 release location\n");
+        fprintf(stderr, "Enter desired spacing:");
+        if (fscanf(stdin, "%lf", voxdim) != 1) fatal("%s", "no spacing given for a single release location");
+    } else {
+        fatal("%s", "Number of release locations cannot be zero");
+    }
+    fvc_point *rp = (fvc_point *)calloc((size_t)cnt, sizeof(fvc_point));
+    check(fvc_seed_list(c, xyz, cnt, 1, rp), c, "fvc_seed_list");
+    int32_t found = 0;
+    for (int q = 0; q < cnt; q++) found += !rp[q].LeftDomain;
+    if (found < cnt) { /* the reference rewrites the file with the interior points (tracers.c:970-990) */
+        FILE *o = fopen(path, "wb");
+        if (!o) fatal("Could not open file %s", path);
+        fwrite(&found, 4, 1, o);
+        for (int q = 0; q < cnt; q++)
+            if (!rp[q].LeftDomain) fwrite(xyz + 3 * q, 8, 3, o);
+        fclose(o);
+    }
+    free(xyz);
+    *nloc = cnt;
+    return rp;
+}
+
+typedef struct {
+    double start, stop;
+    int32_t loc, elem; /* release location, element copied at creation */
+} snode_t;
+
+typedef struct {
+    snode_t *v;
+    int64_t n, cap;
+} snodes_t;
+
+static void snode_add(snodes_t *L, const fvc_deck *d, int32_t loc, int32_t elem, double start) {
+    if (L->n == L->cap) {
+        L->cap = L->cap ? 2 * L->cap : 1024;
+        L->v = (snode_t *)realloc(L->v, sizeof(snode_t) * (size_t)L->cap);
+        if (!L->v) fatal("%s", "Malloc failed for nnode");
+    }
+    snode_t *s = &L->v[L->n++];
+    s->start = start;
+    s->stop = fmin(d->Output_TEnd, start + d->Trace_IntTLength);
+    s->loc = loc;
+    s->elem = elem;
+}
+
+static double vdot3(const double *a, const double *b) {
+    double sum = 0;
+    for (int i = 0; i < 3; i++) sum = sum + a[i] * b[i];
+    return sum;
+}
+
+static double sign_of(double a, double b) { return b >= 0.0 ? fabs(a) : -fabs(a); } /* macros.h:32 */
+
+/* Release times per location from the flow through it (GenerateStaggeredRelease,
+ * tracers.c:418-796): a new point is released whenever the previous one has
+ * travelled voxdim, with the signed speed linear in time over each data
+ * interval (linear or quadratic solve).  Velocities at the release points are
+ * GetVelocity at the interval bounds, on the device.  Returns nodes grouped
+ * by location in release order (the reference's linked lists). */
+static snodes_t stagger_schedule(const fvc_deck *d, fvc_ctx *c, slices_t *sl, const fvc_point *rp, int64_t nloc,
+                                 double voxdim) {
+    int64_t nn = 0;
+    snodes_t *per = (snodes_t *)calloc((size_t)nloc, sizeof(snodes_t));
+    double *voxdim0 = (double *)calloc((size_t)nloc, sizeof(double)), *u0 = (double *)calloc((size_t)nloc * 3, 8);
+    int *ptdone = (int *)calloc((size_t)nloc, sizeof(int));
+    for (int64_t ss = 0; ss < nloc; ss++)
+        if (!rp[ss].LeftDomain) {
+            snode_add(&per[ss], d, (int32_t)ss, rp[ss].ElementIndex, d->Output_TStart);
+            nn++;
+        }
+    fvc_particles *P = NULL;
+    check(fvc_particles_create(c, nloc, &P), c, "fvc_particles_create");
+    check(fvc_particles_set(P, 0, rp, nloc), c, "fvc_particles_set");
+    fvc_point *us = (fvc_point *)malloc(sizeof(fvc_point) * (size_t)nloc);
+    fvc_point *ue = (fvc_point *)malloc(sizeof(fvc_point) * (size_t)nloc);
+    for (int df = d->Data_FirstFrame; df < d->Data_LastFrame; df++) {
+        int lo = slot_for(sl, df, -1);
+        int hi = slot_for(sl, df + 1, lo);
+        const double tmin = d->Data_TMin + df * d->Data_TDelta, tmax = tmin + d->Data_TDelta;
+        /* GetVelocity_* at both bounds; it walks from, and updates, each point's element */
+        check(fvc_launch_velocity(c, P, 0, nloc, tmin, lo, hi, tmin, tmax), c, "fvc_launch_velocity");
+        check(fvc_particles_get(P, 0, us, nloc), c, "fvc_particles_get");
+        if (df == d->Data_FirstFrame)
+            for (int64_t ss = 0; ss < nloc; ss++)
+                if (!rp[ss].LeftDomain) memcpy(u0 + 3 * ss, us[ss].V, 3 * sizeof(double));
+        if (tmin >= d->Trace_ReleaseTMax) break;
+        check(fvc_launch_velocity(c, P, 0, nloc, tmax, lo, hi, tmin, tmax), c, "fvc_launch_velocity");
+        check(fvc_particles_get(P, 0, ue, nloc), c, "fvc_particles_get");
+        for (int64_t ss = 0; ss < nloc; ss++) {
+            if (rp[ss].LeftDomain || ptdone[ss]) continue;
+            snodes_t *L = &per[ss];
+            const int32_t elem = ue[ss].ElementIndex;
+            const double usbar = sign_of(sqrt(vdot3(us[ss].V, us[ss].V)), vdot3(us[ss].V, u0 + 3 * ss));
+            const double uebar = sign_of(sqrt(vdot3(ue[ss].V, ue[ss].V)), vdot3(ue[ss].V, u0 + 3 * ss));
+            double ti, tii, dd;
+            if (usbar < TINY && uebar < TINY) continue; /* reverse or stagnant flow */
+            if (fabs(usbar - uebar) < 1.0e-6) {          /* steady: linear solve (tracers.c:557-620) */
+                if (voxdim0[ss] > TINY)
+                    ti = tmin;
+                else if (voxdim0[ss] < -TINY) {
+                    ti = tmin;
+                    voxdim0[ss] = 0;
+                } else
+                    ti = L->v[L->n - 1].start;
+                for (;;) {
+                    dd = voxdim - voxdim0[ss];
+                    if (dd < 0) fatal("%s", "d < 0");
+                    tii = ti + dd / usbar;
+                    if (tii <= ti) fatal("%s", "Next output time not after previous");
+                    if (tii >= d->Trace_ReleaseTMax) {
+                        ptdone[ss] = 1;
+                        break;
+                    } else if (tii > tmax) {
+                        voxdim0[ss] = voxdim0[ss] + uebar * (tmax - ti);
+                        if (voxdim0[ss] < 0) fatal("%s", "voxdim0 < 0");
+                        break;
+                    }
+                    snode_add(L, d, (int32_t)ss, elem, tii);
+                    nn++;
+                    voxdim0[ss] = 0;
+                    ti = tii;
+                }
+            } else { /* speed linear in time: quadratic solve (tracers.c:621-765) */
+                const double M = (usbar - uebar) / (tmin - tmax);
+                const double N = usbar - M * tmin;
+                if (usbar < 0 && uebar > 0) { /* release when the flow turns positive */
+                    snode_add(L, d, (int32_t)ss, elem, -N / M);
+                    nn++;
+                    voxdim0[ss] = 0;
+                    ti = -N / M;
+                } else {
+                    if (voxdim0[ss] > TINY)
+                        ti = tmin;
+                    else if (voxdim0[ss] < -TINY) {
+                        ti = tmin;
+                        voxdim0[ss] = 0;
+                    } else
+                        ti = L->v[L->n - 1].start;
+                }
+                for (;;) {
+                    const double uibar = M * ti + N;
+                    dd = voxdim - voxdim0[ss];
+                    if (dd < 0) fatal("%s", "d < 0");
+                    const double a = M, b = N + uibar - M * ti, cc = -N * ti - uibar * ti - 2 * dd;
+                    if (b * b - 4 * a * cc < 0) {
+                        if (usbar < uebar) fatal("%s", "Next output time has imaginary roots even with increasing flow");
+                        voxdim0[ss] = -1; /* next interval starts from its own TMin */
+                        break;
+                    }
+                    tii = (-b + sqrt(b * b - 4 * a * cc)) / (2 * a);
+                    if (tii <= ti) fatal("%s", "Next output time not after previous");
+                    if (tii >= d->Trace_ReleaseTMax) {
+                        ptdone[ss] = 1;
+                        break;
+                    } else if (tii > tmax) {
+                        voxdim0[ss] = voxdim0[ss] + 0.5 * (uebar + uibar) * (tmax - ti);
+                        if (voxdim0[ss] < 0) fatal("%s", "voxdim0 < 0");
+                        break;
+                    }
+                    snode_add(L, d, (int32_t)ss, elem, tii);
+                    nn++;
+                    voxdim0[ss] = 0;
+                    ti = tii;
+                }
+            }
+        }
+    }
+    fvc_particles_destroy(P);
+    snodes_t all = {NULL, 0, 0};
+    all.v = (snode_t *)malloc(sizeof(snode_t) * (size_t)(nn > 0 ? nn : 1));
+    all.cap = nn;
+    for (int64_t ss = 0; ss < nloc; ss++) {
+        if (per[ss].n) memcpy(all.v + all.n, per[ss].v, sizeof(snode_t) * (size_t)per[ss].n);
+        all.n += per[ss].n;
+        free(per[ss].v);
+    }
+    free(per);
+    free(voxdim0);
+    free(u0);
+    free(ptdone);
+    free(us);
+    free(ue);
+    return all;
+}
+
+typedef struct {
+    int64_t node;
+    int32_t file;
+    double X[3];
+} srec_t;
+
+static int srec_cmp(const void *a, const void *b) {
+    const srec_t *x = (const srec_t *)a, *y = (const srec_t *)b;
+    if (x->file != y->file) return x->file < y->file ? -1 : 1;
+    return x->node < y->node ? -1 : (x->node > y->node);
+}
+
+/* Tracer_StaggeredRelease (tracers.c:293-416) over every data interval: each
+ * launched point integrates to each output time inside the interval, then to
+ * the interval's end or its stop time; positions are written per output time
+ * in location and release order.  Every round of segments is one device call
+ * with a segment per point. */
+static int64_t run_staggered(const fvc_deck *d, fvc_ctx *c, slices_t *sl, const fvc_int_params *ip, int quiet,
+                             double *t_compute) {
+    int64_t nloc = 0, steps_total = 0;
+    double voxdim = 0.0;
+    fvc_point *rp = stagger_points(d, c, &nloc, &voxdim);
+    if (!quiet) printf("Determining release times...");
+    snodes_t S = stagger_schedule(d, c, sl, rp, nloc, voxdim);
+    if (!quiet) printf("OK!\nTotal number of points to be released is %lld\n", (long long)S.n);
+    const int64_t n = S.n;
+    const int TR = d->Output_TRes;
+    const double TS = d->Output_TStart, DT = d->Output_TDelta;
+    FILE **of = (FILE **)calloc((size_t)TR, sizeof(FILE *));
+    char path[3 * FVC_STR];
+    for (int ss = 0; ss < TR; ss++) { /* main.c:92-106 */
+        snprintf(path, sizeof path, "%s%s.%d.bin", d->Path_Output, d->Trace_OutFilePrefix, ss);
+        if (!(of[ss] = fopen(path, "wb"))) fatal("Could not open file %s", path);
+        double tt = TS + ss * DT;
+        fwrite(&tt, 8, 1, of[ss]);
+    }
+    fvc_particles *P = NULL;
+    check(fvc_particles_create(c, n > 0 ? n : 1, &P), c, "fvc_particles_create");
+    fvc_point *recs = (fvc_point *)calloc((size_t)(n > 0 ? n : 1), sizeof(fvc_point));
+    for (int64_t k = 0; k < n; k++) {
+        memcpy(recs[k].X, rp[S.v[k].loc].X, sizeof recs[k].X);
+        recs[k].ElementIndex = S.v[k].elem;
+    }
+    if (n) check(fvc_particles_set(P, 0, recs, n), c, "fvc_particles_set");
+    double *X = (double *)malloc(sizeof(double) * 3 * (size_t)(n > 0 ? n : 1));
+    int32_t *left = (int32_t *)calloc((size_t)(n > 0 ? n : 1), sizeof(int32_t));
+    for (int64_t k = 0; k < n; k++) memcpy(X + 3 * k, recs[k].X, sizeof recs[k].X);
+    int *status = (int *)calloc((size_t)(n > 0 ? n : 1), sizeof(int));
+    int *nout = (int *)calloc((size_t)(n > 0 ? n : 1), sizeof(int)), *first_ii = (int *)calloc((size_t)(n > 0 ? n : 1), sizeof(int));
+    double *tt = (double *)malloc(sizeof(double) * 2 * (size_t)(n > 0 ? n : 1)), *t2last = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    unsigned char *act = (unsigned char *)calloc((size_t)(n > 0 ? n : 1), 1);
+    srec_t *rec = NULL;
+    int64_t nrec = 0, caprec = 0;
+#define SREC(k_, file_)                                                                                                \
+    do {                                                                                                               \
+        if (nrec == caprec) {                                                                                          \
+            caprec = caprec ? 2 * caprec : 4096;                                                                       \
+            rec = (srec_t *)realloc(rec, sizeof(srec_t) * (size_t)caprec);                                             \
+        }                                                                                                              \
+        rec[nrec].node = (k_);                                                                                         \
+        rec[nrec].file = (file_);                                                                                      \
+        memcpy(rec[nrec].X, X + 3 * (k_), 3 * sizeof(double));                                                        \
+        nrec++;                                                                                                        \
+    } while (0)
+    const double NaN = nan("");
+    for (int df = d->Data_FirstFrame; df != d->Data_LastFrame; df++) {
+        int lo = slot_for(sl, df, -1);
+        int hi = slot_for(sl, df + 1, lo);
+        const double tmin = d->Data_TMin + df * d->Data_TDelta, tmax = tmin + d->Data_TDelta;
+        nrec = 0;
+        int maxout = 0, any = 0;
+        for (int64_t k = 0; k < n; k++) {
+            const snode_t *r = &S.v[k];
+            act[k] = 0;
+            tt[2 * k] = NaN;
+            tt[2 * k + 1] = NaN;
+            if (status[k] == 2 || !(status[k] == 1 || (r->start >= tmin && r->start <= tmax))) continue;
+            if (left[k] && !ip->extrapolate) continue;
+            if (status[k] == 0) {
+                status[k] = 1;
+                if (fabs(r->start - TS) < TINY) SREC(k, 0);
+            }
+            double t1 = fmax(r->start, tmin), t2 = fmin(r->stop, tmax);
+            int oc = 0;
+            for (int ii = 0; ii < TR; ii++)
+                if ((TS + ii * DT > t1 + TINY) && (TS + ii * DT <= t2 + TINY)) oc++;
+            nout[k] = oc;
+            if (oc) {
+                int ii;
+                double next = 0.0;
+                for (ii = 0; ii < TR; ii++) {
+                    next = TS + ii * DT;
+                    if (next > t1 + TINY) break;
+                }
+                first_ii[k] = ii;
+                t2 = next;
+            }
+            tt[2 * k] = t1;
+            tt[2 * k + 1] = t2;
+            t2last[k] = t2;
+            act[k] = 1;
+            any = 1;
+            if (oc > maxout) maxout = oc;
+        }
+        if (any) {
+            double t0 = now_s();
+            /* round 0: to the first output time or the segment end */
+            check(fvc_advect_times(c, P, 0, n, tt, lo, hi, tmin, tmax, ip), c, "fvc_advect_times");
+            int64_t st = 0;
+            fvc_last_step_count(c, &st);
+            steps_total += st;
+            check(fvc_particles_get_xyz(P, 0, X, left, n), c, "fvc_particles_get_xyz");
+            for (int64_t k = 0; k < n; k++)
+                if (act[k] && nout[k] && (!left[k] || ip->extrapolate)) {
+                    if (first_ii[k] >= TR) fatal("%s", "output index out of range");
+                    SREC(k, first_ii[k]);
+                }
+            /* later output times inside the interval (t2 += Output_TDelta, as tracers.c:366-368) */
+            for (int r = 1; r < maxout; r++) {
+                int more = 0;
+                for (int64_t k = 0; k < n; k++) {
+                    tt[2 * k] = NaN;
+                    if (act[k] && nout[k] > r && (!left[k] || ip->extrapolate)) {
+                        tt[2 * k] = t2last[k];
+                        tt[2 * k + 1] = t2last[k] + DT;
+                        t2last[k] = tt[2 * k + 1];
+                        more = 1;
+                    }
+                }
+                if (!more) break;
+                check(fvc_advect_times(c, P, 0, n, tt, lo, hi, tmin, tmax, ip), c, "fvc_advect_times");
+                fvc_last_step_count(c, &st);
+                steps_total += st;
+                check(fvc_particles_get_xyz(P, 0, X, left, n), c, "fvc_particles_get_xyz");
+                for (int64_t k = 0; k < n; k++)
+                    if (tt[2 * k] == tt[2 * k] && (!left[k] || ip->extrapolate)) {
+                        const int file = (int)((tt[2 * k + 1] + TINY) / DT);
+                        if (file < 0 || file >= TR) fatal("%s", "output index out of range");
+                        SREC(k, file);
+                    }
+            }
+            /* remainder of the interval, up to the stop time */
+            int more = 0;
+            for (int64_t k = 0; k < n; k++) {
+                tt[2 * k] = NaN;
+                if (act[k] && nout[k] && (!left[k] || ip->extrapolate)) {
+                    const double t1 = t2last[k], t2 = fmin(S.v[k].stop, tmax);
+                    if (t2 - t1 > 0) {
+                        tt[2 * k] = t1;
+                        tt[2 * k + 1] = t2;
+                        more = 1;
+                    } else if (t2 - t1 < -TINY) {
+                        fatal("%s", "Integration of point beyond requested stop time or range of loaded data");
+                    }
+                }
+            }
+            if (more) {
+                check(fvc_advect_times(c, P, 0, n, tt, lo, hi, tmin, tmax, ip), c, "fvc_advect_times");
+                fvc_last_step_count(c, &st);
+                steps_total += st;
+                check(fvc_particles_get_xyz(P, 0, X, left, n), c, "fvc_particles_get_xyz");
+            }
+            *t_compute += now_s() - t0;
+            for (int64_t k = 0; k < n; k++)
+                if (act[k] && S.v[k].stop <= tmax) status[k] = 2;
+        }
+        /* this interval's records: per output file, in location and release order */
+        if (nrec) {
+            qsort(rec, (size_t)nrec, sizeof(srec_t), srec_cmp);
+            for (int64_t q = 0; q < nrec; q++) fwrite(rec[q].X, 8, 3, of[rec[q].file]);
+        }
+        if (!quiet) printf(" %lld tracers integrated.\n", (long long)0);
+    }
+#undef SREC
+    for (int ss = 0; ss < TR; ss++) fclose(of[ss]);
+    free(of);
+    free(rec);
+    free(act);
+    free(tt);
+    free(t2last);
+    free(nout);
+    free(first_ii);
+    free(status);
+    free(left);
+    free(X);
+    free(recs);
+    free(S.v);
+    free(rp);
+    fvc_particles_destroy(P);
+    return steps_total;
+}
+
 int main(int argc, char **argv) {
     const char *deck_path = NULL;
     int precision = FVC_PREC_F64, nproc = (int)sysconf(_SC_NPROCESSORS_ONLN), quiet = 0;
@@ -187,10 +594,9 @@ int main(int argc, char **argv) {
         fvc_deck_derive(&d, err, sizeof err))
         fatal("%s", err);
     if (d.legacy_96 && !quiet) fprintf(stderr, "Note: deck has 96 entries (no Num_threads); read as Num_threads = 0\n");
-    if (d.Particle_Radius > TINY || d.Int_Type == 2 || d.Trace_ReleaseStrategy == 1 || d.Trace_CETCompute ||
-        d.Trace_RTCompute || d.Trace_APCompute || d.VelOut_Compute || d.Int_NormalFlow)
-        fatal("%s", "deck selects a mode outside the advection path (inertial, RKF, staggered, CET, RT, AP, VelOut, "
-                    "normal flow)");
+    if (d.Particle_Radius > TINY || d.Int_Type == 2 || d.Trace_CETCompute || d.Trace_RTCompute || d.Trace_APCompute ||
+        d.VelOut_Compute || d.Int_NormalFlow)
+        fatal("%s", "deck selects a mode outside the advection path (inertial, RKF, CET, RT, AP, VelOut, normal flow)");
     const int dir = d.Int_TimeDirection;
 
     /* mesh (mesh.c:21-213) */
@@ -304,6 +710,8 @@ int main(int argc, char **argv) {
         free(wot);
         for (int ss = 0; ss < ns; ss++) fvc_ftle_destroy(f[ss]);
         free(f);
+    } else if (d.Trace_Compute && d.Trace_ReleaseStrategy == 1) {
+        steps_total += run_staggered(&d, c, &sl, &ip, quiet, &t_compute);
     } else if (d.Trace_Compute) {
         /* seeds: ReadInTraceMesh / GenerateTracerMesh location chains */
         int64_t n = 0;

@@ -249,6 +249,14 @@ int fvc_deck_check(const fvc_deck *d, char *err, int errlen) {
         if (d->FTLE_IntTLength < FVC_TINY) FAIL("FTLE_IntTLength must be positive");
     }
     if (d->Trace_Compute) {
+        if (d->Trace_ReleaseStrategy == 1) { /* parameters.c:500-511 */
+            if (d->Trace_AlwaysOutput)
+                fprintf(stderr, "\nWarning: Trace_AlwaysOutput not supported for Trace_ReleaseStrategy = 1\n\n");
+            if (d->Int_TimeDirection < 0) FAIL("Staggered tracer release not supported for backward time integration");
+            if (d->Trace_ReleaseTMax < 0) FAIL("Trace_ReleaseTMax < 0");
+            if (d->Dimensions == 2) FAIL("Staggered release not supported for 2D analysis");
+            if (d->Trace_InFileFormat != 4) FAIL("Trace_InFileFormat must be 4 for Trace_ReleaseStrategy = 1");
+        }
         if (d->Trace_GenerateMesh) {
             for (int a = 0; a < 3; a++) {
                 if (d->Trace_Bounds[2 * a] > d->Trace_Bounds[2 * a + 1]) FAIL("Trace_CartMesh Min > Max");

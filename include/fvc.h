@@ -149,6 +149,14 @@ int fvc_launch_velocity(fvc_ctx *ctx, fvc_particles *p, int64_t begin, int64_t c
  * hold the loaded interval [tmin, tmax] (main.c:188-195). */
 int fvc_advect(fvc_ctx *ctx, fvc_particles *p, int64_t begin, int64_t count, double t1, double t2,
                int32_t lo, int32_t hi, double tmin, double tmax, const fvc_int_params *ip);
+
+/* Advect(&pt, t1_i, t2_i) with a segment of its own for every point i of
+ * [begin, begin+count): t12[2q], t12[2q+1] for point begin+q; a NaN t1 skips
+ * the point.  The staggered release integrates each released point this way
+ * between output times (ompVC/tracers.c:336-397).  Same filter as fvc_advect;
+ * RK4 or Euler; segments must lie in [tmin, tmax]. */
+int fvc_advect_times(fvc_ctx *ctx, fvc_particles *p, int64_t begin, int64_t count, const double *t12,
+                     int32_t lo, int32_t hi, double tmin, double tmax, const fvc_int_params *ip);
 /* Points advanced by the last fvc_advect (particle-steps, the step in which a
  * point exits included, floating-point micro-steps below 1e-12*h excluded). */
 int fvc_last_step_count(fvc_ctx *ctx, int64_t *particle_steps);

@@ -63,6 +63,33 @@ def gen_pipe(d):
             f.write("%.17g %.17g %.17g\n" % tuple(p))
 
 
+def release_line():
+    """21 release points across the pipe at z = 0.5, 0.09 apart (the spacing
+    that sets the release interval)."""
+    return np.array([[-0.9 + 0.09 * i, 0.013, 0.5] for i in range(21)])
+
+
+def gen_pipe_stagger(d):
+    gen_pipe(d)
+    host.write_binary_list(os.path.join(d, "release.bin"), release_line())
+
+
+def gen_pipe_steady(d):
+    """The pipe with every slice equal to the t=0 field (steady flow: linear
+    release-time solves)."""
+    t = host.pipe(*PIPE_DIMS, 1.0, 10.0)
+    xyz, conn, adj = host.tube_mesh(t)
+    host.write_unstruct_mesh(d, "pipe", xyz, conn, adj)
+    v0 = host.tube_velocity(t, xyz, 0.0, 0.01, 1234, 0)
+    for s in range(50):
+        host.write_vel_slice(os.path.join(d, "pipe_vel.%d.bin" % s), s * 0.02, v0)
+    host.write_binary_list(os.path.join(d, "release.bin"), release_line())
+
+
+STAGGER = dict(PIPE, Num_threads=1, Output_TStart=0.0, Output_TRes=9, Output_TDelta=0.05, Int_TimeDirection=1,
+               Trace_Compute=1, Trace_ReleaseStrategy=1, Trace_ReleaseTMax=0.3, Trace_GenerateMesh=0,
+               Trace_InFile="release.bin", Trace_InFileFormat=4, Trace_IntTLength=0.25)
+
 CASES = {
     # 2-D Cartesian FTLE over T=10 (the CPU-runnable workload's physics, 128x64 grid)
     "dg_ftle": dict(gen=gen_dg, outputs=["dgftle.0.bin", "dgftle_noT.0.bin", "dgftle_Cartesian.bin"],
@@ -99,6 +126,13 @@ CASES = {
                         deck=dict(PIPE, Output_TStart=0.0, Output_TRes=1, Output_TDelta=0.1, Int_TimeDirection=1,
                                   Trace_Compute=1, Trace_GenerateMesh=0, Trace_InFile="probes.txt",
                                   Trace_InFileFormat=1, Trace_IntTLength=0.0, Trace_OutFilePrefix="ploc")),
+    # staggered release (tracers.c:293-796) from 21 points, unsteady flow (quadratic release-time solves);
+    # one OpenMP thread so that the reference writes records in location order
+    "pipe_staggered": dict(gen=gen_pipe_stagger, outputs=["pst.%d.bin" % k for k in range(9)],
+                           deck=dict(STAGGER, Trace_OutFilePrefix="pst")),
+    # staggered release in steady flow (linear release-time solves)
+    "pipe_staggered_steady": dict(gen=gen_pipe_steady, outputs=["pss.%d.bin" % k for k in range(9)],
+                                  deck=dict(STAGGER, Trace_OutFilePrefix="pss")),
     # 3-D unstructured FTLE with outside masks and early exits, 2 slides
     "pipe_ftle": dict(gen=gen_pipe, outputs=["pft.0.bin", "pft.1.bin", "pft_noT.0.bin", "pft_noT.1.bin",
                                              "pft_Cartesian.bin"],

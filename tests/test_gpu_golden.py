@@ -29,7 +29,8 @@ def run_driver(case, tmp_path, precision="f64"):
     return man, data, out
 
 
-@pytest.mark.parametrize("case", ["dg_tracers", "pipe_tracers", "pipe_locate"])
+@pytest.mark.parametrize("case", ["dg_tracers", "pipe_tracers", "pipe_locate", "pipe_staggered",
+                                  "pipe_staggered_steady"])
 def test_gpu_trajectories_bit_identical(case, tmp_path):
     man, data, out = run_driver(case, tmp_path)
     fx = cases.fixture_dir(case)
@@ -78,3 +79,23 @@ def test_gpu_f32_driver_within_tolerance(case, tmp_path, precision):
     if a.shape == b.shape:
         err = np.abs(a - b).max()
         assert err <= 1e-5 * size, err
+
+
+def test_gpu_staggered_rewrites_release_file_without_outside_points(tmp_path):
+    """Release points outside the mesh are dropped from Trace_InFile, which is
+    rewritten with the interior points (tracers.c:970-990); the run goes on
+    with the interior ones."""
+    deck_path, data, out = cases.prepare("pipe_staggered", str(tmp_path))
+    pts = cases.release_line()
+    allp = np.concatenate([pts[:5], [[1.5, 0.0, 0.5], [0.0, 0.0, -3.0]], pts[5:]])
+    host.write_binary_list(os.path.join(data, "release.bin"), allp)
+    r = subprocess.run([FLOWVC, deck_path, "--precision", "f64", "--quiet"], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    raw = np.fromfile(os.path.join(data, "release.bin"), np.uint8)
+    assert int(raw[:4].view(np.int32)[0]) == pts.shape[0]
+    np.testing.assert_array_equal(raw[4:].view(np.float64).reshape(-1, 3), pts)
+    # same outputs as the run without the outside points
+    fx = cases.fixture_dir("pipe_staggered")
+    for f in cases.load_manifest("pipe_staggered")["outputs"]:
+        assert open(os.path.join(fx, f), "rb").read() == open(os.path.join(out, f), "rb").read(), f
