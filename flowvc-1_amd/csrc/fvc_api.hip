@@ -795,6 +795,13 @@ extern "C" int fvc_seed_grid(fvc_ctx *c, const double mn[3], const double del[3]
     return seed_common(c, 1, nullptr, n, mn, del, res, lsc, out);
 }
 
+extern "C" int fvc_seed_velout_grid(fvc_ctx *c, const double mn[3], const double del[3], const int32_t res[3],
+                                    int32_t lsc, fvc_point *out) {
+    if (!c || !mn || !del || !res || !out || res[0] < 1 || res[1] < 1 || res[2] < 1) return FVC_E_INVALID;
+    int64_t n = (int64_t)res[0] * res[1] * res[2];
+    return seed_common(c, 3, nullptr, n, mn, del, res, lsc, out);
+}
+
 /* ------------------------------------------------------------------------ */
 /* FTLE slides                                                               */
 /* ------------------------------------------------------------------------ */

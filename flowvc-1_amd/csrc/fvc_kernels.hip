@@ -423,7 +423,13 @@ struct ChainGeom {
     int res[3];
 };
 
-/* chain-order point i -> coordinates (ftle.c:412-420, tracers.c:1344-1346) */
+/* Chains 2 and 3 walk rows and planes with the guess reset to the row's
+ * (plane's) first found element at each row (plane) end: chain 2 is the FTLE
+ * grid (z fastest, ftle.c:456-516), chain 3 the velocity output grid (x
+ * fastest, velout.c:58-163). */
+__host__ __device__ inline int64_t chain_rowlen(const ChainGeom &g) { return g.chain == 2 ? g.res[2] : g.res[0]; }
+
+/* chain-order point i -> coordinates (ftle.c:412-420, tracers.c:1344-1346, velout.c:67-69) */
 __device__ inline void chain_point(const ChainGeom &g, int64_t i, double X[3]) {
 #pragma clang fp contract(off)
     if (g.chain == 0) {
@@ -433,7 +439,7 @@ __device__ inline void chain_point(const ChainGeom &g, int64_t i, double X[3]) {
         return;
     }
     int ii, jj, kk;
-    if (g.chain == 1) { /* k, j, i order: x fastest */
+    if (g.chain == 1 || g.chain == 3) { /* k, j, i order: x fastest */
         ii = (int)(i % g.res[0]);
         jj = (int)((i / g.res[0]) % g.res[1]);
         kk = (int)(i / ((int64_t)g.res[0] * g.res[1]));
@@ -468,10 +474,11 @@ __global__ __launch_bounds__(BLOCK) void k_seed_prepare(DevMesh m, ChainGeom g, 
     if (active) {
         double X[3];
         chain_point(g, i, X);
-        if (g.chain == 2) {
-            const int kk = (int)(i % g.res[2]), jj = (int)((i / g.res[2]) % g.res[1]);
+        if (g.chain >= 2) {
+            const int64_t rl = chain_rowlen(g);
+            const int kk = (int)(i % rl), jj = (int)((i / rl) % g.res[1]);
             if (kk == 0) flags |= SF_ROWSTART | (jj == 0 ? SF_PLANESTART : 0);
-            if (kk == g.res[2] - 1) flags |= SF_ROWEND | (jj == g.res[1] - 1 ? SF_PLANEEND : 0);
+            if (kk == rl - 1) flags |= SF_ROWEND | (jj == g.res[1] - 1 ? SF_PLANEEND : 0);
         }
         if (outside_domain(m, X[0], X[1], X[2]))
             flags |= SF_OUTB;
@@ -492,11 +499,11 @@ __global__ __launch_bounds__(BLOCK) void k_seed_prepare(DevMesh m, ChainGeom g, 
         atomicMin(tp.misc, (int)i);
         atomicMax(tp.misc + 1, wmax);
     }
-    if (g.chain == 2 && found) {
+    if (g.chain >= 2 && found) {
         /* one atomic per row (plane) and wave: the wave's first found point in it */
         const unsigned long long below = F & ((1ull << lane) - 1ull);
         const int64_t prev = below ? i - lane + (63 - __clzll(below)) : -1;
-        const int64_t rowlen = g.res[2], planelen = rowlen * g.res[1];
+        const int64_t rowlen = chain_rowlen(g), planelen = rowlen * g.res[1];
         if (prev < 0 || prev / rowlen != i / rowlen) atomicMin(tp.rowfirst + i / rowlen, (int)i);
         if (prev < 0 || prev / planelen != i / planelen) atomicMin(tp.planefirst + i / planelen, (int)i);
     }
@@ -552,7 +559,7 @@ __global__ __launch_bounds__(CHAIN_STAGE) void k_seed_chain(DevMesh m, ChainGeom
                 }
             }
             if (g.chain == 0) lt = left ? -1.0 : 0.0;
-            if (g.chain == 2 && left) lt = -1.0;
+            if (g.chain >= 2 && left) lt = -1.0;
             s_elem[threadIdx.x] = elem;
             s_left[threadIdx.x] = left;
             s_lt[threadIdx.x] = lt;
@@ -601,7 +608,7 @@ __global__ __launch_bounds__(CHAIN_STAGE) void k_seed_chain(DevMesh m, ChainGeom
                             const int last = 63 - __clzll(sh), first = __ffsll((long long)sh) - 1;
                             guess = ce[last];
                             if (g.chain == 0) found += __popcll(sh);
-                            if (g.chain == 2) {
+                            if (g.chain >= 2) {
                                 const int e1 = ce[first];
                                 if (jguess < 0) jguess = e1;
                                 if (iguess < 0) iguess = e1;
@@ -645,7 +652,7 @@ __global__ __launch_bounds__(CHAIN_STAGE) void k_seed_chain(DevMesh m, ChainGeom
                             for (int l = 0; l < lim; l++) {
                                 const int fl = __shfl(my.y, l, 64);
                                 const int rl = __shfl(r, l, 64);
-                                if (g.chain == 2 && (fl & SF_ROWSTART)) {
+                                if (g.chain >= 2 && (fl & SF_ROWSTART)) {
                                     jg = -1;
                                     if (fl & SF_PLANESTART) ig = -1;
                                 }
@@ -653,12 +660,12 @@ __global__ __launch_bounds__(CHAIN_STAGE) void k_seed_chain(DevMesh m, ChainGeom
                                 if (!(fl & SF_OUTB) && rl >= 0) {
                                     gs = rl;
                                     fc++;
-                                    if (g.chain == 2) {
+                                    if (g.chain >= 2) {
                                         if (jg < 0) jg = rl;
                                         if (ig < 0) ig = rl;
                                     }
                                 }
-                                if (g.chain == 2 && (fl & SF_ROWEND)) {
+                                if (g.chain >= 2 && (fl & SF_ROWEND)) {
                                     if (jg >= 0) gs = jg;
                                     if ((fl & SF_PLANEEND) && ig >= 0) gs = ig;
                                 }
@@ -688,7 +695,7 @@ __global__ __launch_bounds__(CHAIN_STAGE) void k_seed_chain(DevMesh m, ChainGeom
                             const bool left = outb || r < 0;
                             double lt = (!outb && r < 0) ? -1.0 : 0.0;
                             if (g.chain == 0) lt = left ? -1.0 : 0.0;
-                            if (g.chain == 2 && left) lt = -1.0;
+                            if (g.chain >= 2 && left) lt = -1.0;
                             s_elem[base - stage + lane] = left ? -1 : r;
                             s_left[base - stage + lane] = left ? 1 : 0;
                             s_lt[base - stage + lane] = lt;
@@ -708,7 +715,7 @@ __global__ __launch_bounds__(CHAIN_STAGE) void k_seed_chain(DevMesh m, ChainGeom
                         const int emin = __shfl(my.x, l, 64), flags = __shfl(my.y, l, 64);
                         int elem = -1, left = (flags & SF_OUTB) ? 1 : 0;
                         double lt = 0.0;
-                        if (g.chain == 2 && (flags & SF_ROWSTART)) {
+                        if (g.chain >= 2 && (flags & SF_ROWSTART)) {
                             jguess = -1;
                             if (flags & SF_PLANESTART) iguess = -1;
                         }
@@ -742,7 +749,7 @@ __global__ __launch_bounds__(CHAIN_STAGE) void k_seed_chain(DevMesh m, ChainGeom
                                     } else {
                                         elem = seed;
                                         guess = seed;
-                                        if (g.chain == 2) jguess = iguess = seed;
+                                        if (g.chain >= 2) jguess = iguess = seed;
                                     }
                                 } else {
                                     if (direct)
@@ -759,7 +766,7 @@ __global__ __launch_bounds__(CHAIN_STAGE) void k_seed_chain(DevMesh m, ChainGeom
                                     } else {
                                         elem = index;
                                         guess = index;
-                                        if (g.chain == 2) {
+                                        if (g.chain >= 2) {
                                             if (jguess < 0) jguess = index;
                                             if (iguess < 0) iguess = index;
                                         }
@@ -769,8 +776,8 @@ __global__ __launch_bounds__(CHAIN_STAGE) void k_seed_chain(DevMesh m, ChainGeom
                             if (err && lane == 0) raise_err(eb, err, i);
                         }
                         if (g.chain == 0) lt = left ? -1.0 : 0.0;
-                        if (g.chain == 2 && left) lt = -1.0;
-                        if (g.chain == 2 && (flags & SF_ROWEND)) {
+                        if (g.chain >= 2 && left) lt = -1.0;
+                        if (g.chain >= 2 && (flags & SF_ROWEND)) {
                             if (jguess >= 0) guess = jguess;
                             if ((flags & SF_PLANEEND) && iguess >= 0) guess = iguess;
                         }
@@ -833,8 +840,8 @@ __global__ __launch_bounds__(BLOCK) void k_tab_src(ChainGeom g, int64_t n, const
     const int2 my = active ? info[i] : make_int2(-1, SF_OUTB);
     const bool found = active && !(my.y & SF_OUTB) && my.x >= 0;
     int rv = -1; /* row-end event value */
-    if (g.chain == 2 && active && (my.y & SF_ROWEND)) {
-        const int64_t rowlen = g.res[2], planelen = rowlen * g.res[1];
+    if (g.chain >= 2 && active && (my.y & SF_ROWEND)) {
+        const int64_t rowlen = chain_rowlen(g), planelen = rowlen * g.res[1];
         const int pf = planefirst[i / planelen], rf = rowfirst[i / rowlen];
         if ((my.y & SF_PLANEEND) && pf < n) rv = pf;
         else if (rf < n) rv = rf;
@@ -1100,7 +1107,8 @@ int launch_seed_chain(fvc_ctx *c, int chain, int64_t n, const double *dxyz, cons
     /* point indices of the tabulated chain are ints; 0x7f7f7f7f marks "none" */
     const bool tab = uns && lsc && n < 0x7f000000LL;
     const int64_t nch = (n + 63) / 64;
-    const int64_t nrows = chain == 2 ? n / g.res[2] : 1, nplanes = chain == 2 ? n / ((int64_t)g.res[2] * g.res[1]) : 1;
+    const int64_t nrows = chain >= 2 ? n / chain_rowlen(g) : 1,
+                  nplanes = chain >= 2 ? n / (chain_rowlen(g) * g.res[1]) : 1;
     size_t bytes = sizeof(int2) * (size_t)n;
     if (tab) bytes += sizeof(int) * (4 * (size_t)n + 2 * (size_t)nch + (size_t)nrows + (size_t)nplanes + 2);
     char *base = (char *)scratch(c, bytes);

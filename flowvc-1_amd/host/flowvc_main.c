@@ -11,9 +11,9 @@
  * file is uploaded once into its own slot (the reference re-reads two slices
  * per frame, velocity.c:61-234).  --nproc emulates the host processor count
  * that selects ompVC's tracer output loop (parameters.c:411-421).
- * Staggered tracer release follows tracers.c:293-1000.  Modes outside the
- * advection path (inertial particles, RKF, exposure/residence time, strain
- * rate, VelOut, normal flow) stop with an error.
+ * Staggered tracer release follows tracers.c:293-1000 and velocity output
+ * velout.c:21-435.  Modes outside the advection path (inertial particles,
+ * RKF, exposure/residence time, strain rate, normal flow) stop with an error.
  */
 #include <math.h>
 #include <stdio.h>
@@ -158,6 +158,123 @@ static double *read_seed_list(const fvc_deck *d, int64_t *n) {
         fatal("Unsupported value for Trace_InFileFormat in %s", path);
     }
     return xyz;
+}
+
+/* ------------------------------------------------------------------------ */
+/* velocity output (velout.c:21-435)                                        */
+/* ------------------------------------------------------------------------ */
+/* Interpolated velocity at every output point and output time: a generated
+ * grid (GenerateVelOutMesh) or a point list (ReadInVelOutMesh) located once,
+ * then GetVelocity at each Output time inside the loaded interval (main.c:
+ * 205-213), each point walking on from its element of the previous output. */
+static void run_velout(const fvc_deck *d, fvc_ctx *c, slices_t *sl, int quiet) {
+    const int dir = d->Int_TimeDirection;
+    int64_t n = 0;
+    fvc_point *pts = NULL;
+    char path[3 * FVC_STR];
+    double mn[3] = {d->VelOut_Bounds[0], d->VelOut_Bounds[2], d->VelOut_Bounds[4]};
+    double mx[3] = {d->VelOut_Bounds[1], d->VelOut_Bounds[3], d->VelOut_Bounds[5]};
+    int32_t res[3] = {d->VelOut_Res[0], d->VelOut_Res[1], d->VelOut_Res[2]};
+    double del[3];
+    if (d->VelOut_GenerateMesh) { /* velout.c:28-48 */
+        for (int a = 0; a < 2; a++) del[a] = res[a] == 1 ? 0 : (mx[a] - mn[a]) / (res[a] - 1);
+        if (res[2] == 1)
+            del[2] = 0;
+        else if (d->Dimensions == 3)
+            del[2] = (mx[2] - mn[2]) / (res[2] - 1);
+        else {
+            fprintf(stderr, "Warning: VelOut_CartMesh.ZRes being set to 1 since velocity data is 2D\n");
+            res[2] = 1;
+            del[2] = 0;
+        }
+        n = (int64_t)res[0] * res[1] * res[2];
+        pts = (fvc_point *)calloc((size_t)(n > 0 ? n : 1), sizeof(fvc_point));
+        check(fvc_seed_velout_grid(c, mn, del, res, d->LocalSearchChecking, pts), c, "fvc_seed_velout_grid");
+    } else { /* velout.c:188-293 */
+        snprintf(path, sizeof path, "%s%s", d->Path_Data, d->VelOut_InFile);
+        FILE *f = fopen(path, "r");
+        if (!f) fatal("Could not open file %s", path);
+        int cnt = 0;
+        double *xyz = NULL;
+        if (d->VelOut_InFileFormat == 1) {
+            if (fscanf(f, "%d\n", &cnt) != 1) fatal("Could not read the number of points from %s", path);
+            xyz = (double *)calloc((size_t)3 * (cnt > 0 ? cnt : 1), sizeof(double));
+            for (int q = 0; q < cnt; q++) {
+                int ok = d->Dimensions == 3 ? fscanf(f, "%lf %lf %lf\n", xyz + 3 * q, xyz + 3 * q + 1, xyz + 3 * q + 2) == 3
+                                            : fscanf(f, "%lf %lf\n", xyz + 3 * q, xyz + 3 * q + 1) == 2;
+                if (!ok) fatal("Could not read coordinates from %s", path);
+            }
+        } else if (d->VelOut_InFileFormat == 2) { /* VTK polydata: 4 header lines, then POINTS n type */
+            char buf[1024];
+            for (int q = 0; q < 4; q++)
+                if (!fgets(buf, sizeof buf, f)) fatal("Reached end of file in %s", path);
+            if (fscanf(f, "%*s %d %*s\n", &cnt) != 1) fatal("Could not read the number of points from %s", path);
+            xyz = (double *)calloc((size_t)3 * (cnt > 0 ? cnt : 1), sizeof(double));
+            for (int q = 0; q < cnt; q++)
+                if (fscanf(f, "%lf %lf %lf", xyz + 3 * q, xyz + 3 * q + 1, xyz + 3 * q + 2) < 3)
+                    fatal("Failed to read coordinates from %s", path);
+        } else {
+            fatal("%s", "VelOut_InFileFormat 3 (VTK unstructured connectivity output) is not supported by this driver");
+        }
+        fclose(f);
+        n = cnt;
+        pts = (fvc_point *)calloc((size_t)(n > 0 ? n : 1), sizeof(fvc_point));
+        check(fvc_seed_list(c, xyz, n, d->LocalSearchChecking, pts), c, "fvc_seed_list");
+        free(xyz);
+    }
+    if (!quiet) {
+        int64_t found = 0;
+        for (int64_t q = 0; q < n; q++) found += !pts[q].LeftDomain;
+        printf("  %lld of %lld located in domain\n", (long long)found, (long long)n);
+    }
+    fvc_particles *P = NULL;
+    check(fvc_particles_create(c, n > 0 ? n : 1, &P), c, "fvc_particles_create");
+    if (n) check(fvc_particles_set(P, 0, pts, n), c, "fvc_particles_set");
+    const int TR = d->Output_TRes;
+    int *done = (int *)calloc((size_t)TR, sizeof(int));
+    fvc_point *cur = (fvc_point *)calloc((size_t)(n > 0 ? n : 1), sizeof(fvc_point));
+    for (int df = d->Data_FirstFrame; df != d->Data_LastFrame; df += dir) {
+        int flo = dir > 0 ? df : df - 1;
+        int lo = slot_for(sl, flo, -1);
+        int hi = slot_for(sl, flo + 1, lo);
+        const double tmin = d->Data_TMin + flo * d->Data_TDelta, tmax = tmin + d->Data_TDelta;
+        for (int ss = 0; ss < TR; ss++) {
+            const double t = d->Output_TStart + dir * ss * d->Output_TDelta; /* velout.c:183-184 */
+            if (!(t - tmin > -TINY && t - tmax < TINY) || done[ss]) continue;
+            if (ss == 0) { /* mesh description with the first output (velout.c:318-350) */
+                if (d->VelOut_GenerateMesh) {
+                    snprintf(path, sizeof path, "%s%s_Cartesian.bin", d->Path_Output, d->VelOut_FilePrefix);
+                    if (fvc_write_cart_header(path, mn, mx, res)) fatal("Could not open file %s", path);
+                } else {
+                    snprintf(path, sizeof path, "%s%s_coordinates.bin", d->Path_Output, d->VelOut_FilePrefix);
+                    FILE *o = fopen(path, "wb");
+                    if (!o) fatal("Could not open file %s", path);
+                    int32_t n32 = (int32_t)n;
+                    fwrite(&n32, 4, 1, o);
+                    for (int64_t q = 0; q < n; q++) fwrite(pts[q].X, 8, 3, o);
+                    fclose(o);
+                }
+            }
+            if (n) {
+                check(fvc_launch_velocity(c, P, 0, n, t, lo, hi, tmin, tmax), c, "fvc_launch_velocity");
+                check(fvc_particles_get(P, 0, cur, n), c, "fvc_particles_get");
+            }
+            snprintf(path, sizeof path, "%s%s.%d.bin", d->Path_Output, d->VelOut_FilePrefix, ss);
+            FILE *o = fopen(path, "wb");
+            if (!o) fatal("Could not open file %s", path);
+            fwrite(&t, 8, 1, o);
+            const double zero[3] = {0.0, 0.0, 0.0};
+            for (int64_t q = 0; q < n; q++) fwrite(pts[q].LeftDomain ? zero : cur[q].V, 8, 3, o);
+            fclose(o);
+            for (int64_t q = 0; q < n; q++) pts[q].LeftDomain = cur[q].LeftDomain;
+            done[ss] = 1;
+            if (!quiet) printf("Outputting velocity field at time %f...OK!\n", t);
+        }
+    }
+    free(done);
+    free(cur);
+    free(pts);
+    fvc_particles_destroy(P);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -595,8 +712,8 @@ int main(int argc, char **argv) {
         fatal("%s", err);
     if (d.legacy_96 && !quiet) fprintf(stderr, "Note: deck has 96 entries (no Num_threads); read as Num_threads = 0\n");
     if (d.Particle_Radius > TINY || d.Int_Type == 2 || d.Trace_CETCompute || d.Trace_RTCompute || d.Trace_APCompute ||
-        d.VelOut_Compute || d.Int_NormalFlow)
-        fatal("%s", "deck selects a mode outside the advection path (inertial, RKF, CET, RT, AP, VelOut, normal flow)");
+        d.Int_NormalFlow)
+        fatal("%s", "deck selects a mode outside the advection path (inertial, RKF, CET, RT, AP, normal flow)");
     const int dir = d.Int_TimeDirection;
 
     /* mesh (mesh.c:21-213) */
@@ -644,6 +761,7 @@ int main(int argc, char **argv) {
     double t_compute = 0.0;
     int64_t steps_total = 0;
 
+    if (d.VelOut_Compute) run_velout(&d, c, &sl, quiet);
     if (d.FTLE_Compute) {
         if (!d.FTLE_GenerateMesh) fatal("%s", "FTLE_GenerateMesh = 0 (IC reuse) is not supported by this driver");
         int ns = d.Output_TRes;

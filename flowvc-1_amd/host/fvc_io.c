@@ -248,6 +248,16 @@ int fvc_deck_check(const fvc_deck *d, char *err, int errlen) {
         if (d->Dimensions == 3 && d->FTLE_Res[2] < 2) FAIL("FTLE_CartMesh.ZRes insufficient");
         if (d->FTLE_IntTLength < FVC_TINY) FAIL("FTLE_IntTLength must be positive");
     }
+    if (d->VelOut_Compute) { /* parameters.c:546-563 */
+        if (d->VelOut_GenerateMesh) {
+            for (int a = 0; a < 3; a++) {
+                if (d->VelOut_Bounds[2 * a] > d->VelOut_Bounds[2 * a + 1]) FAIL("VelOut_CartMesh Min > Max");
+                if (d->VelOut_Res[a] < 1) FAIL("VelOut_CartMesh Res < 1");
+            }
+        }
+        if (d->VelOut_InFileFormat != 1 && d->VelOut_InFileFormat != 2 && d->VelOut_InFileFormat != 3)
+            FAIL("Unsupported value for VelOut_InFileFormat");
+    }
     if (d->Trace_Compute) {
         if (d->Trace_ReleaseStrategy == 1) { /* parameters.c:500-511 */
             if (d->Trace_AlwaysOutput)

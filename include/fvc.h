@@ -172,6 +172,12 @@ int fvc_locate(fvc_ctx *ctx, const double *xyz, const int32_t *guess, int64_t n,
 int fvc_seed_list(fvc_ctx *ctx, const double *xyz, int64_t n, int32_t local_search_checking, fvc_point *out);
 int fvc_seed_grid(fvc_ctx *ctx, const double mn[3], const double del[3], const int32_t res[3],
                   int32_t local_search_checking, fvc_point *out);
+/* Location chain of GenerateVelOutMesh (ompVC/velout.c:21-186) for the
+ * velocity output grid: k,j,i order (x fastest), centre seed, guess reset to
+ * the first found element of each x-row and each z-plane at their ends.
+ * Writes X, LeftDomain and ElementIndex of res[0]*res[1]*res[2] records. */
+int fvc_seed_velout_grid(fvc_ctx *ctx, const double mn[3], const double del[3], const int32_t res[3],
+                         int32_t local_search_checking, fvc_point *out);
 
 /* ---- FTLE (ompVC/ftle.c) ---------------------------------------------------- */
 /* Grid generation, outside masks and the location chain of InitializeFTLEArray

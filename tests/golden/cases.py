@@ -86,6 +86,42 @@ def gen_pipe_steady(d):
     host.write_binary_list(os.path.join(d, "release.bin"), release_line())
 
 
+def velout_points():
+    """Listed velocity output points: interior, on voxel faces and vertices,
+    and outside the mesh."""
+    t = host.pipe(*PIPE_DIMS, 1.0, 10.0)
+    pts = list(host.tube_seeds(t, 60, 0.0, 10.0, 11))
+    dx, dz = 2.0 / PIPE_DIMS[0], 10.0 / PIPE_DIMS[2]
+    for i in range(2, 10):
+        pts.append([-1.0 + i * dx, -1.0 + 5 * dx, 7 * dz])
+        pts.append([-1.0 + 5 * dx, -1.0 + i * dx, 13 * dz + 0.37 * dz])
+    pts += [[0.99, 0.99, 2.0], [0.0, 0.0, 10.5], [0.0, 0.0, 0.0]]
+    return np.array(pts)
+
+
+def gen_pipe_velout(d):
+    gen_pipe(d)
+    with open(os.path.join(d, "velpts.txt"), "w") as f:
+        p = velout_points()
+        f.write("%d\n" % p.shape[0])
+        for q in p:
+            f.write("%.17g %.17g %.17g\n" % tuple(q))
+
+
+def gen_dg_velout(d):
+    gen_dg(d)
+    rng = np.random.default_rng(9)
+    p = np.concatenate([rng.uniform([-0.1, -0.1], [2.1, 1.1], size=(200, 2)), [[0.0, 0.0], [2.0, 1.0], [1.0, 0.5]]])
+    with open(os.path.join(d, "velpts.txt"), "w") as f:
+        f.write("%d\n" % p.shape[0])
+        for q in p:
+            f.write("%.17g %.17g\n" % tuple(q))
+
+
+VELOUT = {"VelOut_Compute": 1, "VelOut_MeshBounds_XMin": -1.0, "VelOut_MeshBounds_XMax": 1.0,
+          "VelOut_MeshBounds_YMin": -1.0, "VelOut_MeshBounds_YMax": 1.0, "VelOut_MeshBounds_ZMin": 0.0,
+          "VelOut_MeshBounds_ZMax": 10.0}
+
 STAGGER = dict(PIPE, Num_threads=1, Output_TStart=0.0, Output_TRes=9, Output_TDelta=0.05, Int_TimeDirection=1,
                Trace_Compute=1, Trace_ReleaseStrategy=1, Trace_ReleaseTMax=0.3, Trace_GenerateMesh=0,
                Trace_InFile="release.bin", Trace_InFileFormat=4, Trace_IntTLength=0.25)
@@ -133,6 +169,27 @@ CASES = {
     # staggered release in steady flow (linear release-time solves)
     "pipe_staggered_steady": dict(gen=gen_pipe_steady, outputs=["pss.%d.bin" % k for k in range(9)],
                                   deck=dict(STAGGER, Trace_OutFilePrefix="pss")),
+    # velocity output on a generated grid whose points are the mesh's voxel corners (every interior point on
+    # element faces: the grid chain's guesses decide the elements), 5 output times
+    "pipe_velout_grid": dict(gen=gen_pipe, outputs=["pvg.%d.bin" % k for k in range(5)] + ["pvg_Cartesian.bin"],
+                             deck=dict(PIPE, **VELOUT, Output_TStart=0.0, Output_TRes=5, Output_TDelta=0.05,
+                                       Int_TimeDirection=1, VelOut_GenerateMesh=1, VelOut_FilePrefix="pvg",
+                                       VelOut_MeshBounds_XRes=13, VelOut_MeshBounds_YRes=13,
+                                       VelOut_MeshBounds_ZRes=41)),
+    # velocity output at listed points (ASCII list), with tracers in the same run
+    "pipe_velout_list": dict(gen=gen_pipe_velout, outputs=["pvl.%d.bin" % k for k in range(4)] +
+                             ["pvl_coordinates.bin"] + ["pvlt.%d.bin" % k for k in range(4)],
+                             ic="pvlt.IC",
+                             deck=dict(PIPE, **VELOUT, Output_TStart=0.0, Output_TRes=4, Output_TDelta=0.03,
+                                       Int_TimeDirection=1, VelOut_GenerateMesh=0, VelOut_InFile="velpts.txt",
+                                       VelOut_InFileFormat=1, VelOut_FilePrefix="pvl", Trace_Compute=1,
+                                       Trace_GenerateMesh=0, Trace_InFile="seeds.bin", Trace_InFileFormat=4,
+                                       Trace_IntTLength=0.09, Trace_OutFilePrefix="pvlt")),
+    # 2-D Cartesian velocity output at listed points (some outside the data), backward output times
+    "dg_velout_list": dict(gen=gen_dg_velout, outputs=["dgv.%d.bin" % k for k in range(6)] + ["dgv_coordinates.bin"],
+                           deck=dict(DG, **VELOUT, Output_TStart=10.0, Output_TRes=6, Output_TDelta=1.5,
+                                     Int_TimeDirection=-1, VelOut_GenerateMesh=0, VelOut_InFile="velpts.txt",
+                                     VelOut_InFileFormat=1, VelOut_FilePrefix="dgv")),
     # 3-D unstructured FTLE with outside masks and early exits, 2 slides
     "pipe_ftle": dict(gen=gen_pipe, outputs=["pft.0.bin", "pft.1.bin", "pft_noT.0.bin", "pft_noT.1.bin",
                                              "pft_Cartesian.bin"],

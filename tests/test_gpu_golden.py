@@ -1,6 +1,7 @@
 """GPU driver (flowvc, FP64 mode, through the C ABI) against the reference's
-golden outputs: tracer positions and .IC element indices bit-identical; FTLE
-values within the eigen-solver rounding (rtol 1e-12, atol 1e-12)."""
+golden outputs: tracer positions (normal and staggered release), velocity
+output files and .IC element indices bit-identical; FTLE values within the
+eigen-solver rounding (rtol 1e-12, atol 1e-12)."""
 import os
 import subprocess
 
@@ -30,7 +31,8 @@ def run_driver(case, tmp_path, precision="f64"):
 
 
 @pytest.mark.parametrize("case", ["dg_tracers", "pipe_tracers", "pipe_locate", "pipe_staggered",
-                                  "pipe_staggered_steady"])
+                                  "pipe_staggered_steady", "pipe_velout_grid", "pipe_velout_list",
+                                  "dg_velout_list"])
 def test_gpu_trajectories_bit_identical(case, tmp_path):
     man, data, out = run_driver(case, tmp_path)
     fx = cases.fixture_dir(case)
