@@ -272,6 +272,7 @@ extern "C" int fvc_create(const fvc_mesh_desc *md, const fvc_params *pp, fvc_ctx
     c->slices.assign(pp->max_slices, nullptr);
     c->slice_scale.assign(pp->max_slices, 1.0f);
     c->slice_time.assign(pp->max_slices, 0.0);
+    c->sr_slices.assign(pp->max_slices, nullptr);
     memset(&c->mesh, 0, sizeof c->mesh);
     DevMesh &m = c->mesh;
     m.dims = md->dims;
@@ -313,6 +314,8 @@ extern "C" int fvc_create(const fvc_mesh_desc *md, const fvc_params *pp, fvc_ctx
             m.cres[2] = 1;
         }
         c->nvel = (int64_t)m.cres[0] * m.cres[1] * m.cres[2];
+        /* strain-rate cells (strainrate.c:86-91): kmax = 1 in 2-D or for a single z layer */
+        c->nsr = (int64_t)(m.cres[0] - 1) * (m.cres[1] - 1) * ((md->dims == 2 || m.cres[2] == 1) ? 1 : m.cres[2] - 1);
     } else {
         if (md->num_nodes <= 0 || md->num_elems <= 0 || !md->coords || !md->conn || !md->adj ||
             md->num_elems > INT32_MAX || md->num_nodes > INT32_MAX) {
@@ -371,6 +374,7 @@ extern "C" int fvc_create(const fvc_mesh_desc *md, const fvc_params *pp, fvc_ctx
             return rc;
         }
         c->nvel = md->num_nodes;
+        c->nsr = md->num_elems;
     }
     c->vel_stride = c->precision == FVC_PREC_F64 ? 32 : (c->precision == FVC_PREC_F32 ? 16 : 8);
     if (hipDeviceSynchronize() != hipSuccess) {
@@ -386,6 +390,8 @@ extern "C" void fvc_destroy(fvc_ctx *c) {
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     for (void *s : c->slices)
+        if (s) hipFree(s);
+    for (double *s : c->sr_slices)
         if (s) hipFree(s);
     hipFree(c->X64);
     hipFree(c->X32);
@@ -504,6 +510,33 @@ extern "C" int fvc_load_slice(fvc_ctx *c, int32_t slot, double timestamp, const 
         return set_err(c, FVC_E_HIP, "slice upload");
     c->slice_scale[slot] = scale;
     c->slice_time[slot] = timestamp;
+    return FVC_OK;
+}
+
+extern "C" int fvc_load_strain_slice(fvc_ctx *c, int32_t slot, double timestamp, const double *sr, int64_t n) {
+    (void)timestamp; /* read but unused, as the reference's (strainrate.c:112-115) */
+    if (!c || !sr || slot < 0 || slot >= c->max_slices) return FVC_E_INVALID;
+    if (n != c->nsr) return set_err(c, FVC_E_INVALID, "strain-rate slice length differs from the element (cell) count");
+    if (!c->sr_slices[slot]) {
+        if (hipMalloc((void **)&c->sr_slices[slot], sizeof(double) * (size_t)std::max<int64_t>(n, 1)) != hipSuccess)
+            return set_err(c, FVC_E_NOMEM, "strain-rate slice alloc");
+        c->bytes_used += (int64_t)(sizeof(double) * (size_t)n);
+    }
+    if (hipMemcpyAsync(c->sr_slices[slot], sr, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, c->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+        return set_err(c, FVC_E_HIP, "strain-rate slice upload");
+    return FVC_OK;
+}
+
+/* strain-rate slices lo, hi into the field when the activation potential is integrated */
+static int bind_strain(fvc_ctx *c, int lo, int hi, const fvc_int_params *ip, DevField &f, PtrSet &ps,
+                       fvc_particles *p) {
+    if (!ip->activation_potential) return FVC_OK;
+    if (!c->sr_slices[lo] || !c->sr_slices[hi]) return set_err(c, FVC_E_STATE, "strain-rate slot not loaded");
+    f.sr_lo = c->sr_slices[lo];
+    f.sr_hi = c->sr_slices[hi];
+    ps.scalar = p->scalar;
     return FVC_OK;
 }
 
@@ -666,7 +699,10 @@ extern "C" int fvc_advect(fvc_ctx *c, fvc_particles *p, int64_t begin, int64_t c
     if (!(ip->time_step > 1.0e-10)) return set_err(c, FVC_E_INVALID, "Int_TimeStep must be positive");
     int rc = check_slots(c, lo, hi, tmin, tmax);
     if (rc) return rc;
-    return launch_advect(c, ptrs(p), nullptr, begin, count, t1, t2, bind_field(c, lo, hi, tmin, tmax), *ip);
+    DevField f = bind_field(c, lo, hi, tmin, tmax);
+    PtrSet ps = ptrs(p);
+    if ((rc = bind_strain(c, lo, hi, ip, f, ps, p))) return rc;
+    return launch_advect(c, ps, nullptr, begin, count, t1, t2, f, *ip);
 }
 
 extern "C" int fvc_advect_times(fvc_ctx *c, fvc_particles *p, int64_t begin, int64_t count, const double *t12,
@@ -688,10 +724,13 @@ extern "C" int fvc_advect_times(fvc_ctx *c, fvc_particles *p, int64_t begin, int
         c->last_steps = 0;
         return FVC_OK;
     }
+    DevField f = bind_field(c, lo, hi, tmin, tmax);
+    PtrSet ps = ptrs(p);
+    if ((rc = bind_strain(c, lo, hi, ip, f, ps, p))) return rc;
     double *d = nullptr;
     FVC_HIP(hipMallocAsync((void **)&d, sizeof(double) * 2 * count, c->stream));
     FVC_HIP(hipMemcpyAsync(d, t12, sizeof(double) * 2 * count, hipMemcpyHostToDevice, c->stream));
-    rc = launch_advect_times(c, ptrs(p), begin, count, d, bind_field(c, lo, hi, tmin, tmax), *ip);
+    rc = launch_advect_times(c, ps, begin, count, d, f, *ip);
     (void)hipFreeAsync(d, c->stream);
     return rc;
 }

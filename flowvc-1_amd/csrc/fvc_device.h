@@ -62,6 +62,8 @@ struct DevField {
     const void *lo, *hi;
     float scale_lo, scale_hi;
     double tmin, tmax;
+    /* strain-rate slices at tmin, tmax when the activation potential is integrated, else null */
+    const double *sr_lo = nullptr, *sr_hi = nullptr;
 };
 
 struct ErrBuf {
@@ -472,6 +474,7 @@ struct PState {
     /* multistep history: velocities at the two previous nodes, count held */
     double F1[3], F2[3];
     int nhist;
+    double scalar; /* activation potential */
 };
 
 __device__ __forceinline__ double sign_of(double a, double b) { return b >= 0.0 ? fabs(a) : -fabs(a); }
@@ -560,6 +563,48 @@ struct IState {
 
 enum { INT_EULER = 0, INT_RK4 = 1, INT_AB2 = 3, INT_AB3 = 4 };
 
+/* Strain rate at the point (GetStrainRate_Unstructured, ompVC/strainrate.c:
+ * 232-246; GetStrainRate_Cartesian, :196-230), linear in time between the two
+ * loaded strain-rate slices.  Unstructured: the reference first walks from
+ * the point's element (search acceptance, d > -TINY); the element held here
+ * was located by that same acceptance at this same position (the step's
+ * final locate, the seed chain or a launch walk that keeps it), so that walk
+ * returns it unchanged and is not repeated. */
+template <int DIM, int UNSTRUCT>
+__device__ inline double strain_rate(const DevMesh &m, const DevField &f, double t, const PState &p, int &err) {
+#pragma clang fp contract(off)
+    const double tloc = (t - f.tmin) / (f.tmax - f.tmin);
+    if ((tloc > 1 + TINY) || (tloc < 0 - TINY)) {
+        err = E_TLOC;
+        return 0.0;
+    }
+    int64_t idx;
+    if (UNSTRUCT) {
+        if (p.elem < 0 || p.elem >= m.ne) {
+            err = E_ELEM;
+            return 0.0;
+        }
+        idx = p.elem;
+    } else {
+        if (outside_domain(m, p.X[0], p.X[1], p.X[2])) {
+            err = E_LEFT;
+            return 0.0;
+        }
+        int cell[3] = {0, 0, 0};
+        for (int a = 0; a < DIM; a++) {
+            cell[a] = (int)floor((p.X[a] - m.cmin[a]) / m.cdel[a]);
+            if (cell[a] == m.cres[a] - 1) cell[a] = m.cres[a] - 2;
+            if (cell[a] < 0 || cell[a] > m.cres[a] - 2) { /* the reference asserts here */
+                err = E_ELEM;
+                return 0.0;
+            }
+        }
+        const int64_t imax = m.cres[0] - 1, jmax = m.cres[1] - 1;
+        idx = ((int64_t)cell[2] * jmax + cell[1]) * imax + cell[0];
+    }
+    return (1 - tloc) * f.sr_lo[idx] + tloc * f.sr_hi[idx];
+}
+
 /* Returns 1 when the interval is finished (end reached, point exited or an
  * error), with *stop the stop time; 0 when max_steps were taken first.
  *   RK4   ompVC/integration.c:317-462
@@ -607,6 +652,10 @@ __device__ int interval_run(const DevMesh &m, const DevField &f, PState &p, ISta
      * d == -TINY, which voids the reuse), so the first stage takes the
      * locate's element and natural coordinates without walking again. */
     int c_elem = -1;
+    /* activation potential (integration.c:355-360, 449-455): Scalar gains
+     * h (S0 + S1) / 2 per step that stays in the domain, S0 and S1 the strain
+     * rate at the step's start and end; only the generic integrator build */
+    const bool ap = GEN && f.sr_lo != nullptr;
     using CT = typename VelT<MODE>::T; /* reduced-precision interpolation reads (float) r, s, t */
     CT c_r = 0, c_s = 0, c_t = 0;
     while ((is.tc - b) * (b - a) < 0.0) {
@@ -614,7 +663,11 @@ __device__ int interval_run(const DevMesh &m, const DevField &f, PState &p, ISta
         if ((is.tc + is.h - b) * (is.tc + is.h - a) > 0.0) is.h = sign_of(b - is.tc, b - a);
         const double h = is.h, tc = is.tc;
         const bool micro = fabs(h) <= hmin;
-        double inc[3], f0[3];
+        double inc[3], f0[3], sr0 = 0.0;
+        if (ap) {
+            sr0 = strain_rate<DIM, UNSTRUCT>(m, f, tc, p, err);
+            if (err) break;
+        }
         bool ab_step = false;
         if (itype == INT_EULER) {
             double ve[3];
@@ -691,6 +744,11 @@ __device__ int interval_run(const DevMesh &m, const DevField &f, PState &p, ISta
             for (int c = 0; c < 3; c++) p.X[c] = p.X[c] - inc[c];
             stop = p.left_time;
             return 1;
+        }
+        if (ap) {
+            const double sr1 = strain_rate<DIM, UNSTRUCT>(m, f, is.tc, p, err);
+            if (err) break;
+            p.scalar += (h * ((sr0 + sr1) / 2));
         }
         if (itype == INT_AB2 || itype == INT_AB3) {
             if (!micro) {

@@ -30,6 +30,9 @@ struct fvc_ctx {
     std::vector<double> slice_time;
     int64_t nvel;       /* nodes per slice */
     size_t vel_stride;  /* bytes per node */
+    /* resident strain-rate slices (activation potential), FP64, one per slot */
+    std::vector<double *> sr_slices;
+    int64_t nsr; /* values per strain-rate slice: elements, or Cartesian cells */
     /* errors and counters */
     int *d_err;
     int64_t *d_where;
@@ -98,6 +101,7 @@ struct PtrSet {
     double *left_time;
     double *H[6];
     int *nhist;
+    double *scalar = nullptr; /* LagrangianPoint.Scalar (activation potential) */
 };
 
 inline PtrSet ptrs(fvc_particles *p) {
@@ -111,7 +115,7 @@ inline PtrSet ptrs(fvc_particles *p) {
     s.left = p->left;
     s.left_time = p->left_time;
     s.nhist = p->nhist;
-    return s;
+    return s; /* scalar stays null: only activation-potential runs carry it */
 }
 
 /* one FTLE grid's device arrays (grid index P = (i*res1 + j)*res2 + k) */

@@ -65,6 +65,7 @@ __device__ inline void load_state(const PtrSet &p, int64_t i, PState &s, bool hi
     s.left = p.left[i];
     s.left_time = p.left_time[i];
     s.nhist = 0;
+    s.scalar = p.scalar ? p.scalar[i] : 0.0;
     if (hist) {
         for (int c = 0; c < 3; c++) {
             s.F1[c] = p.H[c][i];
@@ -82,6 +83,7 @@ __device__ inline void store_state(const PtrSet &p, int64_t i, const PState &s, 
     p.elem[i] = s.elem;
     p.left[i] = s.left;
     p.left_time[i] = s.left_time;
+    if (p.scalar) p.scalar[i] = s.scalar;
     if (hist) {
         for (int c = 0; c < 3; c++) {
             p.H[c][i] = s.F1[c];
@@ -246,7 +248,7 @@ int launch_advect_times(fvc_ctx *c, PtrSet p, int64_t begin, int64_t count, cons
                         const fvc_int_params &ip) {
     FVC_HIP(hipMemsetAsync(c->d_steps, 0, sizeof(unsigned long long), c->stream));
     if (count > 0) {
-        if (ip.int_type == FVC_INT_RK4)
+        if (ip.int_type == FVC_INT_RK4 && !f.sr_lo)
             FVC_DISPATCH(c->precision, c->mesh.dims, c->mesh.mesh_type, k_advect_times_rk4,
                          <<<grid_for(count, BLOCK), BLOCK, 0, c->stream>>>(c->mesh, f, p, begin, count, dtt,
                                                                            ip.int_type, ip.time_step, ip.extrapolate,
@@ -285,7 +287,7 @@ int launch_advect(fvc_ctx *c, PtrSet p, const int *index, int64_t begin, int64_t
         int block = ip.block > 0 ? ip.block : BLOCK;
         if (block != BLOCK) block = BLOCK; /* kernels are compiled for 256-lane blocks */
         timing_begin(c);
-        if (ip.int_type == FVC_INT_RK4)
+        if (ip.int_type == FVC_INT_RK4 && !f.sr_lo) /* the activation potential is in the generic build only */
             FVC_DISPATCH(c->precision, c->mesh.dims, c->mesh.mesh_type, k_advect,
                          <<<grid_for(count, block), block, 0, c->stream>>>(c->mesh, f, p, index, begin, count, t1, t2,
                                                                            ip.int_type, ip.time_step, ip.extrapolate,

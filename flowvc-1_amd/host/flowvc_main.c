@@ -12,8 +12,10 @@
  * per frame, velocity.c:61-234).  --nproc emulates the host processor count
  * that selects ompVC's tracer output loop (parameters.c:411-421).
  * Staggered tracer release follows tracers.c:293-1000 and velocity output
- * velout.c:21-435.  Modes outside the advection path (inertial particles,
- * RKF, exposure/residence time, strain rate, normal flow) stop with an error.
+ * velout.c:21-435; the activation potential (Trace_APCompute) integrates
+ * the strain-rate data along tracer paths (strainrate.c, integration.c).
+ * Modes outside the advection path (inertial particles, RKF, exposure and
+ * residence time, normal flow) stop with an error.
  */
 #include <math.h>
 #include <stdio.h>
@@ -56,6 +58,8 @@ typedef struct {
     double *buf;
     int next_victim;
     double io_s;
+    int64_t nsr;   /* strain-rate values per frame when the activation potential is integrated, else 0 */
+    double *srbuf;
 } slices_t;
 
 static int file_of(const fvc_deck *d, int frame) {
@@ -86,6 +90,17 @@ static int slot_for(slices_t *s, int frame, int keep) {
     double t0 = now_s(), ts;
     if (fvc_read_vel_slice(path, &ts, s->buf, s->n)) fatal("Could not load velocity data from %s", path);
     check(fvc_load_slice(s->c, q, ts, s->buf, s->n), s->c, "fvc_load_slice");
+    if (s->nsr) { /* LoadUnstructStrainRateDataFrame / LoadCartStrainRateDataFrame (strainrate.c:53-173) */
+        snprintf(path, sizeof path, "%s%s_strain-rate.%d.bin", s->d->Path_Data, s->d->Data_InFilePrefix,
+                 s->d->Data_SuffixTMin + f * s->d->Data_SuffixTDelta);
+        FILE *fs = fopen(path, "rb");
+        double st;
+        if (!fs) fatal("Could not open file %s", path);
+        if (fread(&st, 8, 1, fs) != 1) fatal("Could not read time stamp from file %s", path);
+        if (fread(s->srbuf, 8, (size_t)s->nsr, fs) != (size_t)s->nsr) fatal("Could not read strain rate data from %s", path);
+        fclose(fs);
+        check(fvc_load_strain_slice(s->c, q, st, s->srbuf, s->nsr), s->c, "fvc_load_strain_slice");
+    }
     s->io_s += now_s() - t0;
     s->slot_file[q] = f;
     return q;
@@ -711,9 +726,12 @@ int main(int argc, char **argv) {
         fvc_deck_derive(&d, err, sizeof err))
         fatal("%s", err);
     if (d.legacy_96 && !quiet) fprintf(stderr, "Note: deck has 96 entries (no Num_threads); read as Num_threads = 0\n");
-    if (d.Particle_Radius > TINY || d.Int_Type == 2 || d.Trace_CETCompute || d.Trace_RTCompute || d.Trace_APCompute ||
-        d.Int_NormalFlow)
-        fatal("%s", "deck selects a mode outside the advection path (inertial, RKF, CET, RT, AP, normal flow)");
+    if (d.Particle_Radius > TINY || d.Int_Type == 2 || d.Trace_CETCompute || d.Trace_RTCompute || d.Int_NormalFlow)
+        fatal("%s", "deck selects a mode outside the advection path (inertial, RKF, CET, RT, normal flow)");
+    const int ap = d.Trace_Compute && d.Trace_APCompute;
+    if (ap && d.Trace_ReleaseStrategy == 1)
+        fatal("%s", "Trace_APCompute with the staggered release is not supported (the reference's OutputAP reads "
+                    "release files that the staggered release never writes)");
     const int dir = d.Int_TimeDirection;
 
     /* mesh (mesh.c:21-213) */
@@ -757,7 +775,13 @@ int main(int argc, char **argv) {
     sl.slot_file = (int *)malloc(sizeof(int) * sl.nslots);
     for (int q = 0; q < sl.nslots; q++) sl.slot_file[q] = -1;
     sl.buf = (double *)malloc(sizeof(double) * 3 * (size_t)sl.n);
-    fvc_int_params ip = {d.Int_Type == 1 ? FVC_INT_RK4 : FVC_INT_EULER, d.Int_Extrapolate, d.Int_TimeStep, 0, 0};
+    if (ap) {
+        sl.nsr = d.Data_MeshType == 1 ? md.num_elems
+                                      : (int64_t)(md.cart_res[0] - 1) * (md.cart_res[1] - 1) *
+                                            ((d.Dimensions == 2 || md.cart_res[2] == 1) ? 1 : md.cart_res[2] - 1);
+        sl.srbuf = (double *)malloc(sizeof(double) * (size_t)(sl.nsr > 0 ? sl.nsr : 1));
+    }
+    fvc_int_params ip = {d.Int_Type == 1 ? FVC_INT_RK4 : FVC_INT_EULER, d.Int_Extrapolate, d.Int_TimeStep, 0, 0, ap};
     double t_compute = 0.0;
     int64_t steps_total = 0;
 
@@ -908,6 +932,7 @@ int main(int argc, char **argv) {
                                         fwrite(&tt, 8, 1, of[outframe]);
                                     }
                                     fwrite(tmp[q].X, 8, 3, of[outframe]);
+                                    if (ap) fwrite(&tmp[q].Scalar, 8, 1, of[outframe]); /* tracers.c:136-138 */
                                     cnt[outframe]++;
                                 }
                             }
@@ -934,6 +959,33 @@ int main(int argc, char **argv) {
             }
             if (of[i]) fclose(of[i]);
         }
+        if (ap) { /* OutputAP (strainrate.c:248-395): mesh description, then each release's Scalar */
+            if (d.Trace_GenerateMesh) {
+                snprintf(path, sizeof path, "%s%s_Cartesian.bin", d.Path_Output, d.Trace_OutFilePrefix);
+                double cmn[3] = {d.Trace_Cart.XMin, d.Trace_Cart.YMin, d.Trace_Cart.ZMin};
+                double cmx[3] = {d.Trace_Cart.XMax, d.Trace_Cart.YMax, d.Trace_Cart.ZMax};
+                int32_t cres[3] = {d.Trace_Cart.XRes, d.Trace_Cart.YRes, d.Trace_Cart.ZRes};
+                if (fvc_write_cart_header(path, cmn, cmx, cres)) fatal("Could not open file %s", path);
+            } else {
+                snprintf(path, sizeof path, "%s%s_coordinates.bin", d.Path_Output, d.Trace_OutFilePrefix);
+                FILE *o = fopen(path, "wb");
+                if (!o) fatal("Could not open file %s", path);
+                int32_t n32 = (int32_t)n;
+                fwrite(&n32, 4, 1, o);
+                for (int64_t q = 0; q < n; q++) fwrite(ic[q].X, 8, 3, o);
+                fclose(o);
+            }
+            for (int ss = 0; ss < nl; ss++) {
+                check(fvc_particles_get(P, (int64_t)ss * n, tmp, n), c, "fvc_particles_get");
+                snprintf(path, sizeof path, "%s%s_AP.%d.bin", d.Path_Output, d.Trace_OutFilePrefix, ss);
+                FILE *o = fopen(path, "wb");
+                if (!o) fatal("Could not open file %s", path);
+                double tt = d.Output_TStart + ss * d.Trace_LaunchTimeSpacing;
+                fwrite(&tt, 8, 1, o);
+                for (int64_t q = 0; q < n; q++) fwrite(&tmp[q].Scalar, 8, 1, o);
+                fclose(o);
+            }
+        }
         free(tmp);
         free(cnt);
         free(of);
@@ -949,6 +1001,7 @@ int main(int argc, char **argv) {
     fvc_destroy(c);
     free(sl.slot_file);
     free(sl.buf);
+    free(sl.srbuf);
     free(xyz);
     free(conn);
     free(adj);

@@ -98,6 +98,10 @@ typedef struct fvc_int_params {
     double time_step;    /* Int_TimeStep */
     int32_t block;       /* threads per block; 0 = default */
     int32_t steps_per_launch; /* 0 = whole interval in one launch */
+    /* Trace_APCompute: integrate the strain rate of the loaded strain-rate
+     * slices into each point's Scalar (ompVC/integration.c:355-360, 449-455);
+     * fvc_advect and fvc_advect_times only */
+    int32_t activation_potential;
 } fvc_int_params;
 
 typedef struct fvc_ctx fvc_ctx;
@@ -122,6 +126,12 @@ int fvc_memory(fvc_ctx *ctx, int64_t *used, int64_t *free_bytes);
  * (ompVC/velocity.c:61-234): callers name the two slots to interpolate between.
  * BF16/F16 precision stores the slice scaled by its max-abs value. */
 int fvc_load_slice(fvc_ctx *ctx, int32_t slot, double timestamp, const double *uvw, int64_t n);
+/* Strain-rate data of a frame for the activation potential
+ * (LoadUnstructStrainRateDataFrame / LoadCartStrainRateDataFrame, ompVC/
+ * strainrate.c:53-173): one value per element, or per Cartesian cell in k,j,i
+ * order (i fastest; (XRes-1)(YRes-1)(ZRes-1 or 1) values), kept in FP64 in
+ * `slot` beside that slot's velocity. */
+int fvc_load_strain_slice(fvc_ctx *ctx, int32_t slot, double timestamp, const double *sr, int64_t n);
 
 /* ---- particles (device-resident LagrangianPoint state) ------------------- */
 int fvc_particles_create(fvc_ctx *ctx, int64_t capacity, fvc_particles **out);

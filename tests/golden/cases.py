@@ -86,6 +86,30 @@ def gen_pipe_steady(d):
     host.write_binary_list(os.path.join(d, "release.bin"), release_line())
 
 
+def strain_pipe(ne, s):
+    """Per-element strain rate of frame s (any field serves the integration's parity)."""
+    e = np.arange(ne, dtype=np.float64)
+    return 0.5 + 0.4 * np.sin(0.001 * e + 0.3 * s)
+
+
+def gen_pipe_ap(d):
+    gen_pipe(d)
+    ne = int(np.fromfile(os.path.join(d, "pipe_connectivity.bin"), np.int32, count=1)[0])
+    for s in range(50):
+        with open(os.path.join(d, "pipe_strain-rate.%d.bin" % s), "wb") as f:
+            np.array([s * 0.02]).tofile(f)
+            strain_pipe(ne, s).tofile(f)
+
+
+def gen_dg_ap(d):
+    gen_dg(d)
+    i = np.arange(63 * 31, dtype=np.float64)
+    for s in range(21):
+        with open(os.path.join(d, "dg_strain-rate.%d.bin" % s), "wb") as f:
+            np.array([s * 0.5]).tofile(f)
+            (1.0 + 0.5 * np.cos(0.05 * i + 0.3 * s)).tofile(f)
+
+
 def velout_points():
     """Listed velocity output points: interior, on voxel faces and vertices,
     and outside the mesh."""
@@ -169,6 +193,33 @@ CASES = {
     # staggered release in steady flow (linear release-time solves)
     "pipe_staggered_steady": dict(gen=gen_pipe_steady, outputs=["pss.%d.bin" % k for k in range(9)],
                                   deck=dict(STAGGER, Trace_OutFilePrefix="pss")),
+    # activation potential (strain rate integrated along the paths): unstructured list seeds, one release
+    "pipe_tracers_ap": dict(gen=gen_pipe_ap, outputs=["pap.%d.bin" % k for k in range(6)] +
+                            ["pap_AP.0.bin", "pap_coordinates.bin"], ic="pap.IC",
+                            deck=dict(PIPE, Output_TStart=0.0, Output_TRes=6, Output_TDelta=0.04, Int_TimeDirection=1,
+                                      Trace_Compute=1, Trace_GenerateMesh=0, Trace_InFile="seeds.bin",
+                                      Trace_InFileFormat=4, Trace_IntTLength=0.15, Trace_APCompute=1,
+                                      Trace_OutFilePrefix="pap")),
+    # two releases: the reference's OutputAP reads each release file into a fresh allocation it then drops
+    # (tracers.c:1649-1656, strainrate.c:383-384) and writes the stale array, so its _AP.1 repeats release 0;
+    # the records with their Scalar in the position files are compared, and _AP.1 is checked against them
+    "pipe_tracers_ap2": dict(gen=gen_pipe_ap, outputs=["pap2.%d.bin" % k for k in range(6)] +
+                             ["pap2_AP.0.bin", "pap2_AP.1.bin", "pap2_coordinates.bin"], ic="pap2.IC",
+                             skip=["pap2_AP.1.bin"],
+                             deck=dict(PIPE, Output_TStart=0.0, Output_TRes=6, Output_TDelta=0.04,
+                                       Int_TimeDirection=1, Trace_Compute=1, Trace_GenerateMesh=0,
+                                       Trace_InFile="seeds.bin", Trace_InFileFormat=4, Trace_IntTLength=0.15,
+                                       Trace_NumLaunchTimes=2, Trace_LaunchTimeSpacing=0.04, Trace_APCompute=1,
+                                       Trace_OutFilePrefix="pap2")),
+    # activation potential on the 2-D Cartesian grid (per-cell strain rate), generated seeds (2048 records: the
+    # reference leaves LeftDomainTime of generated seeds unset, see dg_tracers)
+    "dg_tracers_ap": dict(gen=gen_dg_ap, outputs=["dgap.%d.bin" % k for k in range(6)] +
+                          ["dgap_AP.0.bin", "dgap_Cartesian.bin"],
+                          deck=dict(DG, Output_TStart=0.0, Output_TRes=6, Output_TDelta=2.0, Int_TimeDirection=1,
+                                    Trace_Compute=1, Trace_GenerateMesh=1, Trace_MeshBounds_XMin=0.0,
+                                    Trace_MeshBounds_XMax=2.0, Trace_MeshBounds_YMin=0.0, Trace_MeshBounds_YMax=1.0,
+                                    Trace_MeshBounds_XRes=64, Trace_MeshBounds_YRes=32, Trace_MeshBounds_ZRes=1,
+                                    Trace_IntTLength=10.0, Trace_APCompute=1, Trace_OutFilePrefix="dgap")),
     # velocity output on a generated grid whose points are the mesh's voxel corners (every interior point on
     # element faces: the grid chain's guesses decide the elements), 5 output times
     "pipe_velout_grid": dict(gen=gen_pipe, outputs=["pvg.%d.bin" % k for k in range(5)] + ["pvg_Cartesian.bin"],
