@@ -94,6 +94,7 @@ extern "C" const char *fvc_strerror(int code) {
     case FVC_E_STATE: return "call out of sequence";
     case FVC_E_LEFT: return "Attempting to integrate point that already left domain";
     case FVC_E_NODEVICE: return "no usable GPU device";
+    case FVC_E_CET_OUTSIDE: return "Point X0 passed to ComputeExposureTime appears to be outside of velocity domain";
     default: return "unknown error";
     }
 }
@@ -393,6 +394,8 @@ extern "C" void fvc_destroy(fvc_ctx *c) {
         if (s) hipFree(s);
     for (double *s : c->sr_slices)
         if (s) hipFree(s);
+    if (c->cet_lo) hipFree(c->cet_lo);
+    if (c->cet_enc) hipFree(c->cet_enc);
     hipFree(c->X64);
     hipFree(c->X32);
     hipFree(c->conn);
@@ -529,9 +532,59 @@ extern "C" int fvc_load_strain_slice(fvc_ctx *c, int32_t slot, double timestamp,
     return FVC_OK;
 }
 
-/* strain-rate slices lo, hi into the field when the activation potential is integrated */
+extern "C" int fvc_cet_reset(fvc_ctx *c) {
+    if (!c) return FVC_E_INVALID;
+    if (c->mesh.mesh_type != 1) return set_err(c, FVC_E_INVALID, "exposure times need an unstructured mesh");
+    const size_t ne = (size_t)c->mesh.ne;
+    if (!c->cet_lo) {
+        if (hipMalloc((void **)&c->cet_lo, 2 * sizeof(unsigned long long) * ne) != hipSuccess ||
+            hipMalloc((void **)&c->cet_enc, sizeof(int) * ne) != hipSuccess)
+            return set_err(c, FVC_E_NOMEM, "exposure-time arrays");
+        c->cet_hi = c->cet_lo + ne;
+        c->bytes_used += (int64_t)(20 * ne);
+    }
+    FVC_HIP(hipMemsetAsync(c->cet_lo, 0, 2 * sizeof(unsigned long long) * ne, c->stream));
+    FVC_HIP(hipMemsetAsync(c->cet_enc, 0, sizeof(int) * ne, c->stream));
+    FVC_HIP(hipStreamSynchronize(c->stream));
+    return FVC_OK;
+}
+
+extern "C" int fvc_cet_launch(fvc_ctx *c, fvc_particles *p, int64_t begin, int64_t count) {
+    if (!c || !p || begin < 0 || count < 0 || begin + count > p->cap) return FVC_E_INVALID;
+    if (!c->cet_lo) return set_err(c, FVC_E_STATE, "fvc_cet_reset not called");
+    int rc = launch_cet_encounters(c, ptrs(p), begin, count, c->cet_enc);
+    return rc ? rc : check_err(c, "fvc_cet_launch");
+}
+
+extern "C" int fvc_cet_get(fvc_ctx *c, double *cetsum, int32_t *encounters) {
+    if (!c) return FVC_E_INVALID;
+    if (!c->cet_lo) return set_err(c, FVC_E_STATE, "fvc_cet_reset not called");
+    const size_t ne = (size_t)c->mesh.ne;
+    if (cetsum) {
+        std::vector<unsigned long long> w(2 * ne);
+        FVC_HIP(hipMemcpy(w.data(), c->cet_lo, sizeof(unsigned long long) * 2 * ne, hipMemcpyDeviceToHost));
+        for (size_t e = 0; e < ne; e++) { /* exact 128-bit sum, rounded once */
+            const unsigned __int128 v = ((unsigned __int128)w[ne + e] << 64) | w[e];
+            cetsum[e] = ldexp((double)v, -CET_FRAC);
+        }
+    }
+    if (encounters) FVC_HIP(hipMemcpy(encounters, c->cet_enc, sizeof(int) * ne, hipMemcpyDeviceToHost));
+    return FVC_OK;
+}
+
+/* strain-rate slices lo, hi (activation potential) and the exposure-time
+ * accumulators into the field when requested */
 static int bind_strain(fvc_ctx *c, int lo, int hi, const fvc_int_params *ip, DevField &f, PtrSet &ps,
                        fvc_particles *p) {
+    if (ip->exposure_substeps) {
+        if (ip->exposure_substeps < 2) return set_err(c, FVC_E_INVALID, "Trace_CETSubsteps must be greater than 1");
+        if (c->mesh.mesh_type != 1) return set_err(c, FVC_E_INVALID, "exposure times need an unstructured mesh");
+        if (!c->cet_lo) return set_err(c, FVC_E_STATE, "fvc_cet_reset not called");
+        f.cet_lo = c->cet_lo;
+        f.cet_hi = c->cet_hi;
+        f.cet_enc = c->cet_enc;
+        f.cet_sub = ip->exposure_substeps;
+    }
     if (!ip->activation_potential) return FVC_OK;
     if (!c->sr_slices[lo] || !c->sr_slices[hi]) return set_err(c, FVC_E_STATE, "strain-rate slot not loaded");
     f.sr_lo = c->sr_slices[lo];

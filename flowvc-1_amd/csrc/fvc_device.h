@@ -33,7 +33,7 @@ enum Mode { P_F64 = 0, P_F32 = 1, P_BF16 = 2, P_F16 = 3 };
 enum Accept { ACC_SEARCH = 0, ACC_INTERP = 1 }; /* d > -TINY (mesh.c:549) / d+TINY >= 0 (velocity.c:545) */
 
 /* device error codes (match include/fvc.h) */
-enum { E_OK = 0, E_INDET = -4, E_TLOC = -5, E_ELEM = -6, E_WALK = -7, E_EIG = -8, E_LEFT = -11 };
+enum { E_OK = 0, E_INDET = -4, E_TLOC = -5, E_ELEM = -6, E_WALK = -7, E_EIG = -8, E_LEFT = -11, E_CET = -13 };
 
 struct DevMesh {
     int dims, mesh_type;
@@ -64,6 +64,10 @@ struct DevField {
     double tmin, tmax;
     /* strain-rate slices at tmin, tmax when the activation potential is integrated, else null */
     const double *sr_lo = nullptr, *sr_hi = nullptr;
+    /* exposure time accumulators (per element: 128-bit fixed-point time, encounters), else null */
+    unsigned long long *cet_lo = nullptr, *cet_hi = nullptr;
+    int *cet_enc = nullptr;
+    int cet_sub = 0; /* Trace_CETSubsteps */
 };
 
 struct ErrBuf {
@@ -242,6 +246,74 @@ __device__ __forceinline__ int walk(const DevMesh &m, double x, double y, double
     }
     err = E_WALK;
     return -2;
+}
+
+/* ------------------------------------------------------------------------ */
+/* searches (ompVC/mesh.c:304-398, 496-611)                                  */
+/* ------------------------------------------------------------------------ */
+/* lowest-index element with d > -TINY (mesh.c:321-363) and whether a second
+ * element also passes; cell lists are ascending, so the first pass is the
+ * lowest index. */
+template <int DIM>
+__device__ int hash_global(const DevMesh &m, double x, double y, double z, int &second) {
+    second = -1;
+    double fx = (x - m.hlo[0]) * m.hinv[0], fy = (y - m.hlo[1]) * m.hinv[1];
+    double fz = DIM == 3 ? (z - m.hlo[2]) * m.hinv[2] : 0.0;
+    if (!(fx >= 0.0 && fy >= 0.0 && fz >= 0.0)) return -1;
+    if (!(fx < (double)m.hres[0] && fy < (double)m.hres[1] && fz < (double)m.hres[2])) return -1;
+    int ix = (int)fx, iy = (int)fy, iz = (int)fz;
+    int64_t cell = ((int64_t)iz * m.hres[1] + iy) * m.hres[0] + ix;
+    int first = -1;
+    for (int q = m.cell_start[cell]; q < m.cell_start[cell + 1]; q++) {
+        int e = m.cell_elems[q];
+        double r, s, t = 0.0;
+        if (DIM == 3) nat3_exact(m, e, x, y, z, r, s, t);
+        else nat2_exact(m, e, x, y, r, s);
+        int slot;
+        if (decide_exact<DIM>(r, s, t, ACC_SEARCH, slot) == 1) {
+            if (first < 0) first = e;
+            else {
+                second = e;
+                break;
+            }
+        }
+    }
+    return first;
+}
+
+/* every element with d > -TINY, ascending: the first `cap` are written to
+ * `list` (when given), the lowest is returned in `first`, the count is
+ * returned whole */
+template <int DIM>
+__device__ int hash_accepting(const DevMesh &m, double x, double y, double z, int *list, int cap, int &first) {
+    first = -1;
+    double fx = (x - m.hlo[0]) * m.hinv[0], fy = (y - m.hlo[1]) * m.hinv[1];
+    double fz = DIM == 3 ? (z - m.hlo[2]) * m.hinv[2] : 0.0;
+    if (!(fx >= 0.0 && fy >= 0.0 && fz >= 0.0)) return 0;
+    if (!(fx < (double)m.hres[0] && fy < (double)m.hres[1] && fz < (double)m.hres[2])) return 0;
+    int ix = (int)fx, iy = (int)fy, iz = (int)fz;
+    int64_t cell = ((int64_t)iz * m.hres[1] + iy) * m.hres[0] + ix;
+    int cnt = 0;
+    for (int q = m.cell_start[cell]; q < m.cell_start[cell + 1]; q++) {
+        int e = m.cell_elems[q];
+        double r, s, t = 0.0;
+        if (DIM == 3) nat3_exact(m, e, x, y, z, r, s, t);
+        else nat2_exact(m, e, x, y, r, s);
+        int slot;
+        if (decide_exact<DIM>(r, s, t, ACC_SEARCH, slot) == 1) {
+            if (cnt == 0) first = e;
+            if (list && cnt < cap) list[cnt] = e;
+            cnt++;
+        }
+    }
+    return cnt;
+}
+
+template <int DIM>
+__device__ int local_search(const DevMesh &m, double x, double y, double z, int guess, int &err) {
+    double r, s, t;
+    int e = walk<P_F64, DIM>(m, x, y, z, guess, ACC_SEARCH, r, s, t, err);
+    return e == -2 ? -1 : e;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -570,6 +642,77 @@ enum { INT_EULER = 0, INT_RK4 = 1, INT_AB2 = 3, INT_AB3 = 4 };
  * was located by that same acceptance at this same position (the step's
  * final locate, the seed chain or a launch walk that keeps it), so that walk
  * returns it unchanged and is not repeated. */
+/* Exposure-time sums are kept exactly: a time t >= 0 adds floor(t 2^CET_FRAC)
+ * to a 128-bit fixed-point accumulator (two 64-bit words, the carry of the low
+ * word added to the high one), so the per-element totals do not depend on the
+ * order in which the parallel steps arrive; the reference adds in its
+ * threads' arrival order (exposuretime.c:139-176). */
+constexpr int CET_FRAC = 100;
+
+__device__ inline void cet_add(unsigned long long *lo, unsigned long long *hi, double v) {
+    if (!(v > 0.0)) return;
+    int e;
+    const double fr = frexp(v, &e);                            /* v = fr 2^e, fr in [0.5, 1) */
+    const unsigned long long mnt = (unsigned long long)ldexp(fr, 53); /* v = mnt 2^(e-53) */
+    const int sh = e - 53 + CET_FRAC;
+    unsigned long long l = 0, h = 0;
+    if (sh <= -64) return;
+    if (sh < 0) l = mnt >> (-sh);
+    else if (sh < 64) {
+        l = mnt << sh;
+        h = sh ? (mnt >> (64 - sh)) : 0ull;
+    } else if (sh < 128) {
+        h = mnt << (sh - 64);
+    }
+    const unsigned long long old = atomicAdd(lo, l);
+    const unsigned long long carry = (old + l < old) ? 1ull : 0ull;
+    if (h + carry) atomicAdd(hi, h + carry);
+}
+
+/* ComputeExposureTime (exposuretime.c:120-178) for a step from X0 to X1 of
+ * length h, X1 in element e1: the whole step to the element of X0 if that is
+ * e1, else h/(S-1) to the element of each of the S-1 points X0 + a (X1 - X0),
+ * a = s/(S-1), s = 1..S-1, walking from point to point, with an encounter
+ * where the element changes; points in no element are skipped. */
+template <int DIM>
+__device__ inline void cet_step(const DevMesh &m, const DevField &f, const double X0[3], const double X1[3], int e1,
+                                double h, int &err) {
+#pragma clang fp contract(off)
+    int e0 = local_search<DIM>(m, X0[0], X0[1], X0[2], e1, err);
+    if (err) return;
+    if (e0 < 0) {
+        int second;
+        e0 = hash_global<DIM>(m, X0[0], X0[1], X0[2], second);
+        if (e0 < 0) {
+            err = E_CET;
+            return;
+        }
+    }
+    if (e0 == e1) {
+        cet_add(f.cet_lo + e0, f.cet_hi + e0, h);
+        return;
+    }
+    const int S = f.cet_sub;
+    int last = e0;
+    for (int s = 1; s < S; s++) {
+        const double a = ((double)s) / ((double)(S - 1));
+        double Xt[3];
+        for (int i = 0; i < 3; i++) Xt[i] = (1 - a) * X0[i] + a * X1[i];
+        int et = local_search<DIM>(m, Xt[0], Xt[1], Xt[2], last, err);
+        if (err) return;
+        if (et < 0) {
+            int second;
+            et = hash_global<DIM>(m, Xt[0], Xt[1], Xt[2], second);
+            if (et < 0) continue; /* path crosses the outside: no contribution (exposuretime.c:160-165) */
+        }
+        cet_add(f.cet_lo + et, f.cet_hi + et, h / (S - 1));
+        if (et != last) {
+            atomicAdd(f.cet_enc + et, 1);
+            last = et;
+        }
+    }
+}
+
 template <int DIM, int UNSTRUCT>
 __device__ inline double strain_rate(const DevMesh &m, const DevField &f, double t, const PState &p, int &err) {
 #pragma clang fp contract(off)
@@ -656,6 +799,7 @@ __device__ int interval_run(const DevMesh &m, const DevField &f, PState &p, ISta
      * h (S0 + S1) / 2 per step that stays in the domain, S0 and S1 the strain
      * rate at the step's start and end; only the generic integrator build */
     const bool ap = GEN && f.sr_lo != nullptr;
+    const bool cet = GEN && UNSTRUCT && f.cet_lo != nullptr; /* exposure time (integration.c:427-448) */
     using CT = typename VelT<MODE>::T; /* reduced-precision interpolation reads (float) r, s, t */
     CT c_r = 0, c_s = 0, c_t = 0;
     while ((is.tc - b) * (b - a) < 0.0) {
@@ -744,6 +888,12 @@ __device__ int interval_run(const DevMesh &m, const DevField &f, PState &p, ISta
             for (int c = 0; c < 3; c++) p.X[c] = p.X[c] - inc[c];
             stop = p.left_time;
             return 1;
+        }
+        if (cet) { /* the step's start as the reference recomputes it: X1 minus the increment */
+            double X0[3];
+            for (int c = 0; c < 3; c++) X0[c] = p.X[c] - inc[c];
+            cet_step<DIM>(m, f, X0, p.X, p.elem, h, err);
+            if (err) break;
         }
         if (ap) {
             const double sr1 = strain_rate<DIM, UNSTRUCT>(m, f, is.tc, p, err);

@@ -33,6 +33,9 @@ struct fvc_ctx {
     /* resident strain-rate slices (activation potential), FP64, one per slot */
     std::vector<double *> sr_slices;
     int64_t nsr; /* values per strain-rate slice: elements, or Cartesian cells */
+    /* exposure time: per element 128-bit fixed-point sums (low, high words) and encounters */
+    unsigned long long *cet_lo, *cet_hi;
+    int *cet_enc;
     /* errors and counters */
     int *d_err;
     int64_t *d_where;
@@ -135,6 +138,7 @@ FtleDev ftle_dev(fvc_ftle *f);
 DevField bind_field(fvc_ctx *c, int lo, int hi, double tmin, double tmax);
 
 int launch_velocity(fvc_ctx *c, PtrSet p, int64_t begin, int64_t count, double t, const DevField &f);
+int launch_cet_encounters(fvc_ctx *c, PtrSet p, int64_t begin, int64_t count, int *enc);
 int launch_advect_times(fvc_ctx *c, PtrSet p, int64_t begin, int64_t count, const double *dtt, const DevField &f,
                         const fvc_int_params &ip);
 int launch_advect(fvc_ctx *c, PtrSet p, const int *index, int64_t begin, int64_t count, double t1, double t2,

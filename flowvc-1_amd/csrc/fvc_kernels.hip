@@ -227,6 +227,20 @@ static inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + b
 
 static ErrBuf errbuf(fvc_ctx *c) { return ErrBuf{c->d_err, c->d_where}; }
 
+__global__ __launch_bounds__(BLOCK) void k_cet_encounters(PtrSet p, int64_t begin, int64_t count, int *enc) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= count) return;
+    const int64_t i = begin + q;
+    if (!p.left[i] && p.elem[i] >= 0) atomicAdd(enc + p.elem[i], 1);
+}
+
+int launch_cet_encounters(fvc_ctx *c, PtrSet p, int64_t begin, int64_t count, int *enc) {
+    if (count <= 0) return FVC_OK;
+    k_cet_encounters<<<grid_for(count, BLOCK), BLOCK, 0, c->stream>>>(p, begin, count, enc);
+    FVC_HIP(hipGetLastError());
+    return FVC_OK;
+}
+
 int launch_velocity(fvc_ctx *c, PtrSet p, int64_t begin, int64_t count, double t, const DevField &f) {
     if (count <= 0) return FVC_OK;
     FVC_DISPATCH(c->precision, c->mesh.dims, c->mesh.mesh_type, k_launch_vel,
@@ -248,7 +262,7 @@ int launch_advect_times(fvc_ctx *c, PtrSet p, int64_t begin, int64_t count, cons
                         const fvc_int_params &ip) {
     FVC_HIP(hipMemsetAsync(c->d_steps, 0, sizeof(unsigned long long), c->stream));
     if (count > 0) {
-        if (ip.int_type == FVC_INT_RK4 && !f.sr_lo)
+        if (ip.int_type == FVC_INT_RK4 && !f.sr_lo && !f.cet_lo)
             FVC_DISPATCH(c->precision, c->mesh.dims, c->mesh.mesh_type, k_advect_times_rk4,
                          <<<grid_for(count, BLOCK), BLOCK, 0, c->stream>>>(c->mesh, f, p, begin, count, dtt,
                                                                            ip.int_type, ip.time_step, ip.extrapolate,
@@ -287,7 +301,7 @@ int launch_advect(fvc_ctx *c, PtrSet p, const int *index, int64_t begin, int64_t
         int block = ip.block > 0 ? ip.block : BLOCK;
         if (block != BLOCK) block = BLOCK; /* kernels are compiled for 256-lane blocks */
         timing_begin(c);
-        if (ip.int_type == FVC_INT_RK4 && !f.sr_lo) /* the activation potential is in the generic build only */
+        if (ip.int_type == FVC_INT_RK4 && !f.sr_lo && !f.cet_lo) /* strain rate, exposure: generic build only */
             FVC_DISPATCH(c->precision, c->mesh.dims, c->mesh.mesh_type, k_advect,
                          <<<grid_for(count, block), block, 0, c->stream>>>(c->mesh, f, p, index, begin, count, t1, t2,
                                                                            ip.int_type, ip.time_step, ip.extrapolate,
@@ -311,71 +325,6 @@ int launch_advect(fvc_ctx *c, PtrSet p, const int *index, int64_t begin, int64_t
 /* ------------------------------------------------------------------------ */
 /* location                                                                   */
 /* ------------------------------------------------------------------------ */
-/* lowest-index element with d > -TINY (mesh.c:321-363) and whether a second
- * element also passes; cell lists are ascending, so the first pass is the
- * lowest index. */
-template <int DIM>
-__device__ int hash_global(const DevMesh &m, double x, double y, double z, int &second) {
-    second = -1;
-    double fx = (x - m.hlo[0]) * m.hinv[0], fy = (y - m.hlo[1]) * m.hinv[1];
-    double fz = DIM == 3 ? (z - m.hlo[2]) * m.hinv[2] : 0.0;
-    if (!(fx >= 0.0 && fy >= 0.0 && fz >= 0.0)) return -1;
-    if (!(fx < (double)m.hres[0] && fy < (double)m.hres[1] && fz < (double)m.hres[2])) return -1;
-    int ix = (int)fx, iy = (int)fy, iz = (int)fz;
-    int64_t cell = ((int64_t)iz * m.hres[1] + iy) * m.hres[0] + ix;
-    int first = -1;
-    for (int q = m.cell_start[cell]; q < m.cell_start[cell + 1]; q++) {
-        int e = m.cell_elems[q];
-        double r, s, t = 0.0;
-        if (DIM == 3) nat3_exact(m, e, x, y, z, r, s, t);
-        else nat2_exact(m, e, x, y, r, s);
-        int slot;
-        if (decide_exact<DIM>(r, s, t, ACC_SEARCH, slot) == 1) {
-            if (first < 0) first = e;
-            else {
-                second = e;
-                break;
-            }
-        }
-    }
-    return first;
-}
-
-/* every element with d > -TINY, ascending: the first `cap` are written to
- * `list` (when given), the lowest is returned in `first`, the count is
- * returned whole */
-template <int DIM>
-__device__ int hash_accepting(const DevMesh &m, double x, double y, double z, int *list, int cap, int &first) {
-    first = -1;
-    double fx = (x - m.hlo[0]) * m.hinv[0], fy = (y - m.hlo[1]) * m.hinv[1];
-    double fz = DIM == 3 ? (z - m.hlo[2]) * m.hinv[2] : 0.0;
-    if (!(fx >= 0.0 && fy >= 0.0 && fz >= 0.0)) return 0;
-    if (!(fx < (double)m.hres[0] && fy < (double)m.hres[1] && fz < (double)m.hres[2])) return 0;
-    int ix = (int)fx, iy = (int)fy, iz = (int)fz;
-    int64_t cell = ((int64_t)iz * m.hres[1] + iy) * m.hres[0] + ix;
-    int cnt = 0;
-    for (int q = m.cell_start[cell]; q < m.cell_start[cell + 1]; q++) {
-        int e = m.cell_elems[q];
-        double r, s, t = 0.0;
-        if (DIM == 3) nat3_exact(m, e, x, y, z, r, s, t);
-        else nat2_exact(m, e, x, y, r, s);
-        int slot;
-        if (decide_exact<DIM>(r, s, t, ACC_SEARCH, slot) == 1) {
-            if (cnt == 0) first = e;
-            if (list && cnt < cap) list[cnt] = e;
-            cnt++;
-        }
-    }
-    return cnt;
-}
-
-template <int DIM>
-__device__ int local_search(const DevMesh &m, double x, double y, double z, int guess, int &err) {
-    double r, s, t;
-    int e = walk<P_F64, DIM>(m, x, y, z, guess, ACC_SEARCH, r, s, t, err);
-    return e == -2 ? -1 : e;
-}
-
 template <int DIM>
 __global__ __launch_bounds__(BLOCK) void k_locate(DevMesh m, const double *__restrict__ xyz, const int *__restrict__ guess,
                                                  int64_t n, int mode, int *__restrict__ out, ErrBuf eb) {

@@ -33,7 +33,7 @@ class Params(ctypes.Structure):
 class IntParams(ctypes.Structure):
     _fields_ = [("int_type", ctypes.c_int32), ("extrapolate", ctypes.c_int32), ("time_step", ctypes.c_double),
                 ("block", ctypes.c_int32), ("steps_per_launch", ctypes.c_int32),
-                ("activation_potential", ctypes.c_int32)]
+                ("activation_potential", ctypes.c_int32), ("exposure_substeps", ctypes.c_int32)]
 
 
 class Release(ctypes.Structure):

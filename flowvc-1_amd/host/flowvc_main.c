@@ -13,9 +13,10 @@
  * that selects ompVC's tracer output loop (parameters.c:411-421).
  * Staggered tracer release follows tracers.c:293-1000 and velocity output
  * velout.c:21-435; the activation potential (Trace_APCompute) integrates
- * the strain-rate data along tracer paths (strainrate.c, integration.c).
- * Modes outside the advection path (inertial particles, RKF, exposure and
- * residence time, normal flow) stop with an error.
+ * the strain-rate data along tracer paths (strainrate.c, integration.c);
+ * exposure times (exposuretime.c) and residence times (residencetime.c) are
+ * written per element and per seed.  Inertial particles, RKF and normal flow
+ * stop with an error.
  */
 #include <math.h>
 #include <stdio.h>
@@ -173,6 +174,28 @@ static double *read_seed_list(const fvc_deck *d, int64_t *n) {
         fatal("Unsupported value for Trace_InFileFormat in %s", path);
     }
     return xyz;
+}
+
+/* Description of the seed set beside per-seed outputs (OutputRT, OutputAP:
+ * residencetime.c:29-66, strainrate.c:250-290): the generated grid's header,
+ * or the seeds' initial coordinates. */
+static void write_trace_mesh(const fvc_deck *d, const char *prefix, const fvc_point *ic, int64_t n) {
+    char path[3 * FVC_STR];
+    if (d->Trace_GenerateMesh) {
+        snprintf(path, sizeof path, "%s%s_Cartesian.bin", d->Path_Output, prefix);
+        double cmn[3] = {d->Trace_Cart.XMin, d->Trace_Cart.YMin, d->Trace_Cart.ZMin};
+        double cmx[3] = {d->Trace_Cart.XMax, d->Trace_Cart.YMax, d->Trace_Cart.ZMax};
+        int32_t cres[3] = {d->Trace_Cart.XRes, d->Trace_Cart.YRes, d->Trace_Cart.ZRes};
+        if (fvc_write_cart_header(path, cmn, cmx, cres)) fatal("Could not open file %s", path);
+    } else {
+        snprintf(path, sizeof path, "%s%s_coordinates.bin", d->Path_Output, prefix);
+        FILE *o = fopen(path, "wb");
+        if (!o) fatal("Could not open file %s", path);
+        int32_t n32 = (int32_t)n;
+        fwrite(&n32, 4, 1, o);
+        for (int64_t q = 0; q < n; q++) fwrite(ic[q].X, 8, 3, o);
+        fclose(o);
+    }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -726,9 +749,14 @@ int main(int argc, char **argv) {
         fvc_deck_derive(&d, err, sizeof err))
         fatal("%s", err);
     if (d.legacy_96 && !quiet) fprintf(stderr, "Note: deck has 96 entries (no Num_threads); read as Num_threads = 0\n");
-    if (d.Particle_Radius > TINY || d.Int_Type == 2 || d.Trace_CETCompute || d.Trace_RTCompute || d.Int_NormalFlow)
-        fatal("%s", "deck selects a mode outside the advection path (inertial, RKF, CET, RT, normal flow)");
+    if (d.Particle_Radius > TINY || d.Int_Type == 2 || d.Int_NormalFlow)
+        fatal("%s", "deck selects a mode outside the advection path (inertial particles, RKF, normal flow)");
     const int ap = d.Trace_Compute && d.Trace_APCompute;
+    const int cet = d.Trace_Compute && d.Trace_CETCompute, rt = d.Trace_Compute && d.Trace_RTCompute;
+    if (cet && (d.Trace_CETAuxillaryMesh || d.Data_MeshType != 1))
+        fatal("%s", "exposure times are supported on the velocity mesh itself (unstructured, no auxiliary mesh)");
+    if ((cet || rt) && d.Trace_ReleaseStrategy == 1)
+        fatal("%s", "exposure or residence times with the staggered release are not supported");
     if (ap && d.Trace_ReleaseStrategy == 1)
         fatal("%s", "Trace_APCompute with the staggered release is not supported (the reference's OutputAP reads "
                     "release files that the staggered release never writes)");
@@ -781,7 +809,9 @@ int main(int argc, char **argv) {
                                             ((d.Dimensions == 2 || md.cart_res[2] == 1) ? 1 : md.cart_res[2] - 1);
         sl.srbuf = (double *)malloc(sizeof(double) * (size_t)(sl.nsr > 0 ? sl.nsr : 1));
     }
-    fvc_int_params ip = {d.Int_Type == 1 ? FVC_INT_RK4 : FVC_INT_EULER, d.Int_Extrapolate, d.Int_TimeStep, 0, 0, ap};
+    fvc_int_params ip = {d.Int_Type == 1 ? FVC_INT_RK4 : FVC_INT_EULER, d.Int_Extrapolate, d.Int_TimeStep, 0, 0, ap,
+                         cet ? d.Trace_CETSubsteps : 0};
+    if (cet) check(fvc_cet_reset(c), c, "fvc_cet_reset");
     double t_compute = 0.0;
     int64_t steps_total = 0;
 
@@ -910,6 +940,7 @@ int main(int argc, char **argv) {
                         double t0 = now_s();
                         if (status[ss] == 0) {
                             status[ss] = 1;
+                            if (cet) check(fvc_cet_launch(c, P, (int64_t)ss * n, n), c, "fvc_cet_launch");
                             t1 = rstart[ss];
                             check(fvc_launch_velocity(c, P, (int64_t)ss * n, n, rstart[ss], lo, hi, tmin, tmax), c,
                                   "fvc_launch_velocity");
@@ -959,22 +990,64 @@ int main(int argc, char **argv) {
             }
             if (of[i]) fclose(of[i]);
         }
-        if (ap) { /* OutputAP (strainrate.c:248-395): mesh description, then each release's Scalar */
-            if (d.Trace_GenerateMesh) {
-                snprintf(path, sizeof path, "%s%s_Cartesian.bin", d.Path_Output, d.Trace_OutFilePrefix);
-                double cmn[3] = {d.Trace_Cart.XMin, d.Trace_Cart.YMin, d.Trace_Cart.ZMin};
-                double cmx[3] = {d.Trace_Cart.XMax, d.Trace_Cart.YMax, d.Trace_Cart.ZMax};
-                int32_t cres[3] = {d.Trace_Cart.XRes, d.Trace_Cart.YRes, d.Trace_Cart.ZRes};
-                if (fvc_write_cart_header(path, cmn, cmx, cres)) fatal("Could not open file %s", path);
-            } else {
-                snprintf(path, sizeof path, "%s%s_coordinates.bin", d.Path_Output, d.Trace_OutFilePrefix);
+        if (rt) { /* OutputRT (residencetime.c:21-185): mesh description, then each release's residence times */
+            write_trace_mesh(&d, d.Trace_RTOutFilePrefix, ic, n);
+            for (int ss = 0; ss < nl; ss++) {
+                check(fvc_particles_get(P, (int64_t)ss * n, tmp, n), c, "fvc_particles_get");
+                snprintf(path, sizeof path, "%s%s.%d.bin", d.Path_Output, d.Trace_RTOutFilePrefix, ss);
                 FILE *o = fopen(path, "wb");
                 if (!o) fatal("Could not open file %s", path);
-                int32_t n32 = (int32_t)n;
-                fwrite(&n32, 4, 1, o);
-                for (int64_t q = 0; q < n; q++) fwrite(ic[q].X, 8, 3, o);
+                const double t0r = d.Output_TStart + ss * d.Trace_LaunchTimeSpacing;
+                fwrite(&t0r, 8, 1, o);
+                for (int64_t q = 0; q < n; q++) {
+                    double rtv;
+                    if (fabs(tmp[q].LeftDomainTime + 1) < TINY)
+                        rtv = -1.0;
+                    else
+                        rtv = (tmp[q].LeftDomain ? tmp[q].LeftDomainTime : rstop[ss]) - t0r;
+                    fwrite(&rtv, 8, 1, o);
+                }
                 fclose(o);
             }
+        } else if (cet) { /* OutputCET (exposuretime.c:180-300) */
+            const int64_t ne = md.num_elems;
+            double *sum = (double *)malloc(sizeof(double) * (size_t)ne), *cv = (double *)calloc((size_t)ne, 8),
+                   *mv = (double *)calloc((size_t)ne, 8);
+            int32_t *enc = (int32_t *)malloc(sizeof(int32_t) * (size_t)ne);
+            check(fvc_cet_get(c, sum, enc), c, "fvc_cet_get");
+            for (int64_t e = 0; e < ne; e++) {
+                if (!(sum[e] > TINY)) continue;
+                double xc;
+                if (d.Dimensions == 3) {
+                    xc = 1.0; /* pow(vol, (1/3)): the exponent 1/3 is integer division, 0 (exposuretime.c:224) */
+                } else {
+                    const double *p1 = xyz + 3 * conn[4 * e], *p2 = xyz + 3 * conn[4 * e + 1],
+                                 *p3 = xyz + 3 * conn[4 * e + 2];
+                    const double A0 = p2[0] - p1[0], A1 = p2[1] - p1[1], B0 = p3[0] - p1[0], B1 = p3[1] - p1[1];
+                    xc = sqrt(fabs(A0 * B1 - A1 * B0) / 2);
+                }
+                cv[e] = sum[e] / xc / (double)n;
+                mv[e] = sum[e] / xc / enc[e];
+            }
+            const double dummy = -1.0;
+            const char *suf[3] = {"CET", "MET", "ENC"};
+            for (int k = 0; k < 3; k++) {
+                snprintf(path, sizeof path, "%s%s_%s.0.bin", d.Path_Output, d.Trace_RTOutFilePrefix, suf[k]);
+                FILE *o = fopen(path, "wb");
+                if (!o) fatal("Could not open %s", path);
+                fwrite(&dummy, 8, 1, o);
+                if (k == 0) fwrite(cv, 8, (size_t)ne, o);
+                if (k == 1) fwrite(mv, 8, (size_t)ne, o);
+                if (k == 2) fwrite(enc, 4, (size_t)ne, o);
+                fclose(o);
+            }
+            free(sum);
+            free(cv);
+            free(mv);
+            free(enc);
+        }
+        if (ap) { /* OutputAP (strainrate.c:248-395): mesh description, then each release's Scalar */
+            write_trace_mesh(&d, d.Trace_OutFilePrefix, ic, n);
             for (int ss = 0; ss < nl; ss++) {
                 check(fvc_particles_get(P, (int64_t)ss * n, tmp, n), c, "fvc_particles_get");
                 snprintf(path, sizeof path, "%s%s_AP.%d.bin", d.Path_Output, d.Trace_OutFilePrefix, ss);

@@ -280,6 +280,12 @@ int fvc_deck_check(const fvc_deck *d, char *err, int errlen) {
         if (d->Trace_LaunchTimeSpacing < FVC_TINY && d->Trace_NumLaunchTimes > 1)
             FAIL("Trace_LaunchTimeSpacing must be positive");
         if (d->Trace_IntTLength < 0) FAIL("Trace_IntTLength < 0");
+        if (d->Trace_CETCompute) { /* parameters.c:535-542 */
+            if (d->Trace_RTCompute) FAIL("Cannot compute exposure times and residence times concurrently");
+            if (d->Trace_CETSubsteps < 2) FAIL("Trace_CETSubsteps must be greater than 1");
+        }
+        if ((d->Trace_RTCompute || d->Trace_CETCompute) && d->Int_TimeDirection < 0)
+            FAIL("Residence/Exposure time computations not supported for backward time integrations");
     }
     return 0;
 #undef FAIL

@@ -37,6 +37,7 @@ extern "C" {
 #define FVC_E_STATE (-10)                 /* call out of sequence */
 #define FVC_E_LEFT (-11)                  /* ompVC/integration.c:326 integrate a point that left */
 #define FVC_E_NODEVICE (-12)              /* no usable gfx950 device */
+#define FVC_E_CET_OUTSIDE (-13)           /* ompVC/exposuretime.c:138 step start outside the mesh */
 
 /* ---- enums -------------------------------------------------------------- */
 enum fvc_mesh_type { FVC_CARTESIAN = 0, FVC_UNSTRUCTURED = 1 };
@@ -102,6 +103,11 @@ typedef struct fvc_int_params {
      * slices into each point's Scalar (ompVC/integration.c:355-360, 449-455);
      * fvc_advect and fvc_advect_times only */
     int32_t activation_potential;
+    /* Trace_CETSubsteps when exposure times are accumulated (ompVC/
+     * integration.c:427-448, exposuretime.c:120-178), 0 otherwise; needs
+     * fvc_cet_reset first; unstructured meshes, fvc_advect and
+     * fvc_advect_times only */
+    int32_t exposure_substeps;
 } fvc_int_params;
 
 typedef struct fvc_ctx fvc_ctx;
@@ -335,6 +341,17 @@ int fvc_server_stats(fvc_server *s, double out[5]);
  * them, input order) or FTLE values with and without the time factor (k, j,
  * i order, as fvc_ftle_output). */
 int fvc_server_take(fvc_server *s, int64_t id, fvc_point *records, double *ftle_wt, double *ftle_wot);
+
+/* ---- exposure time (ompVC/exposuretime.c) ---------------------------------- */
+/* Zero the per-element exposure-time sums and encounter counts (main.c:84-90). */
+int fvc_cet_reset(fvc_ctx *ctx);
+/* One encounter in the element of every point of the range still in the
+ * domain, at a release (tracers.c:180-188). */
+int fvc_cet_launch(fvc_ctx *ctx, fvc_particles *p, int64_t begin, int64_t count);
+/* Per-element exposure-time sums (CETsum) and encounter counts; either may be
+ * NULL.  The sums are accumulated exactly in fixed point (2^-100 s) and
+ * rounded once here, so they do not depend on the order of the steps. */
+int fvc_cet_get(fvc_ctx *ctx, double *cetsum, int32_t *encounters);
 
 /* ---- Z-order seed sort (cudaVC/ZOrderSort/main.cpp:86-101 ordering) ------- */
 /* Sort n points by the interleaved-bit (x most significant) order of their

@@ -211,6 +211,23 @@ CASES = {
                                        Trace_InFile="seeds.bin", Trace_InFileFormat=4, Trace_IntTLength=0.15,
                                        Trace_NumLaunchTimes=2, Trace_LaunchTimeSpacing=0.04, Trace_APCompute=1,
                                        Trace_OutFilePrefix="pap2")),
+    # exposure times per element (CET, MET, encounters) with 5 substeps per step; one OpenMP thread so that the
+    # reference adds each element's times in one order
+    "pipe_tracers_cet": dict(gen=gen_pipe, outputs=["pct.%d.bin" % k for k in range(3)] +
+                             ["pcet_CET.0.bin", "pcet_MET.0.bin", "pcet_ENC.0.bin"], ic="pct.IC",
+                             close=["pcet_CET.0.bin", "pcet_MET.0.bin"],
+                             deck=dict(PIPE, Num_threads=1, Output_TStart=0.0, Output_TRes=3, Output_TDelta=0.05,
+                                       Int_TimeDirection=1, Trace_Compute=1, Trace_GenerateMesh=0,
+                                       Trace_InFile="seeds.bin", Trace_InFileFormat=4, Trace_IntTLength=0.1,
+                                       Trace_CETCompute=1, Trace_CETSubsteps=5, Trace_RTOutFilePrefix="pcet",
+                                       Trace_OutFilePrefix="pct")),
+    # residence times per seed (the location probes: points near the outlet leave, points outside stay -1)
+    "pipe_tracers_rt": dict(gen=gen_pipe, outputs=["prtt.%d.bin" % k for k in range(4)] +
+                            ["prt.0.bin", "prt_coordinates.bin"], ic="prtt.IC",
+                            deck=dict(PIPE, Output_TStart=0.0, Output_TRes=4, Output_TDelta=0.1, Int_TimeDirection=1,
+                                      Trace_Compute=1, Trace_GenerateMesh=0, Trace_InFile="probes.txt",
+                                      Trace_InFileFormat=1, Trace_IntTLength=0.3, Trace_RTCompute=1,
+                                      Trace_RTOutFilePrefix="prt", Trace_OutFilePrefix="prtt")),
     # activation potential on the 2-D Cartesian grid (per-cell strain rate), generated seeds (2048 records: the
     # reference leaves LeftDomainTime of generated seeds unset, see dg_tracers)
     "dg_tracers_ap": dict(gen=gen_dg_ap, outputs=["dgap.%d.bin" % k for k in range(6)] +
