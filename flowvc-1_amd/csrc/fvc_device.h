@@ -646,7 +646,7 @@ enum { INT_EULER = 0, INT_RK4 = 1, INT_AB2 = 3, INT_AB3 = 4 };
  * to a 128-bit fixed-point accumulator (two 64-bit words, the carry of the low
  * word added to the high one), so the per-element totals do not depend on the
  * order in which the parallel steps arrive; the reference adds in its
- * threads' arrival order (exposuretime.c:139-176). */
+ * threads' arrival order (exposuretime.c:135-164). */
 constexpr int CET_FRAC = 100;
 
 __device__ inline void cet_add(unsigned long long *lo, unsigned long long *hi, double v) {

@@ -1019,7 +1019,7 @@ int main(int argc, char **argv) {
                 if (!(sum[e] > TINY)) continue;
                 double xc;
                 if (d.Dimensions == 3) {
-                    xc = 1.0; /* pow(vol, (1/3)): the exponent 1/3 is integer division, 0 (exposuretime.c:224) */
+                    xc = 1.0; /* pow(vol, (1/3)): the exponent 1/3 is integer division, 0 (exposuretime.c:222) */
                 } else {
                     const double *p1 = xyz + 3 * conn[4 * e], *p2 = xyz + 3 * conn[4 * e + 1],
                                  *p3 = xyz + 3 * conn[4 * e + 2];

@@ -9,7 +9,7 @@
  * n = first, first+step, ..., last.  The renumbered velocity files keep their
  * leading time stamp, so they stay readable by ompVC's slice loader
  * (velocity.c:184-211); the reference writes the stamp to its already-closed
- * input stream (main.cpp:282), so its velocity outputs lack it. */
+ * input stream (main.cpp:287), so its velocity outputs lack it. */
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>

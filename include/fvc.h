@@ -37,7 +37,7 @@ extern "C" {
 #define FVC_E_STATE (-10)                 /* call out of sequence */
 #define FVC_E_LEFT (-11)                  /* ompVC/integration.c:326 integrate a point that left */
 #define FVC_E_NODEVICE (-12)              /* no usable gfx950 device */
-#define FVC_E_CET_OUTSIDE (-13)           /* ompVC/exposuretime.c:138 step start outside the mesh */
+#define FVC_E_CET_OUTSIDE (-13)           /* ompVC/exposuretime.c:129 step start outside the mesh */
 
 /* ---- enums -------------------------------------------------------------- */
 enum fvc_mesh_type { FVC_CARTESIAN = 0, FVC_UNSTRUCTURED = 1 };

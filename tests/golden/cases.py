@@ -201,7 +201,7 @@ CASES = {
                                       Trace_InFileFormat=4, Trace_IntTLength=0.15, Trace_APCompute=1,
                                       Trace_OutFilePrefix="pap")),
     # two releases: the reference's OutputAP reads each release file into a fresh allocation it then drops
-    # (tracers.c:1649-1656, strainrate.c:383-384) and writes the stale array, so its _AP.1 repeats release 0;
+    # (tracers.c:1649-1656, strainrate.c:372) and writes the stale array, so its _AP.1 repeats release 0;
     # the records with their Scalar in the position files are compared, and _AP.1 is checked against them
     "pipe_tracers_ap2": dict(gen=gen_pipe_ap, outputs=["pap2.%d.bin" % k for k in range(6)] +
                              ["pap2_AP.0.bin", "pap2_AP.1.bin", "pap2_coordinates.bin"], ic="pap2.IC",
