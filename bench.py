@@ -557,8 +557,9 @@ def main():
                     help="replay the launch sequence from a captured hipGraph (1) or launch it directly (0)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: every box here has one GPU and this build has no multi-GPU path "
-              "(sharding plan in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the advection path runs on a single device (particles shard by "
+              "Z-order range across devices, but no multi-GPU path is implemented); run with --gpus 1",
+              file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
