@@ -881,7 +881,9 @@ int oc_run_deck(const char *deck_path, int nproc, char *msg, int msglen) {
                                     if ((rc = oc_velocity(m, r->start, &r->pts[q], r->pts[q].V))) goto done;
                                 }
                         }
-                        if ((rc = oc_advect_batch(m, &ip, n, r->pts, t1, t2, 1, 64, NULL))) goto done;
+                        /* LOWERLOOP's parallel loop over the tracers, schedule(dynamic, N/nproc) (tracers.c:218-226) */
+                        if ((rc = oc_advect_batch(m, &ip, n, r->pts, t1, t2, nproc, n / nproc > 0 ? n / nproc : 1, NULL)))
+                            goto done;
                         if (fabs(nextout - t2) < TINY) {
                             for (int64_t q = 0; q < n; q++) {
                                 const oc_point *p = &r->pts[q];
