@@ -418,6 +418,12 @@ extern "C" int fvc_set_timing(fvc_ctx *c, int32_t enable) {
     return FVC_OK;
 }
 
+extern "C" int fvc_set_chain_mode(fvc_ctx *c, int32_t mode) {
+    if (!c || mode < 0 || mode > 1) return FVC_E_INVALID;
+    c->chain_replay = mode;
+    return FVC_OK;
+}
+
 extern "C" int fvc_kernel_time(fvc_ctx *c, double *ms, int64_t *launches, int64_t *steps, int32_t reset) {
     if (!c) return FVC_E_INVALID;
     if (ms) *ms = c->t_ms;

@@ -48,6 +48,7 @@ struct fvc_ctx {
     size_t scratch_bytes;
     /* kernel timing (HIP events on `stream`) */
     int timing;
+    int chain_replay; /* seed chains: 1 = always the point-by-point replay */
     hipEvent_t ev0, ev1;
     double t_ms;
     int64_t t_launches, t_steps;

@@ -1056,7 +1056,7 @@ int launch_seed_chain(fvc_ctx *c, int chain, int64_t n, const double *dxyz, cons
         FVC_HIP(hipStreamSynchronize(c->stream));
     }
     /* point indices of the tabulated chain are ints; 0x7f7f7f7f marks "none" */
-    const bool tab = uns && lsc && n < 0x7f000000LL;
+    const bool tab = uns && lsc && n < 0x7f000000LL && !c->chain_replay;
     const int64_t nch = (n + 63) / 64;
     const int64_t nrows = chain >= 2 ? n / chain_rowlen(g) : 1,
                   nplanes = chain >= 2 ? n / (chain_rowlen(g) * g.res[1]) : 1;

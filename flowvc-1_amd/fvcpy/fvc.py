@@ -138,6 +138,7 @@ def lib():
         L.fvc_server_stats.argtypes = [vp, vp]
         L.fvc_server_take.argtypes = [vp, i64, vp, vp, vp]
         L.fvc_mesh_zorder.argtypes = [i64, vp, i64, vp, vp, vp, vp, vp, vp, vp]
+        L.fvc_set_chain_mode.argtypes = [vp, ctypes.c_int32]
         L.fvc_gather_rows.argtypes = [i64, ctypes.c_int, vp, vp, vp]
         _LIB = L
     return _LIB
@@ -246,6 +247,10 @@ class Context:
         out = np.zeros(xyz.shape[0], POINT_DTYPE)
         _chk(lib().fvc_seed_list(self.h, _p(xyz), xyz.shape[0], lsc, _p(out)), "fvc_seed_list", self.h)
         return out
+
+    def set_chain_mode(self, mode):
+        """0: tabulated seed chains where possible (default); 1: point-by-point replay."""
+        _chk(lib().fvc_set_chain_mode(self.h, mode), "fvc_set_chain_mode", self.h)
 
     def seed_grid(self, mn, dl, res, lsc=1):
         mn, dl = np.asarray(mn, np.float64), np.asarray(dl, np.float64)

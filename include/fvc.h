@@ -227,6 +227,11 @@ int fvc_ftle_reset(fvc_ftle *f);
  * particle-steps since the last reset. */
 int fvc_set_timing(fvc_ctx *ctx, int32_t enable);
 int fvc_kernel_time(fvc_ctx *ctx, double *ms, int64_t *launches, int64_t *particle_steps, int32_t reset);
+/* Seed location chains: 0 (default) tabulates the chain where it can (walk
+ * tables per multi-element point composed by pointer jumping, all points in
+ * parallel), 1 always replays it point by point.  Both give the reference's
+ * result; the replay is the cross-check of the tabulated form. */
+int fvc_set_chain_mode(fvc_ctx *ctx, int32_t mode);
 /* A plain device copy of `bytes` bytes (HBM traffic calibration for counters). */
 int fvc_copy_probe(fvc_ctx *ctx, int64_t bytes);
 

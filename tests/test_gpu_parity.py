@@ -303,3 +303,41 @@ def test_ftle_chain_on_voxel_edges_matches_oracle():
         np.testing.assert_array_equal(g["LeftDomain"], o["Pt"]["LeftDomain"])
         np.testing.assert_array_equal(g["ElementIndex"][inside], o["Pt"]["ElementIndex"][inside])
         np.testing.assert_array_equal(g["LeftDomainTime"], o["Pt"]["LeftDomainTime"])
+
+
+def _chain_both(ctx, fn):
+    ctx.set_chain_mode(1)
+    a = fn()
+    ctx.set_chain_mode(0)
+    b = fn()
+    for k in ("ElementIndex", "LeftDomain", "LeftDomainTime"):
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+    return b
+
+
+def test_tabulated_chains_equal_replay_at_full_size():
+    """Seed chains at the sizes the benchmarks use: the tabulated chain and the
+    point-by-point replay give the same records (FTLE grid of the serving
+    benchmark over the Y-bifurcation, where every inside point lies on a voxel
+    edge; the pipe benchmark's 128x128x640 FTLE grid; a tracer grid and a
+    4M-point list)."""
+    t = host.ybif(184, 65, 256)
+    xyz, conn, adj = host.tube_mesh(t)
+    lo, hi = xyz.min(0), xyz.max(0)
+    ctx = fvc.Context(3, [lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]], "bf16", 2, coords=xyz, conn=conn, adj=adj)
+    mn, mx = [t.lo[0], t.lo[1], t.lo[2]], [t.hi[0], t.hi[1], t.hi[2]]
+    for res in ([93, 33, 129], [70, 25, 97]):
+        print("Y-bifurcation FTLE grid chain res=%s" % res, flush=True)
+        r = _chain_both(ctx, lambda: fvc.FtleSlide(ctx, mn, mx, res, 1, 0.0, 0.01).records())
+        assert (r["LeftDomain"] == 0).sum() > 10000
+    del ctx
+    p = host.pipe(60, 60, 330, 1.0, 10.0)
+    xyz, conn, adj = host.tube_mesh(p)
+    ctx = fvc.Context(3, PIPE_BOUNDS, "f32", 2, coords=xyz, conn=conn, adj=adj)
+    print("pipe FTLE grid chain 128x128x640", flush=True)
+    _chain_both(ctx, lambda: fvc.FtleSlide(ctx, [-1, -1, 0], [1, 1, 10], [128, 128, 640], 1, 0.0, 0.02).records())
+    print("pipe tracer grid chain on voxel corners", flush=True)
+    _chain_both(ctx, lambda: ctx.seed_grid([-1.0, -1.0, 0.0], [2.0 / 60, 2.0 / 60, 10.0 / 330], [61, 61, 331], 1))
+    seeds = np.concatenate([host.tube_seeds(p, 4000000, 0.0, 10.0, 5), query_points(p, 40000, 6)])
+    print("pipe list chain n=%d" % seeds.shape[0], flush=True)
+    _chain_both(ctx, lambda: ctx.seed_list(seeds, 1))
