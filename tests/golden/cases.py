@@ -258,6 +258,46 @@ CASES = {
                            deck=dict(DG, **VELOUT, Output_TStart=10.0, Output_TRes=6, Output_TDelta=1.5,
                                      Int_TimeDirection=-1, VelOut_GenerateMesh=0, VelOut_InFile="velpts.txt",
                                      VelOut_InFileFormat=1, VelOut_FilePrefix="dgv")),
+    # Euler (integration.c:222-311) with extrapolation of points that leave (Advect, integration.c:22-45) and
+    # Trace_AlwaysOutput, so that the extrapolated positions are written; 64x32 generated seeds (see dg_tracers)
+    "dg_tracers_euler": dict(gen=gen_dg, outputs=["dge.%d.bin" % k for k in range(6)],
+                             deck=dict(DG, Int_Type=0, Int_Extrapolate=1, Output_TStart=0.0, Output_TRes=6,
+                                       Output_TDelta=2.0, Int_TimeDirection=1, Trace_Compute=1, Trace_GenerateMesh=1,
+                                       Trace_MeshBounds_XMin=0.0, Trace_MeshBounds_XMax=2.0,
+                                       Trace_MeshBounds_YMin=0.0, Trace_MeshBounds_YMax=1.0,
+                                       Trace_MeshBounds_XRes=64, Trace_MeshBounds_YRes=32, Trace_MeshBounds_ZRes=1,
+                                       Trace_IntTLength=10.0, Trace_AlwaysOutput=1, Trace_OutFilePrefix="dge")),
+    # Euler on the unstructured pipe: walk after each step, exit at the outlet
+    "pipe_tracers_euler": dict(gen=gen_pipe, outputs=["pte.%d.bin" % k for k in range(11)], ic="pte.IC",
+                               deck=dict(PIPE, Int_Type=0, Output_TStart=0.0, Output_TRes=11, Output_TDelta=0.1,
+                                         Int_TimeDirection=1, Trace_Compute=1, Trace_GenerateMesh=0,
+                                         Trace_InFile="seeds.bin", Trace_InFileFormat=4, Trace_IntTLength=1.0,
+                                         Trace_OutFilePrefix="pte")),
+    # RK4 with extrapolation: velocity kept up to date after each step (integration.c:420-425), points that
+    # leave move on with their last velocity, all written (Trace_AlwaysOutput); the location probes, whose
+    # points near the outlet leave
+    "pipe_tracers_extrap": dict(gen=gen_pipe, outputs=["ptx.%d.bin" % k for k in range(6)], ic="ptx.IC",
+                                deck=dict(PIPE, Int_Extrapolate=1, Output_TStart=0.0, Output_TRes=6,
+                                          Output_TDelta=0.1, Int_TimeDirection=1, Trace_Compute=1,
+                                          Trace_GenerateMesh=0, Trace_InFile="probes.txt", Trace_InFileFormat=1,
+                                          Trace_IntTLength=0.5, Trace_AlwaysOutput=1, Trace_OutFilePrefix="ptx")),
+    # 2-D Cartesian FTLE with Euler, 64x32 grid, T=4 (early exits re-advected with Euler)
+    "dg_ftle_euler": dict(gen=gen_dg, outputs=["dgfe.0.bin", "dgfe_noT.0.bin", "dgfe_Cartesian.bin"],
+                          deck=dict(DG, Int_Type=0, Output_TStart=0.0, Output_TRes=1, Output_TDelta=1.0,
+                                    Int_TimeDirection=1, FTLE_Compute=1, FTLE_MeshBounds_XMin=0.0,
+                                    FTLE_MeshBounds_XMax=2.0, FTLE_MeshBounds_YMin=0.0, FTLE_MeshBounds_YMax=1.0,
+                                    FTLE_MeshBounds_XRes=64, FTLE_MeshBounds_YRes=32, FTLE_MeshBounds_ZRes=1,
+                                    FTLE_IntTLength=4.0, FTLE_OutFilePrefix="dgfe", FTLE_ICFile="dgfe_ic.bin")),
+    # 3-D unstructured FTLE with extrapolation: points that leave move on with their last velocity and take part
+    # in the gradient at the end of the slide (no early FTLE, ftle.c:113-170)
+    "pipe_ftle_extrap": dict(gen=gen_pipe, outputs=["pfx.0.bin", "pfx.1.bin", "pfx_noT.0.bin", "pfx_noT.1.bin",
+                                                    "pfx_Cartesian.bin"],
+                             deck=dict(PIPE, Int_Extrapolate=1, Output_TStart=0.0, Output_TRes=2, Output_TDelta=0.1,
+                                       Int_TimeDirection=1, FTLE_Compute=1, FTLE_MeshBounds_XMin=-1.0,
+                                       FTLE_MeshBounds_XMax=1.0, FTLE_MeshBounds_YMin=-1.0, FTLE_MeshBounds_YMax=1.0,
+                                       FTLE_MeshBounds_ZMin=0.0, FTLE_MeshBounds_ZMax=10.0, FTLE_MeshBounds_XRes=12,
+                                       FTLE_MeshBounds_YRes=12, FTLE_MeshBounds_ZRes=30, FTLE_IntTLength=0.5,
+                                       FTLE_OutFilePrefix="pfx", FTLE_ICFile="pfx_ic.bin")),
     # 3-D unstructured FTLE with outside masks and early exits, 2 slides
     "pipe_ftle": dict(gen=gen_pipe, outputs=["pft.0.bin", "pft.1.bin", "pft_noT.0.bin", "pft_noT.1.bin",
                                              "pft_Cartesian.bin"],

@@ -34,7 +34,8 @@ def run_driver(case, tmp_path, precision="f64"):
 @pytest.mark.parametrize("case", ["dg_tracers", "pipe_tracers", "pipe_locate", "pipe_staggered",
                                   "pipe_staggered_steady", "pipe_velout_grid", "pipe_velout_list",
                                   "dg_velout_list", "pipe_tracers_ap", "pipe_tracers_ap2", "dg_tracers_ap",
-                                  "pipe_tracers_cet", "pipe_tracers_rt"])
+                                  "pipe_tracers_cet", "pipe_tracers_rt", "dg_tracers_euler", "pipe_tracers_euler",
+                                  "pipe_tracers_extrap"])
 def test_gpu_trajectories_bit_identical(case, tmp_path):
     man, data, out = run_driver(case, tmp_path)
     fx = cases.fixture_dir(case)
@@ -65,7 +66,7 @@ def test_gpu_trajectories_bit_identical(case, tmp_path):
         np.testing.assert_array_equal(ra["LeftDomainTime"], rb["LeftDomainTime"])
 
 
-@pytest.mark.parametrize("case", ["dg_ftle", "dg_ftle_back", "pipe_ftle"])
+@pytest.mark.parametrize("case", ["dg_ftle", "dg_ftle_back", "pipe_ftle", "dg_ftle_euler", "pipe_ftle_extrap"])
 def test_gpu_ftle_matches_reference(case, tmp_path):
     man, data, out = run_driver(case, tmp_path)
     fx = cases.fixture_dir(case)
