@@ -142,6 +142,102 @@ def gen_dg_velout(d):
             f.write("%.17g %.17g\n" % tuple(q))
 
 
+def write_cart_header(path, mn, mx, res):
+    """Cartesian mesh header: XMin, XMax (double), XRes (int), then y, then z
+    (mesh.c:67-122)."""
+    with open(path, "wb") as f:
+        for a in range(3):
+            np.array([mn[a], mx[a]], "<f8").tofile(f)
+            np.array([res[a]], "<i4").tofile(f)
+
+
+BOX_RES = (25, 13, 13)
+
+
+def box_velocity(t):
+    """3-D unsteady field on [0,2]x[0,1]^2 (a double gyre in x-y, stretched and
+    sheared in z), rows in k, j, i order with i fastest (velocity.c:129-140).
+    It leaves through z=1 where cos(pi x) > 0 and through x at the z-shear."""
+    nx, ny, nz = BOX_RES
+    z, y, x = np.meshgrid(np.linspace(0.0, 1.0, nz), np.linspace(0.0, 1.0, ny), np.linspace(0.0, 2.0, nx),
+                          indexing="ij")
+    A, eps, om = 0.1, 0.25, 2 * np.pi / 10
+    a, b = eps * np.sin(om * t), 1 - 2 * eps * np.sin(om * t)
+    f, df = a * x * x + b * x, 2 * a * x + b
+    u = -np.pi * A * np.sin(np.pi * f) * np.cos(np.pi * y) + 0.05 * np.cos(np.pi * z)
+    v = np.pi * A * np.cos(np.pi * f) * np.sin(np.pi * y) * df
+    w = 0.1 * np.cos(np.pi * x) + 0.05 * np.sin(2 * np.pi * z) * (1 + 0.5 * np.sin(om * t))
+    return np.stack([u.ravel(), v.ravel(), w.ravel()], 1)
+
+
+def gen_box(d):
+    write_cart_header(os.path.join(d, "box_Cartesian.bin"), [0.0, 0.0, 0.0], [2.0, 1.0, 1.0], BOX_RES)
+    for s in range(11):
+        host.write_vel_slice(os.path.join(d, "box_vel.%d.bin" % s), 0.5 * s, box_velocity(0.5 * s))
+
+
+CHAN_CELLS = (32, 8)
+
+
+def chan_mesh():
+    """2-D channel [0,4]x[0,1] of jittered triangles (two per cell, diagonals
+    alternating).  Slot order of the adjacency (mesh.c:593-600): [0] across the
+    edge opposite node 1, [1] opposite node 2, [2] opposite node 0."""
+    nx, ny = CHAN_CELLS
+    X, Y = np.meshgrid(np.linspace(0.0, 4.0, nx + 1), np.linspace(0.0, 1.0, ny + 1), indexing="ij")
+    rng = np.random.default_rng(21)
+    inner = (np.arange(nx + 1)[:, None] % nx != 0) & (np.arange(ny + 1)[None, :] % ny != 0)
+    X = X + inner * rng.uniform(-0.2, 0.2, X.shape) * (4.0 / nx)
+    Y = Y + inner * rng.uniform(-0.2, 0.2, Y.shape) * (1.0 / ny)
+    xyz = np.stack([X.ravel(), Y.ravel(), np.zeros(X.size)], 1)
+    tris = []
+    for i in range(nx):
+        for j in range(ny):
+            a, b, c, e = i * (ny + 1) + j, (i + 1) * (ny + 1) + j, (i + 1) * (ny + 1) + j + 1, i * (ny + 1) + j + 1
+            tris += [[a, b, c], [a, c, e]] if (i + j) % 2 == 0 else [[a, b, e], [b, c, e]]
+    edges = {}
+    for k, (n0, n1, n2) in enumerate(tris):
+        for slot, (p, q) in enumerate(((n0, n2), (n0, n1), (n1, n2))):
+            edges.setdefault((min(p, q), max(p, q)), []).append((k, slot))
+    adj = -np.ones((len(tris), 4), np.int32)
+    for lst in edges.values():
+        if len(lst) == 2:
+            (k1, s1), (k2, s2) = lst
+            adj[k1, s1], adj[k2, s2] = k2, k1
+    conn = np.concatenate([np.array(tris), np.zeros((len(tris), 1), np.int64)], 1).astype(np.int32)
+    return xyz, conn, adj
+
+
+def chan_velocity(xyz, t):
+    """Pulsatile channel flow with a cross-stream wave (zero on the walls)."""
+    x, y = xyz[:, 0], xyz[:, 1]
+    u = 6.0 * y * (1.0 - y) * (1.0 + 0.5 * np.sin(2 * np.pi * t))
+    v = 0.1 * np.sin(np.pi * x) * np.sin(np.pi * y) * np.cos(2 * np.pi * t)
+    return np.stack([u, v, np.zeros_like(u)], 1)
+
+
+def gen_chan(d):
+    xyz, conn, adj = chan_mesh()
+    host.write_unstruct_mesh(d, "chan", xyz, conn, adj)
+    for s in range(21):
+        host.write_vel_slice(os.path.join(d, "chan_vel.%d.bin" % s), 0.05 * s, chan_velocity(xyz, 0.05 * s))
+    rng = np.random.default_rng(17)
+    pts = rng.uniform([0.1, 0.01], [3.9, 0.99], size=(600, 2))
+    # nodes and edge midpoints: points held by several triangles, where the chain's guesses decide
+    pts = np.concatenate([pts, xyz[::7, :2], 0.5 * (xyz[conn[::5, 0], :2] + xyz[conn[::5, 1], :2]),
+                          [[4.5, 0.5], [-0.1, 0.5], [2.0, 1.2]]])
+    host.write_binary_list(os.path.join(d, "seeds.bin"), pts, dims=2)
+
+
+BOX = dict(Dimensions=3, Data_MeshType=0, Data_InFilePrefix="box", Data_TRes=11, Data_TDelta=0.5, Data_TMin=0.0,
+           Data_TPeriodic=0, Data_MeshBounds_XMin=0.0, Data_MeshBounds_XMax=2.0, Data_MeshBounds_YMin=0.0,
+           Data_MeshBounds_YMax=1.0, Data_MeshBounds_ZMin=0.0, Data_MeshBounds_ZMax=1.0, Int_Type=1,
+           Int_TimeStep=0.01)
+CHAN = dict(Dimensions=2, Data_MeshType=1, Data_InFilePrefix="chan", Data_TRes=21, Data_TDelta=0.05, Data_TMin=0.0,
+            Data_TPeriodic=0, Data_MeshBounds_XMin=0.0, Data_MeshBounds_XMax=4.0, Data_MeshBounds_YMin=0.0,
+            Data_MeshBounds_YMax=1.0, Data_MeshBounds_ZMin=0.0, Data_MeshBounds_ZMax=0.0, Int_Type=1,
+            Int_TimeStep=0.005, LocalSearchChecking=1)
+
 VELOUT = {"VelOut_Compute": 1, "VelOut_MeshBounds_XMin": -1.0, "VelOut_MeshBounds_XMax": 1.0,
           "VelOut_MeshBounds_YMin": -1.0, "VelOut_MeshBounds_YMax": 1.0, "VelOut_MeshBounds_ZMin": 0.0,
           "VelOut_MeshBounds_ZMax": 10.0}
@@ -298,6 +394,38 @@ CASES = {
                                        FTLE_MeshBounds_ZMin=0.0, FTLE_MeshBounds_ZMax=10.0, FTLE_MeshBounds_XRes=12,
                                        FTLE_MeshBounds_YRes=12, FTLE_MeshBounds_ZRes=30, FTLE_IntTLength=0.5,
                                        FTLE_OutFilePrefix="pfx", FTLE_ICFile="pfx_ic.bin")),
+    # 3-D Cartesian tracers (trilinear interpolation, velocity.c:330-456) from a generated 16x16x10 grid
+    # (2560 records: see dg_tracers for the reference's unset LeftDomainTime), exits through z=1
+    "box_tracers": dict(gen=gen_box, outputs=["btr.%d.bin" % k for k in range(6)],
+                        deck=dict(BOX, Output_TStart=0.0, Output_TRes=6, Output_TDelta=1.0, Int_TimeDirection=1,
+                                  Trace_Compute=1, Trace_GenerateMesh=1, Trace_MeshBounds_XMin=0.05,
+                                  Trace_MeshBounds_XMax=1.95, Trace_MeshBounds_YMin=0.05, Trace_MeshBounds_YMax=0.95,
+                                  Trace_MeshBounds_ZMin=0.05, Trace_MeshBounds_ZMax=0.95, Trace_MeshBounds_XRes=16,
+                                  Trace_MeshBounds_YRes=16, Trace_MeshBounds_ZRes=10, Trace_IntTLength=5.0,
+                                  Trace_OutFilePrefix="btr")),
+    # 3-D Cartesian FTLE on a 24x12x12 grid over the whole box (points on the walls), early exits, 2 slides
+    "box_ftle": dict(gen=gen_box, outputs=["bft.0.bin", "bft.1.bin", "bft_noT.0.bin", "bft_noT.1.bin",
+                                           "bft_Cartesian.bin"],
+                     deck=dict(BOX, Output_TStart=0.0, Output_TRes=2, Output_TDelta=0.5, Int_TimeDirection=1,
+                               FTLE_Compute=1, FTLE_MeshBounds_XMin=0.0, FTLE_MeshBounds_XMax=2.0,
+                               FTLE_MeshBounds_YMin=0.0, FTLE_MeshBounds_YMax=1.0, FTLE_MeshBounds_ZMin=0.0,
+                               FTLE_MeshBounds_ZMax=1.0, FTLE_MeshBounds_XRes=24, FTLE_MeshBounds_YRes=12,
+                               FTLE_MeshBounds_ZRes=12, FTLE_IntTLength=2.0, FTLE_OutFilePrefix="bft",
+                               FTLE_ICFile="bft_ic.bin")),
+    # 2-D unstructured tracers (triangles, velocity.c:570-620, mesh.c:566-605): random seeds, seeds on nodes and
+    # edges, seeds outside; exits through the outlet
+    "chan_tracers": dict(gen=gen_chan, outputs=["ctr.%d.bin" % k for k in range(6)], ic="ctr.IC",
+                         deck=dict(CHAN, Output_TStart=0.0, Output_TRes=6, Output_TDelta=0.2, Int_TimeDirection=1,
+                                   Trace_Compute=1, Trace_GenerateMesh=0, Trace_InFile="seeds.bin",
+                                   Trace_InFileFormat=4, Trace_IntTLength=1.0, Trace_OutFilePrefix="ctr")),
+    # 2-D unstructured FTLE, 48x12 grid (points on the walls and on mesh edges), backward time, 2 slides
+    "chan_ftle": dict(gen=gen_chan, outputs=["cft.0.bin", "cft.1.bin", "cft_noT.0.bin", "cft_noT.1.bin",
+                                             "cft_Cartesian.bin"],
+                      deck=dict(CHAN, Output_TStart=1.0, Output_TRes=2, Output_TDelta=0.1, Int_TimeDirection=-1,
+                                FTLE_Compute=1, FTLE_MeshBounds_XMin=0.0, FTLE_MeshBounds_XMax=4.0,
+                                FTLE_MeshBounds_YMin=0.0, FTLE_MeshBounds_YMax=1.0, FTLE_MeshBounds_XRes=49,
+                                FTLE_MeshBounds_YRes=13, FTLE_MeshBounds_ZRes=1, FTLE_IntTLength=0.5,
+                                FTLE_OutFilePrefix="cft", FTLE_ICFile="cft_ic.bin")),
     # 3-D unstructured FTLE with outside masks and early exits, 2 slides
     "pipe_ftle": dict(gen=gen_pipe, outputs=["pft.0.bin", "pft.1.bin", "pft_noT.0.bin", "pft_noT.1.bin",
                                              "pft_Cartesian.bin"],

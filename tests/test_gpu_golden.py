@@ -35,7 +35,7 @@ def run_driver(case, tmp_path, precision="f64"):
                                   "pipe_staggered_steady", "pipe_velout_grid", "pipe_velout_list",
                                   "dg_velout_list", "pipe_tracers_ap", "pipe_tracers_ap2", "dg_tracers_ap",
                                   "pipe_tracers_cet", "pipe_tracers_rt", "dg_tracers_euler", "pipe_tracers_euler",
-                                  "pipe_tracers_extrap"])
+                                  "pipe_tracers_extrap", "box_tracers", "chan_tracers"])
 def test_gpu_trajectories_bit_identical(case, tmp_path):
     man, data, out = run_driver(case, tmp_path)
     fx = cases.fixture_dir(case)
@@ -66,7 +66,8 @@ def test_gpu_trajectories_bit_identical(case, tmp_path):
         np.testing.assert_array_equal(ra["LeftDomainTime"], rb["LeftDomainTime"])
 
 
-@pytest.mark.parametrize("case", ["dg_ftle", "dg_ftle_back", "pipe_ftle", "dg_ftle_euler", "pipe_ftle_extrap"])
+@pytest.mark.parametrize("case", ["dg_ftle", "dg_ftle_back", "pipe_ftle", "dg_ftle_euler", "pipe_ftle_extrap",
+                                  "box_ftle", "chan_ftle"])
 def test_gpu_ftle_matches_reference(case, tmp_path):
     man, data, out = run_driver(case, tmp_path)
     fx = cases.fixture_dir(case)
@@ -81,13 +82,14 @@ def test_gpu_ftle_matches_reference(case, tmp_path):
         np.testing.assert_allclose(b, a, rtol=1e-12, atol=1e-12, err_msg=f)
 
 
-@pytest.mark.parametrize("case,precision", [("pipe_tracers", "f32"), ("dg_tracers", "f32")])
+@pytest.mark.parametrize("case,precision", [("pipe_tracers", "f32"), ("dg_tracers", "f32"), ("box_tracers", "f32"),
+                                            ("chan_tracers", "f32")])
 def test_gpu_f32_driver_within_tolerance(case, tmp_path, precision):
     """FP32 velocity/geometry with FP64 state: final positions within 1e-5 of the
     domain size of the reference's FP64 trajectories (points present in both)."""
     man, data, out = run_driver(case, tmp_path, precision)
     fx = cases.fixture_dir(case)
-    size = 10.0 if case.startswith("pipe") else 2.0
+    size = {"pipe": 10.0, "dg": 2.0, "box": 2.0, "chan": 4.0}[case.split("_")[0]]
     last = sorted(man["outputs"], key=lambda s: int(s.split(".")[-2]))[-1]
     _, a = host.read_tracer_output(os.path.join(fx, last))
     _, b = host.read_tracer_output(os.path.join(out, last))

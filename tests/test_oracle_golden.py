@@ -27,7 +27,7 @@ def run_oracle(case, tmp_path, oracle_dir):
 
 
 @pytest.mark.parametrize("case", ["dg_tracers", "pipe_tracers", "pipe_locate", "dg_tracers_euler",
-                                  "pipe_tracers_euler", "pipe_tracers_extrap"])
+                                  "pipe_tracers_euler", "pipe_tracers_extrap", "box_tracers", "chan_tracers"])
 def test_trajectories_bit_identical(case, tmp_path, oracle_built, host_built):
     man, data, out = run_oracle(case, tmp_path, oracle_built)
     fx = cases.fixture_dir(case)
@@ -44,7 +44,8 @@ def test_trajectories_bit_identical(case, tmp_path, oracle_built, host_built):
         np.testing.assert_array_equal(ra["X"], rb["X"])
 
 
-@pytest.mark.parametrize("case", ["dg_ftle", "dg_ftle_back", "pipe_ftle", "dg_ftle_euler", "pipe_ftle_extrap"])
+@pytest.mark.parametrize("case", ["dg_ftle", "dg_ftle_back", "pipe_ftle", "dg_ftle_euler", "pipe_ftle_extrap",
+                                  "box_ftle", "chan_ftle"])
 def test_ftle_matches_reference(case, tmp_path, oracle_built, host_built):
     man, data, out = run_oracle(case, tmp_path, oracle_built)
     fx = cases.fixture_dir(case)
