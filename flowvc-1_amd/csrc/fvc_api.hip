@@ -650,6 +650,16 @@ extern "C" int64_t fvc_particles_capacity(const fvc_particles *p) { return p ? p
 extern "C" int fvc_particles_set(fvc_particles *p, int64_t begin, const fvc_point *recs, int64_t n) {
     if (!p || !recs || begin < 0 || n < 0 || begin + n > p->cap) return FVC_E_INVALID;
     fvc_ctx *c = p->ctx;
+    /* element ids start walks on the device: outside [-1, ne) they would
+     * address memory beyond the mesh (-1 is ompVC's "no element" and raises
+     * FVC_E_ELEM_NEG1 when such a point is integrated).  Points flagged as
+     * left hold no element: ompVC leaves the id of points outside the domain
+     * unset in its stored grids (ftle.c:425-455), so it is kept as -1 here. */
+    const bool uns = c->mesh.mesh_type == 1;
+    if (uns)
+        for (int64_t i = 0; i < n; i++)
+            if (!recs[i].LeftDomain && (recs[i].ElementIndex < -1 || recs[i].ElementIndex >= c->mesh.ne))
+                return set_err(c, FVC_E_INVALID, "particle record with an element id outside the mesh");
     std::vector<double> d(8 * (size_t)n);
     std::vector<int> iv(3 * (size_t)n);
     for (int64_t i = 0; i < n; i++) {
@@ -659,7 +669,7 @@ extern "C" int fvc_particles_set(fvc_particles *p, int64_t begin, const fvc_poin
         }
         d[6 * n + i] = recs[i].LeftDomainTime;
         d[7 * n + i] = recs[i].Scalar;
-        iv[i] = recs[i].ElementIndex;
+        iv[i] = (uns && recs[i].LeftDomain) ? -1 : recs[i].ElementIndex;
         iv[n + i] = recs[i].AuxElementIndex;
         iv[2 * n + i] = recs[i].LeftDomain;
     }
@@ -918,28 +928,18 @@ __global__ void k_ftle_grid(double *X0, double *X1, double *X2, int *elem, int *
     lt[P] = clt[P];
 }
 
-extern "C" int fvc_ftle_create(fvc_ctx *c, const double mn[3], const double mx[3], const int32_t res[3], int32_t lsc,
-                               double start_time, double stop_time, fvc_ftle **out) {
-    if (!c || !mn || !mx || !res || !out) return FVC_E_INVALID;
-    *out = nullptr;
-    const int dims = c->mesh.dims;
-    if (res[0] < 2 || res[1] < 2 || (dims == 3 && res[2] < 2)) return FVC_E_INVALID;
+/* Storage of one slide of res[0]*res[1]*res[2] grid points: particle
+ * records, per-point FTLE values and flags, one frame's staging and the
+ * grid's Z-order. */
+static int ftle_alloc(fvc_ctx *c, const double mn[3], const double del[3], const int32_t res[3], double start_time,
+                      double stop_time, fvc_ftle **out) {
     fvc_ftle *f = new fvc_ftle();
     f->ctx = c;
     for (int a = 0; a < 3; a++) {
         f->mn[a] = mn[a];
-        f->mx[a] = mx[a];
+        f->del[a] = del[a];
         f->res[a] = res[a];
-    }
-    /* parameters.c:310-320 */
-    f->del[0] = (f->mx[0] - f->mn[0]) / (f->res[0] - 1);
-    f->del[1] = (f->mx[1] - f->mn[1]) / (f->res[1] - 1);
-    if (dims == 3)
-        f->del[2] = (f->mx[2] - f->mn[2]) / (f->res[2] - 1);
-    else {
-        f->mn[2] = f->mx[2] = 0.0;
-        f->del[2] = 0.0;
-        f->res[2] = 1;
+        f->mx[a] = mn[a] + (res[a] - 1) * del[a];
     }
     f->n = (int64_t)f->res[0] * f->res[1] * f->res[2];
     f->start_time = start_time;
@@ -975,6 +975,56 @@ extern "C" int fvc_ftle_create(fvc_ctx *c, const double mn[3], const double mx[3
         fvc_ftle_destroy(f);
         return rc;
     }
+    *out = f;
+    return FVC_OK;
+}
+
+/* Snapshot of the initial state (for fvc_ftle_reset) and the count of
+ * points inside. */
+static int ftle_finish(fvc_ftle *f) {
+    fvc_ctx *c = f->ctx;
+    fvc_particles *p = f->pts;
+    size_t n = (size_t)f->n;
+    int rc;
+    if ((rc = fvc_particles_create(c, f->n, &f->init_pts)) || (rc = particles_d2d(f->init_pts, 0, p, 0, f->n)))
+        return rc;
+    if (hipMalloc((void **)&f->init_have, sizeof(int) * n) != hipSuccess ||
+        hipMalloc((void **)&f->init_w, sizeof(double) * 2 * n) != hipSuccess)
+        return set_err(c, FVC_E_NOMEM, "FTLE snapshot alloc");
+    c->bytes_used += (int64_t)(n * (4 + 16));
+    hipMemcpyAsync(f->init_have, f->have, sizeof(int) * n, hipMemcpyDeviceToDevice, c->stream);
+    hipMemcpyAsync(f->init_w, f->wt, sizeof(double) * 2 * n, hipMemcpyDeviceToDevice, c->stream);
+    /* points inside = not left */
+    hipStreamSynchronize(c->stream);
+    std::vector<int> lh(n);
+    hipMemcpy(lh.data(), p->left, sizeof(int) * n, hipMemcpyDeviceToHost);
+    int64_t inside = 0;
+    for (size_t q = 0; q < n; q++) inside += !lh[q];
+    f->inside = inside;
+    return FVC_OK;
+}
+
+extern "C" int fvc_ftle_create(fvc_ctx *c, const double mn[3], const double mx[3], const int32_t res[3], int32_t lsc,
+                               double start_time, double stop_time, fvc_ftle **out) {
+    if (!c || !mn || !mx || !res || !out) return FVC_E_INVALID;
+    *out = nullptr;
+    const int dims = c->mesh.dims;
+    if (res[0] < 2 || res[1] < 2 || (dims == 3 && res[2] < 2)) return FVC_E_INVALID;
+    double g_mn[3] = {mn[0], mn[1], mn[2]}, del[3];
+    int32_t g_res[3] = {res[0], res[1], res[2]};
+    /* parameters.c:310-320 */
+    del[0] = (mx[0] - mn[0]) / (res[0] - 1);
+    del[1] = (mx[1] - mn[1]) / (res[1] - 1);
+    if (dims == 3)
+        del[2] = (mx[2] - mn[2]) / (res[2] - 1);
+    else {
+        g_mn[2] = 0.0;
+        del[2] = 0.0;
+        g_res[2] = 1;
+    }
+    fvc_ftle *f = nullptr;
+    int rc = ftle_alloc(c, g_mn, del, g_res, start_time, stop_time, &f);
+    if (rc) return rc;
     /* location chain of InitializeFTLEArray, then the grid records */
     int *ce = f->elem_new, *cl = f->exited; /* reuse staging as chain output */
     double *clt = f->exit_time;
@@ -987,32 +1037,44 @@ extern "C" int fvc_ftle_create(fvc_ctx *c, const double mn[3], const double mx[3
     k_ftle_grid<<<(unsigned)((f->n + 255) / 256), 256, 0, c->stream>>>(
         p->X[0], p->X[1], p->X[2], p->elem, p->left, p->left_time, f->n, f->mn[0], f->mn[1], f->mn[2], f->del[0],
         f->del[1], f->del[2], f->res[1], f->res[2], ce, cl, clt);
-    if ((rc = launch_ftle_mask(f))) {
+    if ((rc = launch_ftle_mask(f)) || (rc = ftle_finish(f))) {
         fvc_ftle_destroy(f);
         return rc;
     }
-    /* snapshot of the initial state for fvc_ftle_reset */
-    if ((rc = fvc_particles_create(c, f->n, &f->init_pts)) || (rc = particles_d2d(f->init_pts, 0, p, 0, f->n))) {
-        fvc_ftle_destroy(f);
-        return rc;
-    }
-    if (hipMalloc((void **)&f->init_have, sizeof(int) * n) != hipSuccess ||
-        hipMalloc((void **)&f->init_w, sizeof(double) * 2 * n) != hipSuccess) {
-        fvc_ftle_destroy(f);
-        return set_err(c, FVC_E_NOMEM, "FTLE snapshot alloc");
-    }
-    c->bytes_used += (int64_t)(n * (4 + 16));
-    hipMemcpyAsync(f->init_have, f->have, sizeof(int) * n, hipMemcpyDeviceToDevice, c->stream);
-    hipMemcpyAsync(f->init_w, f->wt, sizeof(double) * 2 * n, hipMemcpyDeviceToDevice, c->stream);
-    /* points inside = not left */
-    hipStreamSynchronize(c->stream);
-    std::vector<int> lh(n);
-    hipMemcpy(lh.data(), p->left, sizeof(int) * n, hipMemcpyDeviceToHost);
-    int64_t inside = 0;
-    for (size_t q = 0; q < n; q++) inside += !lh[q];
-    f->inside = inside;
     *out = f;
     return check_err(c, "fvc_ftle_create");
+}
+
+extern "C" int fvc_ftle_create_from(fvc_ctx *c, const double mn[3], const double del[3], const int32_t res[3],
+                                    const fvc_point *recs, const int32_t *have, const double *ftle_wt,
+                                    const double *ftle_wot, double start_time, double stop_time, fvc_ftle **out) {
+    if (!c || !mn || !del || !res || !recs || !have || !ftle_wt || !ftle_wot || !out) return FVC_E_INVALID;
+    *out = nullptr;
+    const int dims = c->mesh.dims;
+    if (res[0] < 1 || res[1] < 1 || res[2] < 1 || (dims == 2 && res[2] != 1)) return FVC_E_INVALID;
+    const int64_t n = (int64_t)res[0] * res[1] * res[2];
+    if (n > 0x7fffffffLL) return FVC_E_INVALID;
+    fvc_ftle *f = nullptr;
+    int rc = ftle_alloc(c, mn, del, res, start_time, stop_time, &f);
+    if (rc) return rc;
+    std::vector<int> hv((size_t)n);
+    for (int64_t q = 0; q < n; q++) hv[q] = have[q] != 0;
+    if ((rc = fvc_particles_set(f->pts, 0, recs, n))) {
+        fvc_ftle_destroy(f);
+        return rc;
+    }
+    if (hipMemcpy(f->have, hv.data(), sizeof(int) * n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(f->wt, ftle_wt, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(f->wot, ftle_wot, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess) {
+        fvc_ftle_destroy(f);
+        return set_err(c, FVC_E_HIP, "FTLE state upload failed");
+    }
+    if ((rc = ftle_finish(f))) {
+        fvc_ftle_destroy(f);
+        return rc;
+    }
+    *out = f;
+    return check_err(c, "fvc_ftle_create_from");
 }
 
 extern "C" void fvc_ftle_destroy(fvc_ftle *f) {
@@ -1116,6 +1178,20 @@ extern "C" int fvc_ftle_points(fvc_ftle *f, int64_t *n, int64_t *inside) {
 extern "C" int fvc_ftle_records(fvc_ftle *f, fvc_point *recs) {
     if (!f || !recs) return FVC_E_INVALID;
     return fvc_particles_get(f->pts, 0, recs, f->n);
+}
+
+extern "C" int fvc_ftle_initial(fvc_ftle *f, fvc_point *recs, int32_t *have, double *ftle_wt, double *ftle_wot) {
+    if (!f || !f->init_pts) return FVC_E_INVALID;
+    fvc_ctx *c = f->ctx;
+    const size_t n = (size_t)f->n;
+    int rc;
+    if (recs && (rc = fvc_particles_get(f->init_pts, 0, recs, f->n))) return rc;
+    std::vector<double> w(2 * n);
+    FVC_HIP(hipMemcpy(w.data(), f->init_w, sizeof(double) * 2 * n, hipMemcpyDeviceToHost));
+    if (have) FVC_HIP(hipMemcpy(have, f->init_have, sizeof(int) * n, hipMemcpyDeviceToHost));
+    if (ftle_wt) memcpy(ftle_wt, w.data(), sizeof(double) * n);
+    if (ftle_wot) memcpy(ftle_wot, w.data() + n, sizeof(double) * n);
+    return check_err(c, "fvc_ftle_initial");
 }
 
 /* ------------------------------------------------------------------------ */

@@ -364,7 +364,8 @@ int launch_locate(fvc_ctx *c, const double *dxyz, const int *dguess, int64_t n, 
 /* seed chains                                                                */
 /* ------------------------------------------------------------------------ */
 /* per-point chain flags: outside the bounding box, inside more than one
- * element, and (FTLE chain) the first / last point of a k-row or j-plane */
+ * element, and (grid chains) row and plane starts and the row / plane ends
+ * where the guess is reset */
 enum { SF_OUTB = 1, SF_MULTI = 2, SF_ROWSTART = 4, SF_PLANESTART = 8, SF_ROWEND = 16, SF_PLANEEND = 32 };
 
 struct ChainGeom {
@@ -374,10 +375,12 @@ struct ChainGeom {
     int res[3];
 };
 
-/* Chains 2 and 3 walk rows and planes with the guess reset to the row's
- * (plane's) first found element at each row (plane) end: chain 2 is the FTLE
- * grid (z fastest, ftle.c:456-516), chain 3 the velocity output grid (x
- * fastest, velout.c:58-163). */
+/* Chains 2 and 3 walk rows and planes and reset the guess at row ends.
+ * Chain 3, the velocity output grid (x fastest, velout.c:58-163), sets it to
+ * the row's first found element after every row, then to the plane's after
+ * every plane.  Chain 2, the FTLE grid (z fastest, ftle.c:456-601), sets it
+ * only after each plane, to the first found element of the plane's last row
+ * (its plane guess is applied after the whole grid, when no point follows). */
 __host__ __device__ inline int64_t chain_rowlen(const ChainGeom &g) { return g.chain == 2 ? g.res[2] : g.res[0]; }
 
 /* chain-order point i -> coordinates (ftle.c:412-420, tracers.c:1344-1346, velout.c:67-69) */
@@ -412,8 +415,8 @@ __device__ __forceinline__ int sf_count(int flags) { return (flags >> SF_CNT_SHI
 struct TabPrep {
     int *mflag;      /* 1 where a found point lies in more than one element */
     int *misc;       /* [0] first found point, [1] largest element count */
-    int *rowfirst;   /* FTLE chain: first found point of each k-row */
-    int *planefirst; /* FTLE chain: first found point of each j-plane */
+    int *rowfirst;   /* grid chains: first found point of each row */
+    int *planefirst; /* grid chains: first found point of each plane */
 };
 
 template <int DIM>
@@ -429,7 +432,12 @@ __global__ __launch_bounds__(BLOCK) void k_seed_prepare(DevMesh m, ChainGeom g, 
             const int64_t rl = chain_rowlen(g);
             const int kk = (int)(i % rl), jj = (int)((i / rl) % g.res[1]);
             if (kk == 0) flags |= SF_ROWSTART | (jj == 0 ? SF_PLANESTART : 0);
-            if (kk == rl - 1) flags |= SF_ROWEND | (jj == g.res[1] - 1 ? SF_PLANEEND : 0);
+            if (kk == rl - 1) {
+                if (g.chain == 3)
+                    flags |= SF_ROWEND | (jj == g.res[1] - 1 ? SF_PLANEEND : 0);
+                else if (jj == g.res[1] - 1) /* FTLE: one reset per plane, from its last row */
+                    flags |= SF_ROWEND;
+            }
         }
         if (outside_domain(m, X[0], X[1], X[2]))
             flags |= SF_OUTB;
@@ -463,7 +471,7 @@ __global__ __launch_bounds__(BLOCK) void k_seed_prepare(DevMesh m, ChainGeom g, 
 /* One wave replays the chain over chunks of 64 consecutive points.  A chunk
  * whose results cannot depend on the chain state (every point outside, or
  * inside exactly one element with LocalSearchChecking, after the first found
- * point, and no FTLE row boundary) is resolved by all lanes at once and the
+ * point, and no row boundary of a grid chain) is resolved by all lanes at once and the
  * state (last found element, first found element of the row and plane) is
  * advanced with ballots; any other chunk is replayed point by point. */
 constexpr int CHAIN_STAGE = 1024; /* points staged through LDS per round */
@@ -488,7 +496,7 @@ __global__ __launch_bounds__(CHAIN_STAGE) void k_seed_chain(DevMesh m, ChainGeom
     const bool uns = m.mesh_type == 1;
     int found = 0;                 /* list chain */
     int seed = seed0, guess = seed0 >= 0 ? seed0 : -1;
-    int iguess = -1, jguess = -1;  /* FTLE chain */
+    int iguess = -1, jguess = -1;  /* grid chains */
     for (int64_t stage = 0; stage < n; stage += CHAIN_STAGE) {
         {
             const int64_t i = stage + threadIdx.x;
@@ -539,7 +547,8 @@ __global__ __launch_bounds__(CHAIN_STAGE) void k_seed_chain(DevMesh m, ChainGeom
                 const bool state_ok = g.chain == 0 ? found > 0 : seed >= 0;
                 if (s_flag[ck][0] && (state_ok || s_flag[ck][1])) {
                     /* advance the state segment by segment: a segment ends at a
-                     * row end (FTLE chain), so a row start can only open one */
+                     * row-end event (grid chains) or just before a row start, so a
+                     * row start can only open one */
                     const unsigned long long hits = s_mask[ck][0], rowend = s_mask[ck][1],
                                              rowstart = s_mask[ck][2], planestart = s_mask[ck][3],
                                              planeend = s_mask[ck][4];
@@ -548,7 +557,9 @@ __global__ __launch_bounds__(CHAIN_STAGE) void k_seed_chain(DevMesh m, ChainGeom
                     while (a < 64) {
                         const unsigned long long from_a = ~0ull << a;
                         const unsigned long long re = rowend & from_a;
-                        const int b = re ? __ffsll((long long)re) - 1 : 63;
+                        const unsigned long long rs = rowstart & (from_a << 1); /* row starts after a */
+                        int b = re ? __ffsll((long long)re) - 1 : 63;
+                        if (rs) b = min(b, __ffsll((long long)rs) - 2);
                         const unsigned long long seg = (b == 63 ? ~0ull : ((1ull << (b + 1)) - 1ull)) & from_a;
                         if ((rowstart >> a) & 1ull) {
                             jguess = -1;
@@ -777,8 +788,9 @@ constexpr int TAB_SEED = -2;  /* source: the centre seed element */
 constexpr int TAB_NONE = -3;  /* source: none (the first found point) */
 
 /* Source of every point: the last event before it in chain order that set
- * the guess, as a point index.  A found point sets it to itself; an FTLE row
- * end sets it to the row's first found point, a plane end to the plane's.
+ * the guess, as a point index.  A found point sets it to itself; a row-end
+ * event of a grid chain sets it to the row's first found point, a plane-end
+ * event to the plane's.
  * Within each 64-point chunk from ballots; -1 where no event precedes the
  * point inside its chunk.  chunk_last[c]: the chunk's last event, or -1. */
 __global__ __launch_bounds__(BLOCK) void k_tab_src(ChainGeom g, int64_t n, const int2 *__restrict__ info,

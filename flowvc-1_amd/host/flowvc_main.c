@@ -146,34 +146,74 @@ static void write_ftle(const fvc_deck *d, int outnum, double start, const double
     }
 }
 
-static double *read_seed_list(const fvc_deck *d, int64_t *n) {
-    char path[3 * FVC_STR];
-    snprintf(path, sizeof path, "%s%s", d->Path_Data, d->Trace_InFile);
+/* One seed file of ReadInTraceMesh (tracers.c:1010-1104): binary list, ASCII
+ * list, or legacy VTK (four header lines, "POINTS n type", coordinates). */
+static double *read_seed_file(const fvc_deck *d, const char *path, int64_t *n) {
     double *xyz = NULL;
-    if (d->Trace_InFileFormat == 4) { /* tracers.c:1020-1035 */
+    int cnt = 0;
+    if (d->Trace_InFileFormat == 4) { /* tracers.c:1012-1036 */
         FILE *f = fopen(path, "rb");
-        int32_t cnt;
-        if (!f || fread(&cnt, 4, 1, f) != 1) fatal("Could not open file %s", path);
-        xyz = (double *)calloc((size_t)3 * cnt, sizeof(double));
+        if (!f) fatal("Could not open file %s", path);
+        int32_t c32;
+        if (fread(&c32, 4, 1, f) != 1 || c32 < 0) fatal("Could not read Trace_NumTracers from %s", path);
+        cnt = c32;
+        xyz = (double *)calloc((size_t)3 * (cnt > 0 ? cnt : 1), sizeof(double));
         for (int q = 0; q < cnt; q++)
             if (fread(xyz + 3 * q, 8, (size_t)d->Dimensions, f) != (size_t)d->Dimensions)
                 fatal("Could not read tracer coordinates from %s", path);
         fclose(f);
-        *n = cnt;
-    } else if (d->Trace_InFileFormat == 1) { /* tracers.c:1048-1063 */
+    } else if (d->Trace_InFileFormat == 1) { /* tracers.c:1039-1066 */
         FILE *f = fopen(path, "r");
-        int cnt;
-        if (!f || fscanf(f, "%d", &cnt) != 1) fatal("Could not open file %s", path);
-        xyz = (double *)calloc((size_t)3 * cnt, sizeof(double));
+        if (!f) fatal("Could not open file %s", path);
+        if (fscanf(f, "%d", &cnt) != 1 || cnt < 0) fatal("Could not read Trace_NumTracers from %s", path);
+        xyz = (double *)calloc((size_t)3 * (cnt > 0 ? cnt : 1), sizeof(double));
         for (int q = 0; q < cnt; q++)
             for (int a = 0; a < d->Dimensions; a++)
                 if (fscanf(f, "%lf", xyz + 3 * q + a) != 1) fatal("Could not read tracer coordinates from %s", path);
         fclose(f);
-        *n = cnt;
+    } else if (d->Trace_InFileFormat == 2 || d->Trace_InFileFormat == 3) { /* tracers.c:1067-1100 */
+        if (d->Dimensions == 2) fatal("%s", "The specified Trace_InFileFormat is not currently supported for 2D analysis");
+        FILE *f = fopen(path, "r");
+        if (!f) fatal("Could not open file %s", path);
+        char buf[4096];
+        for (int q = 0; q < 4; q++)
+            if (!fgets(buf, sizeof buf, f)) fatal("%s", "Skipping of first four header lines failed.");
+        if (fscanf(f, "%*s %d %*s\n", &cnt) != 1 || cnt < 0) fatal("Could not read Trace_NumTracers from %s", path);
+        xyz = (double *)calloc((size_t)3 * (cnt > 0 ? cnt : 1), sizeof(double));
+        for (int q = 0; q < cnt; q++)
+            if (fscanf(f, "%lf %lf %lf", xyz + 3 * q, xyz + 3 * q + 1, xyz + 3 * q + 2) != 3)
+                fatal("Could not read tracer coordinates from %s", path);
+        fclose(f);
     } else {
-        fatal("Unsupported value for Trace_InFileFormat in %s", path);
+        fatal("%s", "Unsupported value for Trace_InFileFormat in ReadInTraceMesh()");
     }
+    *n = cnt;
     return xyz;
+}
+
+/* A release's stored records (the .IC files, tracers.c:1258-1276, read back
+ * by CreateNewTraceLaunch, tracers.c:1579-1631): int32 count, then
+ * LagrangianPoint records. */
+static fvc_point *read_ic_file(const char *path, int64_t *n) {
+    FILE *f = fopen(path, "rb");
+    if (!f) fatal("Could not open file %s", path);
+    int32_t cnt;
+    if (fread(&cnt, 4, 1, f) != 1 || cnt < 0) fatal("Could not read Trace_NumTracers from %s", path);
+    fvc_point *r = (fvc_point *)calloc((size_t)(cnt > 0 ? cnt : 1), sizeof(fvc_point));
+    if (fread(r, sizeof(fvc_point), (size_t)cnt, f) != (size_t)cnt)
+        fatal("Could not completely read TraceIC_Array data from %s", path);
+    fclose(f);
+    *n = cnt;
+    return r;
+}
+
+static void write_ic_file(const char *path, const fvc_point *ic, int64_t n) {
+    FILE *o = fopen(path, "wb");
+    if (!o) fatal("Could not open file %s", path);
+    int32_t n32 = (int32_t)n;
+    if (fwrite(&n32, 4, 1, o) != 1 || fwrite(ic, sizeof(fvc_point), (size_t)n, o) != (size_t)n)
+        fatal("Could not completely write TraceIC_Array to %s", path);
+    fclose(o);
 }
 
 /* Description of the seed set beside per-seed outputs (OutputRT, OutputAP:
@@ -817,12 +857,53 @@ int main(int argc, char **argv) {
 
     if (d.VelOut_Compute) run_velout(&d, c, &sl, quiet);
     if (d.FTLE_Compute) {
-        if (!d.FTLE_GenerateMesh) fatal("%s", "FTLE_GenerateMesh = 0 (IC reuse) is not supported by this driver");
         int ns = d.Output_TRes;
         fvc_ftle **f = (fvc_ftle **)calloc((size_t)ns, sizeof(fvc_ftle *));
         double mn[3] = {d.FTLE_Bounds[0], d.FTLE_Bounds[2], d.FTLE_Bounds[4]};
         double mx[3] = {d.FTLE_Bounds[1], d.FTLE_Bounds[3], d.FTLE_Bounds[5]};
         int32_t res[3] = {d.FTLE_Res[0], d.FTLE_Res[1], d.FTLE_Res[2]};
+        char icpath[3 * FVC_STR];
+        snprintf(icpath, sizeof icpath, "%s%s", d.Path_Data, d.FTLE_ICFile);
+        /* FTLE_GenerateMesh = 0: every slide starts from the records of FTLE_ICFile, whose resolution replaces
+         * the parameters' while origin and spacing stay (ftle.c:626-660) */
+        fvc_point *icp = NULL;
+        int32_t *ich = NULL;
+        double *icw = NULL, *icwo = NULL;
+        if (!d.FTLE_GenerateMesh) {
+            FILE *fi = fopen(icpath, "rb");
+            if (!fi) {
+                fprintf(stderr, "Error: Could not open file %s (file must be generated by flowVC with FTLE_GenerateMesh = 0)\n",
+                        icpath);
+                exit(1);
+            }
+            int32_t r3[3];
+            if (fread(r3, 4, 3, fi) != 3) fatal("Could not completely read FTLE grid resolution from %s", icpath);
+            if (r3[0] < 1 || r3[1] < 1 || r3[2] < 1 || (int64_t)r3[0] * r3[1] * r3[2] > 0x7fffffffLL)
+                fatal("FTLE grid resolution in %s out of range", icpath);
+            const int64_t ni = (int64_t)r3[0] * r3[1] * r3[2];
+            fvc_ftle_rec *rec = (fvc_ftle_rec *)malloc(sizeof(fvc_ftle_rec) * (size_t)ni);
+            icp = (fvc_point *)malloc(sizeof(fvc_point) * (size_t)ni);
+            ich = (int32_t *)malloc(sizeof(int32_t) * (size_t)ni);
+            icw = (double *)malloc(sizeof(double) * (size_t)ni);
+            icwo = (double *)malloc(sizeof(double) * (size_t)ni);
+            if (!rec || !icp || !ich || !icw || !icwo) fatal("%s", "Malloc failed for FTLE_MeshPt_temp");
+            if (fread(rec, sizeof(fvc_ftle_rec), (size_t)ni, fi) != (size_t)ni)
+                fatal("Could not completely read FTLE_MeshPt_temp data from %s", icpath);
+            fclose(fi);
+            for (int64_t q = 0; q < ni; q++) {
+                memcpy(&icp[q], &rec[q].Pt, sizeof(fvc_point));
+                ich[q] = rec[q].HaveFTLE;
+                icw[q] = rec[q].FTLEwT;
+                icwo[q] = rec[q].FTLEwoT;
+            }
+            free(rec);
+            d.FTLE_Cart.XRes = r3[0];
+            d.FTLE_Cart.YRes = r3[1];
+            d.FTLE_Cart.ZRes = r3[2];
+        }
+        const double gmn[3] = {d.FTLE_Cart.XMin, d.FTLE_Cart.YMin, d.FTLE_Cart.ZMin};
+        const double gdel[3] = {d.FTLE_Cart.XDelta, d.FTLE_Cart.YDelta, d.FTLE_Cart.ZDelta};
+        const int32_t gres[3] = {d.FTLE_Cart.XRes, d.FTLE_Cart.YRes, d.FTLE_Cart.ZRes};
         for (int ss = 0; ss < ns; ss++) {
             /* ftle.c:672-685 */
             double start = d.Output_TStart + ss * dir * d.Output_TDelta, stop;
@@ -835,8 +916,44 @@ int main(int argc, char **argv) {
             } else {
                 stop = d.Output_TStart + ss * dir * d.Output_TDelta + (double)dir * d.FTLE_IntTLength;
             }
-            check(fvc_ftle_create(c, mn, mx, res, d.LocalSearchChecking, start, stop, &f[ss]), c, "fvc_ftle_create");
+            if (d.FTLE_GenerateMesh)
+                check(fvc_ftle_create(c, mn, mx, res, d.LocalSearchChecking, start, stop, &f[ss]), c, "fvc_ftle_create");
+            else
+                check(fvc_ftle_create_from(c, gmn, gdel, gres, icp, ich, icw, icwo, start, stop, &f[ss]), c,
+                      "fvc_ftle_create_from");
         }
+        if (d.FTLE_GenerateMesh && ns > 0) { /* FTLE_ICFile: resolution, then the grid's records (ftle.c:611-625) */
+            int64_t ni = 0, nin = 0;
+            fvc_ftle_points(f[0], &ni, &nin);
+            fvc_point *pr = (fvc_point *)malloc(sizeof(fvc_point) * (size_t)(ni > 0 ? ni : 1));
+            int32_t *hv = (int32_t *)malloc(sizeof(int32_t) * (size_t)(ni > 0 ? ni : 1));
+            double *w1 = (double *)malloc(sizeof(double) * (size_t)(ni > 0 ? ni : 1)),
+                   *w2 = (double *)malloc(sizeof(double) * (size_t)(ni > 0 ? ni : 1));
+            fvc_ftle_rec *rec = (fvc_ftle_rec *)calloc((size_t)(ni > 0 ? ni : 1), sizeof(fvc_ftle_rec));
+            if (!pr || !hv || !w1 || !w2 || !rec) fatal("%s", "Malloc failed for FTLE_MeshPt_temp");
+            check(fvc_ftle_initial(f[0], pr, hv, w1, w2), c, "fvc_ftle_initial");
+            for (int64_t q = 0; q < ni; q++) {
+                memcpy(&rec[q].Pt, &pr[q], sizeof(fvc_point));
+                rec[q].HaveFTLE = hv[q];
+                rec[q].FTLEwT = w1[q];
+                rec[q].FTLEwoT = w2[q];
+            }
+            FILE *fo = fopen(icpath, "wb");
+            if (!fo) fatal("Could not open file %s", icpath);
+            const int32_t r3[3] = {gres[0], gres[1], gres[2]};
+            if (fwrite(r3, 4, 3, fo) != 3 || fwrite(rec, sizeof(fvc_ftle_rec), (size_t)ni, fo) != (size_t)ni)
+                fatal("Could not completely write FTLE_MeshPt_temp data to %s", icpath);
+            fclose(fo);
+            free(pr);
+            free(hv);
+            free(w1);
+            free(w2);
+            free(rec);
+        }
+        free(icp);
+        free(ich);
+        free(icw);
+        free(icwo);
         int64_t npts = 0, inside = 0;
         fvc_ftle_points(f[0], &npts, &inside);
         if (!quiet) printf("  %lld of %lld located in domain\n", (long long)inside, (long long)npts);
@@ -885,38 +1002,73 @@ int main(int argc, char **argv) {
     } else if (d.Trace_Compute && d.Trace_ReleaseStrategy == 1) {
         steps_total += run_staggered(&d, c, &sl, &ip, quiet, &t_compute);
     } else if (d.Trace_Compute) {
-        /* seeds: ReadInTraceMesh / GenerateTracerMesh location chains */
-        int64_t n = 0;
-        fvc_point *ic = NULL;
+        /* seeds: GenerateTracerMesh / ReadInTraceMesh location chains, or the
+         * stored .IC records (Trace_InFileFormat 0); with Trace_MultipleInFiles
+         * every release has its own seed file and .IC.<release> */
+        const int nl = d.Trace_NumLaunchTimes;
+        const int multi = !d.Trace_GenerateMesh && d.Trace_MultipleInFiles;
+        /* OutputAP / OutputRT read the single .IC file (strainrate.c:293, residencetime.c:54), which separate
+         * seed files never write; exposure times divide by one release's count (exposuretime.c:242) */
+        if (multi && (ap || rt || cet))
+            fatal("%s", "Trace_MultipleInFiles with activation potential, residence or exposure times is not "
+                        "supported by this driver");
+        fvc_point **ics = (fvc_point **)calloc((size_t)nl, sizeof(fvc_point *));
+        int64_t *rn = (int64_t *)calloc((size_t)nl, sizeof(int64_t)), *roff = (int64_t *)calloc((size_t)nl + 1, 8);
+        char icpath[3 * FVC_STR + 16];
+        snprintf(icpath, sizeof icpath, "%s%s.IC", d.Path_Data, d.Trace_OutFilePrefix);
         if (d.Trace_GenerateMesh) {
             double mn[3] = {d.Trace_Cart.XMin, d.Trace_Cart.YMin, d.Trace_Cart.ZMin};
             double del[3] = {d.Trace_Cart.XDelta, d.Trace_Cart.YDelta, d.Trace_Cart.ZDelta};
             int32_t res[3] = {d.Trace_Cart.XRes, d.Trace_Cart.YRes, d.Trace_Cart.ZRes};
-            n = (int64_t)res[0] * res[1] * res[2];
-            ic = (fvc_point *)calloc((size_t)n, sizeof(fvc_point));
-            check(fvc_seed_grid(c, mn, del, res, d.LocalSearchChecking, ic), c, "fvc_seed_grid");
-        } else {
-            if (d.Trace_MultipleInFiles || d.Trace_InFileFormat == 0)
-                fatal("%s", "Trace_MultipleInFiles / INITIALIZED input are not supported by this driver");
-            double *sx = read_seed_list(&d, &n);
-            ic = (fvc_point *)calloc((size_t)n, sizeof(fvc_point));
-            check(fvc_seed_list(c, sx, n, d.LocalSearchChecking, ic), c, "fvc_seed_list");
-            free(sx);
+            rn[0] = (int64_t)res[0] * res[1] * res[2];
+            ics[0] = (fvc_point *)calloc((size_t)(rn[0] > 0 ? rn[0] : 1), sizeof(fvc_point));
+            check(fvc_seed_grid(c, mn, del, res, d.LocalSearchChecking, ics[0]), c, "fvc_seed_grid");
+            write_ic_file(icpath, ics[0], rn[0]);
+        } else if (d.Trace_InFileFormat != 0) {
+            static const char *ext[5] = {"", "dat", "vtk", "vtk", "bin"};
+            for (int ff = 0; ff < (multi ? nl : 1); ff++) {
+                char sp[3 * FVC_STR + 32], ip2[3 * FVC_STR + 32];
+                if (multi) {
+                    snprintf(sp, sizeof sp, "%s%s.%d.%s", d.Path_Data, d.Trace_InFile, ff,
+                             ext[d.Trace_InFileFormat >= 1 && d.Trace_InFileFormat <= 4 ? d.Trace_InFileFormat : 0]);
+                    snprintf(ip2, sizeof ip2, "%s.%d", icpath, ff);
+                } else {
+                    snprintf(sp, sizeof sp, "%s%s", d.Path_Data, d.Trace_InFile);
+                    snprintf(ip2, sizeof ip2, "%s", icpath);
+                }
+                double *sx = read_seed_file(&d, sp, &rn[ff]);
+                ics[ff] = (fvc_point *)calloc((size_t)(rn[ff] > 0 ? rn[ff] : 1), sizeof(fvc_point));
+                check(fvc_seed_list(c, sx, rn[ff], d.LocalSearchChecking, ics[ff]), c, "fvc_seed_list");
+                free(sx);
+                write_ic_file(ip2, ics[ff], rn[ff]);
+            }
+        } else { /* tracers.c:1006: no reading or location; the releases copy the stored records */
+            for (int ff = 0; ff < (multi ? nl : 1); ff++) {
+                char ip2[3 * FVC_STR + 32];
+                if (multi)
+                    snprintf(ip2, sizeof ip2, "%s.%d", icpath, ff);
+                else
+                    snprintf(ip2, sizeof ip2, "%s", icpath);
+                ics[ff] = read_ic_file(ip2, &rn[ff]);
+            }
         }
-        snprintf(path, sizeof path, "%s%s.IC", d.Path_Data, d.Trace_OutFilePrefix);
-        FILE *o = fopen(path, "wb");
-        if (!o) fatal("Could not open file %s", path);
-        int32_t n32 = (int32_t)n;
-        fwrite(&n32, 4, 1, o);
-        fwrite(ic, sizeof(fvc_point), (size_t)n, o);
-        fclose(o);
-        const int nl = d.Trace_NumLaunchTimes;
+        for (int ss = 0; ss < nl; ss++) {
+            if (!multi && ss > 0) {
+                ics[ss] = ics[0];
+                rn[ss] = rn[0];
+            }
+            roff[ss + 1] = roff[ss] + rn[ss];
+        }
+        const int64_t n = rn[0]; /* the count of every release unless each reads its own file */
+        int64_t nmax = 0;
+        for (int ss = 0; ss < nl; ss++) nmax = rn[ss] > nmax ? rn[ss] : nmax;
+        fvc_point *ic = ics[0];
         fvc_particles *P = NULL;
-        check(fvc_particles_create(c, n * nl > 0 ? n * nl : 1, &P), c, "fvc_particles_create");
+        check(fvc_particles_create(c, roff[nl] > 0 ? roff[nl] : 1, &P), c, "fvc_particles_create");
         double *rstart = (double *)malloc(sizeof(double) * nl), *rstop = (double *)malloc(sizeof(double) * nl);
         int *status = (int *)calloc((size_t)nl, sizeof(int));
         for (int ss = 0; ss < nl; ss++) {
-            check(fvc_particles_set(P, (int64_t)ss * n, ic, n), c, "fvc_particles_set");
+            check(fvc_particles_set(P, roff[ss], ics[ss], rn[ss]), c, "fvc_particles_set");
             rstart[ss] = d.Output_TStart + dir * ss * d.Trace_LaunchTimeSpacing; /* tracers.c:1287-1291 */
             rstop[ss] = dir > 0 ? fmin(rstart[ss] + d.Trace_IntTLength, d.Output_TEnd)
                                 : fmax(rstart[ss] - d.Trace_IntTLength, d.Output_TEnd);
@@ -924,7 +1076,7 @@ int main(int argc, char **argv) {
         int upper = nl >= nproc / 2;
         int64_t *cnt = (int64_t *)calloc((size_t)d.Output_TRes, sizeof(int64_t));
         FILE **of = (FILE **)calloc((size_t)d.Output_TRes, sizeof(FILE *));
-        fvc_point *tmp = (fvc_point *)malloc(sizeof(fvc_point) * (size_t)(n > 0 ? n : 1));
+        fvc_point *tmp = (fvc_point *)malloc(sizeof(fvc_point) * (size_t)(nmax > 0 ? nmax : 1));
         int outframe = 0;
         double nextout = d.Output_TStart;
         for (int df = d.Data_FirstFrame; df != d.Data_LastFrame; df += dir) {
@@ -940,19 +1092,19 @@ int main(int argc, char **argv) {
                         double t0 = now_s();
                         if (status[ss] == 0) {
                             status[ss] = 1;
-                            if (cet) check(fvc_cet_launch(c, P, (int64_t)ss * n, n), c, "fvc_cet_launch");
+                            if (cet) check(fvc_cet_launch(c, P, roff[ss], rn[ss]), c, "fvc_cet_launch");
                             t1 = rstart[ss];
-                            check(fvc_launch_velocity(c, P, (int64_t)ss * n, n, rstart[ss], lo, hi, tmin, tmax), c,
+                            check(fvc_launch_velocity(c, P, roff[ss], rn[ss], rstart[ss], lo, hi, tmin, tmax), c,
                                   "fvc_launch_velocity");
                         }
-                        check(fvc_advect(c, P, (int64_t)ss * n, n, t1, t2, lo, hi, tmin, tmax, &ip), c, "fvc_advect");
+                        check(fvc_advect(c, P, roff[ss], rn[ss], t1, t2, lo, hi, tmin, tmax, &ip), c, "fvc_advect");
                         int64_t st = 0;
                         fvc_last_step_count(c, &st);
                         steps_total += st;
                         t_compute += now_s() - t0;
                         if (fabs(nextout - t2) < TINY) {
-                            check(fvc_particles_get(P, (int64_t)ss * n, tmp, n), c, "fvc_particles_get");
-                            for (int64_t q = 0; q < n; q++) {
+                            check(fvc_particles_get(P, roff[ss], tmp, rn[ss]), c, "fvc_particles_get");
+                            for (int64_t q = 0; q < rn[ss]; q++) {
                                 if (tmp[q].LeftDomainTime > -TINY && (!tmp[q].LeftDomain || d.Trace_AlwaysOutput)) {
                                     if (!of[outframe]) {
                                         snprintf(path, sizeof path, "%s%s.%d.bin", d.Path_Output, d.Trace_OutFilePrefix,
@@ -993,7 +1145,7 @@ int main(int argc, char **argv) {
         if (rt) { /* OutputRT (residencetime.c:21-185): mesh description, then each release's residence times */
             write_trace_mesh(&d, d.Trace_RTOutFilePrefix, ic, n);
             for (int ss = 0; ss < nl; ss++) {
-                check(fvc_particles_get(P, (int64_t)ss * n, tmp, n), c, "fvc_particles_get");
+                check(fvc_particles_get(P, roff[ss], tmp, n), c, "fvc_particles_get");
                 snprintf(path, sizeof path, "%s%s.%d.bin", d.Path_Output, d.Trace_RTOutFilePrefix, ss);
                 FILE *o = fopen(path, "wb");
                 if (!o) fatal("Could not open file %s", path);
@@ -1049,7 +1201,7 @@ int main(int argc, char **argv) {
         if (ap) { /* OutputAP (strainrate.c:248-395): mesh description, then each release's Scalar */
             write_trace_mesh(&d, d.Trace_OutFilePrefix, ic, n);
             for (int ss = 0; ss < nl; ss++) {
-                check(fvc_particles_get(P, (int64_t)ss * n, tmp, n), c, "fvc_particles_get");
+                check(fvc_particles_get(P, roff[ss], tmp, n), c, "fvc_particles_get");
                 snprintf(path, sizeof path, "%s%s_AP.%d.bin", d.Path_Output, d.Trace_OutFilePrefix, ss);
                 FILE *o = fopen(path, "wb");
                 if (!o) fatal("Could not open file %s", path);
@@ -1065,7 +1217,10 @@ int main(int argc, char **argv) {
         free(rstart);
         free(rstop);
         free(status);
-        free(ic);
+        for (int ss = 0; ss < (multi ? nl : 1); ss++) free(ics[ss]);
+        free(ics);
+        free(rn);
+        free(roff);
         fvc_particles_destroy(P);
     }
     if (!quiet)

@@ -280,6 +280,9 @@ int fvc_deck_check(const fvc_deck *d, char *err, int errlen) {
         if (d->Trace_LaunchTimeSpacing < FVC_TINY && d->Trace_NumLaunchTimes > 1)
             FAIL("Trace_LaunchTimeSpacing must be positive");
         if (d->Trace_IntTLength < 0) FAIL("Trace_IntTLength < 0");
+        if (d->Trace_InFileFormat == 0 && d->Trace_RTCompute && !d->Trace_GenerateMesh) /* parameters.c:543-544 */
+            fprintf(stderr, "  Warning: Trace_InFileFormat = 0 not recommended when Trace_RTCompute = 1 as connectivity "
+                            "data will be lost.\n");
         if (d->Trace_CETCompute) { /* parameters.c:535-542 */
             if (d->Trace_RTCompute) FAIL("Cannot compute exposure times and residence times concurrently");
             if (d->Trace_CETSubsteps < 2) FAIL("Trace_CETSubsteps must be greater than 1");

@@ -142,7 +142,10 @@ int fvc_load_strain_slice(fvc_ctx *ctx, int32_t slot, double timestamp, const do
 /* ---- particles (device-resident LagrangianPoint state) ------------------- */
 int fvc_particles_create(fvc_ctx *ctx, int64_t capacity, fvc_particles **out);
 void fvc_particles_destroy(fvc_particles *p);
-/* copy n host records into positions [begin, begin+n) / back out */
+/* copy n host records into positions [begin, begin+n) / back out.  On an
+ * unstructured mesh a record not flagged LeftDomain whose element id lies
+ * outside [-1, number of elements) is rejected (FVC_E_INVALID); records
+ * flagged LeftDomain are stored with element id -1. */
 int fvc_particles_set(fvc_particles *p, int64_t begin, const fvc_point *recs, int64_t n);
 int fvc_particles_get(fvc_particles *p, int64_t begin, fvc_point *recs, int64_t n);
 int64_t fvc_particles_capacity(const fvc_particles *p);
@@ -218,6 +221,19 @@ int fvc_ftle_records(fvc_ftle *f, fvc_point *recs);
 /* Restore the slide to its state right after fvc_ftle_create (unlaunched,
  * located grid, masks), from a device-resident copy. */
 int fvc_ftle_reset(fvc_ftle *f);
+/* The slide's state right after creation, as InitializeFTLEArray stores it in
+ * FTLE_ICFile (ompVC/ftle.c:611-625): per grid point, index
+ * (i*res1 + j)*res2 + k, the point record, HaveFTLE and the FTLE values the
+ * outside masks set.  Any output pointer may be NULL. */
+int fvc_ftle_initial(fvc_ftle *f, fvc_point *recs, int32_t *have, double *ftle_wt, double *ftle_wot);
+/* A slide from stored initial state instead of generating and locating the
+ * grid: the FTLE_GenerateMesh = 0 branch of InitializeFTLEArray
+ * (ompVC/ftle.c:626-660), which takes the resolution from the file and the
+ * origin and spacing from the parameters.  Records are checked as
+ * fvc_particles_set checks them. */
+int fvc_ftle_create_from(fvc_ctx *ctx, const double mn[3], const double del[3], const int32_t res[3],
+                         const fvc_point *recs, const int32_t *have, const double *ftle_wt, const double *ftle_wot,
+                         double start_time, double stop_time, fvc_ftle **out);
 
 /* ---- measurement ------------------------------------------------------------ */
 /* When enabled, every launch of the integration kernels (fvc_advect's and the

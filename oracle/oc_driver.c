@@ -88,13 +88,12 @@ oc_ftle *oc_ftle_create(const oc_mesh *m, const double min[3], const double max[
             for (int k = 0; k < f->res[2]; k++)
                 if (oc_test_outside_domain(m, f->pts[FIDX(f, i, j, k)].Pt.X)) mask_outside(f, i, j, k);
     if (m->mesh_type == 1) {
-        int seed = -1, guess = -1, iguess, jguess;
+        int seed = -1, guess = -1, jguess = -1;
         double xs[3] = {f->min[0] + ((int)(f->res[0] / 2)) * f->del[0], f->min[1] + ((int)(f->res[1] / 2)) * f->del[1],
                         f->min[2] + ((int)(f->res[2] / 2)) * f->del[2]};
         seed = oc_global_search(m, xs);
         if (seed >= 0) guess = seed;
         for (int i = 0; i < f->res[0]; i++) {
-            iguess = -1;
             for (int j = 0; j < f->res[1]; j++) {
                 jguess = -1;
                 for (int k = 0; k < f->res[2]; k++) {
@@ -106,7 +105,7 @@ oc_ftle *oc_ftle_create(const oc_mesh *m, const double min[3], const double max[
                             mask_outside(f, i, j, k);
                         else {
                             q->Pt.ElementIndex = seed;
-                            guess = jguess = iguess = seed;
+                            guess = jguess = seed;
                         }
                         continue;
                     }
@@ -124,13 +123,13 @@ oc_ftle *oc_ftle_create(const oc_mesh *m, const double min[3], const double max[
                         q->Pt.ElementIndex = index;
                         guess = index;
                         if (jguess < 0) jguess = index;
-                        if (iguess < 0) iguess = index;
                     }
                 }
-                if (jguess >= 0) guess = jguess;
             }
-            if (iguess >= 0) guess = iguess;
+            /* after each plane, the first found element of its last row (ftle.c:597-598) */
+            if (jguess >= 0) guess = jguess;
         }
+        /* the plane guess of ftle.c:600-601 is applied after the whole grid, when no point follows */
     }
     return f;
 }

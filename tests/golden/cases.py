@@ -132,6 +132,15 @@ def gen_pipe_velout(d):
             f.write("%.17g %.17g %.17g\n" % tuple(q))
 
 
+def gen_pipe_multi(d):
+    """The pipe with one binary seed file per release (Trace_MultipleInFiles,
+    tracers.c:1007-1016): 300, 500 and 200 seeds."""
+    gen_pipe(d)
+    t = host.pipe(*PIPE_DIMS, 1.0, 10.0)
+    for ff, (cnt, seed) in enumerate(((300, 5), (500, 6), (200, 7))):
+        host.write_binary_list(os.path.join(d, "mseeds.%d.bin" % ff), host.tube_seeds(t, cnt, 0.0, 10.0 / 3.0, seed))
+
+
 def gen_dg_velout(d):
     gen_dg(d)
     rng = np.random.default_rng(9)
@@ -261,7 +270,7 @@ CASES = {
                                    FTLE_Compute=1, FTLE_MeshBounds_XMin=0.0, FTLE_MeshBounds_XMax=2.0,
                                    FTLE_MeshBounds_YMin=0.0, FTLE_MeshBounds_YMax=1.0, FTLE_MeshBounds_XRes=64,
                                    FTLE_MeshBounds_YRes=32, FTLE_MeshBounds_ZRes=1, FTLE_IntTLength=4.0,
-                                   FTLE_OutFilePrefix="dgb", FTLE_ICFile="dgb_ic.bin")),
+                                   FTLE_OutFilePrefix="dgb", FTLE_ICFile="dgb_ic.bin"), keep=["dgb_ic.bin"]),
     # 2-D Cartesian tracers from a generated 64x32 grid, 6 outputs.  The reference
     # leaves LeftDomainTime of generated Cartesian tracers uninitialised
     # (tracers.c:1340-1532); 2048 records (160 KB) come from a fresh zeroed
@@ -377,6 +386,14 @@ CASES = {
                                           Output_TDelta=0.1, Int_TimeDirection=1, Trace_Compute=1,
                                           Trace_GenerateMesh=0, Trace_InFile="probes.txt", Trace_InFileFormat=1,
                                           Trace_IntTLength=0.5, Trace_AlwaysOutput=1, Trace_OutFilePrefix="ptx")),
+    # three releases, each from its own seed file and .IC.<release> (Trace_MultipleInFiles)
+    "pipe_tracers_multi": dict(gen=gen_pipe_multi, outputs=["ptm.%d.bin" % k for k in range(6)], ic="ptm.IC.0",
+                               keep=["ptm.IC.1", "ptm.IC.2"],
+                               deck=dict(PIPE, Output_TStart=0.0, Output_TRes=6, Output_TDelta=0.05,
+                                         Int_TimeDirection=1, Trace_Compute=1, Trace_GenerateMesh=0,
+                                         Trace_InFile="mseeds", Trace_InFileFormat=4, Trace_MultipleInFiles=1,
+                                         Trace_NumLaunchTimes=3, Trace_LaunchTimeSpacing=0.05,
+                                         Trace_IntTLength=0.15, Trace_OutFilePrefix="ptm")),
     # 2-D Cartesian FTLE with Euler, 64x32 grid, T=4 (early exits re-advected with Euler)
     "dg_ftle_euler": dict(gen=gen_dg, outputs=["dgfe.0.bin", "dgfe_noT.0.bin", "dgfe_Cartesian.bin"],
                           deck=dict(DG, Int_Type=0, Output_TStart=0.0, Output_TRes=1, Output_TDelta=1.0,
@@ -426,6 +443,17 @@ CASES = {
                                 FTLE_MeshBounds_YMin=0.0, FTLE_MeshBounds_YMax=1.0, FTLE_MeshBounds_XRes=49,
                                 FTLE_MeshBounds_YRes=13, FTLE_MeshBounds_ZRes=1, FTLE_IntTLength=0.5,
                                 FTLE_OutFilePrefix="cft", FTLE_ICFile="cft_ic.bin")),
+    # FTLE grid on the pipe's voxel corners (two voxels apart): every inside point is a mesh vertex held by many
+    # elements, so each element id follows the location chain's guesses (ftle.c:456-601); kept: the grid records
+    "pipe_ftle_vertices": dict(gen=gen_pipe, outputs=["pfv.0.bin", "pfv_noT.0.bin", "pfv_Cartesian.bin"],
+                               keep=["pfv_ic.bin"],
+                               deck=dict(PIPE, Output_TStart=0.0, Output_TRes=1, Output_TDelta=0.1,
+                                         Int_TimeDirection=1, FTLE_Compute=1, FTLE_MeshBounds_XMin=-1.0,
+                                         FTLE_MeshBounds_XMax=1.0, FTLE_MeshBounds_YMin=-1.0,
+                                         FTLE_MeshBounds_YMax=1.0, FTLE_MeshBounds_ZMin=0.0,
+                                         FTLE_MeshBounds_ZMax=10.0, FTLE_MeshBounds_XRes=7, FTLE_MeshBounds_YRes=7,
+                                         FTLE_MeshBounds_ZRes=21, FTLE_IntTLength=0.05, FTLE_OutFilePrefix="pfv",
+                                         FTLE_ICFile="pfv_ic.bin")),
     # 3-D unstructured FTLE with outside masks and early exits, 2 slides
     "pipe_ftle": dict(gen=gen_pipe, outputs=["pft.0.bin", "pft.1.bin", "pft_noT.0.bin", "pft_noT.1.bin",
                                              "pft_Cartesian.bin"],
@@ -434,7 +462,7 @@ CASES = {
                                 FTLE_MeshBounds_YMin=-1.0, FTLE_MeshBounds_YMax=1.0, FTLE_MeshBounds_ZMin=0.0,
                                 FTLE_MeshBounds_ZMax=10.0, FTLE_MeshBounds_XRes=12, FTLE_MeshBounds_YRes=12,
                                 FTLE_MeshBounds_ZRes=30, FTLE_IntTLength=0.5, FTLE_OutFilePrefix="pft",
-                                FTLE_ICFile="pft_ic.bin")),
+                                FTLE_ICFile="pft_ic.bin"), keep=["pft_ic.bin"]),
 }
 
 # loop choice of ompVC depends on the host processor count (parameters.c:398-421);
@@ -450,15 +478,16 @@ def sha256(path):
     return h.hexdigest()
 
 
-def prepare(case, workdir):
-    """Generate inputs for `case` under workdir/data and return (deck_path, out_dir)."""
+def prepare(case, workdir, **over):
+    """Generate inputs for `case` under workdir/data and return (deck_path,
+    data_dir, out_dir); keyword arguments override deck entries."""
     c = CASES[case]
     data = os.path.join(workdir, "data")
     out = os.path.join(workdir, "out")
     os.makedirs(data, exist_ok=True)
     os.makedirs(out, exist_ok=True)
     c["gen"](data)
-    dk = dict(c["deck"], Path_Data=data + "/", Path_Output=out + "/")
+    dk = dict(c["deck"], Path_Data=data + "/", Path_Output=out + "/", **over)
     deck_path = os.path.join(workdir, case + ".in")
     deck.write_deck(deck_path, **dk)
     return deck_path, data, out
@@ -466,7 +495,7 @@ def prepare(case, workdir):
 
 def input_digests(data):
     return {f: sha256(os.path.join(data, f)) for f in sorted(os.listdir(data))
-            if not f.endswith(".IC") and not f.endswith("_ic.bin") and "_slide" not in f and "_OUT" not in f
+            if not f.endswith(".IC") and ".IC." not in f and not f.endswith("_ic.bin") and "_slide" not in f and "_OUT" not in f
             and not f.startswith("pft.") and not f.startswith("dgftle.") and not f.startswith("dgb.")}
 
 

@@ -4,7 +4,8 @@ by oracle/ref.mk into oracle/_ref/) on the cases of cases.py.
 Run in the authoring container only (needs /root/reference):
     make -f oracle/ref.mk && python tests/golden/make_goldens.py
 Each case directory receives the reference's output files, the .IC file where
-the case has one, and manifest.json with input digests.
+the case has one, the files a case lists under "keep" (further .IC files,
+the FTLE_ICFile), and manifest.json with input digests.
 """
 import json
 import os
@@ -41,8 +42,10 @@ def main(selected):
                 shutil.copy(os.path.join(out, f), fx)
             if c.get("ic"):
                 shutil.copy(os.path.join(data, c["ic"]), fx)
+            for f in c.get("keep", []):  # further files the reference writes beside the inputs
+                shutil.copy(os.path.join(data, f), fx)
             man = dict(case=name, nproc=cases.GOLDEN_NPROC, outputs=[f for f in produced if not f.endswith(".info")],
-                       ic=c.get("ic"), inputs=cases.input_digests(data),
+                       ic=c.get("ic"), keep=c.get("keep", []), inputs=cases.input_digests(data),
                        producer="ompVC (ompVC/*.c built by oracle/ref.mk, gcc -O3 -fopenmp)")
             with open(os.path.join(fx, "manifest.json"), "w") as f:
                 json.dump(man, f, indent=1, sort_keys=True)

@@ -4,6 +4,7 @@ potentials, residence times, exposure-time encounter counts, velocity output
 files and .IC element indices bit-identical; exposure-time sums to 1e-12; FTLE values within the
 eigen-solver rounding (rtol 1e-12, atol 1e-12)."""
 import os
+import shutil
 import subprocess
 
 import numpy as np
@@ -35,7 +36,7 @@ def run_driver(case, tmp_path, precision="f64"):
                                   "pipe_staggered_steady", "pipe_velout_grid", "pipe_velout_list",
                                   "dg_velout_list", "pipe_tracers_ap", "pipe_tracers_ap2", "dg_tracers_ap",
                                   "pipe_tracers_cet", "pipe_tracers_rt", "dg_tracers_euler", "pipe_tracers_euler",
-                                  "pipe_tracers_extrap", "box_tracers", "chan_tracers"])
+                                  "pipe_tracers_extrap", "box_tracers", "chan_tracers", "pipe_tracers_multi"])
 def test_gpu_trajectories_bit_identical(case, tmp_path):
     man, data, out = run_driver(case, tmp_path)
     fx = cases.fixture_dir(case)
@@ -56,9 +57,9 @@ def test_gpu_trajectories_bit_identical(case, tmp_path):
         a = np.fromfile(os.path.join(fx, f), np.uint8)
         b = np.fromfile(os.path.join(out, f), np.uint8)
         assert a.size == b.size and np.array_equal(a, b), f
-    if man.get("ic"):
-        ra = host.read_ic(os.path.join(fx, man["ic"]))
-        rb = host.read_ic(os.path.join(data, man["ic"]))
+    for ic in ([man["ic"]] if man.get("ic") else []) + [k for k in man.get("keep", []) if ".IC" in k]:
+        ra = host.read_ic(os.path.join(fx, ic))
+        rb = host.read_ic(os.path.join(data, ic))
         np.testing.assert_array_equal(ra["LeftDomain"], rb["LeftDomain"])
         inside = ra["LeftDomain"] == 0
         np.testing.assert_array_equal(ra["ElementIndex"][inside], rb["ElementIndex"][inside])
@@ -67,7 +68,7 @@ def test_gpu_trajectories_bit_identical(case, tmp_path):
 
 
 @pytest.mark.parametrize("case", ["dg_ftle", "dg_ftle_back", "pipe_ftle", "dg_ftle_euler", "pipe_ftle_extrap",
-                                  "box_ftle", "chan_ftle"])
+                                  "box_ftle", "chan_ftle", "pipe_ftle_vertices"])
 def test_gpu_ftle_matches_reference(case, tmp_path):
     man, data, out = run_driver(case, tmp_path)
     fx = cases.fixture_dir(case)
@@ -138,3 +139,119 @@ def test_gpu_activation_potential_per_release(tmp_path):
             k += 1
         assert k < ap1.size - 1, "record Scalar missing from _AP.1"
         k += 1
+
+
+def _run_deck(deck_path):
+    r = subprocess.run([FLOWVC, deck_path, "--precision", "f64", "--nproc", str(cases.GOLDEN_NPROC), "--quiet"],
+                       capture_output=True, text=True, timeout=300)
+    return r
+
+
+def _same_files(case, out, names=None):
+    fx = cases.fixture_dir(case)
+    for f in names or cases.load_manifest(case)["outputs"]:
+        assert open(os.path.join(fx, f), "rb").read() == open(os.path.join(out, f), "rb").read(), f
+
+
+@pytest.mark.parametrize("case", ["pipe_tracers", "pipe_tracers_multi"])
+def test_gpu_tracers_from_reference_ic_files(case, tmp_path):
+    """Trace_InFileFormat = 0: no seed file is read or located, every release
+    copies the stored records (tracers.c:1006, 1579-1631).  Given the
+    reference's own .IC files, the outputs equal the reference's run that
+    wrote them, byte for byte."""
+    deck_path, data, out = cases.prepare(case, str(tmp_path), Trace_InFileFormat=0)
+    man = cases.load_manifest(case)
+    for ic in [man["ic"]] + [k for k in man.get("keep", []) if ".IC" in k]:
+        shutil.copy(os.path.join(cases.fixture_dir(case), ic), os.path.join(data, ic))
+    print("case %s from stored .IC" % case, flush=True)
+    r = _run_deck(deck_path)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    _same_files(case, out)
+
+
+def test_gpu_tracers_from_vtk_seed_file(tmp_path):
+    """Legacy VTK seed files (Trace_InFileFormat 2/3: four header lines,
+    POINTS n type, coordinates) hold the pipe seeds: outputs equal the
+    binary-list run.  The reference's reader stops on every such file (a
+    stray semicolon makes its header-skip failure unconditional,
+    tracers.c:1082-1083); here the file is read as written."""
+    deck_path, data, out = cases.prepare("pipe_tracers", str(tmp_path), Trace_InFileFormat=2,
+                                         Trace_InFile="seeds.vtk")
+    raw = np.fromfile(os.path.join(data, "seeds.bin"), np.uint8)
+    pts = raw[4:].view(np.float64).reshape(-1, 3)
+    with open(os.path.join(data, "seeds.vtk"), "w") as f:
+        f.write("# vtk DataFile Version 3.0\nseeds\nASCII\nDATASET POLYDATA\nPOINTS %d double\n" % pts.shape[0])
+        for q in pts:
+            f.write("%.17g %.17g %.17g\n" % tuple(q))
+    print("pipe_tracers from a VTK seed file", flush=True)
+    r = _run_deck(deck_path)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    _same_files("pipe_tracers", out)
+    ic = host.read_ic(os.path.join(data, "ptr.IC"))
+    ref_ic = host.read_ic(os.path.join(cases.fixture_dir("pipe_tracers"), "ptr.IC"))
+    np.testing.assert_array_equal(ic["X"], ref_ic["X"])
+
+
+def test_gpu_tracer_input_errors(tmp_path):
+    """VTK seed files in 2-D and a missing stored .IC stop the run with the
+    reference's messages (tracers.c:1068-1071, 1594-1595)."""
+    deck_path, data, out = cases.prepare("dg_tracers", str(tmp_path / "a"), Trace_GenerateMesh=0,
+                                         Trace_InFileFormat=3, Trace_InFile="x.vtk")
+    r = _run_deck(deck_path)
+    assert r.returncode == 1 and "not currently supported for 2D analysis" in r.stderr, r.stderr[-2000:]
+    deck_path, data, out = cases.prepare("pipe_tracers", str(tmp_path / "b"), Trace_InFileFormat=0)
+    r = _run_deck(deck_path)
+    assert r.returncode == 1 and "Could not open file" in r.stderr and "ptr.IC" in r.stderr, r.stderr[-2000:]
+
+
+FTLE_REC = np.dtype({"names": ["X", "V", "ElementIndex", "Aux", "LeftDomain", "LeftDomainTime", "Scalar", "HaveFTLE",
+                               "FTLEwT", "FTLEwoT"],
+                     "formats": [("<f8", 3), ("<f8", 3), "<i4", "<i4", "<i4", "<f8", "<f8", "<i4", "<f8", "<f8"],
+                     "offsets": [0, 24, 48, 52, 56, 64, 72, 80, 88, 96], "itemsize": 104})
+
+
+def _read_ftle_ic(path):
+    raw = open(path, "rb").read()
+    return np.frombuffer(raw[:12], "<i4"), np.frombuffer(raw[12:], FTLE_REC)
+
+
+@pytest.mark.parametrize("case", ["dg_ftle_back", "pipe_ftle", "pipe_ftle_vertices"])
+def test_gpu_ftle_ic_file_matches_reference(case, tmp_path):
+    """FTLE_ICFile (ftle.c:611-625): resolution, then per grid point the
+    initial record, HaveFTLE and the outside masks' values, as the
+    reference writes them (fields the reference leaves unset are skipped)."""
+    man, data, out = run_driver(case, tmp_path)
+    name = cases.CASES[case]["keep"][0]
+    ra, a = _read_ftle_ic(os.path.join(cases.fixture_dir(case), name))
+    rb, b = _read_ftle_ic(os.path.join(data, name))
+    np.testing.assert_array_equal(ra, rb)
+    assert a.size == b.size == int(np.prod(ra))
+    for k in ("X", "LeftDomain", "LeftDomainTime", "HaveFTLE", "FTLEwT", "FTLEwoT"):
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+    if case.startswith("pipe"):
+        inside = a["LeftDomain"] == 0
+        assert inside.sum() > 500
+        np.testing.assert_array_equal(a["ElementIndex"][inside], b["ElementIndex"][inside])
+
+
+@pytest.mark.parametrize("case", ["dg_ftle_back", "pipe_ftle", "pipe_ftle_vertices"])
+def test_gpu_ftle_from_reference_ic_file(case, tmp_path):
+    """FTLE_GenerateMesh = 0: the slides start from the reference's own
+    FTLE_ICFile (ftle.c:626-660), whose unset fields hold whatever the
+    reference's heap held; the FTLE files equal the reference's."""
+    deck_path, data, out = cases.prepare(case, str(tmp_path), FTLE_GenerateMesh=0)
+    name = cases.CASES[case]["keep"][0]
+    shutil.copy(os.path.join(cases.fixture_dir(case), name), os.path.join(data, name))
+    print("case %s from the reference's FTLE IC file" % case, flush=True)
+    r = _run_deck(deck_path)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    fx = cases.fixture_dir(case)
+    for f in cases.load_manifest(case)["outputs"]:
+        a = np.fromfile(os.path.join(fx, f))
+        b = np.fromfile(os.path.join(out, f))
+        if f.endswith("_Cartesian.bin"):
+            assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
+            continue
+        assert a.shape == b.shape, f
+        np.testing.assert_array_equal(a == -1.0, b == -1.0)
+        np.testing.assert_allclose(b, a, rtol=1e-12, atol=1e-12, err_msg=f)

@@ -341,3 +341,27 @@ def test_tabulated_chains_equal_replay_at_full_size():
     seeds = np.concatenate([host.tube_seeds(p, 4000000, 0.0, 10.0, 5), query_points(p, 40000, 6)])
     print("pipe list chain n=%d" % seeds.shape[0], flush=True)
     _chain_both(ctx, lambda: ctx.seed_list(seeds, 1))
+
+
+def test_particle_records_with_foreign_element_ids(pipe):
+    """Records from files carry element ids the kernels walk from: an id
+    outside [-1, elements) on a point not flagged as left is refused before
+    anything reaches the device; points flagged as left are stored without
+    an element (ompVC leaves those ids unset in its stored grids)."""
+    t, xyz, conn, adj, slices = pipe
+    ctx = fvc.Context(3, PIPE_BOUNDS, "f64", 2, coords=xyz, conn=conn, adj=adj)
+    recs = ctx.seed_list(host.tube_seeds(t, 64, 0.0, 10.0, 3), 1)
+    assert (recs["LeftDomain"] == 0).all()
+    P = fvc.Particles(ctx, recs.shape[0])
+    for bad in (conn.shape[0], -2, 2 ** 31 - 1):
+        r = recs.copy()
+        r["ElementIndex"][5] = bad
+        with pytest.raises(fvc.FvcError):
+            P.set(r)
+    r = recs.copy()
+    r["LeftDomain"][7] = 1
+    r["ElementIndex"][7] = -2072131723
+    P.set(r)
+    back = P.get()
+    assert back["ElementIndex"][7] == -1
+    np.testing.assert_array_equal(np.delete(back["ElementIndex"], 7), np.delete(recs["ElementIndex"], 7))
