@@ -386,6 +386,18 @@ CASES = {
                                           Output_TDelta=0.1, Int_TimeDirection=1, Trace_Compute=1,
                                           Trace_GenerateMesh=0, Trace_InFile="probes.txt", Trace_InFileFormat=1,
                                           Trace_IntTLength=0.5, Trace_AlwaysOutput=1, Trace_OutFilePrefix="ptx")),
+    # generated tracer grid on the pipe (tracers.c:1298-1560): x, y on voxel corners two voxels apart, z one voxel
+    # apart from 0.5 below to 0.5 above the pipe, so inside points are mesh vertices and the end planes lie outside
+    # the bounding box; Trace_AlwaysOutput writes those too (the reference leaves their LeftDomainTime unset:
+    # 2205 records, 176 KB, come from fresh zeroed memory)
+    "pipe_tracers_grid": dict(gen=gen_pipe, outputs=["ptg.%d.bin" % k for k in range(3)], ic="ptg.IC",
+                              deck=dict(PIPE, Output_TStart=0.0, Output_TRes=3, Output_TDelta=0.05,
+                                        Int_TimeDirection=1, Trace_Compute=1, Trace_GenerateMesh=1,
+                                        Trace_MeshBounds_XMin=-1.0, Trace_MeshBounds_XMax=1.0,
+                                        Trace_MeshBounds_YMin=-1.0, Trace_MeshBounds_YMax=1.0,
+                                        Trace_MeshBounds_ZMin=-0.5, Trace_MeshBounds_ZMax=10.5,
+                                        Trace_MeshBounds_XRes=7, Trace_MeshBounds_YRes=7, Trace_MeshBounds_ZRes=45,
+                                        Trace_IntTLength=0.1, Trace_AlwaysOutput=1, Trace_OutFilePrefix="ptg")),
     # three releases, each from its own seed file and .IC.<release> (Trace_MultipleInFiles)
     "pipe_tracers_multi": dict(gen=gen_pipe_multi, outputs=["ptm.%d.bin" % k for k in range(6)], ic="ptm.IC.0",
                                keep=["ptm.IC.1", "ptm.IC.2"],

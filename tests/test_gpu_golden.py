@@ -36,7 +36,8 @@ def run_driver(case, tmp_path, precision="f64"):
                                   "pipe_staggered_steady", "pipe_velout_grid", "pipe_velout_list",
                                   "dg_velout_list", "pipe_tracers_ap", "pipe_tracers_ap2", "dg_tracers_ap",
                                   "pipe_tracers_cet", "pipe_tracers_rt", "dg_tracers_euler", "pipe_tracers_euler",
-                                  "pipe_tracers_extrap", "box_tracers", "chan_tracers", "pipe_tracers_multi"])
+                                  "pipe_tracers_extrap", "box_tracers", "chan_tracers", "pipe_tracers_multi",
+                                  "pipe_tracers_grid"])
 def test_gpu_trajectories_bit_identical(case, tmp_path):
     man, data, out = run_driver(case, tmp_path)
     fx = cases.fixture_dir(case)
