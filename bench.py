@@ -523,6 +523,93 @@ def bench_traffic(args, np, fvc, host):
                       "ftle_particle_steps_per_launch": fsteps / max(1, args.frames)}), flush=True)
 
 
+def dg_slide(fvc, np, ctx, res, T=10.0):
+    """One forward FTLE slide of the double gyre over [0, T]: 2T frames of 0.5
+    between consecutive slices, RK4 dt=0.01.  Returns the slide, FTLE (with
+    integration time T), particle-steps and seconds."""
+    F = fvc.FtleSlide(ctx, [0.0, 0.0, 0.0], [2.0, 1.0, 0.0], res, 1, 0.0, T)
+    fvc.lib().fvc_sync(ctx.h)
+    t0 = time.perf_counter()
+    steps = 0
+    for f in range(int(round(2 * T))):
+        F.frame(1, f, f + 1, 0.5 * f, 0.5 * (f + 1), 0.01)
+        steps += ctx.last_steps()
+    wt, _ = F.output(T)
+    fvc.lib().fvc_sync(ctx.h)
+    return F, wt, steps, time.perf_counter() - t0
+
+
+def bench_dg(args, np, fvc, host):
+    """The CPU-runnable configuration and its precision variants: double gyre
+    (A=0.1, eps=0.25, omega=2 pi/10) on 256x128 nodes, 21 slices over [0, 10];
+    FTLE on a 512x256 grid (and a 1024x1024 grid, the 1M-seed comparison set)
+    with RK4 dt=0.01 over T=10.  FP64 first (bit-identical to the oracle), then
+    FP32, BF16 and FP16 slices; errors against FP64: FTLE relative error,
+    final flow-map positions, velocity-grid cell and exit agreement."""
+    nx, ny = 256, 128
+    t0 = time.time()
+    slices = [host.double_gyre_slice(nx, ny, 0.5 * s) for s in range(21)]
+    log("double gyre slices: %.1f s" % (time.time() - t0))
+    out = {"workload": "dg", "config": {"workload": "double gyre 256x128 nodes, 21 slices, RK4 dt=0.01 over T=10",
+                                        "ftle_grids": [[512, 256], [1024, 1024]]}, "results": []}
+    dx, dy = 2.0 / (nx - 1), 1.0 / (ny - 1)
+    for res in ([512, 256, 1], [1024, 1024, 1]):
+        ref = None
+        for prec in ("f64", "f32", "bf16", "f16"):
+            ctx = fvc.Context(2, [0.0, 2.0, 0.0, 1.0, 0.0, 0.0], prec, 21, cart_min=[0.0, 0.0, 0.0],
+                              cart_max=[2.0, 1.0, 0.0], cart_res=[nx, ny, 1])
+            for s, sl in enumerate(slices):
+                ctx.load_slice(s, 0.5 * s, sl)
+            dg_slide(fvc, np, ctx, res)  # warm-up
+            F, wt, steps, secs = dg_slide(fvc, np, ctx, res)
+            rec = F.records()
+            r = {"grid": res[:2], "precision": prec, "seeds": F.n, "seconds": secs, "particle_steps": int(steps),
+                 "particle_steps_per_s": steps / secs, "ftle_seeds_per_s": F.n / secs}
+            if prec == "f64":
+                ref = (wt, rec)
+            else:
+                w64, r64 = ref
+                big = np.abs(w64) > 1e-3
+                rel = np.abs(wt[big] - w64[big]) / np.abs(w64[big])
+                both = (rec["LeftDomain"] == 0) & (r64["LeftDomain"] == 0)
+                dpos = np.linalg.norm(rec["X"][both] - r64["X"][both], axis=1)
+                cell = lambda X: np.stack([np.floor(X[:, 0] / dx), np.floor(X[:, 1] / dy)], 1)
+                r.update(ftle_rel_p50=float(np.percentile(rel, 50)), ftle_rel_p99=float(np.percentile(rel, 99)),
+                         ftle_rel_p999=float(np.percentile(rel, 99.9)), ftle_rel_max=float(rel.max()),
+                         ftle_abs_max=float(np.abs(wt - w64).max()), pos_err_max=float(dpos.max()),
+                         pos_err_p99=float(np.percentile(dpos, 99)),
+                         cell_agreement=float((cell(rec["X"][both]) == cell(r64["X"][both])).all(1).mean()),
+                         exit_agreement=float((rec["LeftDomain"] == r64["LeftDomain"]).mean()))
+            out["results"].append(r)
+            log(json.dumps(r))
+            F.close()
+            ctx.close()
+    if not args.no_cpu:  # the oracle on every 16th point of the 512x256 grid, all 20 frames
+        sys.path.insert(0, os.path.join(HERE, "oracle"))
+        import oracle
+        om = oracle.Mesh(2, [0.0, 2.0, 0.0, 1.0, 0.0, 0.0], cart_min=[0.0, 0.0, 0.0], cart_max=[2.0, 1.0, 0.0],
+                         cart_res=[nx, ny, 1])
+        g = np.stack(np.meshgrid(np.linspace(0.0, 2.0, 512), np.linspace(0.0, 1.0, 256), indexing="ij"), -1)
+        pts = np.zeros(512 * 256 // 16, oracle.POINT_DTYPE)
+        pts["X"][:, :2] = g.reshape(-1, 2)[::16]
+        cores = min(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), os.cpu_count() or 1, 16)
+        t_int, steps = 0.0, 0
+        for f in range(20):
+            om.set_slot(0, slices[f])
+            om.set_slot(1, slices[f + 1])
+            om.set_interval(0.5 * f, 0.5 * (f + 1))
+            t1 = time.perf_counter()
+            rc = om.advect(pts, 0.5 * f, 0.5 * (f + 1), 0.01, nthreads=cores, chunk=max(1, pts.shape[0] // cores))
+            t_int += time.perf_counter() - t1
+            if rc:
+                raise RuntimeError("oracle advect failed (%d)" % rc)
+            steps += om.last_steps.value
+        out["cpu_baseline"] = {"value": steps / t_int, "unit": "particle-steps/s", "cores": cores, "kind": "port",
+                               "sample": "%d of the 512x256 FTLE grid points (every 16th), T=10, FP64 oracle"
+                                         % pts.shape[0]}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -539,9 +626,10 @@ def main():
                     help="element and node numbering of the pipe mesh: renumbered in Z-order by fvc_mesh_zorder "
                          "before upload (cudaVC's ZOrderSort step; untimed setup), or as generated (voxel scan order)")
     ap.add_argument("--no-f64-check", action="store_true")
-    ap.add_argument("--workload", default="pipe", choices=["pipe", "batch", "serve", "traffic"],
+    ap.add_argument("--workload", default="pipe", choices=["pipe", "batch", "serve", "traffic", "dg"],
                     help="pipe: tracers + FTLE grid over one period; batch: uneven releases in one persistent "
-                         "kernel; serve: Poisson request stream on the Y-bifurcation")
+                         "kernel; serve: Poisson request stream on the Y-bifurcation; dg: the 2-D double-gyre "
+                         "FTLE and its BF16/FP16 variants")
     ap.add_argument("--frames", type=int, default=4, help="traffic workload: tracer frames after the copy")
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-window", type=float, default=15.0)
@@ -573,6 +661,9 @@ def main():
         return
     if args.workload == "traffic":
         bench_traffic(args, np, fvc, host)
+        return
+    if args.workload == "dg":
+        bench_dg(args, np, fvc, host)
         return
     res = [int(v) for v in args.ftle.split(",")]
     tube, xyz, conn, adj, slices, seeds = build_inputs(args, host, np)

@@ -365,3 +365,33 @@ def test_particle_records_with_foreign_element_ids(pipe):
     back = P.get()
     assert back["ElementIndex"][7] == -1
     np.testing.assert_array_equal(np.delete(back["ElementIndex"], 7), np.delete(recs["ElementIndex"], 7))
+
+
+@pytest.mark.parametrize("precision,pos_tol,ftle_p99", [("f32", 2e-5, 1e-3), ("f16", 0.2, 2e-2), ("bf16", 0.6, 0.15)])
+def test_double_gyre_ftle_reduced_precision(precision, pos_tol, ftle_p99):
+    """The 2-D double-gyre FTLE (256x128 nodes, 21 slices, RK4 dt=0.01 over
+    T=10) on a 256x128 grid against the FP64 path (bit-identical to the
+    oracle): FP32 meets 1e-5 of the domain size (2) in final position and 1e-3
+    relative FTLE at the 99th percentile; the 16-bit slices meet the looser
+    bounds stated here (the flow is chaotic: separations grow up to ~100x over T)."""
+    nx, ny = 256, 128
+    slices = [host.double_gyre_slice(nx, ny, 0.5 * s) for s in range(21)]
+    out = {}
+    for prec in ("f64", precision):
+        ctx = fvc.Context(2, [0.0, 2.0, 0.0, 1.0, 0.0, 0.0], prec, 21, cart_min=[0.0, 0.0, 0.0],
+                          cart_max=[2.0, 1.0, 0.0], cart_res=[nx, ny, 1])
+        for s, sl in enumerate(slices):
+            ctx.load_slice(s, 0.5 * s, sl)
+        print("double gyre FTLE %s" % prec, flush=True)
+        F = fvc.FtleSlide(ctx, [0.0, 0.0, 0.0], [2.0, 1.0, 0.0], [256, 128, 1], 1, 0.0, 10.0)
+        for f in range(20):
+            F.frame(1, f, f + 1, 0.5 * f, 0.5 * (f + 1), 0.01)
+        wt, _ = F.output(10.0)
+        out[prec] = (wt, F.records())
+    (w64, r64), (w, r) = out["f64"], out[precision]
+    np.testing.assert_array_equal(r["LeftDomain"], r64["LeftDomain"])
+    both = r64["LeftDomain"] == 0
+    assert both.sum() > 30000
+    assert np.linalg.norm(r["X"][both] - r64["X"][both], axis=1).max() <= pos_tol
+    big = np.abs(w64) > 1e-3
+    assert np.percentile(np.abs(w[big] - w64[big]) / np.abs(w64[big]), 99) <= ftle_p99
