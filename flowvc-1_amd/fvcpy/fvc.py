@@ -115,6 +115,8 @@ def lib():
         L.fvc_ftle_output.argtypes = [vp, d, vp, vp]
         L.fvc_ftle_points.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]
         L.fvc_ftle_records.argtypes = [vp, vp]
+        L.fvc_ftle_initial.argtypes = [vp, vp, vp, vp, vp]
+        L.fvc_ftle_create_from.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, d, d, ctypes.POINTER(vp)]
         L.fvc_zsort.argtypes = [vp, vp, i64, vp, vp, vp]
         L.fvc_particles_copy.argtypes = [vp, i64, vp, i64, i64]
         L.fvc_particles_get_xyz.argtypes = [vp, i64, vp, vp, i64]
@@ -341,17 +343,45 @@ class Particles:
 
 
 class FtleSlide:
-    def __init__(self, ctx, mn, mx, res, lsc, start, stop):
+    def __init__(self, ctx, mn, mx, res, lsc, start, stop, _handle=None):
         self.ctx = ctx
-        mn, mx = np.asarray(mn, np.float64), np.asarray(mx, np.float64)
-        res = np.asarray(res, np.int32)
-        h = ctypes.c_void_p()
-        _chk(lib().fvc_ftle_create(ctx.h, _p(mn), _p(mx), _p(res), lsc, start, stop, ctypes.byref(h)),
-             "fvc_ftle_create", ctx.h)
+        if _handle is None:
+            mn, mx = np.asarray(mn, np.float64), np.asarray(mx, np.float64)
+            res = np.asarray(res, np.int32)
+            h = ctypes.c_void_p()
+            _chk(lib().fvc_ftle_create(ctx.h, _p(mn), _p(mx), _p(res), lsc, start, stop, ctypes.byref(h)),
+                 "fvc_ftle_create", ctx.h)
+        else:
+            h = _handle
         self.h = h
         n, ins = ctypes.c_int64(), ctypes.c_int64()
         lib().fvc_ftle_points(h, ctypes.byref(n), ctypes.byref(ins))
         self.n, self.inside = n.value, ins.value
+
+    @classmethod
+    def from_records(cls, ctx, mn, dl, res, recs, have, ftle_wt, ftle_wot, start, stop):
+        """A slide started from stored initial state (FTLE_GenerateMesh = 0)."""
+        mn, dl = np.asarray(mn, np.float64), np.asarray(dl, np.float64)
+        res = np.asarray(res, np.int32)
+        recs = np.ascontiguousarray(recs, POINT_DTYPE)
+        have = np.ascontiguousarray(have, np.int32)
+        ftle_wt = np.ascontiguousarray(ftle_wt, np.float64)
+        ftle_wot = np.ascontiguousarray(ftle_wot, np.float64)
+        n = int(np.prod(res))
+        if not (recs.size == have.size == ftle_wt.size == ftle_wot.size == n):
+            raise ValueError("initial state arrays must hold res[0]*res[1]*res[2] entries")
+        h = ctypes.c_void_p()
+        _chk(lib().fvc_ftle_create_from(ctx.h, _p(mn), _p(dl), _p(res), _p(recs), _p(have), _p(ftle_wt),
+                                        _p(ftle_wot), start, stop, ctypes.byref(h)), "fvc_ftle_create_from", ctx.h)
+        return cls(ctx, None, None, None, None, None, None, _handle=h)
+
+    def initial(self):
+        """The slide's state right after creation: (records, HaveFTLE, FTLEwT, FTLEwoT)."""
+        recs = np.zeros(self.n, POINT_DTYPE)
+        have = np.zeros(self.n, np.int32)
+        wt, wot = np.zeros(self.n), np.zeros(self.n)
+        _chk(lib().fvc_ftle_initial(self.h, _p(recs), _p(have), _p(wt), _p(wot)), "fvc_ftle_initial", self.ctx.h)
+        return recs, have, wt, wot
 
     def frame(self, direction, lo, hi, tmin, tmax, dt, int_type=INT_RK4, extrapolate=0):
         ip = IntParams(int_type, extrapolate, dt, 0, 0)

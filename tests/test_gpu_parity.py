@@ -395,3 +395,35 @@ def test_double_gyre_ftle_reduced_precision(precision, pos_tol, ftle_p99):
     assert np.linalg.norm(r["X"][both] - r64["X"][both], axis=1).max() <= pos_tol
     big = np.abs(w64) > 1e-3
     assert np.percentile(np.abs(w[big] - w64[big]) / np.abs(w64[big]), 99) <= ftle_p99
+
+
+def test_ftle_slide_from_stored_initial_state(pipe):
+    """A slide rebuilt from another slide's initial state (the FTLE_ICFile
+    round trip: fvc_ftle_initial, fvc_ftle_create_from) integrates to the
+    same FTLE, bit for bit, and the rebuilt slide reports the same initial
+    state; grids whose size disagrees with the arrays are refused."""
+    t, xyz, conn, adj, slices = pipe
+    ctx = fvc.Context(3, PIPE_BOUNDS, "f64", 3, coords=xyz, conn=conn, adj=adj)
+    for s in range(3):
+        ctx.load_slice(s, s * 0.02, slices[s])
+    mn, mx, res = [-1.0, -1.0, 0.0], [1.0, 1.0, 10.0], [13, 13, 41]
+    dl = [(mx[a] - mn[a]) / (res[a] - 1) for a in range(3)]
+    A = fvc.FtleSlide(ctx, mn, mx, res, 1, 0.0, 0.04)
+    init = A.initial()
+    assert (init[0]["LeftDomain"] == 0).sum() > 3000 and init[1].sum() > 0
+    B = fvc.FtleSlide.from_records(ctx, mn, dl, res, *init, 0.0, 0.04)
+    for k, v in zip(("recs", "have", "wt", "wot"), B.initial()):
+        if k == "recs":
+            for f in ("X", "ElementIndex", "LeftDomain", "LeftDomainTime"):
+                np.testing.assert_array_equal(v[f], init[0][f], err_msg=f)
+        else:
+            np.testing.assert_array_equal(v, init[("recs", "have", "wt", "wot").index(k)], err_msg=k)
+    out = []
+    for F in (A, B):
+        for f in range(2):
+            F.frame(1, f, f + 1, 0.02 * f, 0.02 * (f + 1), 1e-3)
+        out.append(F.output(0.04))
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+    with pytest.raises(ValueError):
+        fvc.FtleSlide.from_records(ctx, mn, dl, [13, 13, 40], *init, 0.0, 0.04)
