@@ -398,6 +398,27 @@ CASES = {
                                         Trace_MeshBounds_ZMin=-0.5, Trace_MeshBounds_ZMax=10.5,
                                         Trace_MeshBounds_XRes=7, Trace_MeshBounds_YRes=7, Trace_MeshBounds_ZRes=45,
                                         Trace_IntTLength=0.1, Trace_AlwaysOutput=1, Trace_OutFilePrefix="ptg")),
+    # backward-time tracers on the double gyre: two releases 1.0 apart from t=10 back to t=0 (tracers.c:1287-1291)
+    "dg_tracers_back": dict(gen=gen_dg, outputs=["dgtb.%d.bin" % k for k in range(5)],  # no release reaches t=0
+                            deck=dict(DG, Output_TStart=10.0, Output_TRes=6, Output_TDelta=2.0, Int_TimeDirection=-1,
+                                      Trace_Compute=1, Trace_GenerateMesh=1, Trace_MeshBounds_XMin=0.0,
+                                      Trace_MeshBounds_XMax=2.0, Trace_MeshBounds_YMin=0.0,
+                                      Trace_MeshBounds_YMax=1.0, Trace_MeshBounds_XRes=64, Trace_MeshBounds_YRes=32,
+                                      Trace_MeshBounds_ZRes=1, Trace_NumLaunchTimes=2, Trace_LaunchTimeSpacing=1.0,
+                                      Trace_IntTLength=8.0, Trace_OutFilePrefix="dgtb")),
+    # backward-time tracers on the pipe from t=1 (the periodic data's last frame wraps to file 0), exits
+    # through the inlet
+    "pipe_tracers_back": dict(gen=gen_pipe, outputs=["ptb.%d.bin" % k for k in range(6)], ic="ptb.IC",
+                              deck=dict(PIPE, Output_TStart=1.0, Output_TRes=6, Output_TDelta=0.1,
+                                        Int_TimeDirection=-1, Trace_Compute=1, Trace_GenerateMesh=0,
+                                        Trace_InFile="seeds.bin", Trace_InFileFormat=4, Trace_IntTLength=0.5,
+                                        Trace_OutFilePrefix="ptb")),
+    # tracers over 1.6 periods of the periodic pipe data: frames past t=1 read the files again (velocity.c:184-190)
+    "pipe_tracers_periods": dict(gen=gen_pipe, outputs=["ptp.%d.bin" % k for k in range(5)], ic="ptp.IC",
+                                 deck=dict(PIPE, Output_TStart=0.2, Output_TRes=5, Output_TDelta=0.35,
+                                           Int_TimeDirection=1, Trace_Compute=1, Trace_GenerateMesh=0,
+                                           Trace_InFile="seeds.bin", Trace_InFileFormat=4, Trace_IntTLength=1.4,
+                                           Trace_OutFilePrefix="ptp")),
     # three releases, each from its own seed file and .IC.<release> (Trace_MultipleInFiles)
     "pipe_tracers_multi": dict(gen=gen_pipe_multi, outputs=["ptm.%d.bin" % k for k in range(6)], ic="ptm.IC.0",
                                keep=["ptm.IC.1", "ptm.IC.2"],
