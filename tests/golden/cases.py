@@ -427,6 +427,16 @@ CASES = {
                                          Trace_InFile="mseeds", Trace_InFileFormat=4, Trace_MultipleInFiles=1,
                                          Trace_NumLaunchTimes=3, Trace_LaunchTimeSpacing=0.05,
                                          Trace_IntTLength=0.15, Trace_OutFilePrefix="ptm")),
+    # FTLE written every 4 frames during the integration (FTLE_ComputeVariation, ftle.c:188-191, 1253-1325): files
+    # numbered in output order, integration time so far, then the full-length FTLE
+    "dg_ftle_var": dict(gen=gen_dg, outputs=["dgv%s.%d.bin" % (x, k) for x in ("", "_noT") for k in range(4)] +
+                        ["dgv_Cartesian.bin"],
+                        deck=dict(DG, Output_TStart=0.0, Output_TRes=1, Output_TDelta=1.0, Int_TimeDirection=1,
+                                  FTLE_Compute=1, FTLE_MeshBounds_XMin=0.0, FTLE_MeshBounds_XMax=2.0,
+                                  FTLE_MeshBounds_YMin=0.0, FTLE_MeshBounds_YMax=1.0, FTLE_MeshBounds_XRes=64,
+                                  FTLE_MeshBounds_YRes=32, FTLE_MeshBounds_ZRes=1, FTLE_IntTLength=6.0,
+                                  FTLE_ComputeVariation=1, FTLE_VariationOutFreq=4, FTLE_OutFilePrefix="dgv",
+                                  FTLE_ICFile="dgv_ic.bin")),
     # 2-D Cartesian FTLE with Euler, 64x32 grid, T=4 (early exits re-advected with Euler)
     "dg_ftle_euler": dict(gen=gen_dg, outputs=["dgfe.0.bin", "dgfe_noT.0.bin", "dgfe_Cartesian.bin"],
                           deck=dict(DG, Int_Type=0, Output_TStart=0.0, Output_TRes=1, Output_TDelta=1.0,

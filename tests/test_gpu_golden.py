@@ -70,7 +70,8 @@ def test_gpu_trajectories_bit_identical(case, tmp_path):
 
 
 @pytest.mark.parametrize("case", ["dg_ftle", "dg_ftle_back", "pipe_ftle", "dg_ftle_euler", "pipe_ftle_extrap",
-                                  "box_ftle", "chan_ftle", "pipe_ftle_vertices"])
+                                  "box_ftle", "chan_ftle", "pipe_ftle_vertices",
+                                  "dg_ftle_var"])
 def test_gpu_ftle_matches_reference(case, tmp_path):
     man, data, out = run_driver(case, tmp_path)
     fx = cases.fixture_dir(case)
