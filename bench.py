@@ -6,7 +6,8 @@ L=10 voxelised on 60x60x330 hexes, Kuhn 6-tet split (5.6M tets, 0.98M nodes),
 50 velocity slices per period of u_z = 2(1-r^2)(1+0.5 sin 2 pi t) plus swirl
 u_theta = 0.1 r (1-r^2) sin(pi z/10) and N(0, 0.01^2) noise - with
   * 4M tracers uniform in the inlet third, Z-order sorted, RK4 dt=1e-3 over one
-    period (50 frames x 20 steps), exits compacted by the location walk, and
+    period (50 frames x 20 steps; particles that leave keep their slots, and
+    the live ones are listed once more than an eighth of the lanes idle), and
   * a 128x128x640 FTLE grid over the same period (flow-map gradient, largest
     Cauchy-Green eigenvalue), outputs copied back to host memory.
 Inputs are resident in HBM when the timed region starts (mesh, all 50 slices,

@@ -285,7 +285,7 @@ extern "C" int fvc_create(const fvc_mesh_desc *md, const fvc_params *pp, fvc_ctx
         delete c;
         return FVC_E_HIP;
     }
-    if ((rc = dev_alloc(c, &c->d_err, 1)) || (rc = dev_alloc(c, &c->d_where, 1)) || (rc = dev_alloc(c, &c->d_steps, 1))) {
+    if ((rc = dev_alloc(c, &c->d_err, 1)) || (rc = dev_alloc(c, &c->d_where, 1)) || (rc = dev_alloc(c, &c->d_steps, 2))) {
         fvc_destroy(c);
         return rc;
     }
@@ -642,7 +642,14 @@ extern "C" void fvc_particles_destroy(fvc_particles *p) {
     if (!p) return;
     hipStreamSynchronize(p->ctx->stream);
     hipFree(p->buf);
+    if (p->act) hipFree(p->act);
     delete p;
+}
+
+extern "C" int fvc_particles_active(const fvc_particles *p, int64_t *listed) {
+    if (!p || !listed) return FVC_E_INVALID;
+    *listed = p->act_begin >= 0 ? p->act_n : -1;
+    return FVC_OK;
 }
 
 extern "C" int64_t fvc_particles_capacity(const fvc_particles *p) { return p ? p->cap : 0; }
@@ -652,14 +659,15 @@ extern "C" int fvc_particles_set(fvc_particles *p, int64_t begin, const fvc_poin
     fvc_ctx *c = p->ctx;
     /* element ids start walks on the device: outside [-1, ne) they would
      * address memory beyond the mesh (-1 is ompVC's "no element" and raises
-     * FVC_E_ELEM_NEG1 when such a point is integrated).  Points flagged as
-     * left hold no element: ompVC leaves the id of points outside the domain
-     * unset in its stored grids (ftle.c:425-455), so it is kept as -1 here. */
+     * FVC_E_ELEM_NEG1 when such a point is integrated).  On points flagged as
+     * left an id outside that range is stored as -1: ompVC leaves the id of
+     * points outside the domain unset in its stored grids (ftle.c:425-455). */
     const bool uns = c->mesh.mesh_type == 1;
     if (uns)
         for (int64_t i = 0; i < n; i++)
             if (!recs[i].LeftDomain && (recs[i].ElementIndex < -1 || recs[i].ElementIndex >= c->mesh.ne))
                 return set_err(c, FVC_E_INVALID, "particle record with an element id outside the mesh");
+    p->act_begin = -1; /* written state may bring particles back */
     std::vector<double> d(8 * (size_t)n);
     std::vector<int> iv(3 * (size_t)n);
     for (int64_t i = 0; i < n; i++) {
@@ -669,7 +677,8 @@ extern "C" int fvc_particles_set(fvc_particles *p, int64_t begin, const fvc_poin
         }
         d[6 * n + i] = recs[i].LeftDomainTime;
         d[7 * n + i] = recs[i].Scalar;
-        iv[i] = (uns && recs[i].LeftDomain) ? -1 : recs[i].ElementIndex;
+        const int e = recs[i].ElementIndex;
+        iv[i] = (uns && recs[i].LeftDomain && (e < -1 || e >= c->mesh.ne)) ? -1 : e;
         iv[n + i] = recs[i].AuxElementIndex;
         iv[2 * n + i] = recs[i].LeftDomain;
     }
@@ -714,6 +723,7 @@ extern "C" int fvc_particles_get(fvc_particles *p, int64_t begin, fvc_point *rec
 
 static int particles_d2d(fvc_particles *dst, int64_t db, const fvc_particles *src, int64_t sb, int64_t n) {
     fvc_ctx *c = dst->ctx;
+    dst->act_begin = -1; /* copied-in state may bring particles back */
     double *ds[8] = {dst->X[0], dst->X[1], dst->X[2], dst->V[0], dst->V[1], dst->V[2], dst->left_time, dst->scalar};
     double *ss[8] = {src->X[0], src->X[1], src->X[2], src->V[0], src->V[1], src->V[2], src->left_time, src->scalar};
     for (int a = 0; a < 8; a++)
@@ -771,7 +781,27 @@ extern "C" int fvc_advect(fvc_ctx *c, fvc_particles *p, int64_t begin, int64_t c
     DevField f = bind_field(c, lo, hi, tmin, tmax);
     PtrSet ps = ptrs(p);
     if ((rc = bind_strain(c, lo, hi, ip, f, ps, p))) return rc;
-    return launch_advect(c, ps, nullptr, begin, count, t1, t2, f, *ip);
+    /* Particles that left the domain stay in their slots and their lanes
+     * idle.  Once more than an eighth of a range's lanes idle, the range's
+     * particles still to integrate are listed and later calls on the same
+     * range launch over the list only (every particle is integrated exactly
+     * as without the list). */
+    const bool listed = p->act_begin == begin && p->act_count == count && p->act_extrap == ip->extrapolate;
+    const int64_t launched = listed ? p->act_n : count;
+    if ((rc = launch_advect(c, ps, listed ? p->act : nullptr, begin, launched, t1, t2, f, *ip))) return rc;
+    constexpr int64_t LIST_MIN = 4096; /* smaller ranges launch whole */
+    if (count >= LIST_MIN && c->last_idle * 8 > launched) {
+        if (!p->act && dev_alloc(c, &p->act, (size_t)p->cap)) return set_err(c, FVC_E_NOMEM, "active list alloc");
+        const int64_t steps = c->last_steps;
+        int64_t n = 0;
+        if ((rc = launch_active_list(c, ps, begin, count, ip->extrapolate, p->act, &n))) return rc;
+        p->act_begin = begin;
+        p->act_count = count;
+        p->act_n = n;
+        p->act_extrap = ip->extrapolate;
+        c->last_steps = steps;
+    }
+    return FVC_OK;
 }
 
 extern "C" int fvc_advect_times(fvc_ctx *c, fvc_particles *p, int64_t begin, int64_t count, const double *t12,

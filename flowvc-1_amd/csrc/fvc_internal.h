@@ -41,6 +41,7 @@ struct fvc_ctx {
     int64_t *d_where;
     unsigned long long *d_steps;
     int64_t last_steps;
+    int64_t last_idle; /* lanes of the last fvc_advect launch whose particle was not integrated */
     std::string last_error;
     int64_t bytes_used;
     /* scratch (grown on demand) */
@@ -64,6 +65,12 @@ struct fvc_particles {
     double *left_time, *scalar;
     double *H[6]; /* multistep history F1 (x,y,z), F2 (x,y,z) */
     int *nhist;
+    /* particles still to integrate in [act_begin, act_begin + act_count), as
+     * indices relative to act_begin (built by fvc_advect, dropped by any
+     * write of particle state from outside the integrators) */
+    int *act = nullptr;
+    int64_t act_begin = -1, act_count = 0, act_n = 0;
+    int act_extrap = 0;
 };
 
 struct fvc_ftle {
@@ -142,6 +149,7 @@ int launch_velocity(fvc_ctx *c, PtrSet p, int64_t begin, int64_t count, double t
 int launch_cet_encounters(fvc_ctx *c, PtrSet p, int64_t begin, int64_t count, int *enc);
 int launch_advect_times(fvc_ctx *c, PtrSet p, int64_t begin, int64_t count, const double *dtt, const DevField &f,
                         const fvc_int_params &ip);
+int launch_active_list(fvc_ctx *c, PtrSet p, int64_t begin, int64_t count, int extrap, int *list, int64_t *n);
 int launch_advect(fvc_ctx *c, PtrSet p, const int *index, int64_t begin, int64_t count, double t1, double t2,
                   const DevField &f, const fvc_int_params &ip);
 int launch_locate(fvc_ctx *c, const double *dxyz, const int *dguess, int64_t n, int mode, int *dout);

@@ -56,6 +56,36 @@ __device__ inline void add_steps(unsigned long long *ctr, uint32_t ns) {
     }
 }
 
+/* add_steps, plus the number of the block's lanes whose particle was not
+ * integrated into ctr[1] */
+__device__ inline void add_steps_idle(unsigned long long *ctr, uint32_t ns, bool idle) {
+    __shared__ unsigned int red[BLOCK / 64], ired[BLOCK / 64];
+    unsigned long long v = ns;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const unsigned int ni = (unsigned int)__popcll(__ballot(idle));
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+        red[w] = (unsigned int)v;
+        ired[w] = ni;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long s = 0, si = 0;
+        for (int q = 0; q < (int)(blockDim.x >> 6); q++) {
+            s += red[q];
+            si += ired[q];
+        }
+        if (s) atomicAdd(ctr, s);
+        if (si) atomicAdd(ctr + 1, si);
+    }
+}
+
+/* a particle is integrated when it started inside the domain and has not
+ * left it, or left and is extrapolated (ompVC/tracers.c:109, 225) */
+__device__ __forceinline__ bool particle_live(const PtrSet &p, int64_t i, int extrap) {
+    return p.left_time[i] > -TINY && (!p.left[i] || extrap);
+}
+
 __device__ inline void load_state(const PtrSet &p, int64_t i, PState &s, bool hist = false, bool vel = true) {
     for (int c = 0; c < 3; c++) {
         s.X[c] = p.X[c][i];
@@ -103,11 +133,12 @@ __device__ __forceinline__ void advect_body(const DevMesh &m, const DevField &f,
                                             unsigned long long *steps) {
     int64_t q = xcd_block() * blockDim.x + threadIdx.x;
     uint32_t ns = 0;
+    bool idle = false;
     if (q < count) {
-        int64_t i = index ? (int64_t)index[q] : begin + q;
-        double lt = p.left_time[i];
-        int lf = p.left[i];
-        if (lt > -TINY && (!lf || extrap)) {
+        int64_t i = begin + (index ? (int64_t)index[q] : q);
+        if (!particle_live(p, i, extrap)) {
+            idle = true;
+        } else {
             const bool hist = GEN && itype >= 3;
             PState s;
             load_state(p, i, s, hist, extrap);
@@ -119,7 +150,7 @@ __device__ __forceinline__ void advect_body(const DevMesh &m, const DevField &f,
                 store_state(p, i, s, hist, extrap);
         }
     }
-    add_steps(steps, ns);
+    add_steps_idle(steps, ns, idle);
 }
 
 /* RK4 only: no multistep state in registers.  Occupancy 3 waves/SIMD (at
@@ -294,9 +325,29 @@ __global__ __launch_bounds__(BLOCK) void k_advect_times_gen(DevMesh m, DevField 
     k_advect_times_body<MODE, DIM, UNS, 1>(m, f, p, begin, count, tt, itype, dt, extrap, eb, steps);
 }
 
+__global__ __launch_bounds__(BLOCK) void k_live_flags(PtrSet p, int64_t begin, int64_t count, int extrap,
+                                                      int *__restrict__ flags) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < count) flags[q] = particle_live(p, begin + q, extrap) ? 1 : 0;
+}
+
+/* indices (relative to begin) of the particles of [begin, begin + count)
+ * still to integrate, in order */
+int launch_active_list(fvc_ctx *c, PtrSet p, int64_t begin, int64_t count, int extrap, int *list, int64_t *n) {
+    *n = 0;
+    if (count <= 0) return FVC_OK;
+    int *flags = (int *)scratch(c, sizeof(int) * (size_t)count);
+    if (!flags) return FVC_E_NOMEM;
+    k_live_flags<<<grid_for(count, BLOCK), BLOCK, 0, c->stream>>>(p, begin, count, extrap, flags);
+    FVC_HIP(hipGetLastError());
+    return launch_compact(c, flags, count, list, n, nullptr);
+}
+
+/* index: null for every particle of [begin, begin + count), else `count`
+ * indices relative to begin */
 int launch_advect(fvc_ctx *c, PtrSet p, const int *index, int64_t begin, int64_t count, double t1, double t2,
                   const DevField &f, const fvc_int_params &ip) {
-    FVC_HIP(hipMemsetAsync(c->d_steps, 0, sizeof(unsigned long long), c->stream));
+    FVC_HIP(hipMemsetAsync(c->d_steps, 0, 2 * sizeof(unsigned long long), c->stream));
     if (count > 0) {
         int block = ip.block > 0 ? ip.block : BLOCK;
         if (block != BLOCK) block = BLOCK; /* kernels are compiled for 256-lane blocks */
@@ -314,11 +365,12 @@ int launch_advect(fvc_ctx *c, PtrSet p, const int *index, int64_t begin, int64_t
         timing_end(c);
         FVC_HIP(hipGetLastError());
     }
-    unsigned long long steps = 0;
-    FVC_HIP(hipMemcpyAsync(&steps, c->d_steps, sizeof steps, hipMemcpyDeviceToHost, c->stream));
+    unsigned long long steps[2] = {0, 0};
+    FVC_HIP(hipMemcpyAsync(steps, c->d_steps, sizeof steps, hipMemcpyDeviceToHost, c->stream));
     int rc = check_err(c, "fvc_advect");
-    c->last_steps = (int64_t)steps;
-    if (count > 0) timing_collect(c, (int64_t)steps);
+    c->last_steps = (int64_t)steps[0];
+    c->last_idle = (int64_t)steps[1];
+    if (count > 0) timing_collect(c, (int64_t)steps[0]);
     return rc;
 }
 

@@ -4,6 +4,7 @@ Loading fails loudly when the library is not built; there is no CPU fallback.
 """
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -103,6 +104,7 @@ def lib():
         L.fvc_particles_get.argtypes = [vp, i64, vp, i64]
         L.fvc_particles_capacity.restype = i64
         L.fvc_particles_capacity.argtypes = [vp]
+        L.fvc_particles_active.argtypes = [vp, ctypes.POINTER(i64)]
         L.fvc_launch_velocity.argtypes = [vp, vp, i64, i64, d, i32, i32, d, d]
         L.fvc_advect.argtypes = [vp, vp, i64, i64, d, d, i32, i32, d, d, ctypes.POINTER(IntParams)]
         L.fvc_last_step_count.argtypes = [vp, ctypes.POINTER(i64)]
@@ -216,9 +218,17 @@ class Context:
         self.dims = dims
         self.precision = precision
         self.mesh_type = md.mesh_type
+        self._children = weakref.WeakSet()
+
+    def _adopt(self, child):
+        """Objects on this context are destroyed before it (also when the
+        garbage collector finalises a reference cycle in any order)."""
+        self._children.add(child)
 
     def close(self):
         if self.h:
+            for child in list(self._children):
+                child.close()
             lib().fvc_destroy(self.h)
             self.h = None
 
@@ -295,11 +305,13 @@ class Particles:
         h = ctypes.c_void_p()
         _chk(lib().fvc_particles_create(ctx.h, capacity, ctypes.byref(h)), "fvc_particles_create", ctx.h)
         self.h = h
+        ctx._adopt(self)
         self.capacity = capacity
 
     def close(self):
         if self.h:
-            lib().fvc_particles_destroy(self.h)
+            if self.ctx.h:  # never after the context: its destroy call would reach freed state
+                lib().fvc_particles_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -334,6 +346,13 @@ class Particles:
         _chk(lib().fvc_launch_velocity(self.ctx.h, self.h, begin, count, t, lo, hi, tmin, tmax),
              "fvc_launch_velocity", self.ctx.h)
 
+    def active(self):
+        """Length of the list of particles still to integrate that the next
+        advect over the last range launches, or -1 (whole range)."""
+        n = ctypes.c_int64()
+        _chk(lib().fvc_particles_active(self.h, ctypes.byref(n)), "fvc_particles_active", self.ctx.h)
+        return n.value
+
     def advect(self, t1, t2, lo, hi, tmin, tmax, dt, int_type=INT_RK4, extrapolate=0, begin=0, count=None):
         count = self.capacity - begin if count is None else count
         ip = IntParams(int_type, extrapolate, dt, 0, 0)
@@ -354,6 +373,7 @@ class FtleSlide:
         else:
             h = _handle
         self.h = h
+        ctx._adopt(self)
         n, ins = ctypes.c_int64(), ctypes.c_int64()
         lib().fvc_ftle_points(h, ctypes.byref(n), ctypes.byref(ins))
         self.n, self.inside = n.value, ins.value
@@ -406,7 +426,8 @@ class FtleSlide:
 
     def close(self):
         if self.h:
-            lib().fvc_ftle_destroy(self.h)
+            if self.ctx.h:  # never after the context: its destroy call would reach freed state
+                lib().fvc_ftle_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -432,6 +453,7 @@ class Batch:
         _chk(lib().fvc_batch_create(ctx.h, particles.h, arr, len(releases), t0, frame_dt, nslots, dt,
                                     ctypes.byref(h)), "fvc_batch_create", ctx.h)
         self.h = h
+        ctx._adopt(self)
 
     def reset(self):
         _chk(lib().fvc_batch_reset(self.h), "fvc_batch_reset", self.ctx.h)
@@ -455,7 +477,8 @@ class Batch:
 
     def close(self):
         if self.h:
-            lib().fvc_batch_destroy(self.h)
+            if self.ctx.h:  # never after the context: its destroy call would reach freed state
+                lib().fvc_batch_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -477,6 +500,7 @@ class Server:
         h = ctypes.c_void_p()
         _chk(lib().fvc_server_create(ctx.h, ctypes.byref(sp), ctypes.byref(h)), "fvc_server_create", ctx.h)
         self.h = h
+        ctx._adopt(self)
         self._keep = {}
 
     def capacity(self):
@@ -539,7 +563,8 @@ class Server:
 
     def close(self):
         if self.h:
-            lib().fvc_server_destroy(self.h)
+            if self.ctx.h:  # never after the context: its destroy call would reach freed state
+                lib().fvc_server_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -144,11 +144,20 @@ int fvc_particles_create(fvc_ctx *ctx, int64_t capacity, fvc_particles **out);
 void fvc_particles_destroy(fvc_particles *p);
 /* copy n host records into positions [begin, begin+n) / back out.  On an
  * unstructured mesh a record not flagged LeftDomain whose element id lies
- * outside [-1, number of elements) is rejected (FVC_E_INVALID); records
- * flagged LeftDomain are stored with element id -1. */
+ * outside [-1, number of elements) is rejected (FVC_E_INVALID); on records
+ * flagged LeftDomain such an id is stored as -1. */
 int fvc_particles_set(fvc_particles *p, int64_t begin, const fvc_point *recs, int64_t n);
 int fvc_particles_get(fvc_particles *p, int64_t begin, fvc_point *recs, int64_t n);
 int64_t fvc_particles_capacity(const fvc_particles *p);
+/* Particles that leave the domain keep their slots.  Once more than an
+ * eighth of the lanes of an fvc_advect call over a range (4096 particles or
+ * more) found their particle gone, fvc_advect lists that range's particles
+ * still to integrate and later calls on the same range launch over the list
+ * only.  *listed receives the list's length, or -1 when the next call
+ * launches its whole range.  Writing particle state (fvc_particles_set,
+ * fvc_particles_copy) drops the list.  No counterpart in the reference, whose
+ * loops test every point (ompVC/tracers.c:109, 225). */
+int fvc_particles_active(const fvc_particles *p, int64_t *listed);
 /* device-to-device copy of n records (a release re-launched from resident
  * initial conditions, as CreateNewTraceLaunch copies the .IC file,
  * ompVC/tracers.c:1579-1632) */

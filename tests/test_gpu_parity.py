@@ -347,7 +347,8 @@ def test_particle_records_with_foreign_element_ids(pipe):
     """Records from files carry element ids the kernels walk from: an id
     outside [-1, elements) on a point not flagged as left is refused before
     anything reaches the device; points flagged as left are stored without
-    an element (ompVC leaves those ids unset in its stored grids)."""
+    an element when their id lies outside the mesh (ompVC leaves those ids
+    unset in its stored grids) and keep it otherwise."""
     t, xyz, conn, adj, slices = pipe
     ctx = fvc.Context(3, PIPE_BOUNDS, "f64", 2, coords=xyz, conn=conn, adj=adj)
     recs = ctx.seed_list(host.tube_seeds(t, 64, 0.0, 10.0, 3), 1)
@@ -359,7 +360,7 @@ def test_particle_records_with_foreign_element_ids(pipe):
         with pytest.raises(fvc.FvcError):
             P.set(r)
     r = recs.copy()
-    r["LeftDomain"][7] = 1
+    r["LeftDomain"][7:9] = 1
     r["ElementIndex"][7] = -2072131723
     P.set(r)
     back = P.get()
@@ -427,3 +428,44 @@ def test_ftle_slide_from_stored_initial_state(pipe):
     np.testing.assert_array_equal(out[0][1], out[1][1])
     with pytest.raises(ValueError):
         fvc.FtleSlide.from_records(ctx, mn, dl, [13, 13, 40], *init, 0.0, 0.04)
+
+
+@pytest.mark.parametrize("extrapolate", [0, 1])
+def test_tracer_list_of_live_particles(pipe, extrapolate):
+    """Seeds near the outlet mostly leave within a few frames: fvc_advect
+    then launches over a list of the particles still to integrate, and every
+    record equals the run that launches the whole range each frame (its
+    state rewritten from the host between frames, which drops any list).
+    With extrapolation nothing is idle, so no list is built."""
+    t, xyz, conn, adj, slices = pipe
+    ctx = fvc.Context(3, PIPE_BOUNDS, "f64", 3, coords=xyz, conn=conn, adj=adj)
+    for s in range(3):
+        ctx.load_slice(s, s * 0.02, slices[s])
+    seeds = host.tube_seeds(t, 20000, 9.97, 10.0, 5)
+    recs = ctx.seed_list(seeds, 1)
+    A, B = fvc.Particles(ctx, recs.shape[0]), fvc.Particles(ctx, recs.shape[0])
+    A.set(recs)
+    B.set(recs)
+    listed = []
+    for P in (A, B):
+        P.launch_velocity(0.0, 0, 1, 0.0, 0.02)
+    sa = sb = 0
+    for k in range(8):
+        t1, t2 = 0.0025 * k, 0.0025 * (k + 1)
+        sa += A.advect(t1, t2, 0, 1, 0.0, 0.02, 1e-3, extrapolate=extrapolate)
+        listed.append(A.active())
+        sb += B.advect(t1, t2, 0, 1, 0.0, 0.02, 1e-3, extrapolate=extrapolate)
+        B.set(B.get())
+        assert B.active() == -1
+    print("active-list lengths:", listed, flush=True)
+    a, b = A.get(), B.get()
+    assert sa == sb
+    for f in ("X", "V", "ElementIndex", "LeftDomain", "LeftDomainTime"):
+        np.testing.assert_array_equal(a[f], b[f], err_msg=f)
+    if extrapolate:
+        assert listed == [-1] * 8
+    else:
+        gone = int((a["LeftDomain"] != 0).sum())
+        assert gone > recs.shape[0] // 4
+        assert any(0 <= n < recs.shape[0] for n in listed)
+        assert listed[-1] >= int((a["LeftDomain"] == 0).sum())
