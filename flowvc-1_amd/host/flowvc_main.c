@@ -61,6 +61,9 @@ typedef struct {
     double io_s;
     int64_t nsr;   /* strain-rate values per frame when the activation potential is integrated, else 0 */
     double *srbuf;
+    int32_t nwall;  /* Int_NormalFlow: wall nodes whose velocity every slice replaces */
+    int32_t *wall_ids;
+    double *wall_v; /* their velocity, 3 per node */
 } slices_t;
 
 static int file_of(const fvc_deck *d, int frame) {
@@ -90,6 +93,8 @@ static int slot_for(slices_t *s, int frame, int keep) {
              s->d->Data_SuffixTMin + f * s->d->Data_SuffixTDelta);
     double t0 = now_s(), ts;
     if (fvc_read_vel_slice(path, &ts, s->buf, s->n)) fatal("Could not load velocity data from %s", path);
+    for (int32_t i = 0; i < s->nwall; i++) /* inward flow on the walls (velocity.c:213-228) */
+        for (int a = 0; a < 3; a++) s->buf[3 * (int64_t)s->wall_ids[i] + a] = s->wall_v[3 * i + a];
     check(fvc_load_slice(s->c, q, ts, s->buf, s->n), s->c, "fvc_load_slice");
     if (s->nsr) { /* LoadUnstructStrainRateDataFrame / LoadCartStrainRateDataFrame (strainrate.c:53-173) */
         snprintf(path, sizeof path, "%s%s_strain-rate.%d.bin", s->d->Path_Data, s->d->Data_InFilePrefix,
@@ -789,8 +794,11 @@ int main(int argc, char **argv) {
         fvc_deck_derive(&d, err, sizeof err))
         fatal("%s", err);
     if (d.legacy_96 && !quiet) fprintf(stderr, "Note: deck has 96 entries (no Num_threads); read as Num_threads = 0\n");
-    if (d.Particle_Radius > TINY || d.Int_Type == 2 || d.Int_NormalFlow)
-        fatal("%s", "deck selects a mode outside the advection path (inertial particles, RKF, normal flow)");
+    if (d.Particle_Radius > TINY || d.Int_Type == 2)
+        fatal("%s", "deck selects a mode outside the advection path (inertial particles, RKF)");
+    /* the staggered release loads its first frame with normal flow switched off (tracers.c:488-495, 792-793) */
+    if (d.Int_NormalFlow && d.Data_MeshType == 1 && d.Trace_Compute && d.Trace_ReleaseStrategy == 1)
+        fatal("%s", "normal flow with the staggered release is not supported by this driver");
     const int ap = d.Trace_Compute && d.Trace_APCompute;
     const int cet = d.Trace_Compute && d.Trace_CETCompute, rt = d.Trace_Compute && d.Trace_RTCompute;
     if (cet && (d.Trace_CETAuxillaryMesh || d.Data_MeshType != 1))
@@ -843,6 +851,25 @@ int main(int argc, char **argv) {
     sl.slot_file = (int *)malloc(sizeof(int) * sl.nslots);
     for (int q = 0; q < sl.nslots; q++) sl.slot_file[q] = -1;
     sl.buf = (double *)malloc(sizeof(double) * 3 * (size_t)sl.n);
+    if (d.Int_NormalFlow && d.Data_MeshType == 1) { /* LoadMeshSurfaceNormals (mesh.c:255-300) */
+        snprintf(path, sizeof path, "%s%s_normals.bin", d.Path_Data, d.Data_InFilePrefix);
+        FILE *fn = fopen(path, "rb");
+        if (!fn) fatal("Could not open file %s", path);
+        if (fread(&sl.nwall, 4, 1, fn) != 1 || sl.nwall < 0)
+            fatal("Could not read value for Vel_SurfaceMeshNumNodes from %s", path);
+        sl.wall_ids = (int32_t *)malloc(sizeof(int32_t) * (size_t)(sl.nwall > 0 ? sl.nwall : 1));
+        sl.wall_v = (double *)malloc(sizeof(double) * 3 * (size_t)(sl.nwall > 0 ? sl.nwall : 1));
+        if (fread(sl.wall_ids, 4, (size_t)sl.nwall, fn) != (size_t)sl.nwall)
+            fatal("Could not completely read Vel_SurfaceMeshNodeIDs array from %s", path);
+        if (fread(sl.wall_v, 8, 3 * (size_t)sl.nwall, fn) != 3 * (size_t)sl.nwall)
+            fatal("Could not completely read inward normal vector data from %s", path);
+        fclose(fn);
+        for (int32_t i = 0; i < sl.nwall; i++) {
+            if (sl.wall_ids[i] < 0 || sl.wall_ids[i] >= md.num_nodes) fatal("surface node id out of range in %s", path);
+            for (int a = 0; a < 3; a++) /* Int_TimeDirection * Int_NormalFlowScaling * normal, in that order */
+                sl.wall_v[3 * i + a] = d.Int_TimeDirection * d.Int_NormalFlowScaling * sl.wall_v[3 * i + a];
+        }
+    }
     if (ap) {
         sl.nsr = d.Data_MeshType == 1 ? md.num_elems
                                       : (int64_t)(md.cart_res[0] - 1) * (md.cart_res[1] - 1) *
@@ -1230,6 +1257,8 @@ int main(int argc, char **argv) {
     free(sl.slot_file);
     free(sl.buf);
     free(sl.srbuf);
+    free(sl.wall_ids);
+    free(sl.wall_v);
     free(xyz);
     free(conn);
     free(adj);

@@ -141,6 +141,36 @@ def gen_pipe_multi(d):
         host.write_binary_list(os.path.join(d, "mseeds.%d.bin" % ff), host.tube_seeds(t, cnt, 0.0, 10.0 / 3.0, seed))
 
 
+def pipe_walls():
+    """Wall nodes of the pipe (on a boundary face, strictly between inlet and
+    outlet) and their inward normals, radial towards the axis."""
+    t = host.pipe(*PIPE_DIMS, 1.0, 10.0)
+    xyz, conn, adj = host.tube_mesh(t)
+    opposite = [1, 2, 3, 0]  # adjacency slot -> node opposite the face it crosses
+    nodes = set()
+    for s in range(4):
+        e = np.nonzero(adj[:, s] < 0)[0]
+        keep = [k for k in range(4) if k != opposite[s]]
+        nodes.update(conn[e][:, keep].ravel().tolist())
+    ids = np.array(sorted(nodes), np.int32)
+    z = xyz[ids, 2]
+    ids = ids[(z > 1e-9) & (z < 10.0 - 1e-9)]
+    r = np.hypot(xyz[ids, 0], xyz[ids, 1])
+    nrm = np.stack([-xyz[ids, 0] / r, -xyz[ids, 1] / r, np.zeros(ids.size)], 1)
+    return ids, nrm
+
+
+def gen_pipe_normal(d):
+    """The pipe with <prefix>_normals.bin (mesh.c:255-300): count, node ids,
+    inward normals."""
+    gen_pipe(d)
+    ids, nrm = pipe_walls()
+    with open(os.path.join(d, "pipe_normals.bin"), "wb") as f:
+        np.array([ids.size], "<i4").tofile(f)
+        ids.astype("<i4").tofile(f)
+        np.ascontiguousarray(nrm, "<f8").tofile(f)
+
+
 def gen_dg_velout(d):
     gen_dg(d)
     rng = np.random.default_rng(9)
@@ -419,6 +449,20 @@ CASES = {
                                            Int_TimeDirection=1, Trace_Compute=1, Trace_GenerateMesh=0,
                                            Trace_InFile="seeds.bin", Trace_InFileFormat=4, Trace_IntTLength=1.4,
                                            Trace_OutFilePrefix="ptp")),
+    # inward flow on the walls (Int_NormalFlow, velocity.c:213-228): wall-node velocities of every slice replaced
+    # by 0.2 x the inward normal, forward and backward in time
+    "pipe_tracers_normal": dict(gen=gen_pipe_normal, outputs=["ptn.%d.bin" % k for k in range(6)], ic="ptn.IC",
+                                deck=dict(PIPE, Int_NormalFlow=1, Int_NormalFlowScaling=0.2, Output_TStart=0.0,
+                                          Output_TRes=6, Output_TDelta=0.05, Int_TimeDirection=1, Trace_Compute=1,
+                                          Trace_GenerateMesh=0, Trace_InFile="seeds.bin", Trace_InFileFormat=4,
+                                          Trace_IntTLength=0.25, Trace_OutFilePrefix="ptn")),
+    "pipe_tracers_normal_back": dict(gen=gen_pipe_normal, outputs=["ptnb.%d.bin" % k for k in range(6)],
+                                     ic="ptnb.IC",
+                                     deck=dict(PIPE, Int_NormalFlow=1, Int_NormalFlowScaling=0.2, Output_TStart=0.5,
+                                               Output_TRes=6, Output_TDelta=0.05, Int_TimeDirection=-1,
+                                               Trace_Compute=1, Trace_GenerateMesh=0, Trace_InFile="seeds.bin",
+                                               Trace_InFileFormat=4, Trace_IntTLength=0.25,
+                                               Trace_OutFilePrefix="ptnb")),
     # three releases, each from its own seed file and .IC.<release> (Trace_MultipleInFiles)
     "pipe_tracers_multi": dict(gen=gen_pipe_multi, outputs=["ptm.%d.bin" % k for k in range(6)], ic="ptm.IC.0",
                                keep=["ptm.IC.1", "ptm.IC.2"],

@@ -38,7 +38,7 @@ def run_driver(case, tmp_path, precision="f64"):
                                   "pipe_tracers_cet", "pipe_tracers_rt", "dg_tracers_euler", "pipe_tracers_euler",
                                   "pipe_tracers_extrap", "box_tracers", "chan_tracers", "pipe_tracers_multi",
                                   "pipe_tracers_grid", "dg_tracers_back", "pipe_tracers_back",
-                                  "pipe_tracers_periods"])
+                                  "pipe_tracers_periods", "pipe_tracers_normal", "pipe_tracers_normal_back"])
 def test_gpu_trajectories_bit_identical(case, tmp_path):
     man, data, out = run_driver(case, tmp_path)
     fx = cases.fixture_dir(case)
