@@ -769,16 +769,36 @@ extern "C" int fvc_launch_velocity(fvc_ctx *c, fvc_particles *p, int64_t begin, 
     return launch_velocity(c, ptrs(p), begin, count, t, bind_field(c, lo, hi, tmin, tmax));
 }
 
+/* Runge-Kutta-Fehlberg parameters: positive (parameters.c:462-463); the
+ * activation potential and exposure times are not integrated with it here */
+static int check_rkf(fvc_ctx *c, const fvc_int_params *ip) {
+    if (ip->int_type != FVC_INT_RKF) return FVC_OK;
+    if (!(ip->rkf_min_step >= 1.0e-10) || !(ip->rkf_max_step >= 1.0e-10) || !(ip->rkf_accuracy >= 1.0e-10))
+        return set_err(c, FVC_E_INVALID, "RKF integration parameters must be positive");
+    if (ip->activation_potential || ip->exposure_substeps)
+        return set_err(c, FVC_E_INVALID, "RKF with activation potential or exposure times is not available");
+    return FVC_OK;
+}
+
+static void bind_rkf(DevField &f, const fvc_int_params *ip) {
+    f.rkf_min = ip->rkf_min_step;
+    f.rkf_max = ip->rkf_max_step;
+    f.rkf_tol = ip->rkf_accuracy;
+}
+
 extern "C" int fvc_advect(fvc_ctx *c, fvc_particles *p, int64_t begin, int64_t count, double t1, double t2, int32_t lo,
                           int32_t hi, double tmin, double tmax, const fvc_int_params *ip) {
     if (!c || !p || !ip || begin < 0 || count < 0 || begin + count > p->cap) return FVC_E_INVALID;
     if (ip->int_type != FVC_INT_RK4 && ip->int_type != FVC_INT_EULER && ip->int_type != FVC_INT_AB2 &&
-        ip->int_type != FVC_INT_AB3)
+        ip->int_type != FVC_INT_AB3 && ip->int_type != FVC_INT_RKF)
         return set_err(c, FVC_E_INVALID, "integrator not available in fvc_advect");
-    if (!(ip->time_step > 1.0e-10)) return set_err(c, FVC_E_INVALID, "Int_TimeStep must be positive");
-    int rc = check_slots(c, lo, hi, tmin, tmax);
+    int rc = check_rkf(c, ip);
     if (rc) return rc;
+    if (ip->int_type != FVC_INT_RKF && !(ip->time_step > 1.0e-10))
+        return set_err(c, FVC_E_INVALID, "Int_TimeStep must be positive");
+    if ((rc = check_slots(c, lo, hi, tmin, tmax))) return rc;
     DevField f = bind_field(c, lo, hi, tmin, tmax);
+    bind_rkf(f, ip);
     PtrSet ps = ptrs(p);
     if ((rc = bind_strain(c, lo, hi, ip, f, ps, p))) return rc;
     /* Particles that left the domain stay in their slots and their lanes
@@ -1143,7 +1163,9 @@ extern "C" int fvc_ftle_frame(fvc_ftle *f, int32_t dir, int32_t lo, int32_t hi, 
     if (!(f->status == 1 || (f->start_time <= tmax + T && f->start_time >= tmin - T))) return FVC_OK;
     int rc = check_slots(c, lo, hi, tmin, tmax);
     if (rc) return rc;
-    if (ip->int_type != FVC_INT_RK4 && ip->int_type != FVC_INT_EULER) return FVC_E_INVALID;
+    if (ip->int_type != FVC_INT_RK4 && ip->int_type != FVC_INT_EULER && ip->int_type != FVC_INT_RKF)
+        return FVC_E_INVALID;
+    if ((rc = check_rkf(c, ip))) return rc;
     double t1;
     if (f->status == 0) {
         f->status = 1;

@@ -25,6 +25,8 @@
 #include <hip/hip_fp16.h>
 #include <stdint.h>
 
+#include "fvc_root4.h"
+
 namespace fvc {
 
 constexpr double TINY = 1.0e-10; /* ompVC/macros.h:17 */
@@ -68,6 +70,8 @@ struct DevField {
     unsigned long long *cet_lo = nullptr, *cet_hi = nullptr;
     int *cet_enc = nullptr;
     int cet_sub = 0; /* Trace_CETSubsteps */
+    /* Runge-Kutta-Fehlberg: Int_MinTimeStep, Int_MaxTimeStep, Int_Accuracy */
+    double rkf_min = 0.0, rkf_max = 0.0, rkf_tol = 0.0;
 };
 
 struct ErrBuf {
@@ -633,7 +637,7 @@ struct IState {
     int started;
 };
 
-enum { INT_EULER = 0, INT_RK4 = 1, INT_AB2 = 3, INT_AB3 = 4 };
+enum { INT_EULER = 0, INT_RK4 = 1, INT_RKF = 2, INT_AB2 = 3, INT_AB3 = 4 };
 
 /* Strain rate at the point (GetStrainRate_Unstructured, ompVC/strainrate.c:
  * 232-246; GetStrainRate_Cartesian, :196-230), linear in time between the two
@@ -923,6 +927,111 @@ __device__ int interval_run(const DevMesh &m, const DevField &f, PState &p, ISta
     return 1;
 }
 
+/* Runge-Kutta-Fehlberg 4(5) over [a, b] (ompVC/integration.c:464-699): the
+ * fourth-order result is kept and the step adapted from its difference to
+ * the fifth-order one; the step starts at Int_MaxTimeStep every call and six
+ * probes keep their own elements within the call.  A probe outside stops the
+ * point at the step's start time, without revert; a position outside after
+ * an accepted step is reverted as in RK4.  Accepted steps are counted. */
+template <int MODE, int DIM, int UNSTRUCT>
+__device__ double rkf_run(const DevMesh &m, const DevField &f, PState &p, double a, double b, int extrap, int &err,
+                          uint32_t &nsteps) {
+#pragma clang fp contract(off)
+    if (p.left) {
+        err = E_LEFT;
+        return b;
+    }
+    if (UNSTRUCT && (p.elem < 0 || p.elem >= m.ne)) {
+        err = E_ELEM;
+        return b;
+    }
+    Probe mp[6];
+    for (int q = 0; q < 6; q++) {
+        mp[q].elem = p.elem;
+        mp[q].left = 0;
+    }
+    const double a3 = 3.0 / 32.0, b3 = 9.0 / 32.0;
+    const double a4 = 1932.0 / 2197.0, b4 = -7200.0 / 2197.0, c4 = 7296.0 / 2197.0;
+    const double a5 = 439.0 / 216.0, b5 = -8.0, c5 = 3680.0 / 513.0, d5 = -845.0 / 4104.0;
+    const double a6 = -8.0 / 27.0, b6 = 2.0, c6 = -3544.0 / 2565.0, d6 = 1859.0 / 4104.0, e6 = -11.0 / 40.0;
+    const double a7 = 1.0 / 360.0, b7 = -128.0 / 4275.0, c7 = -2197.0 / 75240.0, d7 = 1.0 / 50.0, e7 = 2.0 / 55.0;
+    const double a8 = 25.0 / 216.0, b8 = 1408.0 / 2565.0, c8 = 2197.0 / 4104.0, d8 = -1.0 / 5.0;
+    const double hmax = f.rkf_max, hmin = f.rkf_min, tol = f.rkf_tol;
+    double h = sign_of(hmax, b - a), tc = a;
+    double k1[3], k2[3], k3[3], k4[3], k5[3], k6[3], Y[3];
+    while ((tc - b) * (b - a) < 0.0) {
+        if ((tc + h - b) * (tc + h - a) > 0.0) h = sign_of(b - tc, b - a);
+        stage_velocity<MODE, DIM, UNSTRUCT>(m, f, tc, p.X, mp[0], k1, err);
+        if (err) return b;
+        for (int c = 0; c < 3; c++) Y[c] = p.X[c] + 0.25 * k1[c] * h;
+        stage_velocity<MODE, DIM, UNSTRUCT>(m, f, tc + 0.25 * h, Y, mp[1], k2, err);
+        if (err) return b;
+        for (int c = 0; c < 3; c++) Y[c] = p.X[c] + (a3 * k1[c] + b3 * k2[c]) * h;
+        stage_velocity<MODE, DIM, UNSTRUCT>(m, f, tc + 0.375 * h, Y, mp[2], k3, err);
+        if (err) return b;
+        for (int c = 0; c < 3; c++) Y[c] = p.X[c] + (a4 * k1[c] + b4 * k2[c] + c4 * k3[c]) * h;
+        stage_velocity<MODE, DIM, UNSTRUCT>(m, f, tc + (12.0 * h) / 13.0, Y, mp[3], k4, err);
+        if (err) return b;
+        for (int c = 0; c < 3; c++) Y[c] = p.X[c] + (a5 * k1[c] + b5 * k2[c] + c5 * k3[c] + d5 * k4[c]) * h;
+        stage_velocity<MODE, DIM, UNSTRUCT>(m, f, tc + h, Y, mp[4], k5, err);
+        if (err) return b;
+        for (int c = 0; c < 3; c++)
+            Y[c] = p.X[c] + (a6 * k1[c] + b6 * k2[c] + c6 * k3[c] + d6 * k4[c] + e6 * k5[c]) * h;
+        stage_velocity<MODE, DIM, UNSTRUCT>(m, f, tc + 0.5 * h, Y, mp[5], k6, err);
+        if (err) return b;
+        if (mp[0].left || mp[1].left || mp[2].left || mp[3].left || mp[4].left || mp[5].left) {
+            p.left = 1;
+            p.left_time = tc;
+            return tc;
+        }
+        double er = 0.0;
+        for (int c = 0; c < 3; c++) {
+            const double ec = fabs(a7 * k1[c] + b7 * k3[c] + c7 * k4[c] + d7 * k5[c] + e7 * k6[c]);
+            er = c == 0 ? ec : fmax(er, ec);
+        }
+        double ss = er == 0.0 ? 0.0 : fvc_root4_rn(fabs((tol * h) / (2 * er)));
+        if (er < tol || fabs(h) <= hmin) {
+            double inc[3];
+            for (int c = 0; c < 3; c++) {
+                inc[c] = (a8 * k1[c] + b8 * k3[c] + c8 * k4[c] + d8 * k5[c]) * h;
+                p.X[c] = p.X[c] + inc[c];
+            }
+            tc = tc + h;
+            nsteps++;
+            bool out;
+            if (UNSTRUCT) {
+                double r, s, t;
+                const int e = walk<MODE, DIM>(m, p.X[0], p.X[1], p.X[2], p.elem, ACC_SEARCH, r, s, t, err);
+                if (e == -2) return b;
+                p.elem = e;
+                out = e < 0;
+            } else {
+                out = outside_domain(m, p.X[0], p.X[1], p.X[2]);
+            }
+            if (out) {
+                p.left = 1;
+                p.left_time = tc - h;
+                for (int c = 0; c < 3; c++) p.X[c] = p.X[c] - inc[c];
+                return p.left_time;
+            }
+            if (extrap) {
+                get_velocity<MODE, DIM>(m, f, tc, p, p.V, err);
+                if (err) return b;
+            }
+            if (fabs(h) > hmin && ss > 1.0) {
+                ss = ss > 4.0 ? 4.0 : ss;
+                h = sign_of(fabs(ss * h) > hmax ? hmax : ss * h, b - a);
+            }
+        } else if (ss < 1.0) {
+            ss = ss < 0.1 ? 0.1 : ss;
+            h = sign_of(fabs(ss * h) < hmin ? hmin : ss * h, b - a);
+        } else {
+            h = sign_of(fabs(0.5 * h) < hmin ? hmin : 0.5 * h, b - a);
+        }
+    }
+    return b;
+}
+
 /* Advect (ompVC/integration.c:22-45) over [t1, t2] in one go */
 template <int MODE, int DIM, int UNSTRUCT, int GEN = 1>
 __device__ __forceinline__ double advect(const DevMesh &m, const DevField &f, PState &p, double t1, double t2, int itype,
@@ -932,10 +1041,14 @@ __device__ __forceinline__ double advect(const DevMesh &m, const DevField &f, PS
         for (int c = 0; c < 3; c++) p.X[c] = p.X[c] + p.V[c] * (t2 - t1);
         return t2;
     }
-    IState is;
-    is.started = 0;
     double stop = t2;
-    interval_run<MODE, DIM, UNSTRUCT, GEN>(m, f, p, is, t1, t2, dt, itype, extrap, 0x7fffffff, err, nsteps, stop);
+    if (GEN && itype == INT_RKF) {
+        stop = rkf_run<MODE, DIM, UNSTRUCT>(m, f, p, t1, t2, extrap, err, nsteps);
+    } else {
+        IState is;
+        is.started = 0;
+        interval_run<MODE, DIM, UNSTRUCT, GEN>(m, f, p, is, t1, t2, dt, itype, extrap, 0x7fffffff, err, nsteps, stop);
+    }
     if ((fabs(stop - t2) > TINY) && extrap)
         for (int c = 0; c < 3; c++) p.X[c] = p.X[c] + p.V[c] * (t2 - stop);
     return stop;

@@ -1680,8 +1680,11 @@ int launch_ftle_frame(fvc_ftle *f, int lo, int hi, double tmin, double tmax, dou
                       const fvc_int_params &ip) {
     fvc_ctx *c = f->ctx;
     DevField fld = bind_field(c, lo, hi, tmin, tmax);
+    fld.rkf_min = ip.rkf_min_step;
+    fld.rkf_max = ip.rkf_max_step;
+    fld.rkf_tol = ip.rkf_accuracy;
     FtleDev d = ftle_dev(f);
-    int itype = ip.int_type == FVC_INT_RK4 ? 1 : 0;
+    const int itype = ip.int_type == FVC_INT_RK4 ? INT_RK4 : ip.int_type == FVC_INT_RKF ? INT_RKF : INT_EULER;
     int *flags = (int *)scratch(c, sizeof(int) * (size_t)f->n);
     if (!flags) return FVC_E_NOMEM;
     k_ftle_flags<<<grid_for(f->n, BLOCK), BLOCK, 0, c->stream>>>(d, f->n, ip.extrapolate, f->zperm, flags);

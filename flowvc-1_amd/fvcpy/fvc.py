@@ -14,7 +14,7 @@ from .host import POINT_DTYPE
 OK = 0
 PREC = {"f64": 0, "f32": 1, "bf16": 2, "f16": 3}
 LOCATE_WALK, LOCATE_GLOBAL, LOCATE_WALK_GLOBAL = 0, 1, 2
-INT_EULER, INT_RK4, INT_AB2, INT_AB3 = 0, 1, 3, 4
+INT_EULER, INT_RK4, INT_RKF, INT_AB2, INT_AB3 = 0, 1, 2, 3, 4
 
 _LIB = None
 
@@ -34,7 +34,8 @@ class Params(ctypes.Structure):
 class IntParams(ctypes.Structure):
     _fields_ = [("int_type", ctypes.c_int32), ("extrapolate", ctypes.c_int32), ("time_step", ctypes.c_double),
                 ("block", ctypes.c_int32), ("steps_per_launch", ctypes.c_int32),
-                ("activation_potential", ctypes.c_int32), ("exposure_substeps", ctypes.c_int32)]
+                ("activation_potential", ctypes.c_int32), ("exposure_substeps", ctypes.c_int32),
+                ("rkf_min_step", ctypes.c_double), ("rkf_max_step", ctypes.c_double), ("rkf_accuracy", ctypes.c_double)]
 
 
 class Release(ctypes.Structure):
@@ -353,9 +354,11 @@ class Particles:
         _chk(lib().fvc_particles_active(self.h, ctypes.byref(n)), "fvc_particles_active", self.ctx.h)
         return n.value
 
-    def advect(self, t1, t2, lo, hi, tmin, tmax, dt, int_type=INT_RK4, extrapolate=0, begin=0, count=None):
+    def advect(self, t1, t2, lo, hi, tmin, tmax, dt, int_type=INT_RK4, extrapolate=0, begin=0, count=None,
+               rkf=(0.0, 0.0, 0.0)):
+        """rkf: (Int_MinTimeStep, Int_MaxTimeStep, Int_Accuracy) for INT_RKF."""
         count = self.capacity - begin if count is None else count
-        ip = IntParams(int_type, extrapolate, dt, 0, 0)
+        ip = IntParams(int_type, extrapolate, dt, 0, 0, 0, 0, *rkf)
         _chk(lib().fvc_advect(self.ctx.h, self.h, begin, count, t1, t2, lo, hi, tmin, tmax, ctypes.byref(ip)),
              "fvc_advect", self.ctx.h)
         return self.ctx.last_steps()
@@ -403,8 +406,8 @@ class FtleSlide:
         _chk(lib().fvc_ftle_initial(self.h, _p(recs), _p(have), _p(wt), _p(wot)), "fvc_ftle_initial", self.ctx.h)
         return recs, have, wt, wot
 
-    def frame(self, direction, lo, hi, tmin, tmax, dt, int_type=INT_RK4, extrapolate=0):
-        ip = IntParams(int_type, extrapolate, dt, 0, 0)
+    def frame(self, direction, lo, hi, tmin, tmax, dt, int_type=INT_RK4, extrapolate=0, rkf=(0.0, 0.0, 0.0)):
+        ip = IntParams(int_type, extrapolate, dt, 0, 0, 0, 0, *rkf)
         done = ctypes.c_int32()
         _chk(lib().fvc_ftle_frame(self.h, direction, lo, hi, tmin, tmax, ctypes.byref(ip), ctypes.byref(done)),
              "fvc_ftle_frame", self.ctx.h)

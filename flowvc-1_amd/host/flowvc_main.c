@@ -794,8 +794,10 @@ int main(int argc, char **argv) {
         fvc_deck_derive(&d, err, sizeof err))
         fatal("%s", err);
     if (d.legacy_96 && !quiet) fprintf(stderr, "Note: deck has 96 entries (no Num_threads); read as Num_threads = 0\n");
-    if (d.Particle_Radius > TINY || d.Int_Type == 2)
-        fatal("%s", "deck selects a mode outside the advection path (inertial particles, RKF)");
+    if (d.Particle_Radius > TINY) fatal("%s", "deck selects a mode outside the advection path (inertial particles)");
+    if (d.Int_Type == 2 && d.Trace_Compute &&
+        (d.Trace_ReleaseStrategy == 1 || d.Trace_APCompute || d.Trace_CETCompute))
+        fatal("%s", "RKF with the staggered release, activation potential or exposure times is not supported");
     /* the staggered release loads its first frame with normal flow switched off (tracers.c:488-495, 792-793) */
     if (d.Int_NormalFlow && d.Data_MeshType == 1 && d.Trace_Compute && d.Trace_ReleaseStrategy == 1)
         fatal("%s", "normal flow with the staggered release is not supported by this driver");
@@ -876,8 +878,10 @@ int main(int argc, char **argv) {
                                             ((d.Dimensions == 2 || md.cart_res[2] == 1) ? 1 : md.cart_res[2] - 1);
         sl.srbuf = (double *)malloc(sizeof(double) * (size_t)(sl.nsr > 0 ? sl.nsr : 1));
     }
-    fvc_int_params ip = {d.Int_Type == 1 ? FVC_INT_RK4 : FVC_INT_EULER, d.Int_Extrapolate, d.Int_TimeStep, 0, 0, ap,
-                         cet ? d.Trace_CETSubsteps : 0};
+    /* Advect's dispatch (integration.c:32-37): 1 RK4, 2 RKF, anything else Euler */
+    fvc_int_params ip = {d.Int_Type == 1 ? FVC_INT_RK4 : d.Int_Type == 2 ? FVC_INT_RKF : FVC_INT_EULER,
+                         d.Int_Extrapolate, d.Int_TimeStep, 0, 0, ap, cet ? d.Trace_CETSubsteps : 0,
+                         d.Int_MinTimeStep, d.Int_MaxTimeStep, d.Int_Accuracy};
     if (cet) check(fvc_cet_reset(c), c, "fvc_cet_reset");
     double t_compute = 0.0;
     int64_t steps_total = 0;

@@ -235,6 +235,9 @@ int fvc_deck_check(const fvc_deck *d, char *err, int errlen) {
     if (d->Output_TRes < 1) FAIL("Output_TRes < 1");
     if (d->Output_TDelta < FVC_TINY && d->Output_TRes > 1) FAIL("Output_TDelta must be positive");
     if (d->Int_TimeStep < FVC_TINY && d->Int_Type != 2) FAIL("Int_TimeStep must be positive");
+    if ((d->Int_MinTimeStep < FVC_TINY || d->Int_MaxTimeStep < FVC_TINY || d->Int_Accuracy < FVC_TINY) &&
+        d->Int_Type == 2) /* parameters.c:462-463 */
+        FAIL("RKF integration parameters must be positive");
     if (d->Int_TimeDirection != 1 && d->Int_TimeDirection != -1)
         FAIL("Unsupported Int_TimeDirection value, must be -1 or 1");
     if (d->FTLE_Compute && d->Trace_Compute)

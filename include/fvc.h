@@ -49,7 +49,7 @@ enum fvc_precision {
     FVC_PREC_F16 = 3   /* as F32 with velocity stored fp16 x per-slice scale */
 };
 /* Int_Type (ompVC/integration.c:32-37) plus multistep extensions */
-enum fvc_int_type { FVC_INT_EULER = 0, FVC_INT_RK4 = 1, FVC_INT_AB2 = 3, FVC_INT_AB3 = 4 };
+enum fvc_int_type { FVC_INT_EULER = 0, FVC_INT_RK4 = 1, FVC_INT_RKF = 2, FVC_INT_AB2 = 3, FVC_INT_AB3 = 4 };
 /* point location modes (ompVC/mesh.c:304-398, :496-611) */
 enum fvc_locate_mode {
     FVC_LOCATE_WALK = 0,        /* Get_Element_Local_Search(X, guess): -1 when the walk leaves */
@@ -108,6 +108,10 @@ typedef struct fvc_int_params {
      * fvc_cet_reset first; unstructured meshes, fvc_advect and
      * fvc_advect_times only */
     int32_t exposure_substeps;
+    /* FVC_INT_RKF (ompVC/integration.c:464-699): Int_MinTimeStep,
+     * Int_MaxTimeStep, Int_Accuracy, each positive; fvc_advect and
+     * fvc_ftle_frame, without activation potential or exposure times */
+    double rkf_min_step, rkf_max_step, rkf_accuracy;
 } fvc_int_params;
 
 typedef struct fvc_ctx fvc_ctx;

@@ -9,7 +9,7 @@
  *   ompVC/mesh.c:304-398 (global search), mesh.c:496-611 (local walk),
  *   ompVC/velocity.c:257-457 (Cartesian interpolation), velocity.c:459-621
  *   (unstructured interpolation), velocity.c:624-669 (domain tests),
- *   ompVC/integration.c:22-45 (Advect), :222-311 (Euler), :317-462 (RK4),
+ *   ompVC/integration.c:22-45 (Advect), :222-311 (Euler), :317-462 (RK4), :464-699 (RKF),
  *   ompVC/ftle.c:24-377 (per-frame FTLE), :383-744 (grid init + location chain),
  *   :788-1048 (early FTLE), :1050-1203 (flow-map gradient), :1244-1329 (output),
  *   ompVC/tracers.c:25-289 (tracer frames), :998-1577 (seed location chains),
@@ -74,9 +74,10 @@ typedef struct oc_mesh {
 } oc_mesh;
 
 typedef struct oc_int {
-    int32_t int_type;    /* 0 Euler, 1 RK4 */
+    int32_t int_type;    /* 0 Euler, 1 RK4, 2 RKF */
     double time_step;    /* Int_TimeStep */
     int32_t extrapolate; /* Int_Extrapolate */
+    double min_step, max_step, accuracy; /* RKF: Int_MinTimeStep, Int_MaxTimeStep, Int_Accuracy */
 } oc_int;
 
 /* construction (arrays are copied) */

@@ -464,6 +464,91 @@ fail:
     return t1;
 }
 
+/* integration.c:464-699: Runge-Kutta-Fehlberg 4(5), the fourth-order result
+ * kept, step size from the difference of the two; six probes that keep their
+ * own elements within the call; step starts at Int_MaxTimeStep each call.
+ * Steps counted: accepted ones. */
+static double rkf(const oc_mesh *m, oc_point *p, double t0, double t1, const oc_int *ip, int *err, int64_t *ns) {
+    oc_point mp[6];
+    if (p->LeftDomain) {
+        *err = OC_E_LEFT;
+        return t1;
+    }
+    memset(mp, 0, sizeof mp);
+    if (m->mesh_type == 1) {
+        if (p->ElementIndex == -1) {
+            *err = OC_E_ELEM_NEG1;
+            return t1;
+        }
+        for (int q = 0; q < 6; q++) mp[q].ElementIndex = p->ElementIndex;
+    }
+    const double a3 = 3.0 / 32.0, b3 = 9.0 / 32.0;
+    const double a4 = 1932.0 / 2197.0, b4 = -7200.0 / 2197.0, c4 = 7296.0 / 2197.0;
+    const double a5 = 439.0 / 216.0, b5 = -8.0, c5 = 3680.0 / 513.0, d5 = -845.0 / 4104.0;
+    const double a6 = -8.0 / 27.0, b6 = 2.0, c6 = -3544.0 / 2565.0, d6 = 1859.0 / 4104.0, e6 = -11.0 / 40.0;
+    const double a7 = 1.0 / 360.0, b7 = -128.0 / 4275.0, c7 = -2197.0 / 75240.0, d7 = 1.0 / 50.0, e7 = 2.0 / 55.0;
+    const double a8 = 25.0 / 216.0, b8 = 1408.0 / 2565.0, c8 = 2197.0 / 4104.0, d8 = -1.0 / 5.0;
+    double h = sign_of(ip->max_step, t1 - t0), hmax = ip->max_step, tol = ip->accuracy, tc = t0, ss;
+    double k1[3], k2[3], k3[3], k4[3], k5[3], k6[3];
+    int rc;
+    while ((tc - t1) * (t1 - t0) < 0.0) {
+        if ((tc + h - t1) * (tc + h - t0) > 0.0) h = sign_of(t1 - tc, t1 - t0);
+        for (int c = 0; c < 3; c++) mp[0].X[c] = p->X[c];
+        if ((rc = oc_velocity(m, tc, &mp[0], k1))) goto fail;
+        for (int c = 0; c < 3; c++) mp[1].X[c] = p->X[c] + 0.25 * k1[c] * h;
+        if ((rc = oc_velocity(m, tc + 0.25 * h, &mp[1], k2))) goto fail;
+        for (int c = 0; c < 3; c++) mp[2].X[c] = p->X[c] + (a3 * k1[c] + b3 * k2[c]) * h;
+        if ((rc = oc_velocity(m, tc + 0.375 * h, &mp[2], k3))) goto fail;
+        for (int c = 0; c < 3; c++) mp[3].X[c] = p->X[c] + (a4 * k1[c] + b4 * k2[c] + c4 * k3[c]) * h;
+        if ((rc = oc_velocity(m, tc + (12.0 * h) / 13.0, &mp[3], k4))) goto fail;
+        for (int c = 0; c < 3; c++) mp[4].X[c] = p->X[c] + (a5 * k1[c] + b5 * k2[c] + c5 * k3[c] + d5 * k4[c]) * h;
+        if ((rc = oc_velocity(m, tc + h, &mp[4], k5))) goto fail;
+        for (int c = 0; c < 3; c++)
+            mp[5].X[c] = p->X[c] + (a6 * k1[c] + b6 * k2[c] + c6 * k3[c] + d6 * k4[c] + e6 * k5[c]) * h;
+        if ((rc = oc_velocity(m, tc + 0.5 * h, &mp[5], k6))) goto fail;
+        if (mp[0].LeftDomain || mp[1].LeftDomain || mp[2].LeftDomain || mp[3].LeftDomain || mp[4].LeftDomain ||
+            mp[5].LeftDomain) {
+            p->LeftDomain = 1;
+            p->LeftDomainTime = tc;
+            return p->LeftDomainTime;
+        }
+        double e[3];
+        for (int c = 0; c < 3; c++) e[c] = fabs(a7 * k1[c] + b7 * k3[c] + c7 * k4[c] + d7 * k5[c] + e7 * k6[c]);
+        const double er = fmax(fmax(e[0], e[1]), e[2]);
+        ss = er == 0.0 ? 0.0 : pow(fabs((tol * h) / (2 * er)), 0.25);
+        if (er < tol || fabs(h) <= ip->min_step) {
+            for (int c = 0; c < 3; c++) p->X[c] = p->X[c] + (a8 * k1[c] + b8 * k3[c] + c8 * k4[c] + d8 * k5[c]) * h;
+            tc = tc + h;
+            (*ns)++;
+            if (m->mesh_type == 1) {
+                p->ElementIndex = oc_local_search(m, p->X, p->ElementIndex, err);
+                if (*err) return t1;
+            }
+            if ((m->mesh_type == 0 && oc_test_outside_domain(m, p->X)) || (m->mesh_type == 1 && p->ElementIndex < 0)) {
+                p->LeftDomain = 1;
+                p->LeftDomainTime = tc - h;
+                for (int c = 0; c < 3; c++)
+                    p->X[c] = p->X[c] - (a8 * k1[c] + b8 * k3[c] + c8 * k4[c] + d8 * k5[c]) * h;
+                return p->LeftDomainTime;
+            }
+            if (ip->extrapolate && (rc = oc_velocity(m, tc, p, p->V))) goto fail;
+            if (fabs(h) > ip->min_step && ss > 1.0) {
+                ss = (ss > 4.0) ? 4.0 : ss;
+                h = sign_of((fabs(ss * h) > hmax) ? hmax : ss * h, t1 - t0);
+            }
+        } else if (ss < 1.0) {
+            ss = (ss < 0.1) ? 0.1 : ss;
+            h = sign_of((fabs(ss * h) < ip->min_step) ? ip->min_step : ss * h, t1 - t0);
+        } else {
+            h = sign_of((fabs(0.5 * h) < ip->min_step) ? ip->min_step : 0.5 * h, t1 - t0);
+        }
+    }
+    return t1;
+fail:
+    *err = rc;
+    return t1;
+}
+
 /* Adams-Bashforth 2/3 (no ompVC counterpart; this project's definition, the
  * same as the GPU path's): fixed step x+ = x + h (3 f0 - f1)/2 (AB2) or
  * x+ = x + h (23 f0 - 16 f1 + 5 f2)/12 (AB3), RK4 steps while history is short
@@ -585,8 +670,9 @@ double oc_advect_counted(const oc_mesh *m, const oc_int *ip, oc_point *p, double
         for (int c = 0; c < 3; c++) p->X[c] = p->X[c] + p->V[c] * (t2 - t1);
         return t2;
     }
-    double stop = ip->int_type == 1 ? rk4(m, p, t1, t2, ip->time_step, ip->extrapolate, err, ns)
-                                    : euler(m, p, t1, t2, ip->time_step, ip->extrapolate, err, ns);
+    double stop = ip->int_type == 1   ? rk4(m, p, t1, t2, ip->time_step, ip->extrapolate, err, ns)
+                  : ip->int_type == 2 ? rkf(m, p, t1, t2, ip, err, ns)
+                                      : euler(m, p, t1, t2, ip->time_step, ip->extrapolate, err, ns);
     if ((fabs(stop - t2) > TINY) && ip->extrapolate)
         for (int c = 0; c < 3; c++) p->X[c] = p->X[c] + p->V[c] * (t2 - stop);
     return stop;

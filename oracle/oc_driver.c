@@ -494,7 +494,7 @@ typedef struct {
     double out_tstart, out_tdelta;
     int out_tres;
     int int_type, direction, extrapolate;
-    double dt;
+    double dt, accuracy, min_step, max_step;
     double particle_radius;
     int lsc;
     int ftle, ftle_gen, ftle_res[3], ftle_var, ftle_varfreq;
@@ -569,6 +569,9 @@ static int read_deck(const char *path, deck_t *d, char *msg, int msglen) {
     d->out_tdelta = D(21);
     d->int_type = I(22);
     d->dt = D(23);
+    d->accuracy = D(24);
+    d->min_step = D(25);
+    d->max_step = D(26);
     d->direction = I(27);
     d->extrapolate = I(30);
     d->particle_radius = D(31);
@@ -752,7 +755,7 @@ int oc_run_deck(const char *deck_path, int nproc, char *msg, int msglen) {
     deck_t d;
     int rc = read_deck(deck_path, &d, msg, msglen);
     if (rc) return rc;
-    if (d.particle_radius > TINY || d.int_type == 2 || d.trace_strategy == 1 || d.trace_cet || d.trace_rt ||
+    if (d.particle_radius > TINY || d.trace_strategy == 1 || d.trace_cet || d.trace_rt ||
         d.trace_ap || d.velout || d.ftle_var) {
         snprintf(msg, msglen, "deck uses a mode outside the advection path");
         return OC_E_PARAM;
@@ -763,7 +766,8 @@ int oc_run_deck(const char *deck_path, int nproc, char *msg, int msglen) {
         snprintf(msg, msglen, "could not load mesh");
         return OC_E_IO;
     }
-    oc_int ip = {d.int_type == 1 ? 1 : 0, d.dt, d.extrapolate};
+    oc_int ip = {d.int_type == 1 || d.int_type == 2 ? d.int_type : 0, d.dt, d.extrapolate, d.min_step, d.max_step,
+                 d.accuracy};
     int dir = d.direction;
     int first, last;
     frames(&d, &first, &last);

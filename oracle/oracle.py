@@ -18,7 +18,8 @@ _LIB = None
 
 
 class OcInt(ctypes.Structure):
-    _fields_ = [("int_type", ctypes.c_int32), ("time_step", ctypes.c_double), ("extrapolate", ctypes.c_int32)]
+    _fields_ = [("int_type", ctypes.c_int32), ("time_step", ctypes.c_double), ("extrapolate", ctypes.c_int32),
+                ("min_step", ctypes.c_double), ("max_step", ctypes.c_double), ("accuracy", ctypes.c_double)]
 
 
 def build():

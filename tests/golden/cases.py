@@ -481,6 +481,27 @@ CASES = {
                                   FTLE_MeshBounds_YRes=32, FTLE_MeshBounds_ZRes=1, FTLE_IntTLength=6.0,
                                   FTLE_ComputeVariation=1, FTLE_VariationOutFreq=4, FTLE_OutFilePrefix="dgv",
                                   FTLE_ICFile="dgv_ic.bin")),
+    # Runge-Kutta-Fehlberg (Int_Type=2, integration.c:464-699): adaptive steps between Int_MinTimeStep and
+    # Int_MaxTimeStep at Int_Accuracy, on the double gyre (generated seeds) and the pipe (exits), and FTLE
+    "dg_tracers_rkf": dict(gen=gen_dg, outputs=["dgr.%d.bin" % k for k in range(6)],
+                           deck=dict(DG, Int_Type=2, Int_Accuracy=1e-6, Int_MinTimeStep=1e-4, Int_MaxTimeStep=0.05,
+                                     Output_TStart=0.0, Output_TRes=6, Output_TDelta=2.0, Int_TimeDirection=1,
+                                     Trace_Compute=1, Trace_GenerateMesh=1, Trace_MeshBounds_XMin=0.0,
+                                     Trace_MeshBounds_XMax=2.0, Trace_MeshBounds_YMin=0.0,
+                                     Trace_MeshBounds_YMax=1.0, Trace_MeshBounds_XRes=64, Trace_MeshBounds_YRes=32,
+                                     Trace_MeshBounds_ZRes=1, Trace_IntTLength=10.0, Trace_OutFilePrefix="dgr")),
+    "pipe_tracers_rkf": dict(gen=gen_pipe, outputs=["ptr2.%d.bin" % k for k in range(6)], ic="ptr2.IC",
+                             deck=dict(PIPE, Int_Type=2, Int_Accuracy=1e-6, Int_MinTimeStep=1e-5, Int_MaxTimeStep=0.01,
+                                       Output_TStart=0.0, Output_TRes=6, Output_TDelta=0.2, Int_TimeDirection=1,
+                                       Trace_Compute=1, Trace_GenerateMesh=0, Trace_InFile="probes.txt",
+                                       Trace_InFileFormat=1, Trace_IntTLength=1.0, Trace_OutFilePrefix="ptr2")),
+    "dg_ftle_rkf": dict(gen=gen_dg, outputs=["dgfr.0.bin", "dgfr_noT.0.bin", "dgfr_Cartesian.bin"],
+                        deck=dict(DG, Int_Type=2, Int_Accuracy=1e-6, Int_MinTimeStep=1e-4, Int_MaxTimeStep=0.05,
+                                  Output_TStart=0.0, Output_TRes=1, Output_TDelta=1.0, Int_TimeDirection=1,
+                                  FTLE_Compute=1, FTLE_MeshBounds_XMin=0.0, FTLE_MeshBounds_XMax=2.0,
+                                  FTLE_MeshBounds_YMin=0.0, FTLE_MeshBounds_YMax=1.0, FTLE_MeshBounds_XRes=64,
+                                  FTLE_MeshBounds_YRes=32, FTLE_MeshBounds_ZRes=1, FTLE_IntTLength=4.0,
+                                  FTLE_OutFilePrefix="dgfr", FTLE_ICFile="dgfr_ic.bin")),
     # 2-D Cartesian FTLE with Euler, 64x32 grid, T=4 (early exits re-advected with Euler)
     "dg_ftle_euler": dict(gen=gen_dg, outputs=["dgfe.0.bin", "dgfe_noT.0.bin", "dgfe_Cartesian.bin"],
                           deck=dict(DG, Int_Type=0, Output_TStart=0.0, Output_TRes=1, Output_TDelta=1.0,

@@ -38,7 +38,8 @@ def run_driver(case, tmp_path, precision="f64"):
                                   "pipe_tracers_cet", "pipe_tracers_rt", "dg_tracers_euler", "pipe_tracers_euler",
                                   "pipe_tracers_extrap", "box_tracers", "chan_tracers", "pipe_tracers_multi",
                                   "pipe_tracers_grid", "dg_tracers_back", "pipe_tracers_back",
-                                  "pipe_tracers_periods", "pipe_tracers_normal", "pipe_tracers_normal_back"])
+                                  "pipe_tracers_periods", "pipe_tracers_normal", "pipe_tracers_normal_back",
+                                  "dg_tracers_rkf", "pipe_tracers_rkf"])
 def test_gpu_trajectories_bit_identical(case, tmp_path):
     man, data, out = run_driver(case, tmp_path)
     fx = cases.fixture_dir(case)
@@ -71,7 +72,7 @@ def test_gpu_trajectories_bit_identical(case, tmp_path):
 
 @pytest.mark.parametrize("case", ["dg_ftle", "dg_ftle_back", "pipe_ftle", "dg_ftle_euler", "pipe_ftle_extrap",
                                   "box_ftle", "chan_ftle", "pipe_ftle_vertices",
-                                  "dg_ftle_var"])
+                                  "dg_ftle_var", "dg_ftle_rkf"])
 def test_gpu_ftle_matches_reference(case, tmp_path):
     man, data, out = run_driver(case, tmp_path)
     fx = cases.fixture_dir(case)
@@ -258,3 +259,4 @@ def test_gpu_ftle_from_reference_ic_file(case, tmp_path):
         assert a.shape == b.shape, f
         np.testing.assert_array_equal(a == -1.0, b == -1.0)
         np.testing.assert_allclose(b, a, rtol=1e-12, atol=1e-12, err_msg=f)
+

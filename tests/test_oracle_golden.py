@@ -33,7 +33,7 @@ def run_oracle(case, tmp_path, oracle_dir):
 @pytest.mark.parametrize("case", ["dg_tracers", "pipe_tracers", "pipe_locate", "dg_tracers_euler",
                                   "pipe_tracers_euler", "pipe_tracers_extrap", "box_tracers", "chan_tracers",
                                   "pipe_tracers_grid", "dg_tracers_back", "pipe_tracers_back",
-                                  "pipe_tracers_periods"])
+                                  "pipe_tracers_periods", "dg_tracers_rkf", "pipe_tracers_rkf"])
 def test_trajectories_bit_identical(case, tmp_path, oracle_built, host_built):
     man, data, out = run_oracle(case, tmp_path, oracle_built)
     fx = cases.fixture_dir(case)
@@ -51,7 +51,8 @@ def test_trajectories_bit_identical(case, tmp_path, oracle_built, host_built):
 
 
 @pytest.mark.parametrize("case", ["dg_ftle", "dg_ftle_back", "pipe_ftle", "dg_ftle_euler", "pipe_ftle_extrap",
-                                  "box_ftle", "chan_ftle", "pipe_ftle_vertices"])
+                                  "box_ftle", "chan_ftle", "pipe_ftle_vertices",
+                                  "dg_ftle_rkf"])
 def test_ftle_matches_reference(case, tmp_path, oracle_built, host_built):
     man, data, out = run_oracle(case, tmp_path, oracle_built)
     fx = cases.fixture_dir(case)
