@@ -769,14 +769,11 @@ extern "C" int fvc_launch_velocity(fvc_ctx *c, fvc_particles *p, int64_t begin, 
     return launch_velocity(c, ptrs(p), begin, count, t, bind_field(c, lo, hi, tmin, tmax));
 }
 
-/* Runge-Kutta-Fehlberg parameters: positive (parameters.c:462-463); the
- * activation potential and exposure times are not integrated with it here */
+/* Runge-Kutta-Fehlberg parameters: positive (parameters.c:462-463) */
 static int check_rkf(fvc_ctx *c, const fvc_int_params *ip) {
     if (ip->int_type != FVC_INT_RKF) return FVC_OK;
     if (!(ip->rkf_min_step >= 1.0e-10) || !(ip->rkf_max_step >= 1.0e-10) || !(ip->rkf_accuracy >= 1.0e-10))
         return set_err(c, FVC_E_INVALID, "RKF integration parameters must be positive");
-    if (ip->activation_potential || ip->exposure_substeps)
-        return set_err(c, FVC_E_INVALID, "RKF with activation potential or exposure times is not available");
     return FVC_OK;
 }
 
@@ -828,9 +825,12 @@ extern "C" int fvc_advect_times(fvc_ctx *c, fvc_particles *p, int64_t begin, int
                                 int32_t lo, int32_t hi, double tmin, double tmax, const fvc_int_params *ip) {
     if (!c || !p || !ip || begin < 0 || count < 0 || begin + count > p->cap || (count > 0 && !t12))
         return FVC_E_INVALID;
-    if (ip->int_type != FVC_INT_RK4 && ip->int_type != FVC_INT_EULER)
+    if (ip->int_type != FVC_INT_RK4 && ip->int_type != FVC_INT_EULER && ip->int_type != FVC_INT_RKF)
         return set_err(c, FVC_E_INVALID, "integrator not available in fvc_advect_times");
-    if (!(ip->time_step > 1.0e-10)) return set_err(c, FVC_E_INVALID, "Int_TimeStep must be positive");
+    int rc0 = check_rkf(c, ip);
+    if (rc0) return rc0;
+    if (ip->int_type != FVC_INT_RKF && !(ip->time_step > 1.0e-10))
+        return set_err(c, FVC_E_INVALID, "Int_TimeStep must be positive");
     int rc = check_slots(c, lo, hi, tmin, tmax);
     if (rc) return rc;
     for (int64_t q = 0; q < count; q++) { /* each segment inside the loaded interval, t1 <= t2 */
@@ -844,6 +844,7 @@ extern "C" int fvc_advect_times(fvc_ctx *c, fvc_particles *p, int64_t begin, int
         return FVC_OK;
     }
     DevField f = bind_field(c, lo, hi, tmin, tmax);
+    bind_rkf(f, ip);
     PtrSet ps = ptrs(p);
     if ((rc = bind_strain(c, lo, hi, ip, f, ps, p))) return rc;
     double *d = nullptr;

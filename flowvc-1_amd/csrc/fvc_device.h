@@ -957,10 +957,17 @@ __device__ double rkf_run(const DevMesh &m, const DevField &f, PState &p, double
     const double a7 = 1.0 / 360.0, b7 = -128.0 / 4275.0, c7 = -2197.0 / 75240.0, d7 = 1.0 / 50.0, e7 = 2.0 / 55.0;
     const double a8 = 25.0 / 216.0, b8 = 1408.0 / 2565.0, c8 = 2197.0 / 4104.0, d8 = -1.0 / 5.0;
     const double hmax = f.rkf_max, hmin = f.rkf_min, tol = f.rkf_tol;
+    const bool ap = f.sr_lo != nullptr;                 /* activation potential (integration.c:534-539, 671-677) */
+    const bool cet = UNSTRUCT && f.cet_lo != nullptr;   /* exposure time (integration.c:650-670) */
     double h = sign_of(hmax, b - a), tc = a;
     double k1[3], k2[3], k3[3], k4[3], k5[3], k6[3], Y[3];
     while ((tc - b) * (b - a) < 0.0) {
         if ((tc + h - b) * (tc + h - a) > 0.0) h = sign_of(b - tc, b - a);
+        double sr0 = 0.0;
+        if (ap) {
+            sr0 = strain_rate<DIM, UNSTRUCT>(m, f, tc, p, err);
+            if (err) return b;
+        }
         stage_velocity<MODE, DIM, UNSTRUCT>(m, f, tc, p.X, mp[0], k1, err);
         if (err) return b;
         for (int c = 0; c < 3; c++) Y[c] = p.X[c] + 0.25 * k1[c] * h;
@@ -1017,6 +1024,17 @@ __device__ double rkf_run(const DevMesh &m, const DevField &f, PState &p, double
             if (extrap) {
                 get_velocity<MODE, DIM>(m, f, tc, p, p.V, err);
                 if (err) return b;
+            }
+            if (cet) {
+                double X0[3];
+                for (int c = 0; c < 3; c++) X0[c] = p.X[c] - inc[c];
+                cet_step<DIM>(m, f, X0, p.X, p.elem, h, err);
+                if (err) return b;
+            }
+            if (ap) {
+                const double sr1 = strain_rate<DIM, UNSTRUCT>(m, f, tc, p, err);
+                if (err) return b;
+                p.scalar += (h * ((sr0 + sr1) / 2));
             }
             if (fabs(h) > hmin && ss > 1.0) {
                 ss = ss > 4.0 ? 4.0 : ss;

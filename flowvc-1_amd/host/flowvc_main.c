@@ -795,9 +795,6 @@ int main(int argc, char **argv) {
         fatal("%s", err);
     if (d.legacy_96 && !quiet) fprintf(stderr, "Note: deck has 96 entries (no Num_threads); read as Num_threads = 0\n");
     if (d.Particle_Radius > TINY) fatal("%s", "deck selects a mode outside the advection path (inertial particles)");
-    if (d.Int_Type == 2 && d.Trace_Compute &&
-        (d.Trace_ReleaseStrategy == 1 || d.Trace_APCompute || d.Trace_CETCompute))
-        fatal("%s", "RKF with the staggered release, activation potential or exposure times is not supported");
     /* the staggered release loads its first frame with normal flow switched off (tracers.c:488-495, 792-793) */
     if (d.Int_NormalFlow && d.Data_MeshType == 1 && d.Trace_Compute && d.Trace_ReleaseStrategy == 1)
         fatal("%s", "normal flow with the staggered release is not supported by this driver");

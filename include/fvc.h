@@ -109,8 +109,8 @@ typedef struct fvc_int_params {
      * fvc_advect_times only */
     int32_t exposure_substeps;
     /* FVC_INT_RKF (ompVC/integration.c:464-699): Int_MinTimeStep,
-     * Int_MaxTimeStep, Int_Accuracy, each positive; fvc_advect and
-     * fvc_ftle_frame, without activation potential or exposure times */
+     * Int_MaxTimeStep, Int_Accuracy, each positive; fvc_advect,
+     * fvc_advect_times and fvc_ftle_frame */
     double rkf_min_step, rkf_max_step, rkf_accuracy;
 } fvc_int_params;
 

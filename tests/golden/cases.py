@@ -495,6 +495,26 @@ CASES = {
                                        Output_TStart=0.0, Output_TRes=6, Output_TDelta=0.2, Int_TimeDirection=1,
                                        Trace_Compute=1, Trace_GenerateMesh=0, Trace_InFile="probes.txt",
                                        Trace_InFileFormat=1, Trace_IntTLength=1.0, Trace_OutFilePrefix="ptr2")),
+    # RKF with the staggered release, the activation potential and exposure times (integration.c:534-539, 650-677)
+    "pipe_staggered_rkf": dict(gen=gen_pipe_stagger, outputs=["psr.%d.bin" % k for k in range(9)],
+                               deck=dict(STAGGER, Int_Type=2, Int_Accuracy=1e-6, Int_MinTimeStep=1e-5,
+                                         Int_MaxTimeStep=0.01, Trace_OutFilePrefix="psr")),
+    "pipe_tracers_rkf_ap": dict(gen=gen_pipe_ap, outputs=["pra.%d.bin" % k for k in range(6)] +
+                                ["pra_AP.0.bin", "pra_coordinates.bin"], ic="pra.IC",
+                                deck=dict(PIPE, Int_Type=2, Int_Accuracy=1e-6, Int_MinTimeStep=1e-5,
+                                          Int_MaxTimeStep=0.01, Output_TStart=0.0, Output_TRes=6, Output_TDelta=0.04,
+                                          Int_TimeDirection=1, Trace_Compute=1, Trace_GenerateMesh=0,
+                                          Trace_InFile="seeds.bin", Trace_InFileFormat=4, Trace_IntTLength=0.15,
+                                          Trace_APCompute=1, Trace_OutFilePrefix="pra")),
+    "pipe_tracers_rkf_cet": dict(gen=gen_pipe, outputs=["prc.%d.bin" % k for k in range(3)] +
+                                 ["prcet_CET.0.bin", "prcet_MET.0.bin", "prcet_ENC.0.bin"], ic="prc.IC",
+                                 close=["prcet_CET.0.bin", "prcet_MET.0.bin"],
+                                 deck=dict(PIPE, Num_threads=1, Int_Type=2, Int_Accuracy=1e-6, Int_MinTimeStep=1e-5,
+                                           Int_MaxTimeStep=0.01, Output_TStart=0.0, Output_TRes=3,
+                                           Output_TDelta=0.05, Int_TimeDirection=1, Trace_Compute=1,
+                                           Trace_GenerateMesh=0, Trace_InFile="seeds.bin", Trace_InFileFormat=4,
+                                           Trace_IntTLength=0.1, Trace_CETCompute=1, Trace_CETSubsteps=5,
+                                           Trace_RTOutFilePrefix="prcet", Trace_OutFilePrefix="prc")),
     "dg_ftle_rkf": dict(gen=gen_dg, outputs=["dgfr.0.bin", "dgfr_noT.0.bin", "dgfr_Cartesian.bin"],
                         deck=dict(DG, Int_Type=2, Int_Accuracy=1e-6, Int_MinTimeStep=1e-4, Int_MaxTimeStep=0.05,
                                   Output_TStart=0.0, Output_TRes=1, Output_TDelta=1.0, Int_TimeDirection=1,
