@@ -598,7 +598,15 @@ static int64_t run_staggered(const fvc_deck *d, fvc_ctx *c, slices_t *sl, const 
     double voxdim = 0.0;
     fvc_point *rp = stagger_points(d, c, &nloc, &voxdim);
     if (!quiet) printf("Determining release times...");
+    /* release times come from the velocity as read: ompVC determines them
+     * before it loads the wall normals (main.c:55-67, tracers.c:488-495) */
+    const int32_t nwall = sl->nwall;
+    sl->nwall = 0;
     snodes_t S = stagger_schedule(d, c, sl, rp, nloc, voxdim);
+    if (nwall) { /* integration reads every slice with the wall velocities */
+        sl->nwall = nwall;
+        for (int q = 0; q < sl->nslots; q++) sl->slot_file[q] = -1;
+    }
     if (!quiet) printf("OK!\nTotal number of points to be released is %lld\n", (long long)S.n);
     const int64_t n = S.n;
     const int TR = d->Output_TRes;
@@ -795,9 +803,6 @@ int main(int argc, char **argv) {
         fatal("%s", err);
     if (d.legacy_96 && !quiet) fprintf(stderr, "Note: deck has 96 entries (no Num_threads); read as Num_threads = 0\n");
     if (d.Particle_Radius > TINY) fatal("%s", "deck selects a mode outside the advection path (inertial particles)");
-    /* the staggered release loads its first frame with normal flow switched off (tracers.c:488-495, 792-793) */
-    if (d.Int_NormalFlow && d.Data_MeshType == 1 && d.Trace_Compute && d.Trace_ReleaseStrategy == 1)
-        fatal("%s", "normal flow with the staggered release is not supported by this driver");
     const int ap = d.Trace_Compute && d.Trace_APCompute;
     const int cet = d.Trace_Compute && d.Trace_CETCompute, rt = d.Trace_Compute && d.Trace_RTCompute;
     if (cet && (d.Trace_CETAuxillaryMesh || d.Data_MeshType != 1))

@@ -164,6 +164,15 @@ def gen_pipe_normal(d):
     """The pipe with <prefix>_normals.bin (mesh.c:255-300): count, node ids,
     inward normals."""
     gen_pipe(d)
+    write_normals(d)
+
+
+def gen_pipe_stagger_normal(d):
+    gen_pipe_stagger(d)
+    write_normals(d)
+
+
+def write_normals(d):
     ids, nrm = pipe_walls()
     with open(os.path.join(d, "pipe_normals.bin"), "wb") as f:
         np.array([ids.size], "<i4").tofile(f)
@@ -496,6 +505,11 @@ CASES = {
                                        Trace_Compute=1, Trace_GenerateMesh=0, Trace_InFile="probes.txt",
                                        Trace_InFileFormat=1, Trace_IntTLength=1.0, Trace_OutFilePrefix="ptr2")),
     # RKF with the staggered release, the activation potential and exposure times (integration.c:534-539, 650-677)
+    # staggered release with normal flow: release times from the velocity as read, integration with the wall
+    # velocities (main.c:55-67, tracers.c:488-495); release points near the wall
+    "pipe_staggered_normal": dict(gen=gen_pipe_stagger_normal, outputs=["psn.%d.bin" % k for k in range(9)],
+                                  deck=dict(STAGGER, Int_NormalFlow=1, Int_NormalFlowScaling=0.2,
+                                            Trace_OutFilePrefix="psn")),
     "pipe_staggered_rkf": dict(gen=gen_pipe_stagger, outputs=["psr.%d.bin" % k for k in range(9)],
                                deck=dict(STAGGER, Int_Type=2, Int_Accuracy=1e-6, Int_MinTimeStep=1e-5,
                                          Int_MaxTimeStep=0.01, Trace_OutFilePrefix="psr")),

@@ -40,7 +40,7 @@ def run_driver(case, tmp_path, precision="f64"):
                                   "pipe_tracers_grid", "dg_tracers_back", "pipe_tracers_back",
                                   "pipe_tracers_periods", "pipe_tracers_normal", "pipe_tracers_normal_back",
                                   "dg_tracers_rkf", "pipe_tracers_rkf", "pipe_staggered_rkf", "pipe_tracers_rkf_ap",
-                                  "pipe_tracers_rkf_cet"])
+                                  "pipe_tracers_rkf_cet", "pipe_staggered_normal"])
 def test_gpu_trajectories_bit_identical(case, tmp_path):
     man, data, out = run_driver(case, tmp_path)
     fx = cases.fixture_dir(case)
