@@ -105,6 +105,29 @@ def run_step(P, P_ic, F, ctx, n_tr, res):
     return out
 
 
+def tracer_periods(P, P_ic, ctx, nper):
+    """The tracers over `nper` periods of the periodic data (outside the timed
+    steps): seconds, particle-steps and the live-list length (-1: the whole
+    range launches) of each period, as particles leave through the outlet."""
+    P.copy_from(P_ic)
+    P.launch_velocity(0.0, 0, 1, 0.0, 0.02)
+    out = []
+    for k in range(nper):
+        ctx_sync(ctx)
+        t0 = time.perf_counter()
+        steps = 0
+        for tmin, tmax, lo, hi in frames():
+            steps += P.advect(tmin + k, tmax + k, lo, hi, tmin + k, tmax + k, 1e-3)
+        ctx_sync(ctx)
+        secs = time.perf_counter() - t0
+        left = int((P.get()["LeftDomain"] != 0).sum())
+        out.append({"period": k, "seconds": secs, "particle_steps": int(steps), "left_domain": left,
+                    "live_list": P.active()})
+        log("tracer period %d: %.3f s, %d particle-steps, %d left, list %d" %
+            (k, out[-1]["seconds"], steps, left, out[-1]["live_list"]))
+    return out
+
+
 def ctx_sync(ctx):
     from fvcpy import fvc
     fvc.lib().fvc_sync(ctx.h)
@@ -627,6 +650,9 @@ def main():
                     help="element and node numbering of the pipe mesh: renumbered in Z-order by fvc_mesh_zorder "
                          "before upload (cudaVC's ZOrderSort step; untimed setup), or as generated (voxel scan order)")
     ap.add_argument("--no-f64-check", action="store_true")
+    ap.add_argument("--tracer-periods", type=int, default=0,
+                    help="pipe workload: after the timed steps, run the tracers over this many periods and report "
+                         "each period (particles leaving, live-particle lists)")
     ap.add_argument("--workload", default="pipe", choices=["pipe", "batch", "serve", "traffic", "dg"],
                     help="pipe: tracers + FTLE grid over one period; batch: uneven releases in one persistent "
                          "kernel; serve: Poisson request stream on the Y-bifurcation; dg: the 2-D double-gyre "
@@ -829,6 +855,8 @@ def main():
                                           "above_tolerance": int((rel > 1e-3).sum()), "tolerance": 1e-3,
                                           "floor": "1e-3 x median |FTLE|", "points": int(ok.sum())}
         log("FP64 FTLE reference %.1f s" % (time.time() - t0))
+    if args.tracer_periods > 0:
+        result["tracer_periods"] = tracer_periods(P, P_ic, ctx, args.tracer_periods)
     print(json.dumps(result), flush=True)
 
 
