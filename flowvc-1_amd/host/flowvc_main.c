@@ -15,8 +15,9 @@
  * velout.c:21-435; the activation potential (Trace_APCompute) integrates
  * the strain-rate data along tracer paths (strainrate.c, integration.c);
  * exposure times (exposuretime.c) and residence times (residencetime.c) are
- * written per element and per seed.  Inertial particles, RKF and normal flow
- * stop with an error.
+ * written per element and per seed.  RKF (Int_Type 2) and normal flow
+ * (Int_NormalFlow) run as the reference does; inertial particles
+ * (Particle_Radius > 0) stop with an error.
  */
 #include <math.h>
 #include <stdio.h>

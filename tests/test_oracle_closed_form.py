@@ -198,3 +198,48 @@ def test_ab3_is_third_order(pipe_mesh):
         errs.append(np.abs(pts["X"] - exact).max())
     r1, r2 = errs[0] / errs[1], errs[1] / errs[2]
     assert 6.0 < r1 < 10.0 and 6.0 < r2 < 10.0, errs
+
+
+def test_ab2_equals_exact_recurrence(pipe_mesh):
+    """AB2 with one RK4 starting step on u = A x + b, split over two calls
+    (history carried): equals x+ = x + h (3 f(x) - f(x-)) / 2 applied in numpy
+    after the RK4 map of the first step."""
+    om, A, b, q = _rotation_case(pipe_mesh)
+    pts, rc = om.seed_list(q, 1)
+    hist = np.zeros(q.shape[0], oracle.HIST_DTYPE)
+    h = 0.01
+    rc1, n1 = oracle.ab_advect(om, pts, hist, 0.0, 0.25, h, 2)
+    rc2, n2 = oracle.ab_advect(om, pts, hist, 0.25, 0.5, h, 2)
+    assert rc1 == 0 and rc2 == 0
+    f = lambda x: x @ A.T + b
+    hA = h * A
+    T = np.eye(3) + hA + hA @ hA / 2 + hA @ hA @ hA / 6 + hA @ hA @ hA @ hA / 24
+    c = h * (np.eye(3) + hA / 2 + hA @ hA / 6 + hA @ hA @ hA / 24) @ b
+    xs = [q.copy(), q @ T.T + c]
+    while len(xs) < 51:
+        x1, x0 = xs[-1], xs[-2]
+        xs.append(x1 + h * (3 * f(x1) - f(x0)) / 2)
+    np.testing.assert_allclose(pts["X"], xs[50], atol=1e-11)
+    assert n1 + n2 == 50 * q.shape[0]
+    # the recurrence is not AB3's, nor Euler's: both land far from it
+    xe = q.copy()
+    for _ in range(50):
+        xe = xe + h * f(xe)
+    assert np.abs(xe - xs[50]).max() > 1e-4
+
+
+def test_ab2_is_second_order(pipe_mesh):
+    """Errors against the exact rotation fall by about 4 when h halves."""
+    om, A, b, q = _rotation_case(pipe_mesh)
+    q = q[:20]
+    errs = []
+    for h in (0.02, 0.01, 0.005):
+        pts, rc = om.seed_list(q, 1)
+        hist = np.zeros(q.shape[0], oracle.HIST_DTYPE)
+        assert oracle.ab_advect(om, pts, hist, 0.0, 1.0, h, 2)[0] == 0
+        ang = 2.0 * 1.0
+        R = np.array([[np.cos(ang), -np.sin(ang), 0], [np.sin(ang), np.cos(ang), 0], [0, 0, 1]])
+        exact = q @ R.T + [0, 0, 0.5]
+        errs.append(np.abs(pts["X"] - exact).max())
+    r1, r2 = errs[0] / errs[1], errs[1] / errs[2]
+    assert 3.0 < r1 < 5.0 and 3.0 < r2 < 5.0, errs
