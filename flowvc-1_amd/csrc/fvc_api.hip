@@ -293,7 +293,10 @@ extern "C" int fvc_create(const fvc_mesh_desc *md, const fvc_params *pp, fvc_ctx
         fvc_destroy(c);
         return FVC_E_HIP;
     }
-    hipMemset(c->d_err, 0, sizeof(int));
+    if (hipMemset(c->d_err, 0, sizeof(int)) != hipSuccess) {
+        fvc_destroy(c);
+        return FVC_E_HIP;
+    }
     if (md->mesh_type == 0) {
         /* Cartesian grid (mesh.c:96-121) */
         for (int a = 0; a < 3; a++) {
@@ -361,10 +364,13 @@ extern "C" int fvc_create(const fvc_mesh_desc *md, const fvc_params *pp, fvc_ctx
             fvc_destroy(c);
             return rc;
         }
-        hipMemcpy(c->X64, x64.data(), sizeof(double4) * md->num_nodes, hipMemcpyHostToDevice);
-        hipMemcpy(c->X32, x32.data(), sizeof(float4) * md->num_nodes, hipMemcpyHostToDevice);
-        hipMemcpy(c->conn, cn.data(), sizeof(int4) * md->num_elems, hipMemcpyHostToDevice);
-        hipMemcpy(c->adj, ad.data(), sizeof(int4) * md->num_elems, hipMemcpyHostToDevice);
+        if (hipMemcpy(c->X64, x64.data(), sizeof(double4) * md->num_nodes, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(c->X32, x32.data(), sizeof(float4) * md->num_nodes, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(c->conn, cn.data(), sizeof(int4) * md->num_elems, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(c->adj, ad.data(), sizeof(int4) * md->num_elems, hipMemcpyHostToDevice) != hipSuccess) {
+            fvc_destroy(c);
+            return FVC_E_HIP;
+        }
         m.X64 = c->X64;
         m.X32 = c->X32;
         m.conn = c->conn;
@@ -390,6 +396,13 @@ extern "C" void fvc_destroy(fvc_ctx *c) {
     if (!c) return;
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
+    /* objects still alive on the context: servers and batches first (they
+     * hold particle sets), then FTLE slides, then particle sets.  Each release
+     * drops the object (and the objects it owns) from these sets. */
+    while (!c->live_servers.empty()) server_release(*c->live_servers.begin());
+    while (!c->live_batches.empty()) batch_release(*c->live_batches.begin());
+    while (!c->live_ftle.empty()) ftle_release(*c->live_ftle.begin());
+    while (!c->live_particles.empty()) particles_release(*c->live_particles.begin());
     for (void *s : c->slices)
         if (s) hipFree(s);
     for (double *s : c->sr_slices)
@@ -556,7 +569,7 @@ extern "C" int fvc_cet_reset(fvc_ctx *c) {
 }
 
 extern "C" int fvc_cet_launch(fvc_ctx *c, fvc_particles *p, int64_t begin, int64_t count) {
-    if (!c || !p || begin < 0 || count < 0 || begin + count > p->cap) return FVC_E_INVALID;
+    if (!c || !p || p->ctx != c || begin < 0 || count < 0 || begin + count > p->cap) return FVC_E_INVALID;
     if (!c->cet_lo) return set_err(c, FVC_E_STATE, "fvc_cet_reset not called");
     int rc = launch_cet_encounters(c, ptrs(p), begin, count, c->cet_enc);
     return rc ? rc : check_err(c, "fvc_cet_launch");
@@ -620,6 +633,7 @@ extern "C" int fvc_particles_create(fvc_ctx *c, int64_t cap, fvc_particles **out
         delete p;
         return set_err(c, FVC_E_NOMEM, "particle alloc");
     }
+    c->live_particles.insert(p);
     c->bytes_used += (int64_t)bytes;
     double *d = (double *)p->buf;
     for (int a = 0; a < 3; a++) p->X[a] = d + a * cap;
@@ -632,17 +646,31 @@ extern "C" int fvc_particles_create(fvc_ctx *c, int64_t cap, fvc_particles **out
     p->aux = iv + cap;
     p->left = iv + 2 * cap;
     p->nhist = iv + 3 * cap;
-    hipMemsetAsync(p->buf, 0, bytes, c->stream);
-    hipStreamSynchronize(c->stream);
+    if (hipMemsetAsync(p->buf, 0, bytes, c->stream) != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess) {
+        fvc_particles_destroy(p);
+        return set_err(c, FVC_E_HIP, "particle clear");
+    }
     *out = p;
     return FVC_OK;
 }
 
-extern "C" void fvc_particles_destroy(fvc_particles *p) {
-    if (!p) return;
-    hipStreamSynchronize(p->ctx->stream);
+void fvc::particles_release(fvc_particles *p) {
+    fvc_ctx *c = p->ctx;
+    if (!c) return;
+    batches_release_on(c, p); /* a batch advances p: it goes first */
+    hipStreamSynchronize(c->stream);
     hipFree(p->buf);
     if (p->act) hipFree(p->act);
+    p->buf = nullptr;
+    p->act = nullptr;
+    p->act_begin = -1;
+    c->live_particles.erase(p);
+    p->ctx = nullptr;
+}
+
+extern "C" void fvc_particles_destroy(fvc_particles *p) {
+    if (!p) return;
+    particles_release(p);
     delete p;
 }
 
@@ -655,7 +683,7 @@ extern "C" int fvc_particles_active(const fvc_particles *p, int64_t *listed) {
 extern "C" int64_t fvc_particles_capacity(const fvc_particles *p) { return p ? p->cap : 0; }
 
 extern "C" int fvc_particles_set(fvc_particles *p, int64_t begin, const fvc_point *recs, int64_t n) {
-    if (!p || !recs || begin < 0 || n < 0 || begin + n > p->cap) return FVC_E_INVALID;
+    if (!p || !p->ctx || !recs || begin < 0 || n < 0 || begin + n > p->cap) return FVC_E_INVALID;
     fvc_ctx *c = p->ctx;
     /* element ids start walks on the device: outside [-1, ne) they would
      * address memory beyond the mesh (-1 is ompVC's "no element" and raises
@@ -695,7 +723,7 @@ extern "C" int fvc_particles_set(fvc_particles *p, int64_t begin, const fvc_poin
 }
 
 extern "C" int fvc_particles_get(fvc_particles *p, int64_t begin, fvc_point *recs, int64_t n) {
-    if (!p || !recs || begin < 0 || n < 0 || begin + n > p->cap) return FVC_E_INVALID;
+    if (!p || !p->ctx || !recs || begin < 0 || n < 0 || begin + n > p->cap) return FVC_E_INVALID;
     fvc_ctx *c = p->ctx;
     std::vector<double> d(8 * (size_t)n);
     std::vector<int> iv(3 * (size_t)n);
@@ -739,14 +767,14 @@ static int particles_d2d(fvc_particles *dst, int64_t db, const fvc_particles *sr
 }
 
 extern "C" int fvc_particles_copy(fvc_particles *dst, int64_t db, const fvc_particles *src, int64_t sb, int64_t n) {
-    if (!dst || !src || dst->ctx != src->ctx || n < 0 || db < 0 || sb < 0 || db + n > dst->cap || sb + n > src->cap)
+    if (!dst || !src || !dst->ctx || dst->ctx != src->ctx || n < 0 || db < 0 || sb < 0 || db + n > dst->cap || sb + n > src->cap)
         return FVC_E_INVALID;
     if (n == 0) return FVC_OK;
     return particles_d2d(dst, db, src, sb, n);
 }
 
 extern "C" int fvc_particles_get_xyz(fvc_particles *p, int64_t begin, double *xyz, int32_t *left, int64_t n) {
-    if (!p || !xyz || begin < 0 || n < 0 || begin + n > p->cap) return FVC_E_INVALID;
+    if (!p || !p->ctx || !xyz || begin < 0 || n < 0 || begin + n > p->cap) return FVC_E_INVALID;
     fvc_ctx *c = p->ctx;
     std::vector<double> d(3 * (size_t)n);
     for (int a = 0; a < 3; a++)
@@ -763,7 +791,7 @@ extern "C" int fvc_particles_get_xyz(fvc_particles *p, int64_t begin, double *xy
 /* ------------------------------------------------------------------------ */
 extern "C" int fvc_launch_velocity(fvc_ctx *c, fvc_particles *p, int64_t begin, int64_t count, double t, int32_t lo,
                                    int32_t hi, double tmin, double tmax) {
-    if (!c || !p || begin < 0 || count < 0 || begin + count > p->cap) return FVC_E_INVALID;
+    if (!c || !p || p->ctx != c || begin < 0 || count < 0 || begin + count > p->cap) return FVC_E_INVALID;
     int rc = check_slots(c, lo, hi, tmin, tmax);
     if (rc) return rc;
     return launch_velocity(c, ptrs(p), begin, count, t, bind_field(c, lo, hi, tmin, tmax));
@@ -785,7 +813,7 @@ static void bind_rkf(DevField &f, const fvc_int_params *ip) {
 
 extern "C" int fvc_advect(fvc_ctx *c, fvc_particles *p, int64_t begin, int64_t count, double t1, double t2, int32_t lo,
                           int32_t hi, double tmin, double tmax, const fvc_int_params *ip) {
-    if (!c || !p || !ip || begin < 0 || count < 0 || begin + count > p->cap) return FVC_E_INVALID;
+    if (!c || !p || p->ctx != c || !ip || begin < 0 || count < 0 || begin + count > p->cap) return FVC_E_INVALID;
     if (ip->int_type != FVC_INT_RK4 && ip->int_type != FVC_INT_EULER && ip->int_type != FVC_INT_AB2 &&
         ip->int_type != FVC_INT_AB3 && ip->int_type != FVC_INT_RKF)
         return set_err(c, FVC_E_INVALID, "integrator not available in fvc_advect");
@@ -823,7 +851,7 @@ extern "C" int fvc_advect(fvc_ctx *c, fvc_particles *p, int64_t begin, int64_t c
 
 extern "C" int fvc_advect_times(fvc_ctx *c, fvc_particles *p, int64_t begin, int64_t count, const double *t12,
                                 int32_t lo, int32_t hi, double tmin, double tmax, const fvc_int_params *ip) {
-    if (!c || !p || !ip || begin < 0 || count < 0 || begin + count > p->cap || (count > 0 && !t12))
+    if (!c || !p || p->ctx != c || !ip || begin < 0 || count < 0 || begin + count > p->cap || (count > 0 && !t12))
         return FVC_E_INVALID;
     if (ip->int_type != FVC_INT_RK4 && ip->int_type != FVC_INT_EULER && ip->int_type != FVC_INT_RKF)
         return set_err(c, FVC_E_INVALID, "integrator not available in fvc_advect_times");
@@ -986,6 +1014,7 @@ static int ftle_alloc(fvc_ctx *c, const double mn[3], const double del[3], const
                       double stop_time, fvc_ftle **out) {
     fvc_ftle *f = new fvc_ftle();
     f->ctx = c;
+    c->live_ftle.insert(f);
     for (int a = 0; a < 3; a++) {
         f->mn[a] = mn[a];
         f->del[a] = del[a];
@@ -997,7 +1026,7 @@ static int ftle_alloc(fvc_ctx *c, const double mn[3], const double del[3], const
     f->stop_time = stop_time;
     int rc = fvc_particles_create(c, f->n, &f->pts);
     if (rc) {
-        delete f;
+        fvc_ftle_destroy(f);
         return rc;
     }
     size_t n = (size_t)f->n;
@@ -1043,12 +1072,12 @@ static int ftle_finish(fvc_ftle *f) {
         hipMalloc((void **)&f->init_w, sizeof(double) * 2 * n) != hipSuccess)
         return set_err(c, FVC_E_NOMEM, "FTLE snapshot alloc");
     c->bytes_used += (int64_t)(n * (4 + 16));
-    hipMemcpyAsync(f->init_have, f->have, sizeof(int) * n, hipMemcpyDeviceToDevice, c->stream);
-    hipMemcpyAsync(f->init_w, f->wt, sizeof(double) * 2 * n, hipMemcpyDeviceToDevice, c->stream);
+    FVC_HIP(hipMemcpyAsync(f->init_have, f->have, sizeof(int) * n, hipMemcpyDeviceToDevice, c->stream));
+    FVC_HIP(hipMemcpyAsync(f->init_w, f->wt, sizeof(double) * 2 * n, hipMemcpyDeviceToDevice, c->stream));
     /* points inside = not left */
-    hipStreamSynchronize(c->stream);
+    FVC_HIP(hipStreamSynchronize(c->stream));
     std::vector<int> lh(n);
-    hipMemcpy(lh.data(), p->left, sizeof(int) * n, hipMemcpyDeviceToHost);
+    FVC_HIP(hipMemcpy(lh.data(), p->left, sizeof(int) * n, hipMemcpyDeviceToHost));
     int64_t inside = 0;
     for (size_t q = 0; q < n; q++) inside += !lh[q];
     f->inside = inside;
@@ -1128,19 +1157,31 @@ extern "C" int fvc_ftle_create_from(fvc_ctx *c, const double mn[3], const double
     return check_err(c, "fvc_ftle_create_from");
 }
 
-extern "C" void fvc_ftle_destroy(fvc_ftle *f) {
-    if (!f) return;
-    hipStreamSynchronize(f->ctx->stream);
+void fvc::ftle_release(fvc_ftle *f) {
+    fvc_ctx *c = f->ctx;
+    if (!c) return;
+    hipStreamSynchronize(c->stream);
     if (f->buf) hipFree(f->buf);
     if (f->init_have) hipFree(f->init_have);
     if (f->init_w) hipFree(f->init_w);
+    f->buf = nullptr;
+    f->init_have = nullptr;
+    f->init_w = nullptr;
     fvc_particles_destroy(f->init_pts);
     fvc_particles_destroy(f->pts);
+    f->init_pts = f->pts = nullptr;
+    c->live_ftle.erase(f);
+    f->ctx = nullptr;
+}
+
+extern "C" void fvc_ftle_destroy(fvc_ftle *f) {
+    if (!f) return;
+    ftle_release(f);
     delete f;
 }
 
 extern "C" int fvc_ftle_reset(fvc_ftle *f) {
-    if (!f || !f->init_pts) return FVC_E_INVALID;
+    if (!f || !f->ctx || !f->init_pts) return FVC_E_INVALID;
     fvc_ctx *c = f->ctx;
     size_t n = (size_t)f->n;
     int rc = particles_d2d(f->pts, 0, f->init_pts, 0, f->n);
@@ -1156,7 +1197,7 @@ extern "C" int fvc_ftle_reset(fvc_ftle *f) {
 
 extern "C" int fvc_ftle_frame(fvc_ftle *f, int32_t dir, int32_t lo, int32_t hi, double tmin, double tmax,
                               const fvc_int_params *ip, int32_t *done) {
-    if (!f || !ip || (dir != 1 && dir != -1)) return FVC_E_INVALID;
+    if (!f || !f->ctx || !ip || (dir != 1 && dir != -1)) return FVC_E_INVALID;
     fvc_ctx *c = f->ctx;
     if (done) *done = 0;
     if (f->status == 2) return FVC_OK;
@@ -1216,25 +1257,25 @@ int fvc::copy_ftle_kji(fvc_ctx *c, const double *d_wt, const double *d_wot, cons
 }
 
 extern "C" int fvc_ftle_output(fvc_ftle *f, double int_t, double *wt, double *wot) {
-    if (!f) return FVC_E_INVALID;
+    if (!f || !f->ctx) return FVC_E_INVALID;
     int rc = launch_ftle_fill(f, int_t);
     return rc ? rc : copy_ftle_kji(f->ctx, f->wt, f->wot, f->res, wt, wot);
 }
 
 extern "C" int fvc_ftle_points(fvc_ftle *f, int64_t *n, int64_t *inside) {
-    if (!f) return FVC_E_INVALID;
+    if (!f || !f->ctx) return FVC_E_INVALID;
     if (n) *n = f->n;
     if (inside) *inside = f->inside;
     return FVC_OK;
 }
 
 extern "C" int fvc_ftle_records(fvc_ftle *f, fvc_point *recs) {
-    if (!f || !recs) return FVC_E_INVALID;
+    if (!f || !f->ctx || !recs) return FVC_E_INVALID;
     return fvc_particles_get(f->pts, 0, recs, f->n);
 }
 
 extern "C" int fvc_ftle_initial(fvc_ftle *f, fvc_point *recs, int32_t *have, double *ftle_wt, double *ftle_wot) {
-    if (!f || !f->init_pts) return FVC_E_INVALID;
+    if (!f || !f->ctx || !f->init_pts) return FVC_E_INVALID;
     fvc_ctx *c = f->ctx;
     const size_t n = (size_t)f->n;
     int rc;

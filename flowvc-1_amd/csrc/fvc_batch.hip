@@ -361,10 +361,15 @@ struct fvc_batch {
     int4 *mp;
     int ntiles[2];
     int launched; /* release-time velocities done */
-    /* captured launch group */
-    hipGraph_t graph;
-    hipGraphExec_t gexec;
-    int g_block, g_budget, g_grid;
+    /* captured launch groups, one per (block, budget, grid) configuration
+     * run so far; kept until the batch is released, so a sweep captures each
+     * configuration once and never destroys a graph between runs */
+    struct Group {
+        int block, budget, grid0, grid1;
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+    };
+    std::vector<Group> groups;
 };
 
 /* class k's lists live at offset cls_off(k) of each list buffer */
@@ -534,6 +539,7 @@ extern "C" int fvc_batch_create(fvc_ctx *c, fvc_particles *p, const fvc_release 
 
     fvc_batch *b = new fvc_batch();
     b->ctx = c;
+    c->live_batches.insert(b);
     b->p = p;
     b->n = n;
     b->n0 = n0;
@@ -552,7 +558,7 @@ extern "C" int fvc_batch_create(fvc_ctx *c, fvc_particles *p, const fvc_release 
     size_t bytes = 0;
     for (size_t s : sizes) bytes += (s + 255) / 256 * 256;
     if (hipMalloc(&b->buf, bytes) != hipSuccess) {
-        delete b;
+        fvc_batch_destroy(b);
         return set_err(c, FVC_E_NOMEM, "batch: device memory");
     }
     char *q = (char *)b->buf;
@@ -599,26 +605,43 @@ extern "C" int fvc_batch_create(fvc_ctx *c, fvc_particles *p, const fvc_release 
     return FVC_OK;
 }
 
-static void drop_graph(fvc_batch *b) {
-    if (b->gexec) hipGraphExecDestroy(b->gexec);
-    if (b->graph) hipGraphDestroy(b->graph);
-    b->gexec = nullptr;
-    b->graph = nullptr;
+void fvc::batch_release(fvc_batch *b) {
+    fvc_ctx *c = b->ctx;
+    if (!c) return;
+    hipStreamSynchronize(c->stream);
+    for (auto &g : b->groups) {
+        if (g.exec) hipGraphExecDestroy(g.exec);
+        if (g.graph) hipGraphDestroy(g.graph);
+    }
+    b->groups.clear();
+    if (b->buf) hipFree(b->buf);
+    if (b->d_rel) hipFree(b->d_rel);
+    if (b->d_slots) hipFree(b->d_slots);
+    b->buf = nullptr;
+    b->d_rel = nullptr;
+    b->d_slots = nullptr;
+    fvc_particles_destroy(b->snap);
+    b->snap = nullptr;
+    c->live_batches.erase(b);
+    b->ctx = nullptr;
+    b->p = nullptr;
+}
+
+void fvc::batches_release_on(fvc_ctx *c, fvc_particles *p) {
+    std::vector<fvc_batch *> on;
+    for (fvc_batch *b : c->live_batches)
+        if (b->p == p) on.push_back(b);
+    for (fvc_batch *b : on) batch_release(b);
 }
 
 extern "C" void fvc_batch_destroy(fvc_batch *b) {
     if (!b) return;
-    hipStreamSynchronize(b->ctx->stream);
-    drop_graph(b);
-    if (b->buf) hipFree(b->buf);
-    if (b->d_rel) hipFree(b->d_rel);
-    if (b->d_slots) hipFree(b->d_slots);
-    fvc_particles_destroy(b->snap);
+    batch_release(b);
     delete b;
 }
 
 extern "C" int fvc_batch_reset(fvc_batch *b) {
-    if (!b) return FVC_E_INVALID;
+    if (!b || !b->ctx) return FVC_E_INVALID;
     fvc_ctx *c = b->ctx;
     int rc = fvc_particles_copy(b->p, 0, b->snap, 0, b->n);
     if (rc) return rc;
@@ -653,7 +676,7 @@ static int launch_releases(fvc_batch *b) {
 }
 
 extern "C" int fvc_batch_run(fvc_batch *b, const fvc_batch_params *bp, int64_t *particle_steps, int64_t *launches) {
-    if (!b || !bp) return FVC_E_INVALID;
+    if (!b || !b->ctx || !bp) return FVC_E_INVALID;
     fvc_ctx *c = b->ctx;
     const int bs = bp->block;
     if (!block_ok(c->precision, bs))
@@ -668,20 +691,28 @@ extern "C" int fvc_batch_run(fvc_batch *b, const fvc_batch_params *bp, int64_t *
     timing_begin(c); /* with timing on: device time of the whole run */
     int rc0 = launch_releases(b);
     if (rc0) return rc0;
-    if (bp->use_graph && !(b->gexec && b->g_block == bs && b->g_budget == budget && b->g_grid == grid[0])) {
-        drop_graph(b);
-        FVC_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        enqueue_group(b, bs, grid, budget);
-        const hipError_t e = hipStreamEndCapture(c->stream, &b->graph);
-        if (e != hipSuccess) {
-            b->graph = nullptr;
-            c->last_error = std::string("batch: graph capture: ") + hipGetErrorString(e);
-            return FVC_E_HIP;
+    hipGraphExec_t gexec = nullptr;
+    if (bp->use_graph) {
+        for (const auto &g : b->groups)
+            if (g.block == bs && g.budget == budget && g.grid0 == grid[0] && g.grid1 == grid[1]) gexec = g.exec;
+        if (!gexec) {
+            fvc_batch::Group g{bs, budget, grid[0], grid[1], nullptr, nullptr};
+            FVC_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            enqueue_group(b, bs, grid, budget);
+            const hipError_t e = hipStreamEndCapture(c->stream, &g.graph);
+            if (e != hipSuccess) {
+                c->last_error = std::string("batch: graph capture: ") + hipGetErrorString(e);
+                return FVC_E_HIP;
+            }
+            const hipError_t ei = hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0);
+            if (ei != hipSuccess) {
+                hipGraphDestroy(g.graph);
+                c->last_error = std::string("batch: graph instantiate: ") + hipGetErrorString(ei);
+                return FVC_E_HIP;
+            }
+            b->groups.push_back(g);
+            gexec = g.exec;
         }
-        FVC_HIP(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
-        b->g_block = bs;
-        b->g_budget = budget;
-        b->g_grid = grid[0];
     }
     int64_t rounds = 0;
     int active[4] = {(int)b->cls_n0[0], 0, (int)b->cls_n0[1], 0};
@@ -689,7 +720,7 @@ extern "C" int fvc_batch_run(fvc_batch *b, const fvc_batch_params *bp, int64_t *
         if (rounds >= max_rounds + GROUP)
             return set_err(c, FVC_E_STATE, "batch: particles still active after the step bound");
         if (bp->use_graph) {
-            FVC_HIP(hipGraphLaunch(b->gexec, c->stream));
+            FVC_HIP(hipGraphLaunch(gexec, c->stream));
         } else {
             enqueue_group(b, bs, grid, budget);
             FVC_HIP(hipGetLastError());
@@ -714,7 +745,7 @@ extern "C" int fvc_batch_run(fvc_batch *b, const fvc_batch_params *bp, int64_t *
 
 extern "C" int fvc_batch_tune(fvc_batch *b, const int32_t *blocks, int32_t nblocks, const int32_t *budgets,
                               int32_t nbudgets, int32_t use_graph, fvc_batch_params *best, double *best_ms) {
-    if (!b || !blocks || !budgets || nblocks <= 0 || nbudgets <= 0 || !best) return FVC_E_INVALID;
+    if (!b || !b->ctx || !blocks || !budgets || nblocks <= 0 || nbudgets <= 0 || !best) return FVC_E_INVALID;
     fvc_ctx *c = b->ctx;
     double best_t = 1e300;
     fvc_batch_params bp_best{};

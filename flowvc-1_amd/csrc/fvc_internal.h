@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <set>
 #include <string>
 #include <vector>
 
@@ -53,10 +54,16 @@ struct fvc_ctx {
     hipEvent_t ev0, ev1;
     double t_ms;
     int64_t t_launches, t_steps;
+    /* objects created on this context: fvc_destroy releases their device
+     * memory and detaches them (their handles then accept only *_destroy) */
+    std::set<fvc_particles *> live_particles;
+    std::set<fvc_ftle *> live_ftle;
+    std::set<fvc_batch *> live_batches;
+    std::set<fvc_server *> live_servers;
 };
 
 struct fvc_particles {
-    fvc_ctx *ctx;
+    fvc_ctx *ctx; /* null once detached by fvc_destroy */
     int64_t cap;
     void *buf;
     double *X[3];
@@ -74,7 +81,7 @@ struct fvc_particles {
 };
 
 struct fvc_ftle {
-    fvc_ctx *ctx;
+    fvc_ctx *ctx; /* null once detached by fvc_destroy */
     double mn[3], mx[3], del[3];
     int32_t res[3];
     int64_t n;
@@ -179,6 +186,14 @@ int launch_copy_probe(fvc_ctx *c, void *dst, const void *src, int64_t bytes);
 int copy_ftle_kji(fvc_ctx *c, const double *d_wt, const double *d_wot, const int32_t res[3], double *wt, double *wot);
 
 int check_err(fvc_ctx *c, const char *what);
+/* release an object's device memory and detach it from its context (the
+ * handle stays valid for its *_destroy call only); no-ops when detached */
+void particles_release(fvc_particles *p);
+void ftle_release(fvc_ftle *f);
+void batch_release(fvc_batch *b);
+void server_release(fvc_server *s);
+/* batches running over particle set p (released before p) */
+void batches_release_on(fvc_ctx *c, fvc_particles *p);
 void *scratch(fvc_ctx *c, size_t bytes);
 /* bracket the hot kernel launch with events when timing is on */
 inline void timing_begin(fvc_ctx *c) {

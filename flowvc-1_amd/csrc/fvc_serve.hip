@@ -859,12 +859,13 @@ extern "C" int fvc_server_create(fvc_ctx *c, const fvc_server_params *sp_in, fvc
     if (cap < 1024) return srv_err(c, FVC_E_NOMEM, "no device memory for a request pool");
     fvc_server *S = new fvc_server();
     S->ctx = c;
+    c->live_servers.insert(S);
     S->sp = sp;
     S->cap = cap;
     S->ntiles = (int)((cap + TILE - 1) / TILE);
     int rc = fvc_particles_create(c, cap, &S->pool);
     if (rc) {
-        delete S;
+        fvc_server_destroy(S);
         return rc;
     }
     const size_t n = (size_t)cap;
@@ -877,8 +878,7 @@ extern "C" int fvc_server_create(fvc_ctx *c, const fvc_server_params *sp_in, fvc
     size_t bytes = 0;
     for (size_t s : sizes) bytes += (s + 255) / 256 * 256;
     if (hipMalloc(&S->buf, bytes) != hipSuccess) {
-        fvc_particles_destroy(S->pool);
-        delete S;
+        fvc_server_destroy(S);
         return srv_err(c, FVC_E_NOMEM, "server pool");
     }
     char *q = (char *)S->buf;
@@ -960,15 +960,27 @@ extern "C" int fvc_server_create(fvc_ctx *c, const fvc_server_params *sp_in, fvc
     return FVC_OK;
 }
 
-extern "C" void fvc_server_destroy(fvc_server *S) {
-    if (!S) return;
-    hipStreamSynchronize(S->ctx->stream);
+void fvc::server_release(fvc_server *S) {
+    fvc_ctx *c = S->ctx;
+    if (!c) return;
+    hipStreamSynchronize(c->stream);
     for (int par = 0; par < 2; par++) {
         if (S->gexec[par]) hipGraphExecDestroy(S->gexec[par]);
         if (S->graph[par]) hipGraphDestroy(S->graph[par]);
+        S->gexec[par] = nullptr;
+        S->graph[par] = nullptr;
     }
     if (S->buf) hipFree(S->buf);
+    S->buf = nullptr;
     fvc_particles_destroy(S->pool);
+    S->pool = nullptr;
+    c->live_servers.erase(S);
+    S->ctx = nullptr;
+}
+
+extern "C" void fvc_server_destroy(fvc_server *S) {
+    if (!S) return;
+    server_release(S);
     delete S;
 }
 
@@ -982,7 +994,7 @@ extern "C" int fvc_server_capacity(fvc_server *S, int64_t *particles, int64_t *b
 extern "C" double fvc_server_clock(fvc_server *S) { return S ? now_s() - S->t_origin : 0.0; }
 
 extern "C" int fvc_server_submit(fvc_server *S, const fvc_request_desc *rq, int64_t *id) {
-    if (!S || !rq || !id) return FVC_E_INVALID;
+    if (!S || !S->ctx || !rq || !id) return FVC_E_INVALID;
     fvc_ctx *c = S->ctx;
     HostReq R;
     R.d = *rq;
@@ -1019,7 +1031,7 @@ extern "C" int fvc_server_submit(fvc_server *S, const fvc_request_desc *rq, int6
 }
 
 extern "C" int fvc_server_poll(fvc_server *S, int32_t *running, int32_t *queued) {
-    if (!S) return FVC_E_INVALID;
+    if (!S || !S->ctx) return FVC_E_INVALID;
     fvc_ctx *c = S->ctx;
     int rc = FVC_OK;
     const int M = S->sp.max_requests;
@@ -1075,6 +1087,7 @@ extern "C" int fvc_server_poll(fvc_server *S, int32_t *running, int32_t *queued)
             retire(S, R, FVC_RS_DONE);
         } else if ((rc = admit(S, R))) {
             retire(S, R, FVC_RS_FAILED);
+            S->queued.erase(it); /* retired: a later poll must not admit it again */
             return rc;
         }
         it = S->queued.erase(it);
@@ -1109,7 +1122,7 @@ extern "C" int fvc_server_stats(fvc_server *S, double out[5]) {
 }
 
 extern "C" int fvc_server_info(fvc_server *S, int64_t id, fvc_request_info *info) {
-    if (!S || !info) return FVC_E_INVALID;
+    if (!S || !S->ctx || !info) return FVC_E_INVALID;
     auto it = S->reqs.find(id);
     if (it == S->reqs.end()) return srv_err(S->ctx, FVC_E_INVALID, "unknown request");
     const HostReq &R = it->second;
@@ -1124,7 +1137,7 @@ extern "C" int fvc_server_info(fvc_server *S, int64_t id, fvc_request_info *info
 }
 
 extern "C" int fvc_server_take(fvc_server *S, int64_t id, fvc_point *records, double *ftle_wt, double *ftle_wot) {
-    if (!S) return FVC_E_INVALID;
+    if (!S || !S->ctx) return FVC_E_INVALID;
     auto it = S->reqs.find(id);
     if (it == S->reqs.end()) return srv_err(S->ctx, FVC_E_INVALID, "unknown request");
     HostReq &R = it->second;

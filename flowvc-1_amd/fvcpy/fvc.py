@@ -12,6 +12,8 @@ from . import PKG_DIR
 from .host import POINT_DTYPE
 
 OK = 0
+E_INVALID, E_NOMEM, E_HIP, E_INDETERMINATE_NEIGHBOR, E_TLOC_RANGE, E_ELEM_NEG1, E_WALK_LIMIT = -1, -2, -3, -4, -5, -6, -7
+E_ADMISSION, E_STATE = -9, -10
 PREC = {"f64": 0, "f32": 1, "bf16": 2, "f16": 3}
 LOCATE_WALK, LOCATE_GLOBAL, LOCATE_WALK_GLOBAL = 0, 1, 2
 INT_EULER, INT_RK4, INT_RKF, INT_AB2, INT_AB3 = 0, 1, 2, 3, 4
@@ -310,9 +312,8 @@ class Particles:
         self.capacity = capacity
 
     def close(self):
-        if self.h:
-            if self.ctx.h:  # never after the context: its destroy call would reach freed state
-                lib().fvc_particles_destroy(self.h)
+        if self.h:  # also safe after the context's destroy, which detached this object
+            lib().fvc_particles_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -428,9 +429,8 @@ class FtleSlide:
         return wt, wot
 
     def close(self):
-        if self.h:
-            if self.ctx.h:  # never after the context: its destroy call would reach freed state
-                lib().fvc_ftle_destroy(self.h)
+        if self.h:  # also safe after the context's destroy, which detached this object
+            lib().fvc_ftle_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -479,9 +479,8 @@ class Batch:
                 "waves_per_cu": best.waves_per_cu}, ms.value
 
     def close(self):
-        if self.h:
-            if self.ctx.h:  # never after the context: its destroy call would reach freed state
-                lib().fvc_batch_destroy(self.h)
+        if self.h:  # also safe after the context's destroy, which detached this object
+            lib().fvc_batch_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -565,9 +564,8 @@ class Server:
         raise RuntimeError("server still busy after %d ticks" % max_ticks)
 
     def close(self):
-        if self.h:
-            if self.ctx.h:  # never after the context: its destroy call would reach freed state
-                lib().fvc_server_destroy(self.h)
+        if self.h:  # also safe after the context's destroy, which detached this object
+            lib().fvc_server_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -12,7 +12,16 @@
  * Errors: every call returns FVC_OK (0) or a negative code; fvc_strerror()
  * names it.  A caller keeping ompVC's behaviour turns non-zero into exit(1).
  * Threading: one host thread per context; calls on one context are ordered on
- * the context's stream and are synchronous unless named *_async.
+ * the context's stream, and every call returns when its device work is done
+ * (fvc_server_poll returns with one tick of the server enqueued).
+ *
+ * Lifetimes: particle sets, FTLE slides, batches and servers belong to the
+ * context they were created on.  They may be destroyed in any order, before
+ * or after the context: fvc_destroy releases the device memory of every
+ * object still alive on the context and detaches it, and a detached handle
+ * accepts only its *_destroy call (any other call returns FVC_E_INVALID).
+ * Destroying a particle set first releases the batches created over it in
+ * the same way.
  */
 #ifndef FVC_H
 #define FVC_H
@@ -122,6 +131,8 @@ typedef struct fvc_ftle fvc_ftle;
 /* Upload the mesh (LoadMeshData, ompVC/mesh.c:21-43) and build the device
  * spatial hash.  Replaces the mesh globals of ompVC/globals.h:38-47. */
 int fvc_create(const fvc_mesh_desc *mesh, const fvc_params *params, fvc_ctx **out);
+/* Frees the context; objects still alive on it are released and detached
+ * (see Lifetimes above). */
 void fvc_destroy(fvc_ctx *ctx);
 const char *fvc_strerror(int code);
 /* Message of the last failure on this context (HIP error text, offending index). */

@@ -169,3 +169,33 @@ def test_server_steps_counted_and_admission_limits(pipe):
     inside = int(((recs["LeftDomain"] == 0) | (recs["LeftDomainTime"] > 0)).sum())
     assert inf["particle_steps"] >= 10 * (inside - 5)
     S.close()
+
+
+def test_failed_admission_leaves_the_queue(pipe):
+    """A request whose admission fails is retired as failed and leaves the
+    queue: later polls do not admit it again, and requests after it run.
+    The failure is forced from the host: a context whose walks may take at
+    most 8 hops, and a point list without LocalSearchChecking whose second
+    seed lies 9 units down the pipe from the first (its chain walk crosses
+    dozens of elements)."""
+    t, xyz, conn, adj, slices = pipe
+    ctx = fvc.Context(3, PIPE_BOUNDS, "f32", NSLOTS, coords=xyz, conn=conn, adj=adj, walk_limit=8)
+    for s in range(NSLOTS):
+        ctx.load_slice(s, s * FDT, slices[s])
+    S = fvc.Server(ctx, 0.0, FDT, NSLOTS, DT, pool_particles=4096, steps_per_launch=16, lsc=0)
+    bad = S.submit_tracer(np.array([[0.0, 0.0, 0.4], [0.0, 0.0, 9.6]]), 0.0, 0.01)
+    with pytest.raises(fvc.FvcError) as e:
+        S.poll()
+    assert e.value.code == fvc.E_WALK_LIMIT
+    assert S.info(bad)["state"] == fvc.RS_FAILED
+    for _ in range(3):
+        r, q = S.poll()
+        assert r == 0 and q == 0
+        assert S.info(bad)["state"] == fvc.RS_FAILED
+    good = S.submit_tracer(np.array([[0.1, 0.05, 5.0]]), 0.0, 0.01)
+    S.run_until_idle()
+    assert S.info(good)["state"] == fvc.RS_DONE
+    rec = S.take(good)
+    assert rec.shape[0] == 1 and rec["LeftDomain"][0] == 0 and rec["X"][0, 2] > 5.0
+    S.close()
+    ctx.close()
