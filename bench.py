@@ -34,6 +34,28 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def host_cores():
+    """Every processor this process may run on (the reference runs its loops
+    on omp_get_num_procs() threads, ompVC/parameters.c:395)."""
+    return len(os.sched_getaffinity(0))
+
+
+def host_info():
+    """nproc and the lscpu model, sockets and threads of the host."""
+    info = {"nproc": host_cores()}
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)"):
+                info[k] = v
+    except Exception:
+        pass
+    return info
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` on the FP32 pipe workload, from the
     committed FETCH_SIZE / WRITE_SIZE counter passes (profiles/pmc_traffic_pipe.md,
@@ -138,7 +160,7 @@ def cpu_baseline(args, np, xyz, conn, adj, slices, recs_sample):
     as tracers.c:222-223) on a bounded sample of the tracers."""
     sys.path.insert(0, os.path.join(HERE, "oracle"))
     import oracle
-    cores = min(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), os.cpu_count() or 1, 16)
+    cores = host_cores()
     om = oracle.Mesh(3, [-1.0, 1.0, -1.0, 1.0, 0.0, 10.0], coords=xyz, conn=conn, adj=adj)
     pts = np.ascontiguousarray(recs_sample.copy())
     chunk = max(1, pts.shape[0] // cores)
@@ -267,7 +289,7 @@ def bench_batch(args, np, fvc, host):
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(HERE, "oracle"))
         import oracle
-        cores = min(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), os.cpu_count() or 1, 16)
+        cores = host_cores()
         om = oracle.Mesh(3, [-1.0, 1.0, -1.0, 1.0, 0.0, 10.0], coords=xyz, conn=conn, adj=adj)
         stride = max(1, args.batch_cpu_stride)
 
@@ -326,7 +348,7 @@ def bench_batch(args, np, fvc, host):
                 errs.append(np.abs(g["X"][ok] - pts["X"][ok]).max(axis=1))
         err = np.concatenate(errs) if errs else np.zeros(1)
         result["cpu_baseline"] = {
-            "value": steps / t_int, "unit": "particle-steps/s", "cores": cores, "kind": "port",
+            "value": steps / t_int, "unit": "particle-steps/s", "cores": cores, "kind": "port", "host": host_info(),
             "sample": "every %d-th seed of each release, same releases/horizons/integrators, integrate-only "
                       "time %.1f s, oracle FP64 restatement (OpenMP)" % (stride, t_int)}
         result["pos_err_vs_fp64"] = {"max": float(err.max()), "p99": float(np.percentile(err, 99)),
@@ -362,13 +384,15 @@ def serve_requests(np, host, tube, rate, window, seed):
     return out
 
 
-def bench_serve(args, np, fvc, host):
+def bench_serve(args, np, fvc, host, window=None, emit=True):
     """Served mode on the Y-bifurcation: requests arrive by a Poisson process
     and are admitted into the server's HBM pool as memory frees up; p50/p99
     latency (arrival to result) and aggregate particle-steps/s, beside the
     batch-at-a-time path (each request alone, first come first served, the
-    same arithmetic: results are bit-identical)."""
+    same arithmetic: results are bit-identical).  `window` overrides the
+    arrival window; with emit=False the result is returned, not printed."""
     from concurrent.futures import ThreadPoolExecutor
+    window = args.serve_window if window is None else window
     t0 = time.time()
     tube = host.ybif(184, 65, 256)
     xyz, conn, adj = host.tube_mesh(tube)
@@ -380,9 +404,9 @@ def bench_serve(args, np, fvc, host):
     for s, sl in enumerate(slices):
         ctx.load_slice(s, s * 0.01, sl)
     del slices
-    reqs = serve_requests(np, host, tube, args.serve_rate, args.serve_window, 11)
+    reqs = serve_requests(np, host, tube, args.serve_rate, window, 11)
     log("serve setup %.1f s: %d nodes, %d tets, 100 %s slices, %d requests over %.0f s" %
-        (time.time() - t0, xyz.shape[0], conn.shape[0], args.serve_precision, len(reqs), args.serve_window))
+        (time.time() - t0, xyz.shape[0], conn.shape[0], args.serve_precision, len(reqs), window))
     S = fvc.Server(ctx, 0.0, 0.01, 100, 1e-3, hbm_fraction=0.6, steps_per_launch=args.serve_budget,
                    use_graph=not args.serve_no_graph)
     cap, pool_bytes = S.capacity()
@@ -489,7 +513,7 @@ def bench_serve(args, np, fvc, host):
             "workload": "serving on a Y-bifurcation (parent R=1, daughters 0.7 at +/-30 deg; %d tets, %d nodes), "
                         "100 %s slices resident, Poisson arrivals %.0f/s for %.0f s (%d requests), FTLE or tracer, "
                         "seeds log-uniform [1k, 4M], horizon uniform [0.05, 1] period, RK4 dt=1e-3" %
-                        (conn.shape[0], xyz.shape[0], args.serve_precision, args.serve_rate, args.serve_window,
+                        (conn.shape[0], xyz.shape[0], args.serve_precision, args.serve_rate, window,
                          len(reqs)),
             "requests": len(reqs), "rate_per_s": args.serve_rate, "steps_per_launch": args.serve_budget,
         },
@@ -509,7 +533,10 @@ def bench_serve(args, np, fvc, host):
                    "pool_gb": pool_bytes / 1e9, "results_identical_to_alone": "%d/%d" % (same, len(served))},
         "roofline": None,
     }
-    print(json.dumps(result), flush=True)
+    ctx.close()
+    if emit:
+        print(json.dumps(result), flush=True)
+    return result
 
 
 def bench_traffic(args, np, fvc, host):
@@ -616,7 +643,7 @@ def bench_dg(args, np, fvc, host):
         g = np.stack(np.meshgrid(np.linspace(0.0, 2.0, 512), np.linspace(0.0, 1.0, 256), indexing="ij"), -1)
         pts = np.zeros(512 * 256 // 16, oracle.POINT_DTYPE)
         pts["X"][:, :2] = g.reshape(-1, 2)[::16]
-        cores = min(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), os.cpu_count() or 1, 16)
+        cores = host_cores()
         t_int, steps = 0.0, 0
         for f in range(20):
             om.set_slot(0, slices[f])
@@ -629,6 +656,7 @@ def bench_dg(args, np, fvc, host):
                 raise RuntimeError("oracle advect failed (%d)" % rc)
             steps += om.last_steps.value
         out["cpu_baseline"] = {"value": steps / t_int, "unit": "particle-steps/s", "cores": cores, "kind": "port",
+                               "host": host_info(),
                                "sample": "%d of the 512x256 FTLE grid points (every 16th), T=10, FP64 oracle"
                                          % pts.shape[0]}
     print(json.dumps(out), flush=True)
@@ -660,6 +688,9 @@ def main():
     ap.add_argument("--frames", type=int, default=4, help="traffic workload: tracer frames after the copy")
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-window", type=float, default=15.0)
+    ap.add_argument("--serve-leg", type=float, default=5.0,
+                    help="pipe workload: after it, the served Y-bifurcation workload over an arrival window of this "
+                         "many seconds for p50/p99 request latency (0: skip)")
     ap.add_argument("--serve-budget", type=int, default=32)
     ap.add_argument("--serve-precision", default="bf16", choices=["f32", "bf16", "f16"])
     ap.add_argument("--serve-check", type=int, default=40)
@@ -812,7 +843,7 @@ def main():
         live = (pts["LeftDomain"] == 0) & (pts["LeftDomainTime"] > -1e-10) & (gpu_left[sample_idx] == 0)
         err = np.abs(gpu_final[sample_idx][live] - pts["X"][live]).max(axis=1) if live.any() else np.zeros(1)
         result["cpu_baseline"] = {
-            "value": cb["value"], "unit": "particle-steps/s", "cores": cb["cores"], "kind": "port",
+            "value": cb["value"], "unit": "particle-steps/s", "cores": cb["cores"], "kind": "port", "host": host_info(),
             "sample": "%d of the %d tracers (every %d-th after Z-order), one period, integrate-only time %.1f s, "
                       "oracle FP64 restatement of ompVC (OpenMP dynamic chunks)" % (sample_idx.size, n_tr, stride,
                                                                                      cb["seconds"]),
@@ -857,6 +888,21 @@ def main():
         log("FP64 FTLE reference %.1f s" % (time.time() - t0))
     if args.tracer_periods > 0:
         result["tracer_periods"] = tracer_periods(P, P_ic, ctx, args.tracer_periods)
+    if args.serve_leg > 0:
+        # request latency: the served workload (Y-bifurcation, BF16 slices,
+        # Poisson arrivals) over a shorter arrival window, after the pipe's
+        # device memory is released
+        F.close()
+        ctx.close()
+        sr = bench_serve(args, np, fvc, host, window=args.serve_leg, emit=False)
+        result["p50_latency_ms"] = sr["p50_latency_ms"]
+        result["p99_latency_ms"] = sr["p99_latency_ms"]
+        result["serving"] = {"workload": sr["config"]["workload"], "requests": sr["steps"],
+                             "particle_steps_per_s": sr["value"],
+                             "p50_latency_ms": sr["p50_latency_ms"], "p99_latency_ms": sr["p99_latency_ms"],
+                             "batch_at_a_time": sr["batch_at_a_time"],
+                             "results_identical_to_alone": sr["served"]["results_identical_to_alone"],
+                             "makespan_s": sr["served"]["makespan_s"], "peak_running": sr["served"]["peak_running"]}
     print(json.dumps(result), flush=True)
 
 
