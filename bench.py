@@ -28,6 +28,10 @@ sys.path.insert(0, os.path.join(HERE, "flowvc-1_amd"))
 
 BYTES_PER_STEP_F32 = 704  # 5 containment tests x (16 B conn + 4 x 12 B coords) + 4 probes x 4 nodes x 2 slices x 12 B
 HBM_PEAK = 8.0e12
+# VALU issue ceiling of MI355X: 256 CUs x 4 SIMDs, one SIMD-cycle each per
+# cycle at the 2.4 GHz peak engine clock (a wave64 VALU instruction occupies
+# its SIMD for 4 cycles, FP64 multiplies and FMAs for 8)
+VALU_PEAK_SIMD_CYCLES = 256 * 4 * 2.4e9
 
 
 def log(*a):
@@ -56,20 +60,55 @@ def host_info():
     return info
 
 
+def profile_figure(fname, key, kernel):
+    """A figure of `kernel` recorded in a committed counter summary under
+    profiles/ (lines `<key> <kernel> <value>`); None when absent."""
+    try:
+        with open(os.path.join(HERE, "profiles", fname)) as f:
+            for line in f:
+                w = line.split()
+                if len(w) == 3 and w[0] == key and w[1] == kernel:
+                    return float(w[2])
+    except OSError:
+        pass
+    return None
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` on the FP32 pipe workload, from the
     committed FETCH_SIZE / WRITE_SIZE counter passes (profiles/pmc_traffic_pipe.md,
     scaled by the copy-kernel calibration there); None when not measured."""
-    path = os.path.join(HERE, "profiles", "pmc_traffic_pipe.md")
-    try:
-        with open(path) as f:
-            for line in f:
-                w = line.split()
-                if len(w) == 3 and w[0] == "traffic_bytes_per_launch" and w[1] == kernel:
-                    return int(w[2])
-    except OSError:
-        pass
-    return None
+    v = profile_figure("pmc_traffic_pipe.md", "traffic_bytes_per_launch", kernel)
+    return None if v is None else int(v)
+
+
+def roofline_valu(kernel, steps_per_launch, avg_ms, bpp, default_workload):
+    """Roofline of the dominant RK4 kernel against the bound that governs it.
+    Counter passes on the FP32 pipe workload (profiles/pmc_valu_pipe.md)
+    show the vector ALU busy most of each launch while L2 read bandwidth and
+    HBM stay far below their ceilings, so the bound is VALU issue:
+    achieved = VALU busy SIMD-cycles per particle-step (SQ_ACTIVE_INST_VALU x 4
+    over the particle-steps of the profiled launches) x particle-steps of
+    this launch / its measured duration, against 1024 SIMDs x 2.4 GHz.  The
+    algorithmic gather bytes (704 B per particle-step in FP32) are reported
+    beside it with the measured HBM traffic (cache-served: their HBM
+    fraction exceeds 1)."""
+    cyc = profile_figure("pmc_valu_pipe.md", "valu_cycles_per_particle_step", kernel) if default_workload else None
+    secs = avg_ms * 1e-3
+    gather = steps_per_launch * bpp / secs if secs > 0 else 0.0
+    out = {"kernel": kernel, "avg_launch_ms": avg_ms, "particle_steps_per_launch": steps_per_launch,
+           "gather_bytes_per_particle_step": bpp, "gather_GBps_algorithmic": gather / 1e9,
+           "gather_frac_of_hbm_peak": gather / HBM_PEAK,
+           "traffic": measured_traffic(kernel) if default_workload else None,
+           "traffic_source": "profiles/pmc_traffic_pipe.md (FETCH_SIZE x2.000 + WRITE_SIZE x1.000 per launch)"}
+    if cyc is None or secs <= 0:
+        out.update(bound="valu", achieved=None, peak=VALU_PEAK_SIMD_CYCLES / 1e9, unit="G SIMD-cycles/s", frac=None)
+        return out
+    achieved = cyc * steps_per_launch / secs
+    out.update(bound="valu", achieved=achieved / 1e9, peak=VALU_PEAK_SIMD_CYCLES / 1e9, unit="G SIMD-cycles/s",
+               frac=achieved / VALU_PEAK_SIMD_CYCLES, valu_cycles_per_particle_step=cyc,
+               valu_source="profiles/pmc_valu_pipe.md (SQ_ACTIVE_INST_VALU x 4 per particle-step)")
+    return out
 
 
 def build_inputs(args, host, np, with_seeds=True):
@@ -262,7 +301,6 @@ def bench_batch(args, np, fvc, host):
     K = args.steps
     value = tot_steps / wall
     bpp = BYTES_PER_STEP_F32 if args.precision == "f32" else (512 if args.precision in ("bf16", "f16") else 1328)
-    achieved = k_steps * bpp / (k_ms * 1e-3) if k_ms > 0 else 0.0
     result = {
         "metric": "particle-steps/s + FTLE seeds/s at stated FP64 error; p50/p99 request latency",
         "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": K, "warmup": args.warmup,
@@ -279,12 +317,8 @@ def bench_batch(args, np, fvc, host):
         },
         "batch": {"tuned": best, "tune_ms_eighth_batch": best_ms, "launch_groups": launches,
                   "particle_steps_per_run": tot_steps // K},
-        "roofline": {
-            "kernel": "k_batch (whole run: batch kernels + compaction, device time)",
-            "bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK, "traffic": None, "bytes_per_particle_step": bpp,
-            "avg_launch_ms": k_ms,
-        },
+        "roofline": dict(roofline_valu("k_batch", k_steps, k_ms, bpp, args.precision == "f32"),
+                         kernel="k_batch (whole run: batch kernels + compaction, device time)"),
     }
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(HERE, "oracle"))
@@ -788,7 +822,6 @@ def main():
         512 if args.precision in ("bf16", "f16") else 1328)
     avg_ms = dom[1][0] / max(dom[1][1], 1)
     steps_per_launch = dom[1][2] / max(dom[1][1], 1)
-    achieved = steps_per_launch * bpp / (avg_ms * 1e-3) if avg_ms > 0 else 0.0
     result = {
         "metric": "particle-steps/s + FTLE seeds/s at stated FP64 error; p50/p99 request latency",
         "value": value,
@@ -817,20 +850,8 @@ def main():
         "ftle_particle_steps_per_s": ft_steps / ft_t if ft_t > 0 else None,
         "p50_latency_ms": None,
         "p99_latency_ms": None,
-        "roofline": {
-            "kernel": dom[0],
-            "bound": "hbm",
-            "achieved": achieved / 1e9,
-            "peak": HBM_PEAK / 1e9,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK,
-            "traffic": measured_traffic(dom[0]) if (args.precision == "f32" and args.tracers == 4_000_000 and
-                                                    res == [128, 128, 640] and args.nx == 60) else None,
-            "traffic_source": "profiles/pmc_traffic_pipe.md (FETCH_SIZE x2.000 + WRITE_SIZE x1.000 per launch)",
-            "bytes_per_particle_step": bpp,
-            "avg_launch_ms": avg_ms,
-            "particle_steps_per_launch": steps_per_launch,
-        },
+        "roofline": roofline_valu(dom[0], steps_per_launch, avg_ms, bpp, args.precision == "f32" and
+                                  args.tracers == 4_000_000 and res == [128, 128, 640] and args.nx == 60),
     }
 
     # accuracy against FP64 and the CPU baseline on a bounded sample
@@ -880,11 +901,32 @@ def main():
         w64, _ = F64.output(1.0)
         w32 = last["wt"]
         ok = w64 > 0
-        rel = np.abs(w32[ok] - w64[ok]) / np.maximum(np.abs(w64[ok]), 1e-3 * np.median(np.abs(w64[ok])))
+        rel_all = np.zeros_like(w64)
+        rel_all[ok] = np.abs(w32[ok] - w64[ok]) / np.maximum(np.abs(w64[ok]), 1e-3 * np.median(np.abs(w64[ok])))
+        rel = rel_all[ok]
+        # points whose FTLE stencil (the point and its six grid neighbours) left
+        # the domain at different times in the two runs: there the early FTLE
+        # integrates over a different time, whatever the arithmetic precision
+        r32, r64 = F.records(), F64.records()
+        flip = (r32["LeftDomain"] != r64["LeftDomain"]) | (r32["LeftDomainTime"] != r64["LeftDomainTime"])
+        flip = flip.reshape(res[0], res[1], res[2])
+        near = flip.copy()
+        for ax in range(3):
+            lo_s = [slice(None)] * 3
+            hi_s = [slice(None)] * 3
+            lo_s[ax], hi_s[ax] = slice(0, -1), slice(1, None)
+            near[tuple(lo_s)] |= flip[tuple(hi_s)]
+            near[tuple(hi_s)] |= flip[tuple(lo_s)]
+        near_kji = near.transpose(2, 1, 0).ravel()  # FTLE outputs are in k, j, i order
+        same = ok & ~near_kji
+        rs = rel_all[same]
         result["ftle_rel_err_vs_fp64"] = {"p50": float(np.median(rel)), "p99": float(np.percentile(rel, 99)),
                                           "p999": float(np.percentile(rel, 99.9)), "max": float(rel.max()),
                                           "above_tolerance": int((rel > 1e-3).sum()), "tolerance": 1e-3,
-                                          "floor": "1e-3 x median |FTLE|", "points": int(ok.sum())}
+                                          "floor": "1e-3 x median |FTLE|", "points": int(ok.sum()),
+                                          "exit_time_differs_in_stencil": int((ok & near_kji).sum()),
+                                          "same_exit_max": float(rs.max()) if rs.size else None,
+                                          "same_exit_above_tolerance": int((rs > 1e-3).sum())}
         log("FP64 FTLE reference %.1f s" % (time.time() - t0))
     if args.tracer_periods > 0:
         result["tracer_periods"] = tracer_periods(P, P_ic, ctx, args.tracer_periods)

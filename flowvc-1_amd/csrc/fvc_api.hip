@@ -321,10 +321,12 @@ extern "C" int fvc_create(const fvc_mesh_desc *md, const fvc_params *pp, fvc_ctx
         /* strain-rate cells (strainrate.c:86-91): kmax = 1 in 2-D or for a single z layer */
         c->nsr = (int64_t)(m.cres[0] - 1) * (m.cres[1] - 1) * ((md->dims == 2 || m.cres[2] == 1) ? 1 : m.cres[2] - 1);
     } else {
+        /* device gathers address node arrays (32 B per node at most) and
+         * element arrays (16 B per element) with 32-bit byte offsets */
         if (md->num_nodes <= 0 || md->num_elems <= 0 || !md->coords || !md->conn || !md->adj ||
-            md->num_elems > INT32_MAX || md->num_nodes > INT32_MAX) {
+            md->num_elems > (INT64_C(1) << 32) / 16 || md->num_nodes > (INT64_C(1) << 32) / 32) {
             fvc_destroy(c);
-            return FVC_E_INVALID;
+            return set_err(nullptr, FVC_E_INVALID, "");
         }
         const int nn = md->dims == 3 ? 4 : 3;
         for (int64_t e = 0; e < md->num_elems; e++)

@@ -186,7 +186,7 @@ __device__ void batch_particle(const DevMesh &m, const BatchDev &b, int i, int b
 /* GEN = 0: RK4 releases only (no multistep state in registers); 1: any.
  * Held to 3 waves/SIMD (at most 168 VGPRs; a few spill slots). */
 template <int MODE, int BS, int GEN>
-__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(3))) void k_batch(DevMesh m, BatchDev b,
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(FVC_HOT_WAVES))) void k_batch(DevMesh m, BatchDev b,
                                                                                       int budget) {
     const int lane = threadIdx.x & 63;
     const int home = blockIdx.x % NQ;

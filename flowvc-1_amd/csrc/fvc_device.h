@@ -25,7 +25,16 @@
 #include <hip/hip_fp16.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "fvc_root4.h"
+
+/* waves per SIMD the hot persistent and per-interval RK4 kernels are
+ * compiled for (their VGPR budget is 512 / waves, rounded to the
+ * allocation granule) */
+#ifndef FVC_HOT_WAVES
+#define FVC_HOT_WAVES 3
+#endif
 
 namespace fvc {
 
@@ -83,15 +92,26 @@ __device__ inline void raise_err(ErrBuf e, int code, int64_t where) {
     if (atomicCAS(e.code, 0, code) == 0) *e.where = where;
 }
 
+/* Gather of element idx of an unstructured-mesh array with a 32-bit byte
+ * offset: node, connectivity, adjacency and node-velocity arrays are below
+ * 4 GiB (fvc_create checks it), so the address is the array base plus a
+ * zero-extended 32-bit offset, which gfx950's global loads take as a scalar
+ * base and one vector register, without 64-bit address arithmetic. */
+template <typename T>
+__device__ __forceinline__ T ld32(const T *__restrict__ base, uint32_t idx) {
+    const uint32_t off = idx * (uint32_t)sizeof(T);
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + (uint64_t)off);
+}
+
 /* ------------------------------------------------------------------------ */
 /* FP64 natural coordinates in the reference's expression order               */
 /* (ompVC/mesh.c:527-547, velocity.c:497-517)                                */
 /* ------------------------------------------------------------------------ */
-__device__ __forceinline__ void nat3_exact(const DevMesh &m, int e, double x, double y, double z, double &r, double &s,
-                                           double &t) {
+__device__ __forceinline__ void nat3_exact_p(const double4 *__restrict__ X64, const int4 *__restrict__ conn, int e,
+                                             double x, double y, double z, double &r, double &s, double &t) {
 #pragma clang fp contract(off)
-    int4 nd = m.conn[e];
-    double4 p0 = m.X64[nd.x], p1 = m.X64[nd.y], p2 = m.X64[nd.z], p3 = m.X64[nd.w];
+    int4 nd = ld32(conn, e);
+    double4 p0 = ld32(X64, nd.x), p1 = ld32(X64, nd.y), p2 = ld32(X64, nd.z), p3 = ld32(X64, nd.w);
     double x0 = p0.x, y0 = p0.y, z0 = p0.z, x1 = p1.x, y1 = p1.y, z1 = p1.z;
     double x2 = p2.x, y2 = p2.y, z2 = p2.z, x3 = p3.x, y3 = p3.y, z3 = p3.z;
     double c11 = (z3 - z0) * (y2 - y3) - (z2 - z3) * (y3 - y0);
@@ -112,15 +132,25 @@ __device__ __forceinline__ void nat3_exact(const DevMesh &m, int e, double x, do
     t = (c31 * dx + c32 * dy + c33 * dz) / det;
 }
 
-__device__ __forceinline__ void nat2_exact(const DevMesh &m, int e, double x, double y, double &r, double &s) {
+__device__ __forceinline__ void nat2_exact_p(const double4 *__restrict__ X64, const int4 *__restrict__ conn, int e,
+                                             double x, double y, double &r, double &s) {
 #pragma clang fp contract(off)
-    int4 nd = m.conn[e];
-    double4 p0 = m.X64[nd.x], p1 = m.X64[nd.y], p2 = m.X64[nd.z];
+    int4 nd = ld32(conn, e);
+    double4 p0 = ld32(X64, nd.x), p1 = ld32(X64, nd.y), p2 = ld32(X64, nd.z);
     double x0 = p0.x, y0 = p0.y, x1 = p1.x, y1 = p1.y, x2 = p2.x, y2 = p2.y;
     double c11 = y2 - y0, c12 = x0 - x2, c21 = y0 - y1, c22 = x1 - x0;
     double det = (x1 - x0) * (y2 - y0) - (x2 - x0) * (y1 - y0);
     r = (c11 * (x - x0) + c12 * (y - y0)) / det;
     s = (c21 * (x - x0) + c22 * (y - y0)) / det;
+}
+
+__device__ __forceinline__ void nat3_exact(const DevMesh &m, int e, double x, double y, double z, double &r, double &s,
+                                           double &t) {
+    nat3_exact_p(m.X64, m.conn, e, x, y, z, r, s, t);
+}
+
+__device__ __forceinline__ void nat2_exact(const DevMesh &m, int e, double x, double y, double &r, double &s) {
+    nat2_exact_p(m.X64, m.conn, e, x, y, r, s);
 }
 
 __device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
@@ -153,7 +183,7 @@ __device__ __forceinline__ int decide_exact(double r, double s, double t, int ac
 /* ------------------------------------------------------------------------ */
 __device__ __forceinline__ void nat3_fast(const DevMesh &m, int4 nd, double x, double y, double z, float &r, float &s,
                                           float &t) {
-    float4 p0 = m.X32[nd.x], p1 = m.X32[nd.y], p2 = m.X32[nd.z], p3 = m.X32[nd.w];
+    float4 p0 = ld32(m.X32, nd.x), p1 = ld32(m.X32, nd.y), p2 = ld32(m.X32, nd.z), p3 = ld32(m.X32, nd.w);
     float qx = (float)((x - m.centre[0]) - (double)p0.x);
     float qy = (float)((y - m.centre[1]) - (double)p0.y);
     float qz = (float)((z - m.centre[2]) - (double)p0.z);
@@ -170,7 +200,7 @@ __device__ __forceinline__ void nat3_fast(const DevMesh &m, int4 nd, double x, d
 }
 
 __device__ __forceinline__ void nat2_fast(const DevMesh &m, int4 nd, double x, double y, float &r, float &s) {
-    float4 p0 = m.X32[nd.x], p1 = m.X32[nd.y], p2 = m.X32[nd.z];
+    float4 p0 = ld32(m.X32, nd.x), p1 = ld32(m.X32, nd.y), p2 = ld32(m.X32, nd.z);
     float qx = (float)((x - m.centre[0]) - (double)p0.x);
     float qy = (float)((y - m.centre[1]) - (double)p0.y);
     float ax = p1.x - p0.x, ay = p1.y - p0.y, bx = p2.x - p0.x, by = p2.y - p0.y;
@@ -221,7 +251,7 @@ __device__ __forceinline__ int walk(const DevMesh &m, double x, double y, double
             }
             dec = decide_exact<DIM>(r, s, t, acc, slot, tie);
         } else {
-            int4 nd = m.conn[e];
+            int4 nd = ld32(m.conn, e);
             float fr, fs, ft = 0.0f;
             if (DIM == 3) nat3_fast(m, nd, x, y, z, fr, fs, ft);
             else nat2_fast(m, nd, x, y, fr, fs);
@@ -244,8 +274,118 @@ __device__ __forceinline__ int walk(const DevMesh &m, double x, double y, double
             err = E_INDET;
             return -2;
         }
-        const int *ap = reinterpret_cast<const int *>(m.adj + e);
-        e = ap[slot];
+        e = ld32(reinterpret_cast<const int *>(m.adj), 4 * (uint32_t)e + slot);
+        if (e == -1) return -1;
+    }
+    err = E_WALK;
+    return -2;
+}
+
+/* ------------------------------------------------------------------------ */
+/* walk on a query point given in FP32 relative to the mesh centre           */
+/* ------------------------------------------------------------------------ */
+/* The RK4 stage probes of the reduced-precision modes are formed in FP32 as
+ * an anchor A = (float)(X - centre) and a small offset d = D + c h k, D the
+ * FP32 residual of X - centre - A; the probe stands for the point
+ * (centre + A) + d.  The offset keeps FP32 resolution at its own small
+ * magnitude, so probes sit within ~1e-10 of the FP64 probe positions.  An
+ * FP32 decision forms q - node0 as (A - node0) + d, and an undecided one is
+ * recomputed with the FP64 formula at (centre + A) + d, so the element
+ * returned is the one the reference's walk returns for that point. */
+__device__ __forceinline__ void nat3_fast_c(const DevMesh &m, int4 nd, const float A[3], const float d[3], float &r,
+                                            float &s, float &t) {
+    float4 p0 = ld32(m.X32, nd.x), p1 = ld32(m.X32, nd.y), p2 = ld32(m.X32, nd.z), p3 = ld32(m.X32, nd.w);
+    float qx = (A[0] - p0.x) + d[0], qy = (A[1] - p0.y) + d[1], qz = (A[2] - p0.z) + d[2];
+    float ax = p1.x - p0.x, ay = p1.y - p0.y, az = p1.z - p0.z;
+    float bx = p2.x - p0.x, by = p2.y - p0.y, bz = p2.z - p0.z;
+    float cx = p3.x - p0.x, cy = p3.y - p0.y, cz = p3.z - p0.z;
+    float n1x = by * cz - bz * cy, n1y = bz * cx - bx * cz, n1z = bx * cy - by * cx; /* b x c */
+    float n2x = cy * az - cz * ay, n2y = cz * ax - cx * az, n2z = cx * ay - cy * ax; /* c x a */
+    float n3x = ay * bz - az * by, n3y = az * bx - ax * bz, n3z = ax * by - ay * bx; /* a x b */
+    /* hardware reciprocal (1 ulp): the guard band bounds the node-coordinate
+     * rounding, some 100 times larger than this */
+    float inv = __builtin_amdgcn_rcpf(ax * n1x + ay * n1y + az * n1z);
+    r = (qx * n1x + qy * n1y + qz * n1z) * inv;
+    s = (qx * n2x + qy * n2y + qz * n2z) * inv;
+    t = (qx * n3x + qy * n3y + qz * n3z) * inv;
+}
+
+__device__ __forceinline__ void nat2_fast_c(const DevMesh &m, int4 nd, const float A[3], const float d[3], float &r,
+                                            float &s) {
+    float4 p0 = ld32(m.X32, nd.x), p1 = ld32(m.X32, nd.y), p2 = ld32(m.X32, nd.z);
+    float qx = (A[0] - p0.x) + d[0], qy = (A[1] - p0.y) + d[1];
+    float ax = p1.x - p0.x, ay = p1.y - p0.y, bx = p2.x - p0.x, by = p2.y - p0.y;
+    float inv = __builtin_amdgcn_rcpf(ax * by - ay * bx);
+    r = (qx * by - qy * bx) * inv;
+    s = (ax * qy - ay * qx) * inv;
+}
+
+/* the FP64 decision at (centre + A) + d, for the few tests that fall
+ * inside the guard band (node coordinates and connectivity by pointer, so
+ * the call can stay out of line without the mesh descriptor) */
+struct ExactDec {
+    float r, s, t;
+    int slot, dec;
+};
+
+#ifndef FVC_ESC_INLINE
+#define FVC_ESC_ATTR __noinline__
+#else
+#define FVC_ESC_ATTR __forceinline__
+#endif
+
+template <int DIM>
+__device__ FVC_ESC_ATTR ExactDec decide_exact_c(const double4 *__restrict__ X64, const int4 *__restrict__ conn, double cx,
+                                                double cy, double cz, int e, float ax, float ay, float az, float dx,
+                                                float dy, float dz, int acc) {
+#pragma clang fp contract(off)
+    const double x = (cx + (double)ax) + (double)dx, y = (cy + (double)ay) + (double)dy,
+                 z = (cz + (double)az) + (double)dz;
+    double r, s, t = 0.0;
+    if (DIM == 3) nat3_exact_p(X64, conn, e, x, y, z, r, s, t);
+    else nat2_exact_p(X64, conn, e, x, y, r, s);
+    ExactDec o;
+    o.slot = -1;
+    o.dec = decide_exact<DIM>(r, s, t, acc, o.slot);
+    o.r = (float)r;
+    o.s = (float)s;
+    o.t = (float)t;
+    return o;
+}
+
+/* walk<MODE != P_F64> with the query point (centre + A) + d: returns the
+ * accepting element, -1 when an adjacency of -1 is crossed, -2 on error */
+template <int DIM>
+__device__ __forceinline__ int walk_c(const DevMesh &m, const float A[3], const float d[3], int e, int acc, float &r,
+                                      float &s, float &t, int4 &nodes, int &err) {
+    for (int hop = 0; hop < m.walk_limit; hop++) {
+        int slot = -1;
+        const int4 nd = ld32(m.conn, e);
+        nodes = nd;
+        float fr, fs, ft = 0.0f;
+        if (DIM == 3) nat3_fast_c(m, nd, A, d, fr, fs, ft);
+        else nat2_fast_c(m, nd, A, d, fr, fs);
+        int dec = decide_fast<DIM>(fr, fs, ft, m.guard, slot);
+        if (dec == 2) {
+            const ExactDec x = decide_exact_c<DIM>(m.X64, m.conn, m.centre[0], m.centre[1], m.centre[2], e, A[0], A[1],
+                                                   A[2], d[0], d[1], d[2], acc);
+            fr = x.r;
+            fs = x.s;
+            ft = x.t;
+            slot = x.slot;
+            dec = x.dec;
+        }
+        if (dec == 1) {
+            r = fr;
+            s = fs;
+            t = ft;
+            return e;
+        }
+        if (dec < 0) {
+            err = E_INDET;
+            return -2;
+        }
+        e = ld32(reinterpret_cast<const int *>(m.adj), 4 * (uint32_t)e + slot);
         if (e == -1) return -1;
     }
     err = E_WALK;
@@ -340,6 +480,35 @@ struct VelT<P_F16> {
     using T = float;
 };
 
+/* velocity of node n of an unstructured mesh (32-bit offsets, see ld32) */
+template <int MODE>
+__device__ __forceinline__ void load_vel_node(const void *base, float scale, uint32_t n, typename VelT<MODE>::T out[3]) {
+    if constexpr (MODE == P_F64) {
+        const double4 v = ld32(reinterpret_cast<const double4 *>(base), n);
+        out[0] = v.x;
+        out[1] = v.y;
+        out[2] = v.z;
+    } else if constexpr (MODE == P_F32) {
+        const float4 v = ld32(reinterpret_cast<const float4 *>(base), n);
+        out[0] = v.x;
+        out[1] = v.y;
+        out[2] = v.z;
+    } else {
+        const uint2 w = ld32(reinterpret_cast<const uint2 *>(base), n);
+        unsigned short h0 = (unsigned short)(w.x & 0xffffu), h1 = (unsigned short)(w.x >> 16);
+        unsigned short h2 = (unsigned short)(w.y & 0xffffu);
+        if constexpr (MODE == P_BF16) {
+            out[0] = __uint_as_float((unsigned)h0 << 16) * scale;
+            out[1] = __uint_as_float((unsigned)h1 << 16) * scale;
+            out[2] = __uint_as_float((unsigned)h2 << 16) * scale;
+        } else {
+            out[0] = __half2float(__ushort_as_half(h0)) * scale;
+            out[1] = __half2float(__ushort_as_half(h1)) * scale;
+            out[2] = __half2float(__ushort_as_half(h2)) * scale;
+        }
+    }
+}
+
 template <int MODE>
 __device__ __forceinline__ void load_vel(const void *base, float scale, int64_t n, typename VelT<MODE>::T out[3]) {
     if constexpr (MODE == P_F64) {
@@ -381,6 +550,35 @@ template <int MODE, int DIM>
 __device__ __forceinline__ void interp_unstruct(const DevMesh &m, const DevField &f, double tloc, int e, double r,
                                                 double s, double t, double out[3]);
 
+/* reduced-precision modes: time-lerp of the four nodes' velocities and the
+ * reference's fabs(r), fabs(s), fabs(t) weights, in FP32 */
+template <int MODE, int DIM>
+__device__ __forceinline__ void interp_nodes_f(const DevField &f, float tl, int4 nd, float r, float s, float t,
+                                               float out[3]) {
+    using VT = typename VelT<MODE>::T;
+    const int nn = DIM == 3 ? 4 : 3;
+    int ids[4] = {nd.x, nd.y, nd.z, nd.w};
+    float n[4][3];
+    for (int q = 0; q < nn; q++) {
+        VT a[3], b[3];
+        load_vel_node<MODE>(f.lo, f.scale_lo, ids[q], a);
+        load_vel_node<MODE>(f.hi, f.scale_hi, ids[q], b);
+        for (int c = 0; c < 3; c++) n[q][c] = a[c] + tl * (b[c] - a[c]);
+    }
+    float ar = fabsf(r), as = fabsf(s), at = fabsf(t);
+    for (int c = 0; c < 3; c++) {
+        float v = n[0][c] + (n[1][c] - n[0][c]) * ar + (n[2][c] - n[0][c]) * as;
+        if (DIM == 3) v += (n[3][c] - n[0][c]) * at;
+        out[c] = (DIM == 2 && c == 2) ? 0.0f : v;
+    }
+}
+
+template <int MODE, int DIM>
+__device__ __forceinline__ void interp_unstruct_f(const DevMesh &m, const DevField &f, float tl, int e, float r,
+                                                  float s, float t, float out[3]) {
+    interp_nodes_f<MODE, DIM>(f, tl, ld32(m.conn, e), r, s, t, out);
+}
+
 template <int MODE, int DIM>
 __device__ __forceinline__ void vel_unstruct(const DevMesh &m, const DevField &f, double tloc, double x, double y,
                                              double z, Probe &p, double out[3], int &err) {
@@ -409,7 +607,7 @@ __device__ __forceinline__ void vel_unstruct(const DevMesh &m, const DevField &f
 template <int MODE, int DIM>
 __device__ __forceinline__ void interp_unstruct(const DevMesh &m, const DevField &f, double tloc, int e, double r,
                                                 double s, double t, double out[3]) {
-    int4 nd = m.conn[e];
+    int4 nd = ld32(m.conn, e);
     using VT = typename VelT<MODE>::T;
     if constexpr (MODE == P_F64) {
 #pragma clang fp contract(off)
@@ -417,8 +615,8 @@ __device__ __forceinline__ void interp_unstruct(const DevMesh &m, const DevField
         const int nn = DIM == 3 ? 4 : 3;
         int ids[4] = {nd.x, nd.y, nd.z, nd.w};
         for (int q = 0; q < nn; q++) {
-            load_vel<MODE>(f.lo, 1.0f, ids[q], a[q]);
-            load_vel<MODE>(f.hi, 1.0f, ids[q], b[q]);
+            load_vel_node<MODE>(f.lo, 1.0f, ids[q], a[q]);
+            load_vel_node<MODE>(f.hi, 1.0f, ids[q], b[q]);
             for (int c = 0; c < 3; c++) n[q][c] = (1 - tloc) * a[q][c] + tloc * b[q][c];
         }
         double ar = fabs(r), as = fabs(s), at = fabs(t);
@@ -430,22 +628,9 @@ __device__ __forceinline__ void interp_unstruct(const DevMesh &m, const DevField
             out[2] = 0.0;
         }
     } else {
-        const int nn = DIM == 3 ? 4 : 3;
-        int ids[4] = {nd.x, nd.y, nd.z, nd.w};
-        float tl = (float)tloc;
-        float n[4][3];
-        for (int q = 0; q < nn; q++) {
-            VT a[3], b[3];
-            load_vel<MODE>(f.lo, f.scale_lo, ids[q], a);
-            load_vel<MODE>(f.hi, f.scale_hi, ids[q], b);
-            for (int c = 0; c < 3; c++) n[q][c] = a[c] + tl * (b[c] - a[c]);
-        }
-        float ar = fabsf((float)r), as = fabsf((float)s), at = fabsf((float)t);
-        for (int c = 0; c < 3; c++) {
-            float v = n[0][c] + (n[1][c] - n[0][c]) * ar + (n[2][c] - n[0][c]) * as;
-            if (DIM == 3) v += (n[3][c] - n[0][c]) * at;
-            out[c] = (DIM == 2 && c == 2) ? 0.0 : (double)v;
-        }
+        float v[3];
+        interp_unstruct_f<MODE, DIM>(m, f, (float)tloc, e, (float)r, (float)s, (float)t, v);
+        for (int c = 0; c < 3; c++) out[c] = v[c];
     }
 }
 
@@ -620,6 +805,68 @@ __device__ __forceinline__ void stage_velocity(const DevMesh &m, const DevField 
         vel_cart<MODE, DIM>(m, f, tloc, X, lft, out);
         if (lft) pr.left = 1;
     }
+}
+
+/* Reduced-precision modes on unstructured meshes: the stage velocity at an
+ * FP32 probe (anchor A, offset d; the point (centre + A) + d) and at the
+ * FP64 position X, with FP32 results. */
+template <int MODE, int DIM>
+__device__ __forceinline__ void stage_velocity_c(const DevMesh &m, const DevField &f, double tloc, const float A[3],
+                                                 const float d[3], Probe &pr, float out[3], int &err) {
+#pragma clang fp contract(off)
+    out[0] = out[1] = out[2] = 0.0f;
+    if (pr.elem == -1) {
+        err = E_ELEM;
+        return;
+    }
+    if ((tloc > 1 + TINY) || (tloc < 0 - TINY)) {
+        err = E_TLOC;
+        return;
+    }
+    if (pr.elem < 0 || pr.elem >= m.ne) {
+        err = E_ELEM;
+        return;
+    }
+    float r, s, t;
+    int4 nd;
+    const int e = walk_c<DIM>(m, A, d, pr.elem, ACC_INTERP, r, s, t, nd, err);
+    if (e == -2) return;
+    if (e == -1) {
+        pr.elem = -1;
+        pr.left = 1;
+        return;
+    }
+    pr.elem = e;
+    interp_nodes_f<MODE, DIM>(f, (float)tloc, nd, r, s, t, out);
+}
+
+template <int MODE, int DIM>
+__device__ __forceinline__ void stage_velocity_x(const DevMesh &m, const DevField &f, double tloc, const double X[3],
+                                                 Probe &pr, float out[3], int &err) {
+#pragma clang fp contract(off)
+    out[0] = out[1] = out[2] = 0.0f;
+    if (pr.elem == -1) {
+        err = E_ELEM;
+        return;
+    }
+    if ((tloc > 1 + TINY) || (tloc < 0 - TINY)) {
+        err = E_TLOC;
+        return;
+    }
+    if (pr.elem < 0 || pr.elem >= m.ne) {
+        err = E_ELEM;
+        return;
+    }
+    double r, s, t;
+    const int e = walk<MODE, DIM>(m, X[0], X[1], X[2], pr.elem, ACC_INTERP, r, s, t, err);
+    if (e == -2) return;
+    if (e == -1) {
+        pr.elem = -1;
+        pr.left = 1;
+        return;
+    }
+    pr.elem = e;
+    interp_unstruct_f<MODE, DIM>(m, f, (float)tloc, e, (float)r, (float)s, (float)t, out);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -806,12 +1053,21 @@ __device__ int interval_run(const DevMesh &m, const DevField &f, PState &p, ISta
     const bool cet = GEN && UNSTRUCT && f.cet_lo != nullptr; /* exposure time (integration.c:427-448) */
     using CT = typename VelT<MODE>::T; /* reduced-precision interpolation reads (float) r, s, t */
     CT c_r = 0, c_s = 0, c_t = 0;
+    /* Reduced-precision modes on unstructured meshes integrate in FP32: stage
+     * velocities, probes (relative to the mesh centre) and the weighted sum
+     * of the four slopes; the position and time stay FP64, the increment is
+     * h times the FP32 slope sum over 6 in FP64. */
+    constexpr bool FAST = MODE != P_F64 && UNSTRUCT;
+    using KT = typename std::conditional<FAST, float, double>::type;
+    /* FAST: time locations as (t - tmin) times one reciprocal of the frame span */
+    const double inv_span = FAST ? 1.0 / (f.tmax - f.tmin) : 0.0;
     while ((is.tc - b) * (b - a) < 0.0) {
         if (taken == max_steps) return 0;
         if ((is.tc + is.h - b) * (is.tc + is.h - a) > 0.0) is.h = sign_of(b - is.tc, b - a);
         const double h = is.h, tc = is.tc;
         const bool micro = fabs(h) <= hmin;
-        double inc[3], f0[3], sr0 = 0.0;
+        double inc[3], sr0 = 0.0;
+        KT f0[3];
         if (ap) {
             sr0 = strain_rate<DIM, UNSTRUCT>(m, f, tc, p, err);
             if (err) break;
@@ -824,27 +1080,59 @@ __device__ int interval_run(const DevMesh &m, const DevField &f, PState &p, ISta
             for (int c = 0; c < 3; c++) inc[c] = h * ve[c];
         } else {
             if (UNSTRUCT && c_elem >= 0) {
-                const double tloc = (tc - f.tmin) / (f.tmax - f.tmin);
+                const double tloc = FAST ? (tc - f.tmin) * inv_span : (tc - f.tmin) / (f.tmax - f.tmin);
                 if ((tloc > 1 + TINY) || (tloc < 0 - TINY)) {
                     err = E_TLOC;
                     break;
                 }
                 is.mp[0].elem = c_elem;
-                interp_unstruct<MODE, DIM>(m, f, tloc, c_elem, c_r, c_s, c_t, f0);
+                if constexpr (FAST) interp_unstruct_f<MODE, DIM>(m, f, (float)tloc, c_elem, c_r, c_s, c_t, f0);
+                else interp_unstruct<MODE, DIM>(m, f, tloc, c_elem, c_r, c_s, c_t, f0);
             } else {
-                stage_velocity<MODE, DIM, UNSTRUCT>(m, f, tc, p.X, is.mp[0], f0, err);
+                if constexpr (FAST) stage_velocity_x<MODE, DIM>(m, f, (tc - f.tmin) * inv_span, p.X, is.mp[0], f0, err);
+                else stage_velocity<MODE, DIM, UNSTRUCT>(m, f, tc, p.X, is.mp[0], f0, err);
             }
             c_elem = -1;
             if (err) break;
             const bool full = fabs(h - hfull) <= 1e-9 * fabs(hfull);
             if (itype != INT_RK4 && micro) {
-                for (int c = 0; c < 3; c++) inc[c] = h * f0[c];
+                for (int c = 0; c < 3; c++) inc[c] = h * (double)f0[c];
             } else if (itype != INT_RK4 && full && p.nhist >= order - 1) {
                 ab_step = true;
                 if (order == 2)
-                    for (int c = 0; c < 3; c++) inc[c] = h * (3 * f0[c] - p.F1[c]) / 2;
+                    for (int c = 0; c < 3; c++) inc[c] = h * (3 * (double)f0[c] - p.F1[c]) / 2;
                 else
-                    for (int c = 0; c < 3; c++) inc[c] = h * (23 * f0[c] - 16 * p.F1[c] + 5 * p.F2[c]) / 12;
+                    for (int c = 0; c < 3; c++)
+                        inc[c] = h * (23 * (double)f0[c] - 16 * p.F1[c] + 5 * p.F2[c]) / 12;
+            } else if constexpr (FAST) {
+                /* k2, k3, k4 at probes X + h/2 f0, X + h/2 k2, X + h k3 (anchor A,
+                 * offset D + c h k); S = f0 + 2 k2 + 2 k3 + k4 */
+                const float hf = (float)h, hh = 0.5f * hf;
+                const double tl_half = (tc + h / 2 - f.tmin) * inv_span, tl_end = (tc + h - f.tmin) * inv_span;
+                float A[3], D[3], d[3], k[3], S[3];
+                for (int c = 0; c < 3; c++) {
+                    const double xc = p.X[c] - m.centre[c];
+                    A[c] = (float)xc;
+                    D[c] = (float)(xc - (double)A[c]);
+                    S[c] = f0[c];
+                    d[c] = D[c] + hh * f0[c];
+                }
+                stage_velocity_c<MODE, DIM>(m, f, tl_half, A, d, is.mp[1], k, err);
+                if (err) break;
+                for (int c = 0; c < 3; c++) {
+                    S[c] = S[c] + 2.0f * k[c];
+                    d[c] = D[c] + hh * k[c];
+                }
+                stage_velocity_c<MODE, DIM>(m, f, tl_half, A, d, is.mp[2], k, err);
+                if (err) break;
+                for (int c = 0; c < 3; c++) {
+                    S[c] = S[c] + 2.0f * k[c];
+                    d[c] = D[c] + hf * k[c];
+                }
+                stage_velocity_c<MODE, DIM>(m, f, tl_end, A, d, is.mp[3], k, err);
+                if (err) break;
+                const double h6 = h * (1.0 / 6.0);
+                for (int c = 0; c < 3; c++) inc[c] = h6 * (double)(S[c] + k[c]);
             } else {
                 double k2[3], k3[3], k4[3], Y[3];
                 for (int c = 0; c < 3; c++) Y[c] = p.X[c] + 0.5 * f0[c] * h;

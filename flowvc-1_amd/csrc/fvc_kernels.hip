@@ -153,14 +153,17 @@ __device__ __forceinline__ void advect_body(const DevMesh &m, const DevField &f,
     add_steps_idle(steps, ns, idle);
 }
 
-/* RK4 only: no multistep state in registers.  Occupancy 3 waves/SIMD (at
- * most 168 VGPRs) fits this kernel without spills and hides more of the
- * gather latency than the 2 waves its natural allocation allows. */
+/* RK4 without extrapolation: no multistep state or particle velocity in
+ * registers (the launchers send RK4 with extrapolation to k_advect_gen).
+ * Occupancy 3 waves/SIMD (at most 168 VGPRs) fits this kernel without spills
+ * and hides more of the gather latency than the 2 waves its natural
+ * allocation allows. */
 template <int MODE, int DIM, int UNS>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void k_advect(
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(FVC_HOT_WAVES))) void k_advect(
     DevMesh m, DevField f, PtrSet p, const int *__restrict__ index, int64_t begin, int64_t count, double t1, double t2,
     int itype, double dt, int extrap, ErrBuf eb, unsigned long long *steps) {
-    advect_body<MODE, DIM, UNS, 0>(m, f, p, index, begin, count, t1, t2, itype, dt, extrap, eb, steps);
+    (void)extrap;
+    advect_body<MODE, DIM, UNS, 0>(m, f, p, index, begin, count, t1, t2, itype, dt, 0, eb, steps);
 }
 
 /* Euler, AB2, AB3 */
@@ -352,7 +355,8 @@ int launch_advect(fvc_ctx *c, PtrSet p, const int *index, int64_t begin, int64_t
         int block = ip.block > 0 ? ip.block : BLOCK;
         if (block != BLOCK) block = BLOCK; /* kernels are compiled for 256-lane blocks */
         timing_begin(c);
-        if (ip.int_type == FVC_INT_RK4 && !f.sr_lo && !f.cet_lo) /* strain rate, exposure: generic build only */
+        /* strain rate, exposure, extrapolation: generic build only */
+        if (ip.int_type == FVC_INT_RK4 && !ip.extrapolate && !f.sr_lo && !f.cet_lo)
             FVC_DISPATCH(c->precision, c->mesh.dims, c->mesh.mesh_type, k_advect,
                          <<<grid_for(count, block), block, 0, c->stream>>>(c->mesh, f, p, index, begin, count, t1, t2,
                                                                            ip.int_type, ip.time_step, ip.extrapolate,
@@ -1303,10 +1307,11 @@ __device__ __forceinline__ void ftle_advect_body(const DevMesh &m, const DevFiel
 }
 
 template <int MODE, int DIM, int UNS>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void k_ftle_advect(
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(FVC_HOT_WAVES))) void k_ftle_advect(
     DevMesh m, DevField f, FtleDev d, const int *__restrict__ active, int64_t nactive, double t1, double t2, int itype,
     double dt, int extrap, ErrBuf eb, unsigned long long *steps) {
-    ftle_advect_body<MODE, DIM, UNS, 0>(m, f, d, active, nactive, t1, t2, itype, dt, extrap, eb, steps);
+    (void)extrap; /* RK4 without extrapolation; the launcher sends the rest to k_ftle_advect_gen */
+    ftle_advect_body<MODE, DIM, UNS, 0>(m, f, d, active, nactive, t1, t2, itype, dt, 0, eb, steps);
 }
 
 template <int MODE, int DIM, int UNS>
@@ -1694,7 +1699,7 @@ int launch_ftle_frame(fvc_ftle *f, int lo, int hi, double tmin, double tmax, dou
     FVC_HIP(hipMemsetAsync(c->d_steps, 0, sizeof(unsigned long long), c->stream));
     if (f->nactive > 0) {
         timing_begin(c);
-        if (itype == 1)
+        if (itype == 1 && !ip.extrapolate)
             FVC_DISPATCH(c->precision, c->mesh.dims, c->mesh.mesh_type, k_ftle_advect,
                          <<<grid_for(f->nactive, BLOCK), BLOCK, 0, c->stream>>>(c->mesh, fld, d, f->active,
                                                                                 f->nactive, t1, t2, itype,

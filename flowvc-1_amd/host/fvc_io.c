@@ -186,13 +186,32 @@ int fvc_deck_read(const char *path, fvc_deck *d, char *err, int errlen) {
         if (!next_token(&r, eq, sizeof eq, 0) || !next_token(&r, val, sizeof val, 0)) break;
         const kdesc *kd = &deck_keys[k];
         char *p = (char *)d + kd->off;
+        char *end = NULL;
+        /* the reference reads with fscanf into fixed buffers and leaves an
+         * unparsable number unset; both are refused here */
         if (kd->type == KS) {
-            strncpy(p, val, FVC_STR - 1);
-            p[FVC_STR - 1] = 0;
+            if (strlen(val) >= FVC_STR) {
+                fclose(f);
+                seterr(err, errlen, "Value of %s is longer than %d characters", kd->name, FVC_STR - 1);
+                return -3;
+            }
+            strcpy(p, val);
         } else if (kd->type == KI) {
-            *(int32_t *)p = (int32_t)strtol(val, NULL, 10);
+            long v = strtol(val, &end, 10);
+            if (end == val || *end || v < INT32_MIN || v > INT32_MAX) {
+                fclose(f);
+                seterr(err, errlen, "Could not read an integer for %s from \"%s\"", kd->name, val);
+                return -3;
+            }
+            *(int32_t *)p = (int32_t)v;
         } else {
-            *(double *)p = strtod(val, NULL);
+            double v = strtod(val, &end);
+            if (end == val || *end) {
+                fclose(f);
+                seterr(err, errlen, "Could not read a number for %s from \"%s\"", kd->name, val);
+                return -3;
+            }
+            *(double *)p = v;
         }
         k++;
     }

@@ -9,6 +9,7 @@ caught before any comparison.
 import hashlib
 import json
 import os
+import shutil
 import sys
 
 import numpy as np
@@ -277,10 +278,75 @@ def gen_chan(d):
     host.write_binary_list(os.path.join(d, "seeds.bin"), pts, dims=2)
 
 
+STEN_DATA = os.path.join(HERE, "stenosis2d_data")
+STEN_SLICES = (7070, 7080, 7090, 7100)
+
+
+def sten_mesh():
+    """The reference's 2-D stenosis mesh (cudaVC/data/Stenosis2D/bin: 40,279
+    nodes, 79,066 triangles, adjacency with a 0 in the unused fourth slot)."""
+    xyz = np.fromfile(os.path.join(STEN_DATA, "sineflow_coordinates.bin"), "<f8", offset=4).reshape(-1, 3)
+    conn = np.fromfile(os.path.join(STEN_DATA, "sineflow_connectivity.bin"), "<i4", offset=4).reshape(-1, 4)
+    return xyz, conn
+
+
+def gen_sten(d):
+    """The reference's own Stenosis2D data, copied unchanged: mesh files and
+    the first four velocity slices (sineflow_vel.7070 ... 7100, t = 1.414 ...
+    1.420 s); the runs below read no others."""
+    for f in ["sineflow_coordinates.bin", "sineflow_connectivity.bin", "sineflow_adjacency.bin"] + \
+            ["sineflow_vel.%d.bin" % k for k in STEN_SLICES]:
+        shutil.copy(os.path.join(STEN_DATA, f), os.path.join(d, f))
+
+
+def sten_probes():
+    """Location probes on the stenosis: random points over the bounding box
+    (the walls curve through it, so some fall outside), mesh vertices, edge
+    midpoints (points held by two or more triangles, where the chain's guesses
+    decide) and far-apart points."""
+    xyz, conn = sten_mesh()
+    rng = np.random.default_rng(31)
+    pts = [rng.uniform([-0.0171, 0.0], [0.0171, 0.00275], size=(300, 2)), xyz[::211, :2],
+           0.5 * (xyz[conn[::157, 0], :2] + xyz[conn[::157, 1], :2]),
+           [[-0.0171, 0.0001], [0.0171, 0.0027], [0.0, 0.0014], [0.0, -0.001], [0.02, 0.001]]]
+    return np.concatenate(pts)
+
+
+def gen_sten_probes(d):
+    gen_sten(d)
+    p = sten_probes()
+    with open(os.path.join(d, "probes.txt"), "w") as f:
+        f.write("%d\n" % p.shape[0])
+        for q in p:
+            f.write("%.17g %.17g\n" % tuple(q))
+
+
+def gen_sten_velout(d):
+    gen_sten(d)
+    p = sten_probes()[::2]
+    with open(os.path.join(d, "velpts.txt"), "w") as f:
+        f.write("%d\n" % p.shape[0])
+        for q in p:
+            f.write("%.17g %.17g\n" % tuple(q))
+
+
+def gen_sten_seeds(d):
+    """1,200 seeds over the upstream half of the channel and the throat."""
+    gen_sten(d)
+    rng = np.random.default_rng(37)
+    host.write_binary_list(os.path.join(d, "seeds.bin"), rng.uniform([-0.012, 0.0002], [0.002, 0.0026], size=(1200, 2)),
+                           dims=2)
+
+
 BOX = dict(Dimensions=3, Data_MeshType=0, Data_InFilePrefix="box", Data_TRes=11, Data_TDelta=0.5, Data_TMin=0.0,
            Data_TPeriodic=0, Data_MeshBounds_XMin=0.0, Data_MeshBounds_XMax=2.0, Data_MeshBounds_YMin=0.0,
            Data_MeshBounds_YMax=1.0, Data_MeshBounds_ZMin=0.0, Data_MeshBounds_ZMax=1.0, Int_Type=1,
            Int_TimeStep=0.01)
+STEN = dict(Dimensions=2, Data_MeshType=1, Data_InFilePrefix="sineflow", Data_SuffixTMin=7070, Data_SuffixTDelta=10,
+            Data_TRes=101, Data_TDelta=0.002, Data_TMin=1.414, Data_TPeriodic=1, Data_MeshBounds_XMin=-0.01715,
+            Data_MeshBounds_XMax=0.01715, Data_MeshBounds_YMin=0.0, Data_MeshBounds_YMax=0.00275,
+            Data_MeshBounds_ZMin=0.0, Data_MeshBounds_ZMax=0.0, Int_Type=1, Int_TimeStep=0.00007,
+            LocalSearchChecking=1)
 CHAN = dict(Dimensions=2, Data_MeshType=1, Data_InFilePrefix="chan", Data_TRes=21, Data_TDelta=0.05, Data_TMin=0.0,
             Data_TPeriodic=0, Data_MeshBounds_XMin=0.0, Data_MeshBounds_XMax=4.0, Data_MeshBounds_YMin=0.0,
             Data_MeshBounds_YMax=1.0, Data_MeshBounds_ZMin=0.0, Data_MeshBounds_ZMax=0.0, Int_Type=1,
@@ -573,6 +639,22 @@ CASES = {
                                FTLE_ICFile="bft_ic.bin")),
     # 2-D unstructured tracers (triangles, velocity.c:570-620, mesh.c:566-605): random seeds, seeds on nodes and
     # edges, seeds outside; exits through the outlet
+    # the reference's Stenosis2D data (real 2-D triangle mesh and velocity): location chain on random, vertex
+    # and edge-midpoint probes; velocity output at listed points; tracers over two data frames with the
+    # reference deck's time step (28.6 steps a frame: the last one clipped)
+    "sten_locate": dict(gen=gen_sten_probes, outputs=[], ic="sloc.IC",
+                        deck=dict(STEN, Output_TStart=1.414, Output_TRes=1, Output_TDelta=0.002, Int_TimeDirection=1,
+                                  Trace_Compute=1, Trace_GenerateMesh=0, Trace_InFile="probes.txt",
+                                  Trace_InFileFormat=1, Trace_IntTLength=0.0, Trace_OutFilePrefix="sloc")),
+    "sten_velout": dict(gen=gen_sten_velout, outputs=["svo.%d.bin" % k for k in range(4)] + ["svo_coordinates.bin"],
+                        deck=dict(STEN, VelOut_Compute=1, VelOut_MeshBounds_XMin=-0.01715, VelOut_MeshBounds_XMax=0.01715,
+                                  VelOut_MeshBounds_YMin=0.0, VelOut_MeshBounds_YMax=0.00275, Output_TStart=1.414,
+                                  Output_TRes=4, Output_TDelta=0.0013, Int_TimeDirection=1, VelOut_GenerateMesh=0,
+                                  VelOut_InFile="velpts.txt", VelOut_InFileFormat=1, VelOut_FilePrefix="svo")),
+    "sten_tracers": dict(gen=gen_sten_seeds, outputs=["str.%d.bin" % k for k in range(5)], ic="str.IC",
+                         deck=dict(STEN, Output_TStart=1.414, Output_TRes=5, Output_TDelta=0.001, Int_TimeDirection=1,
+                                   Trace_Compute=1, Trace_GenerateMesh=0, Trace_InFile="seeds.bin",
+                                   Trace_InFileFormat=4, Trace_IntTLength=0.004, Trace_OutFilePrefix="str")),
     "chan_tracers": dict(gen=gen_chan, outputs=["ctr.%d.bin" % k for k in range(6)], ic="ctr.IC",
                          deck=dict(CHAN, Output_TStart=0.0, Output_TRes=6, Output_TDelta=0.2, Int_TimeDirection=1,
                                    Trace_Compute=1, Trace_GenerateMesh=0, Trace_InFile="seeds.bin",

@@ -40,7 +40,8 @@ def run_driver(case, tmp_path, precision="f64"):
                                   "pipe_tracers_grid", "dg_tracers_back", "pipe_tracers_back",
                                   "pipe_tracers_periods", "pipe_tracers_normal", "pipe_tracers_normal_back",
                                   "dg_tracers_rkf", "pipe_tracers_rkf", "pipe_staggered_rkf", "pipe_tracers_rkf_ap",
-                                  "pipe_tracers_rkf_cet", "pipe_staggered_normal"])
+                                  "pipe_tracers_rkf_cet", "pipe_staggered_normal", "sten_locate", "sten_velout",
+                                  "sten_tracers"])
 def test_gpu_trajectories_bit_identical(case, tmp_path):
     man, data, out = run_driver(case, tmp_path)
     fx = cases.fixture_dir(case)
@@ -89,13 +90,13 @@ def test_gpu_ftle_matches_reference(case, tmp_path):
 
 
 @pytest.mark.parametrize("case,precision", [("pipe_tracers", "f32"), ("dg_tracers", "f32"), ("box_tracers", "f32"),
-                                            ("chan_tracers", "f32")])
+                                            ("chan_tracers", "f32"), ("sten_tracers", "f32")])
 def test_gpu_f32_driver_within_tolerance(case, tmp_path, precision):
     """FP32 velocity/geometry with FP64 state: final positions within 1e-5 of the
     domain size of the reference's FP64 trajectories (points present in both)."""
     man, data, out = run_driver(case, tmp_path, precision)
     fx = cases.fixture_dir(case)
-    size = {"pipe": 10.0, "dg": 2.0, "box": 2.0, "chan": 4.0}[case.split("_")[0]]
+    size = {"pipe": 10.0, "dg": 2.0, "box": 2.0, "chan": 4.0, "sten": 0.0343}[case.split("_")[0]]
     last = sorted(man["outputs"], key=lambda s: int(s.split(".")[-2]))[-1]
     _, a = host.read_tracer_output(os.path.join(fx, last))
     _, b = host.read_tracer_output(os.path.join(out, last))
@@ -261,3 +262,36 @@ def test_gpu_ftle_from_reference_ic_file(case, tmp_path):
         np.testing.assert_array_equal(a == -1.0, b == -1.0)
         np.testing.assert_allclose(b, a, rtol=1e-12, atol=1e-12, err_msg=f)
 
+
+
+@pytest.mark.parametrize("precision", ["f32", "bf16"])
+def test_gpu_stenosis_locate_reduced_precision(precision):
+    """On the reference's Stenosis2D triangle mesh, the FP32 walk decisions
+    (with FP64 escalation inside the guard band) give, for every probe of the
+    location fixture, the element the reference's FP64 chain gave (the
+    fixture's .IC); a walk and a global search from scratch agree with the
+    oracle's for the same points."""
+    import sys
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle
+    from fvcpy import fvc
+    xyz, conn = cases.sten_mesh()
+    adj = np.fromfile(os.path.join(cases.STEN_DATA, "sineflow_adjacency.bin"), "<i4", offset=4).reshape(-1, 4)
+    bounds = [-0.01715, 0.01715, 0.0, 0.00275, 0.0, 0.0]
+    ctx = fvc.Context(2, bounds, precision, 2, coords=xyz, conn=conn, adj=adj)
+    probes = cases.sten_probes()
+    q = np.concatenate([probes, np.zeros((probes.shape[0], 1))], 1)
+    ref = host.read_ic(os.path.join(cases.fixture_dir("sten_locate"), "sloc.IC"))
+    print("stenosis locate %s: %d probes" % (precision, q.shape[0]), flush=True)
+    got = ctx.seed_list(q, 1)
+    np.testing.assert_array_equal(got["LeftDomain"], ref["LeftDomain"])
+    inside = ref["LeftDomain"] == 0
+    np.testing.assert_array_equal(got["ElementIndex"][inside], ref["ElementIndex"][inside])
+    om = oracle.Mesh(2, bounds, coords=xyz, conn=conn, adj=adj)
+    rng = np.random.default_rng(3)
+    guess = rng.integers(0, conn.shape[0], q.shape[0]).astype(np.int32)
+    w_ref, rc = om.locate_local(q, guess)
+    assert rc == 0
+    np.testing.assert_array_equal(ctx.locate(q, guess, fvc.LOCATE_WALK), w_ref)
+    np.testing.assert_array_equal(ctx.locate(q, None, fvc.LOCATE_GLOBAL), om.locate_global(q))
+    ctx.close()

@@ -33,7 +33,8 @@ def run_oracle(case, tmp_path, oracle_dir):
 @pytest.mark.parametrize("case", ["dg_tracers", "pipe_tracers", "pipe_locate", "dg_tracers_euler",
                                   "pipe_tracers_euler", "pipe_tracers_extrap", "box_tracers", "chan_tracers",
                                   "pipe_tracers_grid", "dg_tracers_back", "pipe_tracers_back",
-                                  "pipe_tracers_periods", "dg_tracers_rkf", "pipe_tracers_rkf"])
+                                  "pipe_tracers_periods", "dg_tracers_rkf", "pipe_tracers_rkf", "sten_locate",
+                                  "sten_tracers"])
 def test_trajectories_bit_identical(case, tmp_path, oracle_built, host_built):
     man, data, out = run_oracle(case, tmp_path, oracle_built)
     fx = cases.fixture_dir(case)
