@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <mutex>
 #include <vector>
 
 #include "fvc_internal.h"
@@ -258,6 +259,8 @@ static float guard_band(const fvc_mesh_desc *md, const double centre[3]) {
 /* ------------------------------------------------------------------------ */
 /* context                                                                   */
 /* ------------------------------------------------------------------------ */
+static int noalloc_scratch(int device, void **ptr);
+
 extern "C" int fvc_create(const fvc_mesh_desc *md, const fvc_params *pp, fvc_ctx **out) {
     if (!md || !pp || !out) return FVC_E_INVALID;
     *out = nullptr;
@@ -292,6 +295,13 @@ extern "C" int fvc_create(const fvc_mesh_desc *md, const fvc_params *pp, fvc_ctx
     if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
         fvc_destroy(c);
         return FVC_E_HIP;
+    }
+    {
+        void *sp = nullptr; /* the no-allocation staging of this device (fvc_locate_noalloc) */
+        if (noalloc_scratch(pp->device, &sp)) {
+            fvc_destroy(c);
+            return FVC_E_HIP;
+        }
     }
     if (hipMemset(c->d_err, 0, sizeof(int)) != hipSuccess) {
         fvc_destroy(c);
@@ -916,6 +926,60 @@ extern "C" int fvc_locate(fvc_ctx *c, const double *xyz, const int32_t *guess, i
     hipFreeAsync(dout, c->stream);
     hipStreamSynchronize(c->stream);
     return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* point location without allocation                                         */
+/* ------------------------------------------------------------------------ */
+/* Staging for fvc_locate_noalloc: FVC_NOALLOC_MAX_POINTS query points (24 B),
+ * guesses (4 B) and results (4 B), each region 256-byte aligned.  Every GPU
+ * the library runs on holds its own copy until the process ends; calls on
+ * one GPU share it. */
+static __device__ __align__(256) unsigned char scratch[(size_t)FVC_NOALLOC_MAX_POINTS * 32 + 1024];
+
+/* scratch's device address on each GPU (its host-side name is only a
+ * stand-in: kernels and copies take these addresses) */
+static std::mutex g_scratch_mu;
+static std::vector<void *> g_scratch_ptr;
+
+static int noalloc_scratch(int device, void **ptr) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (device < 0) return FVC_E_INVALID;
+    if ((size_t)device >= g_scratch_ptr.size()) g_scratch_ptr.resize((size_t)device + 1, nullptr);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        int cur = -1;
+        if (hipGetDevice(&cur) != hipSuccess || cur != device) return FVC_E_HIP;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(::scratch)) != hipSuccess || !p) return FVC_E_HIP;
+        g_scratch_ptr[device] = p;
+    }
+    *ptr = g_scratch_ptr[device];
+    return FVC_OK;
+}
+
+extern "C" int fvc_locate_noalloc(fvc_ctx *c, const double *xyz, const int32_t *guess, int64_t n, int32_t mode,
+                                  int32_t *elem) {
+    if (!c || !xyz || !elem || n < 0 || mode < 0 || mode > 2 || (mode != FVC_LOCATE_GLOBAL && !guess))
+        return FVC_E_INVALID;
+    if (c->mesh.mesh_type != 1) return set_err(c, FVC_E_INVALID, "location needs an unstructured mesh");
+    if (n > FVC_NOALLOC_MAX_POINTS)
+        return set_err(c, FVC_E_INVALID, "fvc_locate_noalloc: more points than FVC_NOALLOC_MAX_POINTS");
+    if (n == 0) return FVC_OK;
+    if (hipSetDevice(c->device) != hipSuccess) return set_err(c, FVC_E_HIP, "fvc_locate_noalloc: hipSetDevice");
+    void *base = nullptr;
+    if (noalloc_scratch(c->device, &base)) return set_err(c, FVC_E_HIP, "fvc_locate_noalloc: scratch address");
+    char *b = (char *)base;
+    const size_t xbytes = ((size_t)n * 24 + 255) / 256 * 256, ibytes = ((size_t)n * 4 + 255) / 256 * 256;
+    double *dx = (double *)b;
+    int *dg = (int *)(b + xbytes);
+    int *dout = (int *)(b + xbytes + ibytes);
+    FVC_HIP(hipMemcpyAsync(dx, xyz, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
+    if (guess) FVC_HIP(hipMemcpyAsync(dg, guess, sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
+    int rc = launch_locate(c, dx, guess ? dg : nullptr, n, mode, dout); /* every result is written */
+    if (rc) return rc;
+    FVC_HIP(hipMemcpyAsync(elem, dout, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
+    FVC_HIP(hipStreamSynchronize(c->stream));
+    return FVC_OK;
 }
 
 static int seed_common(fvc_ctx *c, int chain, const double *xyz, int64_t n, const double mn[3], const double del[3],

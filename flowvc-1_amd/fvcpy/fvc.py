@@ -14,6 +14,7 @@ from .host import POINT_DTYPE
 OK = 0
 E_INVALID, E_NOMEM, E_HIP, E_INDETERMINATE_NEIGHBOR, E_TLOC_RANGE, E_ELEM_NEG1, E_WALK_LIMIT = -1, -2, -3, -4, -5, -6, -7
 E_ADMISSION, E_STATE = -9, -10
+NOALLOC_MAX_POINTS = 1 << 18  # FVC_NOALLOC_MAX_POINTS (include/fvc.h)
 PREC = {"f64": 0, "f32": 1, "bf16": 2, "f16": 3}
 LOCATE_WALK, LOCATE_GLOBAL, LOCATE_WALK_GLOBAL = 0, 1, 2
 INT_EULER, INT_RK4, INT_RKF, INT_AB2, INT_AB3 = 0, 1, 2, 3, 4

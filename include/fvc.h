@@ -208,6 +208,18 @@ int fvc_last_step_count(fvc_ctx *ctx, int64_t *particle_steps);
 /* n query points xyz (n x 3, host), guesses (walk modes), results elem (host).
  * Element indices are those ompVC returns for the same (X, guess). */
 int fvc_locate(fvc_ctx *ctx, const double *xyz, const int32_t *guess, int64_t n, int32_t mode, int32_t *elem);
+/* fvc_locate without allocation: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc): the query points,
+ * guesses and results are staged in a fixed device array of the library
+ * (one per GPU, held until the process ends) instead of memory allocated per
+ * call.  Same arguments and results as fvc_locate; at most
+ * FVC_NOALLOC_MAX_POINTS points per call (more: FVC_E_INVALID).  Calls on one
+ * GPU share the array: calls that overlap on one GPU (several host threads,
+ * contexts on the same device) race. */
+#define FVC_NOALLOC_MAX_POINTS (1 << 18)
+int fvc_locate_noalloc(fvc_ctx *ctx, const double *xyz, const int32_t *guess, int64_t n, int32_t mode,
+                       int32_t *elem);
 /* Seed location chains of ReadInTraceMesh (ompVC/tracers.c:1111-1251) for a
  * point list, and GenerateTracerMesh (tracers.c:1298-1532) for a grid: writes
  * X, LeftDomain, ElementIndex and LeftDomainTime of n records (grid: k,j,i
