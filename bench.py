@@ -258,20 +258,26 @@ def bench_batch(args, np, fvc, host):
     n = recs.shape[0]
     log("batch: %d releases, %d seeds (%d RK4 / %d AB3 releases), setup %.1f s" %
         (len(rel), n, int((types == fvc.INT_RK4).sum()), int((types == fvc.INT_AB3).sum()), time.time() - t0))
-    # autotune on a 1/8 copy (same releases, an eighth of the seeds each)
-    rel_t = release_table(counts, starts, horizons, types, scale=8)
-    idx_t = np.concatenate([np.arange(b0, b0 + rt[1]) for (b0, _, _, _, _), rt in zip(rel, rel_t)])
-    Pt = fvc.Particles(ctx, idx_t.size)
-    Pt.set(recs[idx_t])
-    Bt = fvc.Batch(ctx, Pt, rel_t, 0.0, 0.02, 50, 1e-3)
-    t0 = time.time()
-    blocks = [64, 128, 256, 512, 1024] if args.precision == "f32" else [256]
-    budgets = [1, 4, 16, 64, 256, 0]
-    best, best_ms = Bt.tune(blocks, budgets, args.batch_graph)
-    Bt.close()
-    del Pt
-    log("autotune %.1f s over blocks %s x budgets %s: best %s (%.1f ms on the 1/8 batch)" %
-        (time.time() - t0, blocks, budgets, best, best_ms))
+    if args.batch_config:
+        # fixed block size and steps per launch, no tuning sweep
+        bs, bu = (int(v) for v in args.batch_config.split(","))
+        best, best_ms = {"block": bs, "steps_per_launch": bu, "use_graph": args.batch_graph, "waves_per_cu": 0}, None
+        log("batch configuration given: %s (no autotune)" % best)
+    else:
+        # autotune on a 1/8 copy (same releases, an eighth of the seeds each)
+        rel_t = release_table(counts, starts, horizons, types, scale=8)
+        idx_t = np.concatenate([np.arange(b0, b0 + rt[1]) for (b0, _, _, _, _), rt in zip(rel, rel_t)])
+        Pt = fvc.Particles(ctx, idx_t.size)
+        Pt.set(recs[idx_t])
+        Bt = fvc.Batch(ctx, Pt, rel_t, 0.0, 0.02, 50, 1e-3)
+        t0 = time.time()
+        blocks = [64, 128, 256, 512, 1024] if args.precision == "f32" else [256]
+        budgets = [1, 4, 16, 64, 256, 0]
+        best, best_ms = Bt.tune(blocks, budgets, args.batch_graph)
+        Bt.close()
+        del Pt
+        log("autotune %.1f s over blocks %s x budgets %s: best %s (%.1f ms on the 1/8 batch)" %
+            (time.time() - t0, blocks, budgets, best, best_ms))
     P = fvc.Particles(ctx, n)
     P.set(recs)
     B = fvc.Batch(ctx, P, rel, 0.0, 0.02, 50, 1e-3)
@@ -733,6 +739,8 @@ def main():
                     help="also run every request alone (batch-at-a-time comparison)")
     ap.add_argument("--releases", type=int, default=64)
     ap.add_argument("--batch-cpu-stride", type=int, default=10)
+    ap.add_argument("--batch-config", default="",
+                    help="batch workload: 'block,steps_per_launch' to run without the autotuner sweep")
     ap.add_argument("--batch-graph", type=int, default=1, choices=[0, 1],
                     help="replay the launch sequence from a captured hipGraph (1) or launch it directly (0)")
     args = ap.parse_args()
