@@ -320,33 +320,34 @@ __device__ __forceinline__ void nat2_fast_c(const DevMesh &m, int4 nd, const flo
     s = (ax * qy - ay * qx) * inv;
 }
 
-/* the FP64 decision at (centre + A) + d, for the few tests that fall
- * inside the guard band (node coordinates and connectivity by pointer, so
- * the call can stay out of line without the mesh descriptor) */
+/* the FP64 decision at the double point (x, y, z), for the few tests that
+ * fall inside the guard band (node coordinates and connectivity by pointer,
+ * so the call can stay out of line without the mesh descriptor); tie reports
+ * d == -TINY, where the two acceptance rules differ */
 struct ExactDec {
     float r, s, t;
-    int slot, dec;
+    int slot, dec, tie;
 };
 
-#ifndef FVC_ESC_INLINE
+/* inlined: an out-of-line call costs the hot kernels registers around the
+ * call (build with FVC_ESC_NOINLINE to compare) */
+#ifdef FVC_ESC_NOINLINE
 #define FVC_ESC_ATTR __noinline__
 #else
 #define FVC_ESC_ATTR __forceinline__
 #endif
 
 template <int DIM>
-__device__ FVC_ESC_ATTR ExactDec decide_exact_c(const double4 *__restrict__ X64, const int4 *__restrict__ conn, double cx,
-                                                double cy, double cz, int e, float ax, float ay, float az, float dx,
-                                                float dy, float dz, int acc) {
+__device__ FVC_ESC_ATTR ExactDec decide_exact_at(const double4 *__restrict__ X64, const int4 *__restrict__ conn, int e,
+                                                 double x, double y, double z, int acc) {
 #pragma clang fp contract(off)
-    const double x = (cx + (double)ax) + (double)dx, y = (cy + (double)ay) + (double)dy,
-                 z = (cz + (double)az) + (double)dz;
     double r, s, t = 0.0;
     if (DIM == 3) nat3_exact_p(X64, conn, e, x, y, z, r, s, t);
     else nat2_exact_p(X64, conn, e, x, y, r, s);
     ExactDec o;
     o.slot = -1;
-    o.dec = decide_exact<DIM>(r, s, t, acc, o.slot);
+    o.tie = 0;
+    o.dec = decide_exact<DIM>(r, s, t, acc, o.slot, &o.tie);
     o.r = (float)r;
     o.s = (float)s;
     o.t = (float)t;
@@ -367,8 +368,9 @@ __device__ __forceinline__ int walk_c(const DevMesh &m, const float A[3], const 
         else nat2_fast_c(m, nd, A, d, fr, fs);
         int dec = decide_fast<DIM>(fr, fs, ft, m.guard, slot);
         if (dec == 2) {
-            const ExactDec x = decide_exact_c<DIM>(m.X64, m.conn, m.centre[0], m.centre[1], m.centre[2], e, A[0], A[1],
-                                                   A[2], d[0], d[1], d[2], acc);
+            const ExactDec x = decide_exact_at<DIM>(m.X64, m.conn, e, (m.centre[0] + (double)A[0]) + (double)d[0],
+                                                    (m.centre[1] + (double)A[1]) + (double)d[1],
+                                                    (m.centre[2] + (double)A[2]) + (double)d[2], acc);
             fr = x.r;
             fs = x.s;
             ft = x.t;
@@ -999,6 +1001,222 @@ __device__ inline double strain_rate(const DevMesh &m, const DevField &f, double
     return (1 - tloc) * f.sr_lo[idx] + tloc * f.sr_hi[idx];
 }
 
+
+/* ------------------------------------------------------------------------ */
+/* RK4 of the reduced-precision modes on unstructured meshes, one           */
+/* containment test per lane per loop iteration                              */
+/* ------------------------------------------------------------------------ */
+/* The same integrator as interval_run's RK4 (integration.c:317-462): four
+ * stage probes that keep their own walk elements, the increment
+ * h (f0 + 2 k2 + 2 k3 + k4) / 6, a locate walk at the new position, the exit
+ * revert, the first stage taking the locate's element and coordinates.  It is
+ * written as a state machine: a lane is in one of five phases (first-stage
+ * walk at X, the k2, k3, k4 probe walks, the locate walk), and each pass of
+ * the loop makes one containment test for every lane, followed by the
+ * interpolation of the lanes whose test accepted and the bookkeeping of their
+ * next phase.  Walks of different lengths then overlap instead of making the
+ * whole wavefront repeat the longest one.  Every lane performs the same
+ * floating-point operations, in the same order, as a straight-line walk would.
+ * Query points are anchored FP32 offsets (A, d), see nat3_fast_c; an
+ * undecided test of the first-stage or locate walk is decided in FP64 at the
+ * double position X, a probe's at (centre + A) + d. */
+enum { PH_F0 = 0, PH_K2 = 1, PH_K3 = 2, PH_K4 = 3, PH_LOC = 4, PH_DONE = 5 };
+
+template <int MODE, int DIM>
+__device__ int rk4_sm(const DevMesh &m, const DevField &f, PState &p, IState &is, double a, double b, double dt,
+                      int max_steps, int &err, uint32_t &nsteps, double &stop) {
+#pragma clang fp contract(off)
+    if (!is.started) {
+        if (p.left) {
+            err = E_LEFT;
+            stop = b;
+            return 1;
+        }
+        if (p.elem < 0 || p.elem >= m.ne) {
+            err = E_ELEM;
+            stop = b;
+            return 1;
+        }
+        is.tc = a;
+        is.h = sign_of(dt, b - a);
+        for (int q = 0; q < 4; q++) {
+            is.mp[q].elem = p.elem;
+            is.mp[q].left = 0;
+        }
+        is.started = 1;
+    }
+    const double hmin = 1e-12 * fabs(dt);
+    const double inv_span = 1.0 / (f.tmax - f.tmin);
+    int taken = 0, ret = 1;
+    int phase = PH_F0, e = 0, hops = 0, start = 0;
+    float A[3], D[3], dq[3], S[3] = {0.f, 0.f, 0.f};
+    float S4[3] = {0.f, 0.f, 0.f}; /* f0 + 2 k2 + 2 k3 + k4 of the step (its increment is h/6 S4) */
+    /* anchors of X: A = (float)(X - centre), D its FP32 residual */
+    auto anchor = [&]() {
+        for (int c = 0; c < 3; c++) {
+            const double xc = p.X[c] - m.centre[c];
+            A[c] = (float)xc;
+            D[c] = (float)(xc - (double)A[c]);
+        }
+    };
+    /* time location of a phase: tc, tc + h/2, tc + h */
+    auto tloc_of = [&](int ph) {
+        const double tq = ph == PH_F0 ? is.tc : (ph == PH_K4 ? is.tc + is.h : is.tc + is.h / 2);
+        return (tq - f.tmin) * inv_span;
+    };
+    /* enter a probe phase: its walk starts at the probe's own element */
+    auto enter_probe = [&](int ph) {
+        phase = ph;
+        hops = 0;
+        const double tl = tloc_of(ph);
+        if (is.mp[ph].elem == -1) err = E_ELEM;
+        else if ((tl > 1 + TINY) || (tl < 0 - TINY)) err = E_TLOC;
+        else if (is.mp[ph].elem < 0 || is.mp[ph].elem >= m.ne) err = E_ELEM;
+        e = is.mp[ph].elem;
+    };
+    /* start of a step (the loop head of integration.c:344); false when the
+     * interval is done or the step budget spent */
+    auto begin_step = [&]() -> bool {
+        if (!((is.tc - b) * (b - a) < 0.0)) {
+            stop = b;
+            return false;
+        }
+        if (taken == max_steps) {
+            ret = 0;
+            return false;
+        }
+        if ((is.tc + is.h - b) * (is.tc + is.h - a) > 0.0) is.h = sign_of(b - is.tc, b - a);
+        return true;
+    };
+    anchor();
+    int4 nd = make_int4(0, 0, 0, 0);
+    float r = 0.f, s = 0.f, t = 0.f;
+    int ltie = 0; /* a locate-walk test hit d == -TINY */
+    if (!begin_step()) phase = PH_DONE;
+    else enter_probe(PH_F0);
+    if (err) phase = PH_DONE;
+    while (phase != PH_DONE) {
+        /* ---- one containment test of this lane's walk */
+        nd = ld32(m.conn, e);
+        const bool at_x = phase == PH_F0 || phase == PH_LOC;
+        int slot = -1, dec;
+        if (DIM == 3) nat3_fast_c(m, nd, A, at_x ? D : dq, r, s, t);
+        else nat2_fast_c(m, nd, A, at_x ? D : dq, r, s);
+        dec = decide_fast<DIM>(r, s, t, m.guard, slot);
+        if (dec == 2) {
+            double x, y, z;
+            if (at_x) {
+                x = p.X[0];
+                y = p.X[1];
+                z = p.X[2];
+            } else {
+                x = (m.centre[0] + (double)A[0]) + (double)dq[0];
+                y = (m.centre[1] + (double)A[1]) + (double)dq[1];
+                z = (m.centre[2] + (double)A[2]) + (double)dq[2];
+            }
+            const ExactDec xd =
+                decide_exact_at<DIM>(m.X64, m.conn, e, x, y, z, phase == PH_LOC ? ACC_SEARCH : ACC_INTERP);
+            r = xd.r;
+            s = xd.s;
+            t = xd.t;
+            slot = xd.slot;
+            dec = xd.dec;
+            ltie |= xd.tie;
+        }
+        if (dec < 0) {
+            err = E_INDET;
+            break;
+        }
+        if (dec == 0) { /* across the face */
+            e = ld32(reinterpret_cast<const int *>(m.adj), 4 * (uint32_t)e + slot);
+            if (e != -1) {
+                if (++hops >= m.walk_limit) {
+                    err = E_WALK;
+                    break;
+                }
+                continue; /* the next test of this walk */
+            }
+        }
+        /* ---- the walk ended: accepted at e, or left the mesh (e == -1) */
+        bool want = e >= 0;
+        double tl_i = 0.0;
+        if (phase == PH_LOC) {
+            p.elem = want ? e : -1;
+            /* a probe that left holds element -1 (and its left flag) */
+            const bool out = !want || is.mp[0].elem < 0 || is.mp[1].elem < 0 || is.mp[2].elem < 0 ||
+                             is.mp[3].elem < 0;
+            if (out) { /* exit: back to the step's start (integration.c:430-440) */
+                const double h6 = is.h * (1.0 / 6.0);
+                p.left = 1;
+                p.left_time = is.tc - is.h;
+                for (int c = 0; c < 3; c++) p.X[c] = p.X[c] - h6 * (double)S4[c];
+                stop = p.left_time;
+                break;
+            }
+            /* the next step; its first stage takes this element and these
+             * coordinates when the locate started where the first stage
+             * started and no test met the tie */
+            const bool reuse = !ltie && start == is.mp[0].elem;
+            if (!begin_step()) break;
+            if (!reuse) {
+                enter_probe(PH_F0);
+                if (err) break;
+                continue;
+            }
+            tl_i = tloc_of(PH_F0);
+            if ((tl_i > 1 + TINY) || (tl_i < 0 - TINY)) {
+                err = E_TLOC;
+                break;
+            }
+            is.mp[0].elem = e;
+            phase = PH_F0;
+        } else {
+            if (want) is.mp[phase].elem = e;
+            else { /* the probe left: zero velocity for this stage (velocity.c:588-592) */
+                is.mp[phase].elem = -1;
+                is.mp[phase].left = 1;
+            }
+            tl_i = tloc_of(phase);
+        }
+        /* ---- this stage's velocity and the next phase */
+        float k[3] = {0.f, 0.f, 0.f};
+        if (want) interp_nodes_f<MODE, DIM>(f, (float)tl_i, nd, r, s, t, k);
+        const float hf = (float)is.h, hh = 0.5f * hf;
+        if (phase == PH_F0) {
+            for (int c = 0; c < 3; c++) {
+                S[c] = k[c];
+                dq[c] = D[c] + hh * k[c];
+            }
+            enter_probe(PH_K2);
+        } else if (phase == PH_K2 || phase == PH_K3) {
+            const float w = phase == PH_K2 ? hh : hf;
+            for (int c = 0; c < 3; c++) {
+                S[c] = S[c] + 2.0f * k[c];
+                dq[c] = D[c] + w * k[c];
+            }
+            enter_probe(phase + 1);
+        } else { /* PH_K4: the step's increment, position and time */
+            const double h = is.h, h6 = h * (1.0 / 6.0);
+            for (int c = 0; c < 3; c++) {
+                S4[c] = S[c] + k[c];
+                p.X[c] = p.X[c] + h6 * (double)S4[c];
+            }
+            is.tc = is.tc + h;
+            if (!(fabs(h) <= hmin)) nsteps++;
+            taken++;
+            anchor();
+            phase = PH_LOC;
+            hops = 0;
+            ltie = 0;
+            start = p.elem;
+            e = p.elem; /* a located element: the walks above keep p.elem >= 0 */
+        }
+        if (err) break;
+    }
+    if (err) stop = b;
+    return ret;
+}
+
 /* Returns 1 when the interval is finished (end reached, point exited or an
  * error), with *stop the stop time; 0 when max_steps were taken first.
  *   RK4   ompVC/integration.c:317-462
@@ -1014,6 +1232,11 @@ __device__ int interval_run(const DevMesh &m, const DevField &f, PState &p, ISta
 #pragma clang fp contract(off)
     /* GEN = 0 compiles RK4 only (no multistep state in registers) */
     const int itype = GEN ? itype_rt : INT_RK4;
+    if constexpr (MODE != P_F64 && UNSTRUCT) {
+        /* GEN = 0 builds integrate neither strain rate nor exposure time */
+        if (itype == INT_RK4 && !extrap && (!GEN || (f.sr_lo == nullptr && f.cet_lo == nullptr)))
+            return rk4_sm<MODE, DIM>(m, f, p, is, a, b, dt, max_steps, err, nsteps, stop);
+    }
     if (!is.started) {
         if (p.left) {
             err = E_LEFT;
