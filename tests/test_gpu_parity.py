@@ -104,9 +104,11 @@ def test_advect_f64_bit_exact(pipe):
     assert steps >= 20 * (inside.sum() - ref["LeftDomain"].sum())
 
 
-@pytest.mark.parametrize("precision,tol", [("f32", 1e-5), ("bf16", 2e-3), ("f16", 2e-4)])
+@pytest.mark.parametrize("precision,tol", [("f32", 1e-7), ("bf16", 3e-4), ("f16", 5e-5)])
 def test_advect_reduced_precision_within_tolerance(pipe, precision, tol):
-    """Position error after one frame (20 RK4 steps); tol is absolute (domain 10)."""
+    """Position error after one frame (20 RK4 steps) against the oracle's FP64;
+    tol is absolute (domain 10), about three times the measured maxima
+    (FP32 2.2e-9, BF16 6.5e-5, FP16 8.6e-6 on these 3,000 seeds)."""
     got, ref, _ = _advect_both(pipe, precision)
     same = (got["LeftDomain"] == ref["LeftDomain"])
     assert same.mean() > 0.99
@@ -368,13 +370,17 @@ def test_particle_records_with_foreign_element_ids(pipe):
     np.testing.assert_array_equal(np.delete(back["ElementIndex"], 7), np.delete(recs["ElementIndex"], 7))
 
 
-@pytest.mark.parametrize("precision,pos_tol,ftle_p99", [("f32", 2e-5, 1e-3), ("f16", 0.2, 2e-2), ("bf16", 0.6, 0.15)])
-def test_double_gyre_ftle_reduced_precision(precision, pos_tol, ftle_p99):
+@pytest.mark.parametrize("precision,pos_tol,pos_p99,ftle_p99", [("f32", 2e-5, 2e-7, 1e-5), ("f16", 0.03, 6e-4, 1e-2),
+                                                                ("bf16", 0.25, 5e-3, 0.08)])
+def test_double_gyre_ftle_reduced_precision(precision, pos_tol, pos_p99, ftle_p99):
     """The 2-D double-gyre FTLE (256x128 nodes, 21 slices, RK4 dt=0.01 over
     T=10) on a 256x128 grid against the FP64 path (bit-identical to the
-    oracle): FP32 meets 1e-5 of the domain size (2) in final position and 1e-3
-    relative FTLE at the 99th percentile; the 16-bit slices meet the looser
-    bounds stated here (the flow is chaotic: separations grow up to ~100x over T)."""
+    oracle).  FP32 meets 1e-5 of the domain size (2) in final position and
+    1e-3 relative FTLE.  The bounds are about three times the measured values
+    (FP32: position max 1.2e-6, p99 3.6e-8, FTLE p99 8.0e-7; FP16: 8.6e-3,
+    2.2e-4, 4.3e-3; BF16: 0.081, 1.7e-3, 0.034): the flow is chaotic
+    (separations grow up to ~100x over T), so a velocity rounding of 2^-12
+    (FP16) or 2^-9 (BF16) ends as these position errors."""
     nx, ny = 256, 128
     slices = [host.double_gyre_slice(nx, ny, 0.5 * s) for s in range(21)]
     out = {}
@@ -393,7 +399,8 @@ def test_double_gyre_ftle_reduced_precision(precision, pos_tol, ftle_p99):
     np.testing.assert_array_equal(r["LeftDomain"], r64["LeftDomain"])
     both = r64["LeftDomain"] == 0
     assert both.sum() > 30000
-    assert np.linalg.norm(r["X"][both] - r64["X"][both], axis=1).max() <= pos_tol
+    d = np.linalg.norm(r["X"][both] - r64["X"][both], axis=1)
+    assert d.max() <= pos_tol and np.percentile(d, 99) <= pos_p99, (d.max(), np.percentile(d, 99))
     big = np.abs(w64) > 1e-3
     assert np.percentile(np.abs(w[big] - w64[big]) / np.abs(w64[big]), 99) <= ftle_p99
 
