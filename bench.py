@@ -935,6 +935,11 @@ def main():
                                           "exit_time_differs_in_stencil": int((ok & near_kji).sum()),
                                           "same_exit_max": float(rs.max()) if rs.size else None,
                                           "same_exit_above_tolerance": int((rs > 1e-3).sum())}
+        worst = np.argsort(rel_all)[-3:]  # the largest relative errors: FP64 and FP32 values beside the floor
+        result["ftle_rel_err_vs_fp64"]["largest"] = [
+            {"ftle_fp64": float(w64[q]), "ftle_fp32": float(w32[q]), "rel": float(rel_all[q])} for q in worst]
+        result["ftle_rel_err_vs_fp64"]["floor_value"] = float(1e-3 * np.median(np.abs(w64[ok])))
+        result["ftle_rel_err_vs_fp64"]["median_ftle"] = float(np.median(np.abs(w64[ok])))
         log("FP64 FTLE reference %.1f s" % (time.time() - t0))
     if args.tracer_periods > 0:
         result["tracer_periods"] = tracer_periods(P, P_ic, ctx, args.tracer_periods)

@@ -3,7 +3,7 @@
  *
  * Device layout (per context):
  *   X64  double4[num_nodes]  node coordinates (FP64, exact walk decisions)
- *   X32  float4[num_nodes]   node coordinates minus the mesh centre (FP32 path)
+ *   ER   float4[4 num_elems] FP32 affine frame of each element (FP32 walk tests)
  *   conn int4[num_elems], adj int4[num_elems]
  *   spatial hash: uniform cells over the padded node bounding box, each cell
  *     listing (ascending) the elements whose padded bounding box meets it
@@ -207,53 +207,82 @@ static int build_hash(fvc_ctx *c, const fvc_mesh_desc *md) {
     return FVC_OK;
 }
 
-/* FP32 guard band: bound on |natural coordinate error| of the FP32 path,
- * from the largest inverse-map row norm over elements and the coordinate
- * magnitude after centring. */
-static float guard_band(const fvc_mesh_desc *md, const double centre[3]) {
+/* FP32 affine frames of the elements (the layout fvc_device.h's ld_frame
+ * reads: rows (M[a][0..2], o[a]), then (lo[0..2], g)), from the FP64 nodes
+ * relative to the mesh centre.  Returns the largest row-sum norm of M (the
+ * 1-r-s-t row included) over elements with a finite frame. */
+static double build_frames(const fvc_mesh_desc *md, const double centre[3], std::vector<float4> &ER) {
     const int dims = md->dims;
-    double maxinv = 0.0, maxc = 0.0;
-    for (int64_t n = 0; n < md->num_nodes; n++)
-        for (int a = 0; a < dims; a++) maxc = std::max(maxc, fabs(md->coords[3 * n + a] - centre[a]));
+    const double u = ldexp(1.0, -24);
+    ER.assign((size_t)4 * md->num_elems, make_float4(0.f, 0.f, 0.f, 0.f));
+    double maxrow = 0.0;
     for (int64_t e = 0; e < md->num_elems; e++) {
         const int32_t *nd = md->conn + 4 * e;
-        double E[3][3] = {{0}};
-        for (int q = 1; q <= dims; q++)
-            for (int a = 0; a < dims; a++) E[a][q - 1] = md->coords[3 * nd[q] + a] - md->coords[3 * nd[0] + a];
-        double rowsum;
+        double P[4][3] = {{0}};
+        for (int q = 0; q <= dims; q++)
+            for (int a = 0; a < dims; a++) P[q][a] = md->coords[3 * nd[q] + a] - centre[a];
+        double E[3][3] = {{0}}, M[3][3] = {{0}}; /* E[a][q] = (node q+1 - node 0)[a] */
+        for (int q = 0; q < dims; q++)
+            for (int a = 0; a < dims; a++) E[a][q] = P[q + 1][a] - P[0][a];
+        double det;
         if (dims == 3) {
-            double det = E[0][0] * (E[1][1] * E[2][2] - E[1][2] * E[2][1]) -
-                         E[0][1] * (E[1][0] * E[2][2] - E[1][2] * E[2][0]) +
-                         E[0][2] * (E[1][0] * E[2][1] - E[1][1] * E[2][0]);
-            if (det == 0.0) return 1.0f;
-            double inv[3][3];
-            inv[0][0] = (E[1][1] * E[2][2] - E[1][2] * E[2][1]) / det;
-            inv[0][1] = (E[0][2] * E[2][1] - E[0][1] * E[2][2]) / det;
-            inv[0][2] = (E[0][1] * E[1][2] - E[0][2] * E[1][1]) / det;
-            inv[1][0] = (E[1][2] * E[2][0] - E[1][0] * E[2][2]) / det;
-            inv[1][1] = (E[0][0] * E[2][2] - E[0][2] * E[2][0]) / det;
-            inv[1][2] = (E[0][2] * E[1][0] - E[0][0] * E[1][2]) / det;
-            inv[2][0] = (E[1][0] * E[2][1] - E[1][1] * E[2][0]) / det;
-            inv[2][1] = (E[0][1] * E[2][0] - E[0][0] * E[2][1]) / det;
-            inv[2][2] = (E[0][0] * E[1][1] - E[0][1] * E[1][0]) / det;
-            rowsum = 0.0;
-            for (int r = 0; r < 3; r++) rowsum = std::max(rowsum, fabs(inv[r][0]) + fabs(inv[r][1]) + fabs(inv[r][2]));
-            double l0 = 0.0;
-            for (int a = 0; a < 3; a++) l0 += fabs(inv[0][a] + inv[1][a] + inv[2][a]);
-            rowsum = std::max(rowsum, l0);
+            det = E[0][0] * (E[1][1] * E[2][2] - E[1][2] * E[2][1]) - E[0][1] * (E[1][0] * E[2][2] - E[1][2] * E[2][0]) +
+                  E[0][2] * (E[1][0] * E[2][1] - E[1][1] * E[2][0]);
+            M[0][0] = (E[1][1] * E[2][2] - E[1][2] * E[2][1]) / det;
+            M[0][1] = (E[0][2] * E[2][1] - E[0][1] * E[2][2]) / det;
+            M[0][2] = (E[0][1] * E[1][2] - E[0][2] * E[1][1]) / det;
+            M[1][0] = (E[1][2] * E[2][0] - E[1][0] * E[2][2]) / det;
+            M[1][1] = (E[0][0] * E[2][2] - E[0][2] * E[2][0]) / det;
+            M[1][2] = (E[0][2] * E[1][0] - E[0][0] * E[1][2]) / det;
+            M[2][0] = (E[1][0] * E[2][1] - E[1][1] * E[2][0]) / det;
+            M[2][1] = (E[0][1] * E[2][0] - E[0][0] * E[2][1]) / det;
+            M[2][2] = (E[0][0] * E[1][1] - E[0][1] * E[1][0]) / det;
         } else {
-            double det = E[0][0] * E[1][1] - E[0][1] * E[1][0];
-            if (det == 0.0) return 1.0f;
-            rowsum = std::max(fabs(E[1][1]) + fabs(E[0][1]), fabs(E[1][0]) + fabs(E[0][0])) / fabs(det);
-            rowsum = std::max(rowsum, (fabs(E[1][1] - E[1][0]) + fabs(E[0][0] - E[0][1])) / fabs(det));
+            det = E[0][0] * E[1][1] - E[0][1] * E[1][0];
+            M[0][0] = E[1][1] / det;
+            M[0][1] = -E[0][1] / det;
+            M[1][0] = -E[1][0] / det;
+            M[1][1] = E[0][0] / det;
         }
-        maxinv = std::max(maxinv, rowsum);
+        double rown = 0.0, enorm = 0.0, l0row = 0.0;
+        for (int i = 0; i < dims; i++) {
+            double rs = 0.0, es = 0.0;
+            for (int j = 0; j < dims; j++) {
+                rs += fabs(M[i][j]);
+                es += fabs(E[i][j]);
+            }
+            rown = std::max(rown, rs);
+            enorm = std::max(enorm, es);
+        }
+        for (int j = 0; j < dims; j++) {
+            double cs = 0.0;
+            for (int i = 0; i < dims; i++) cs += M[i][j];
+            l0row += fabs(cs);
+        }
+        rown = std::max(rown, l0row);
+        float o[3] = {0.f, 0.f, 0.f};
+        for (int a = 0; a < dims; a++) o[a] = (float)P[0][a];
+        double lo[3] = {0.0, 0.0, 0.0};
+        for (int i = 0; i < dims; i++)
+            for (int a = 0; a < dims; a++) lo[i] += M[i][a] * ((double)o[a] - P[0][a]);
+        const double g = 4.0 * (18.0 * rown * enorm + 12.0) * u;
+        bool ok = det != 0.0 && std::isfinite(g) && g < 0.25;
+        for (int i = 0; i < dims && ok; i++) {
+            ok = std::isfinite(lo[i]) && fabs(lo[i]) < 1e30;
+            for (int j = 0; j < dims; j++) ok = ok && fabs(M[i][j]) < 1e30;
+        }
+        float4 *R = ER.data() + 4 * e;
+        if (!ok) { /* every test on this element is decided in FP64 */
+            R[3].w = INFINITY;
+            continue;
+        }
+        R[0] = make_float4((float)M[0][0], (float)M[0][1], (float)M[0][2], o[0]);
+        R[1] = make_float4((float)M[1][0], (float)M[1][1], (float)M[1][2], o[1]);
+        R[2] = make_float4((float)M[2][0], (float)M[2][1], (float)M[2][2], o[2]);
+        R[3] = make_float4((float)lo[0], (float)lo[1], (float)lo[2], (float)g);
+        maxrow = std::max(maxrow, rown);
     }
-    /* node rounding (2 ulp-halves per edge) and query rounding, FP32 solve error;
-     * safety factor 8 over the first-order bound */
-    double eps = ldexp(1.0, -24);
-    double g = 8.0 * maxinv * maxc * eps * (2.0 * 4.0 + 8.0);
-    return (float)std::min(0.25, std::max(g, 1e-6));
+    return maxrow;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -332,9 +361,9 @@ extern "C" int fvc_create(const fvc_mesh_desc *md, const fvc_params *pp, fvc_ctx
         c->nsr = (int64_t)(m.cres[0] - 1) * (m.cres[1] - 1) * ((md->dims == 2 || m.cres[2] == 1) ? 1 : m.cres[2] - 1);
     } else {
         /* device gathers address node arrays (32 B per node at most) and
-         * element arrays (16 B per element) with 32-bit byte offsets */
+         * element arrays (64 B per element at most) with 32-bit byte offsets */
         if (md->num_nodes <= 0 || md->num_elems <= 0 || !md->coords || !md->conn || !md->adj ||
-            md->num_elems > (INT64_C(1) << 32) / 16 || md->num_nodes > (INT64_C(1) << 32) / 32) {
+            md->num_elems > (INT64_C(1) << 32) / 64 || md->num_nodes > (INT64_C(1) << 32) / 32) {
             fvc_destroy(c);
             return set_err(nullptr, FVC_E_INVALID, "");
         }
@@ -357,13 +386,12 @@ extern "C" int fvc_create(const fvc_mesh_desc *md, const fvc_params *pp, fvc_ctx
             }
         for (int a = 0; a < 3; a++) m.centre[a] = 0.5 * (lo[a] + hi[a]);
         std::vector<double4> x64(md->num_nodes);
-        std::vector<float4> x32(md->num_nodes);
         for (int64_t n = 0; n < md->num_nodes; n++) {
             const double *p = md->coords + 3 * n;
             x64[n] = make_double4(p[0], p[1], p[2], 0.0);
-            x32[n] = make_float4((float)(p[0] - m.centre[0]), (float)(p[1] - m.centre[1]),
-                                 (float)(p[2] - m.centre[2]), 0.0f);
         }
+        std::vector<float4> er;
+        const double maxrow = build_frames(md, m.centre, er);
         std::vector<int4> cn(md->num_elems), ad(md->num_elems);
         for (int64_t e = 0; e < md->num_elems; e++) {
             const int32_t *q = md->conn + 4 * e, *a = md->adj + 4 * e;
@@ -371,23 +399,23 @@ extern "C" int fvc_create(const fvc_mesh_desc *md, const fvc_params *pp, fvc_ctx
             cn[e] = make_int4(q[0], q[1], q[2], nn == 4 ? q[3] : q[0]);
             ad[e] = make_int4(a[0], a[1], a[2], nn == 4 ? a[3] : -1);
         }
-        if ((rc = dev_alloc(c, &c->X64, md->num_nodes)) || (rc = dev_alloc(c, &c->X32, md->num_nodes)) ||
+        if ((rc = dev_alloc(c, &c->X64, md->num_nodes)) || (rc = dev_alloc(c, &c->ER, 4 * md->num_elems)) ||
             (rc = dev_alloc(c, &c->conn, md->num_elems)) || (rc = dev_alloc(c, &c->adj, md->num_elems))) {
             fvc_destroy(c);
             return rc;
         }
         if (hipMemcpy(c->X64, x64.data(), sizeof(double4) * md->num_nodes, hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(c->X32, x32.data(), sizeof(float4) * md->num_nodes, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(c->ER, er.data(), sizeof(float4) * 4 * md->num_elems, hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(c->conn, cn.data(), sizeof(int4) * md->num_elems, hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(c->adj, ad.data(), sizeof(int4) * md->num_elems, hipMemcpyHostToDevice) != hipSuccess) {
             fvc_destroy(c);
             return FVC_E_HIP;
         }
         m.X64 = c->X64;
-        m.X32 = c->X32;
+        m.ER = c->ER;
         m.conn = c->conn;
         m.adj = c->adj;
-        m.guard = guard_band(md, m.centre);
+        m.gq = (float)(12.0 * ldexp(1.0, -24) * maxrow);
         if ((rc = build_hash(c, md))) {
             fvc_destroy(c);
             return rc;
@@ -422,7 +450,7 @@ extern "C" void fvc_destroy(fvc_ctx *c) {
     if (c->cet_lo) hipFree(c->cet_lo);
     if (c->cet_enc) hipFree(c->cet_enc);
     hipFree(c->X64);
-    hipFree(c->X32);
+    hipFree(c->ER);
     hipFree(c->conn);
     hipFree(c->adj);
     hipFree(c->cell_start);

@@ -10,12 +10,13 @@
  * Precision modes (template MODE):
  *   P_F64  : FP64 everything, expressions in the reference's order with FP
  *            contraction off, so results are bit-identical to ompVC's.
- *   P_F32  : FP32 node coordinates (relative to the mesh centre) and FP32
- *            velocity; FP64 position/time state.  Every walk decision (accept,
- *            exit face) is taken in FP32 only when it is farther than a
- *            guard band from its threshold; otherwise it is recomputed with
- *            the FP64 formula, so the element sequence of a walk equals the
- *            FP64 walk's for the same query point.
+ *   P_F32  : FP32 element frames (an inverse edge matrix and origin per
+ *            element) and FP32 velocity; FP64 position/time state.  Every
+ *            walk decision (accept, exit face) is taken in FP32 only when it
+ *            is farther than the element's guard band from its threshold;
+ *            otherwise it is recomputed with the FP64 formula, so the
+ *            element sequence of a walk equals the FP64 walk's for the same
+ *            query point.
  *   P_BF16/P_F16 : as P_F32 with velocity stored in 16 bits times a
  *            per-slice scale.
  */
@@ -54,12 +55,12 @@ struct DevMesh {
     int cres[3];
     /* unstructured */
     const double4 *__restrict__ X64;
-    const float4 *__restrict__ X32; /* node - centre, FP32 */
+    const float4 *__restrict__ ER; /* FP32 affine frame of each element (4 x float4) */
     const int4 *__restrict__ conn;
     const int4 *__restrict__ adj;
     double centre[3];
     int64_t nn, ne;
-    float guard;    /* FP32 decision guard band on natural coordinates */
+    float gq;       /* FP32 guard coefficient of the query offset: 12 2^-24 max |M| */
     int walk_limit; /* hop limit */
     /* spatial hash */
     const int *__restrict__ cell_start;
@@ -179,49 +180,109 @@ __device__ __forceinline__ int decide_exact(double r, double s, double t, int ac
 }
 
 /* ------------------------------------------------------------------------ */
-/* FP32 natural coordinates relative to node 0 (Cramer's rule)               */
+/* FP32 natural coordinates from per-element affine frames                  */
 /* ------------------------------------------------------------------------ */
-__device__ __forceinline__ void nat3_fast(const DevMesh &m, int4 nd, double x, double y, double z, float &r, float &s,
-                                          float &t) {
-    float4 p0 = ld32(m.X32, nd.x), p1 = ld32(m.X32, nd.y), p2 = ld32(m.X32, nd.z), p3 = ld32(m.X32, nd.w);
-    float qx = (float)((x - m.centre[0]) - (double)p0.x);
-    float qy = (float)((y - m.centre[1]) - (double)p0.y);
-    float qz = (float)((z - m.centre[2]) - (double)p0.z);
-    float ax = p1.x - p0.x, ay = p1.y - p0.y, az = p1.z - p0.z;
-    float bx = p2.x - p0.x, by = p2.y - p0.y, bz = p2.z - p0.z;
-    float cx = p3.x - p0.x, cy = p3.y - p0.y, cz = p3.z - p0.z;
-    float n1x = by * cz - bz * cy, n1y = bz * cx - bx * cz, n1z = bx * cy - by * cx; /* b x c */
-    float n2x = cy * az - cz * ay, n2y = cz * ax - cx * az, n2z = cx * ay - cy * ax; /* c x a */
-    float n3x = ay * bz - az * by, n3y = az * bx - ax * bz, n3z = ax * by - ay * bx; /* a x b */
-    float inv = 1.0f / (ax * n1x + ay * n1y + az * n1z);
-    r = (qx * n1x + qy * n1y + qz * n1z) * inv;
-    s = (qx * n2x + qy * n2y + qz * n2z) * inv;
-    t = (qx * n3x + qy * n3y + qz * n3z) * inv;
-}
+/* Element e's frame is four float4 at ER[4e .. 4e+3], built at fvc_create
+ * from the FP64 nodes (relative to the mesh centre):
+ *   ER[4e+a] = (M[a][0], M[a][1], M[a][2], o[a])   a = 0, 1, 2
+ *   ER[4e+3] = (lo[0], lo[1], lo[2], g)
+ * o = node 0 rounded to FP32 (an FP32 origin, exact in FP32), M = the inverse
+ * of the edge matrix [n1-n0, n2-n0, n3-n0] rounded to FP32, lo the natural
+ * coordinates of o itself (o - node0 is the rounding of node 0), and g the
+ * element's guard coefficient 4 (18 cond + 12) 2^-24, cond = |M| |E| in the
+ * row-sum norm with the 1-r-s-t row included.  2-D frames use the (x, y)
+ * parts of rows 0, 1 and 3.  Natural coordinates are then
+ *   (r, s, t) = lo + M (q - o),
+ * q - o formed as (A - o) + d from the anchor A and offset d of the query
+ * (or rounded once from the FP64 difference), three FMAs per coordinate.
+ * Per coordinate, the rounding of M, of (A - o), of the sum with d and of
+ * the three FMAs err by at most 6 2^-24 cond L + 2^-24 |M| |d| to first
+ * order, L = max(1, |r|, |s|, |t|, |1-r-s-t|); 1-r-s-t sums three such
+ * errors and rounds three times more: (18 cond + 12) 2^-24 L +
+ * 3 2^-24 |M| |d| in all, and the band is four times that.  Node rounding
+ * enters only through lo, so the band scales with the element's own shape,
+ * not with the mesh's extent.  An element whose edge matrix is singular or
+ * whose frame is not finite gets M = 0, lo = 0, g = inf: every test on it
+ * is decided in FP64. */
+struct Frame {
+    float4 a0, a1, a2, a3;
+};
 
-__device__ __forceinline__ void nat2_fast(const DevMesh &m, int4 nd, double x, double y, float &r, float &s) {
-    float4 p0 = ld32(m.X32, nd.x), p1 = ld32(m.X32, nd.y), p2 = ld32(m.X32, nd.z);
-    float qx = (float)((x - m.centre[0]) - (double)p0.x);
-    float qy = (float)((y - m.centre[1]) - (double)p0.y);
-    float ax = p1.x - p0.x, ay = p1.y - p0.y, bx = p2.x - p0.x, by = p2.y - p0.y;
-    float inv = 1.0f / (ax * by - ay * bx);
-    r = (qx * by - qy * bx) * inv;
-    s = (ax * qy - ay * qx) * inv;
-}
-
-/* FP32 decision: 1 accept, 0 move (slot set), 2 undecided (escalate) */
 template <int DIM>
-__device__ __forceinline__ int decide_fast(float r, float s, float t, float g, int &slot) {
-    float l0 = DIM == 3 ? 1.0f - r - s - t : 1.0f - r - s;
+__device__ __forceinline__ Frame ld_frame(const DevMesh &m, int e) {
+    Frame F;
+    const uint32_t b = 4u * (uint32_t)e;
+    F.a0 = ld32(m.ER, b);
+    F.a1 = ld32(m.ER, b + 1);
+    if (DIM == 3) F.a2 = ld32(m.ER, b + 2);
+    else F.a2 = make_float4(0.f, 0.f, 0.f, 0.f);
+    F.a3 = ld32(m.ER, b + 3);
+    return F;
+}
+
+/* (r, s, t) = lo + M q with q = query - o, each an explicit FMA chain */
+template <int DIM>
+__device__ __forceinline__ void frame_nat(const Frame &F, float qx, float qy, float qz, float &r, float &s, float &t) {
+    if (DIM == 3) {
+        r = __builtin_fmaf(F.a0.z, qz, __builtin_fmaf(F.a0.y, qy, __builtin_fmaf(F.a0.x, qx, F.a3.x)));
+        s = __builtin_fmaf(F.a1.z, qz, __builtin_fmaf(F.a1.y, qy, __builtin_fmaf(F.a1.x, qx, F.a3.y)));
+        t = __builtin_fmaf(F.a2.z, qz, __builtin_fmaf(F.a2.y, qy, __builtin_fmaf(F.a2.x, qx, F.a3.z)));
+    } else {
+        r = __builtin_fmaf(F.a0.y, qy, __builtin_fmaf(F.a0.x, qx, F.a3.x));
+        s = __builtin_fmaf(F.a1.y, qy, __builtin_fmaf(F.a1.x, qx, F.a3.y));
+        t = 0.0f;
+    }
+}
+
+/* query given in FP64 (x, y, z): q - o rounded once from the FP64 difference */
+template <int DIM>
+__device__ __forceinline__ void nat_fast(const DevMesh &m, int e, double x, double y, double z, float &r, float &s,
+                                         float &t, float &ge) {
+#pragma clang fp contract(off)
+    const Frame F = ld_frame<DIM>(m, e);
+    const float qx = (float)((x - m.centre[0]) - (double)F.a0.w);
+    const float qy = (float)((y - m.centre[1]) - (double)F.a1.w);
+    const float qz = DIM == 3 ? (float)((z - m.centre[2]) - (double)F.a2.w) : 0.0f;
+    frame_nat<DIM>(F, qx, qy, qz, r, s, t);
+    ge = F.a3.w;
+}
+
+/* query anchored in FP32: the point (centre + A) + d, q - o = (A - o) + d */
+template <int DIM>
+__device__ __forceinline__ void nat_fast_c(const DevMesh &m, int e, const float A[3], const float d[3], float &r,
+                                           float &s, float &t, float &ge) {
+#pragma clang fp contract(off)
+    const Frame F = ld_frame<DIM>(m, e);
+    const float qx = (A[0] - F.a0.w) + d[0];
+    const float qy = (A[1] - F.a1.w) + d[1];
+    const float qz = DIM == 3 ? (A[2] - F.a2.w) + d[2] : 0.0f;
+    frame_nat<DIM>(F, qx, qy, qz, r, s, t);
+    ge = F.a3.w;
+}
+
+/* guard term of a query offset d (its rounding in (A - o) + d) */
+__device__ __forceinline__ float guard_offset(const DevMesh &m, const float d[3]) {
+    return m.gq * fmaxf(fabsf(d[0]), fmaxf(fabsf(d[1]), fabsf(d[2])));
+}
+
+/* FP32 decision: 1 accept, 0 move (slot set), 2 undecided (escalate); the
+ * band is ge L + gq around each threshold */
+template <int DIM>
+__device__ __forceinline__ int decide_fast(float r, float s, float t, float ge, float gq, int &slot) {
+#pragma clang fp contract(off)
+    float l0 = DIM == 3 ? ((1.0f - r) - s) - t : (1.0f - r) - s;
     float v[4] = {r, s, DIM == 3 ? t : l0, l0};
     const int n = DIM == 3 ? 4 : 3;
     int a = 0;
-    float d = v[0];
-    for (int q = 1; q < n; q++)
+    float d = v[0], L = fabsf(v[0]);
+    for (int q = 1; q < n; q++) {
+        L = fmaxf(L, fabsf(v[q]));
         if (v[q] < d) {
             d = v[q];
             a = q;
         }
+    }
+    const float g = __builtin_fmaf(ge, fmaxf(L, 1.0f), gq);
     if (d > g) return 1;
     if (d >= -g) return 2;
     float second = 3.0e38f;
@@ -251,11 +312,9 @@ __device__ __forceinline__ int walk(const DevMesh &m, double x, double y, double
             }
             dec = decide_exact<DIM>(r, s, t, acc, slot, tie);
         } else {
-            int4 nd = ld32(m.conn, e);
-            float fr, fs, ft = 0.0f;
-            if (DIM == 3) nat3_fast(m, nd, x, y, z, fr, fs, ft);
-            else nat2_fast(m, nd, x, y, fr, fs);
-            dec = decide_fast<DIM>(fr, fs, ft, m.guard, slot);
+            float fr, fs, ft, ge;
+            nat_fast<DIM>(m, e, x, y, z, fr, fs, ft, ge);
+            dec = decide_fast<DIM>(fr, fs, ft, ge, 0.0f, slot);
             if (dec == 2) {
                 if (DIM == 3) nat3_exact(m, e, x, y, z, r, s, t);
                 else {
@@ -292,34 +351,6 @@ __device__ __forceinline__ int walk(const DevMesh &m, double x, double y, double
  * FP32 decision forms q - node0 as (A - node0) + d, and an undecided one is
  * recomputed with the FP64 formula at (centre + A) + d, so the element
  * returned is the one the reference's walk returns for that point. */
-__device__ __forceinline__ void nat3_fast_c(const DevMesh &m, int4 nd, const float A[3], const float d[3], float &r,
-                                            float &s, float &t) {
-    float4 p0 = ld32(m.X32, nd.x), p1 = ld32(m.X32, nd.y), p2 = ld32(m.X32, nd.z), p3 = ld32(m.X32, nd.w);
-    float qx = (A[0] - p0.x) + d[0], qy = (A[1] - p0.y) + d[1], qz = (A[2] - p0.z) + d[2];
-    float ax = p1.x - p0.x, ay = p1.y - p0.y, az = p1.z - p0.z;
-    float bx = p2.x - p0.x, by = p2.y - p0.y, bz = p2.z - p0.z;
-    float cx = p3.x - p0.x, cy = p3.y - p0.y, cz = p3.z - p0.z;
-    float n1x = by * cz - bz * cy, n1y = bz * cx - bx * cz, n1z = bx * cy - by * cx; /* b x c */
-    float n2x = cy * az - cz * ay, n2y = cz * ax - cx * az, n2z = cx * ay - cy * ax; /* c x a */
-    float n3x = ay * bz - az * by, n3y = az * bx - ax * bz, n3z = ax * by - ay * bx; /* a x b */
-    /* hardware reciprocal (1 ulp): the guard band bounds the node-coordinate
-     * rounding, some 100 times larger than this */
-    float inv = __builtin_amdgcn_rcpf(ax * n1x + ay * n1y + az * n1z);
-    r = (qx * n1x + qy * n1y + qz * n1z) * inv;
-    s = (qx * n2x + qy * n2y + qz * n2z) * inv;
-    t = (qx * n3x + qy * n3y + qz * n3z) * inv;
-}
-
-__device__ __forceinline__ void nat2_fast_c(const DevMesh &m, int4 nd, const float A[3], const float d[3], float &r,
-                                            float &s) {
-    float4 p0 = ld32(m.X32, nd.x), p1 = ld32(m.X32, nd.y), p2 = ld32(m.X32, nd.z);
-    float qx = (A[0] - p0.x) + d[0], qy = (A[1] - p0.y) + d[1];
-    float ax = p1.x - p0.x, ay = p1.y - p0.y, bx = p2.x - p0.x, by = p2.y - p0.y;
-    float inv = __builtin_amdgcn_rcpf(ax * by - ay * bx);
-    r = (qx * by - qy * bx) * inv;
-    s = (ax * qy - ay * qx) * inv;
-}
-
 /* the FP64 decision at the double point (x, y, z), for the few tests that
  * fall inside the guard band (node coordinates and connectivity by pointer,
  * so the call can stay out of line without the mesh descriptor); tie reports
@@ -359,14 +390,13 @@ __device__ FVC_ESC_ATTR ExactDec decide_exact_at(const double4 *__restrict__ X64
 template <int DIM>
 __device__ __forceinline__ int walk_c(const DevMesh &m, const float A[3], const float d[3], int e, int acc, float &r,
                                       float &s, float &t, int4 &nodes, int &err) {
+    const float gd = guard_offset(m, d);
     for (int hop = 0; hop < m.walk_limit; hop++) {
         int slot = -1;
-        const int4 nd = ld32(m.conn, e);
-        nodes = nd;
-        float fr, fs, ft = 0.0f;
-        if (DIM == 3) nat3_fast_c(m, nd, A, d, fr, fs, ft);
-        else nat2_fast_c(m, nd, A, d, fr, fs);
-        int dec = decide_fast<DIM>(fr, fs, ft, m.guard, slot);
+        nodes = ld32(m.conn, e);
+        float fr, fs, ft, ge;
+        nat_fast_c<DIM>(m, e, A, d, fr, fs, ft, ge);
+        int dec = decide_fast<DIM>(fr, fs, ft, ge, gd, slot);
         if (dec == 2) {
             const ExactDec x = decide_exact_at<DIM>(m.X64, m.conn, e, (m.centre[0] + (double)A[0]) + (double)d[0],
                                                     (m.centre[1] + (double)A[1]) + (double)d[1],
@@ -1017,7 +1047,7 @@ __device__ inline double strain_rate(const DevMesh &m, const DevField &f, double
  * next phase.  Walks of different lengths then overlap instead of making the
  * whole wavefront repeat the longest one.  Every lane performs the same
  * floating-point operations, in the same order, as a straight-line walk would.
- * Query points are anchored FP32 offsets (A, d), see nat3_fast_c; an
+ * Query points are anchored FP32 offsets (A, d), see nat_fast_c; an
  * undecided test of the first-stage or locate walk is decided in FP64 at the
  * double position X, a probe's at (centre + A) + d. */
 enum { PH_F0 = 0, PH_K2 = 1, PH_K3 = 2, PH_K4 = 3, PH_LOC = 4, PH_DONE = 5 };
@@ -1050,6 +1080,7 @@ __device__ int rk4_sm(const DevMesh &m, const DevField &f, PState &p, IState &is
     int taken = 0, ret = 1;
     int phase = PH_F0, e = 0, hops = 0, start = 0;
     float A[3], D[3], dq[3], S[3] = {0.f, 0.f, 0.f};
+    float gD = 0.f, gdq = 0.f; /* guard terms of the offsets D and dq */
     float S4[3] = {0.f, 0.f, 0.f}; /* f0 + 2 k2 + 2 k3 + k4 of the step (its increment is h/6 S4) */
     /* anchors of X: A = (float)(X - centre), D its FP32 residual */
     auto anchor = [&]() {
@@ -1058,6 +1089,7 @@ __device__ int rk4_sm(const DevMesh &m, const DevField &f, PState &p, IState &is
             A[c] = (float)xc;
             D[c] = (float)(xc - (double)A[c]);
         }
+        gD = guard_offset(m, D);
     };
     /* time location of a phase: tc, tc + h/2, tc + h */
     auto tloc_of = [&](int ph) {
@@ -1100,9 +1132,9 @@ __device__ int rk4_sm(const DevMesh &m, const DevField &f, PState &p, IState &is
         nd = ld32(m.conn, e);
         const bool at_x = phase == PH_F0 || phase == PH_LOC;
         int slot = -1, dec;
-        if (DIM == 3) nat3_fast_c(m, nd, A, at_x ? D : dq, r, s, t);
-        else nat2_fast_c(m, nd, A, at_x ? D : dq, r, s);
-        dec = decide_fast<DIM>(r, s, t, m.guard, slot);
+        float ge;
+        nat_fast_c<DIM>(m, e, A, at_x ? D : dq, r, s, t, ge);
+        dec = decide_fast<DIM>(r, s, t, ge, at_x ? gD : gdq, slot);
         if (dec == 2) {
             double x, y, z;
             if (at_x) {
@@ -1187,6 +1219,7 @@ __device__ int rk4_sm(const DevMesh &m, const DevField &f, PState &p, IState &is
                 S[c] = k[c];
                 dq[c] = D[c] + hh * k[c];
             }
+            gdq = guard_offset(m, dq);
             enter_probe(PH_K2);
         } else if (phase == PH_K2 || phase == PH_K3) {
             const float w = phase == PH_K2 ? hh : hf;
@@ -1194,6 +1227,7 @@ __device__ int rk4_sm(const DevMesh &m, const DevField &f, PState &p, IState &is
                 S[c] = S[c] + 2.0f * k[c];
                 dq[c] = D[c] + w * k[c];
             }
+            gdq = guard_offset(m, dq);
             enter_probe(phase + 1);
         } else { /* PH_K4: the step's increment, position and time */
             const double h = is.h, h6 = h * (1.0 / 6.0);

@@ -20,7 +20,7 @@ struct fvc_ctx {
     fvc::DevMesh mesh;
     /* device allocations */
     double4 *X64;
-    float4 *X32;
+    float4 *ER; /* element frames, 4 per element */
     int4 *conn, *adj;
     int *cell_start, *cell_elems;
     int64_t ncells, nentries;
