@@ -582,27 +582,99 @@ template <int MODE, int DIM>
 __device__ __forceinline__ void interp_unstruct(const DevMesh &m, const DevField &f, double tloc, int e, double r,
                                                 double s, double t, double out[3]);
 
-/* reduced-precision modes: time-lerp of the four nodes' velocities and the
- * reference's fabs(r), fabs(s), fabs(t) weights, in FP32 */
+/* reduced-precision modes: velocities of an element's nodes in the two
+ * bound slices as loaded (FP32 values, or 16-bit words before scaling), so
+ * the gather can be issued before the containment test that decides whether
+ * it is used */
+struct F3 {
+    float x, y, z;
+};
+
+template <int MODE, int DIM>
+struct NodeVel {
+    static constexpr int NN = DIM == 3 ? 4 : 3;
+    typename std::conditional<MODE == P_F32, F3, uint2>::type lo[NN], hi[NN];
+};
+
+template <int MODE>
+__device__ __forceinline__ void ld_vel_word(const void *base, uint32_t n, F3 &w) {
+    const float4 v = ld32(reinterpret_cast<const float4 *>(base), n);
+    w.x = v.x;
+    w.y = v.y;
+    w.z = v.z;
+}
+
+template <int MODE>
+__device__ __forceinline__ void ld_vel_word(const void *base, uint32_t n, uint2 &w) {
+    w = ld32(reinterpret_cast<const uint2 *>(base), n);
+}
+
+template <int MODE, int DIM>
+__device__ __forceinline__ void gather_vel(const DevField &f, int4 nd, NodeVel<MODE, DIM> &g) {
+    const uint32_t ids[4] = {(uint32_t)nd.x, (uint32_t)nd.y, (uint32_t)nd.z, (uint32_t)nd.w};
+    for (int q = 0; q < NodeVel<MODE, DIM>::NN; q++) {
+        ld_vel_word<MODE>(f.lo, ids[q], g.lo[q]);
+        ld_vel_word<MODE>(f.hi, ids[q], g.hi[q]);
+    }
+}
+
+template <int MODE>
+__device__ __forceinline__ void node_comp(const F3 &w, float v[3]) {
+    v[0] = w.x;
+    v[1] = w.y;
+    v[2] = w.z;
+}
+
+template <int MODE>
+__device__ __forceinline__ void node_comp(const uint2 &w, float v[3]) {
+    const unsigned short h0 = (unsigned short)(w.x & 0xffffu), h1 = (unsigned short)(w.x >> 16);
+    const unsigned short h2 = (unsigned short)(w.y & 0xffffu);
+    if constexpr (MODE == P_BF16) {
+        v[0] = __uint_as_float((unsigned)h0 << 16);
+        v[1] = __uint_as_float((unsigned)h1 << 16);
+        v[2] = __uint_as_float((unsigned)h2 << 16);
+    } else {
+        v[0] = __half2float(__ushort_as_half(h0));
+        v[1] = __half2float(__ushort_as_half(h1));
+        v[2] = __half2float(__ushort_as_half(h2));
+    }
+}
+
+/* the reference's weights fabs(r), fabs(s), fabs(t) and 1 minus their sum
+ * (velocity.c:574-581, n0 + (n1-n0)|r| + (n2-n0)|s| + (n3-n0)|t|) applied to
+ * each slice, then the time weights (1 - tl) and tl, which carry the 16-bit
+ * slices' scales; FP32 throughout */
+template <int MODE, int DIM>
+__device__ __forceinline__ void interp_gathered(const DevField &f, const NodeVel<MODE, DIM> &g, float tl, float r,
+                                                float s, float t, float out[3]) {
+#pragma clang fp contract(off)
+    constexpr int NN = NodeVel<MODE, DIM>::NN;
+    float w[4];
+    w[1] = fabsf(r);
+    w[2] = fabsf(s);
+    w[3] = DIM == 3 ? fabsf(t) : 0.0f;
+    w[0] = DIM == 3 ? ((1.0f - w[1]) - w[2]) - w[3] : (1.0f - w[1]) - w[2];
+    float lo[3], hi[3];
+    for (int q = 0; q < NN; q++) {
+        float a[3], b[3];
+        node_comp<MODE>(g.lo[q], a);
+        node_comp<MODE>(g.hi[q], b);
+        for (int c = 0; c < 3; c++) {
+            lo[c] = q == 0 ? w[0] * a[c] : __builtin_fmaf(w[q], a[c], lo[c]);
+            hi[c] = q == 0 ? w[0] * b[c] : __builtin_fmaf(w[q], b[c], hi[c]);
+        }
+    }
+    const float tlo = MODE == P_F32 ? 1.0f - tl : (1.0f - tl) * f.scale_lo;
+    const float thi = MODE == P_F32 ? tl : tl * f.scale_hi;
+    for (int c = 0; c < 3; c++) out[c] = (DIM == 2 && c == 2) ? 0.0f : __builtin_fmaf(thi, hi[c], tlo * lo[c]);
+}
+
 template <int MODE, int DIM>
 __device__ __forceinline__ void interp_nodes_f(const DevField &f, float tl, int4 nd, float r, float s, float t,
                                                float out[3]) {
-    using VT = typename VelT<MODE>::T;
-    const int nn = DIM == 3 ? 4 : 3;
-    int ids[4] = {nd.x, nd.y, nd.z, nd.w};
-    float n[4][3];
-    for (int q = 0; q < nn; q++) {
-        VT a[3], b[3];
-        load_vel_node<MODE>(f.lo, f.scale_lo, ids[q], a);
-        load_vel_node<MODE>(f.hi, f.scale_hi, ids[q], b);
-        for (int c = 0; c < 3; c++) n[q][c] = a[c] + tl * (b[c] - a[c]);
-    }
-    float ar = fabsf(r), as = fabsf(s), at = fabsf(t);
-    for (int c = 0; c < 3; c++) {
-        float v = n[0][c] + (n[1][c] - n[0][c]) * ar + (n[2][c] - n[0][c]) * as;
-        if (DIM == 3) v += (n[3][c] - n[0][c]) * at;
-        out[c] = (DIM == 2 && c == 2) ? 0.0f : v;
-    }
+    NodeVel<MODE, DIM> g;
+    gather_vel<MODE, DIM>(f, nd, g);
+    interp_gathered<MODE, DIM>(f, g, tl, r, s, t, out);
 }
 
 template <int MODE, int DIM>
@@ -1079,9 +1151,10 @@ __device__ int rk4_sm(const DevMesh &m, const DevField &f, PState &p, IState &is
     const double inv_span = 1.0 / (f.tmax - f.tmin);
     int taken = 0, ret = 1;
     int phase = PH_F0, e = 0, hops = 0, start = 0;
+    /* S: f0, then f0 + 2 k2, f0 + 2 k2 + 2 k3, and from the k4 stage to the
+     * locate's end the step's f0 + 2 k2 + 2 k3 + k4 (increment h/6 S) */
     float A[3], D[3], dq[3], S[3] = {0.f, 0.f, 0.f};
     float gD = 0.f, gdq = 0.f; /* guard terms of the offsets D and dq */
-    float S4[3] = {0.f, 0.f, 0.f}; /* f0 + 2 k2 + 2 k3 + k4 of the step (its increment is h/6 S4) */
     /* anchors of X: A = (float)(X - centre), D its FP32 residual */
     auto anchor = [&]() {
         for (int c = 0; c < 3; c++) {
@@ -1121,15 +1194,16 @@ __device__ int rk4_sm(const DevMesh &m, const DevField &f, PState &p, IState &is
         return true;
     };
     anchor();
-    int4 nd = make_int4(0, 0, 0, 0);
     float r = 0.f, s = 0.f, t = 0.f;
     int ltie = 0; /* a locate-walk test hit d == -TINY */
     if (!begin_step()) phase = PH_DONE;
     else enter_probe(PH_F0);
     if (err) phase = PH_DONE;
     while (phase != PH_DONE) {
-        /* ---- one containment test of this lane's walk */
-        nd = ld32(m.conn, e);
+        /* ---- one containment test of this lane's walk (the element's
+         * connectivity is loaded beside its frame, for the velocity gather
+         * when the walk ends here) */
+        const int4 nd = ld32(m.conn, e);
         const bool at_x = phase == PH_F0 || phase == PH_LOC;
         int slot = -1, dec;
         float ge;
@@ -1181,7 +1255,7 @@ __device__ int rk4_sm(const DevMesh &m, const DevField &f, PState &p, IState &is
                 const double h6 = is.h * (1.0 / 6.0);
                 p.left = 1;
                 p.left_time = is.tc - is.h;
-                for (int c = 0; c < 3; c++) p.X[c] = p.X[c] - h6 * (double)S4[c];
+                for (int c = 0; c < 3; c++) p.X[c] = p.X[c] - h6 * (double)S[c];
                 stop = p.left_time;
                 break;
             }
@@ -1232,8 +1306,8 @@ __device__ int rk4_sm(const DevMesh &m, const DevField &f, PState &p, IState &is
         } else { /* PH_K4: the step's increment, position and time */
             const double h = is.h, h6 = h * (1.0 / 6.0);
             for (int c = 0; c < 3; c++) {
-                S4[c] = S[c] + k[c];
-                p.X[c] = p.X[c] + h6 * (double)S4[c];
+                S[c] = S[c] + k[c];
+                p.X[c] = p.X[c] + h6 * (double)S[c];
             }
             is.tc = is.tc + h;
             if (!(fabs(h) <= hmin)) nsteps++;
