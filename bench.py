@@ -26,7 +26,7 @@ import time
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(HERE, "flowvc-1_amd"))
 
-BYTES_PER_STEP_F32 = 704  # 5 containment tests x (16 B conn + 4 x 12 B coords) + 4 probes x 4 nodes x 2 slices x 12 B
+BYTES_PER_STEP_F32 = 704  # 4 containment tests x (16 B conn + 64 B element frame) + 4 stages x 4 nodes x 2 slices x 12 B
 HBM_PEAK = 8.0e12
 # VALU issue ceiling of MI355X: 256 CUs x 4 SIMDs, one SIMD-cycle each per
 # cycle at the 2.4 GHz peak engine clock (a wave64 VALU instruction occupies
@@ -83,16 +83,17 @@ def measured_traffic(kernel):
 
 
 def roofline_valu(kernel, steps_per_launch, avg_ms, bpp, default_workload):
-    """Roofline of the dominant RK4 kernel against the bound that governs it.
-    Counter passes on the FP32 pipe workload (profiles/pmc_valu_pipe.md)
-    show the vector ALU busy most of each launch while L2 read bandwidth and
-    HBM stay far below their ceilings, so the bound is VALU issue:
-    achieved = VALU busy SIMD-cycles per particle-step (SQ_ACTIVE_INST_VALU x 4
-    over the particle-steps of the profiled launches) x particle-steps of
-    this launch / its measured duration, against 1024 SIMDs x 2.4 GHz.  The
-    algorithmic gather bytes (704 B per particle-step in FP32) are reported
-    beside it with the measured HBM traffic (cache-served: their HBM
-    fraction exceeds 1)."""
+    """Roofline of the dominant RK4 kernel against the nearest ceiling.
+    Counter passes on the FP32 pipe workload (profiles/pmc_valu_pipe.md,
+    pmc_l2_pipe.md, pmc_traffic_pipe.md) put VALU issue highest (56-64%
+    busy), L2-to-L1 reads at 38% of the L2's bandwidth and HBM near 3%; the
+    rest of each launch is gather latency at 4 waves per SIMD.  The bound
+    reported is VALU issue: achieved = VALU busy SIMD-cycles per
+    particle-step (SQ_ACTIVE_INST_VALU x 4 over the particle-steps of the
+    profiled launches) x particle-steps of this launch / its measured
+    duration, against 1024 SIMDs x 2.4 GHz.  The algorithmic gather bytes
+    (704 B per particle-step in FP32) are reported beside it with the
+    measured HBM traffic (cache-served: their HBM fraction exceeds 1)."""
     cyc = profile_figure("pmc_valu_pipe.md", "valu_cycles_per_particle_step", kernel) if default_workload else None
     secs = avg_ms * 1e-3
     gather = steps_per_launch * bpp / secs if secs > 0 else 0.0

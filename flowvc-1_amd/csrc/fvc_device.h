@@ -104,6 +104,22 @@ __device__ __forceinline__ T ld32(const T *__restrict__ base, uint32_t idx) {
     return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + (uint64_t)off);
 }
 
+/* A wave claims the next 64 entries of a persistent kernel's work queue
+ * (a counter per queue, reset before the launch) and returns their offset;
+ * an offset >= len means the queue is exhausted.  Lane 0 reads the counter
+ * first and skips the atomic once it has passed len, so the waves that
+ * arrive after the queue ran out (most of a launch with little work) do not
+ * all contend on one address; the counter only grows, so a stale read only
+ * costs an atomic. */
+__device__ __forceinline__ int claim64(int *queue, int len) {
+    int base = 0;
+    if ((threadIdx.x & 63) == 0) {
+        base = __hip_atomic_load(queue, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (base < len) base = atomicAdd(queue, 64);
+    }
+    return __shfl(base, 0, 64);
+}
+
 /* ------------------------------------------------------------------------ */
 /* FP64 natural coordinates in the reference's expression order               */
 /* (ompVC/mesh.c:527-547, velocity.c:497-517)                                */

@@ -221,9 +221,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(FVC_HOT_WAVE
         const int q = (home + k) % NQ;
         const int seg_lo = (int)((int64_t)n * q / NQ), seg_hi = (int)((int64_t)n * (q + 1) / NQ);
         for (;;) {
-            int base = 0;
-            if (lane == 0) base = atomicAdd(&S.queue[q], 64);
-            base = __shfl(base, 0, 64);
+            const int base = claim64(&S.queue[q], seg_hi - seg_lo);
             if (seg_lo + base >= seg_hi) break;
             const int idx = seg_lo + base + lane;
             int r = -1, fin = 0;
