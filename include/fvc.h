@@ -129,7 +129,10 @@ typedef struct fvc_ftle fvc_ftle;
 
 /* ---- context ------------------------------------------------------------- */
 /* Upload the mesh (LoadMeshData, ompVC/mesh.c:21-43) and build the device
- * spatial hash.  Replaces the mesh globals of ompVC/globals.h:38-47. */
+ * spatial hash.  Replaces the mesh globals of ompVC/globals.h:38-47.
+ * Device memory of an unstructured mesh: 32 B per node, 96 B per element
+ * (connectivity, adjacency and the 64-byte FP32 frame the reduced-precision
+ * walks test against), plus the hash; at most 2^26 elements and 2^27 nodes. */
 int fvc_create(const fvc_mesh_desc *mesh, const fvc_params *params, fvc_ctx **out);
 /* Frees the context; objects still alive on it are released and detached
  * (see Lifetimes above). */
