@@ -1167,6 +1167,13 @@ __device__ int rk4_sm(const DevMesh &m, const DevField &f, PState &p, IState &is
     const double inv_span = 1.0 / (f.tmax - f.tmin);
     int taken = 0, ret = 1;
     int phase = PH_F0, e = 0, hops = 0, start = 0;
+    /* the stage probes' walk elements and left flags, in registers (selected
+     * by phase; written back to the interval state at the end) */
+    int mpe[4], mpl = 0;
+    for (int q = 0; q < 4; q++) {
+        mpe[q] = is.mp[q].elem;
+        mpl |= is.mp[q].left << q;
+    }
     /* S: f0, then f0 + 2 k2, f0 + 2 k2 + 2 k3, and from the k4 stage to the
      * locate's end the step's f0 + 2 k2 + 2 k3 + k4 (increment h/6 S) */
     float A[3], D[3], dq[3], S[3] = {0.f, 0.f, 0.f};
@@ -1190,10 +1197,11 @@ __device__ int rk4_sm(const DevMesh &m, const DevField &f, PState &p, IState &is
         phase = ph;
         hops = 0;
         const double tl = tloc_of(ph);
-        if (is.mp[ph].elem == -1) err = E_ELEM;
+        const int pe = ph == PH_F0 ? mpe[0] : (ph == PH_K2 ? mpe[1] : (ph == PH_K3 ? mpe[2] : mpe[3]));
+        if (pe == -1) err = E_ELEM;
         else if ((tl > 1 + TINY) || (tl < 0 - TINY)) err = E_TLOC;
-        else if (is.mp[ph].elem < 0 || is.mp[ph].elem >= m.ne) err = E_ELEM;
-        e = is.mp[ph].elem;
+        else if (pe < 0 || pe >= m.ne) err = E_ELEM;
+        e = pe;
     };
     /* start of a step (the loop head of integration.c:344); false when the
      * interval is done or the step budget spent */
@@ -1211,7 +1219,8 @@ __device__ int rk4_sm(const DevMesh &m, const DevField &f, PState &p, IState &is
     };
     anchor();
     float r = 0.f, s = 0.f, t = 0.f;
-    int ltie = 0; /* a locate-walk test hit d == -TINY */
+    int ltie = 0;             /* a locate-walk test hit d == -TINY */
+    int pleft = 0;            /* a probe walk of this step left the mesh */
     if (!begin_step()) phase = PH_DONE;
     else enter_probe(PH_F0);
     if (err) phase = PH_DONE;
@@ -1260,14 +1269,13 @@ __device__ int rk4_sm(const DevMesh &m, const DevField &f, PState &p, IState &is
             }
         }
         /* ---- the walk ended: accepted at e, or left the mesh (e == -1) */
-        bool want = e >= 0;
-        double tl_i = 0.0;
+        const bool want = e >= 0;
+        bool from_loc = false;
         if (phase == PH_LOC) {
             p.elem = want ? e : -1;
-            /* a probe that left holds element -1 (and its left flag) */
-            const bool out = !want || is.mp[0].elem < 0 || is.mp[1].elem < 0 || is.mp[2].elem < 0 ||
-                             is.mp[3].elem < 0;
-            if (out) { /* exit: back to the step's start (integration.c:430-440) */
+            /* a probe of this step that left (its element is -1 and its left
+             * flag set) ends the step as an exit too */
+            if (!want || pleft) { /* exit: back to the step's start (integration.c:430-440) */
                 const double h6 = is.h * (1.0 / 6.0);
                 p.left = 1;
                 p.left_time = is.tc - is.h;
@@ -1278,53 +1286,48 @@ __device__ int rk4_sm(const DevMesh &m, const DevField &f, PState &p, IState &is
             /* the next step; its first stage takes this element and these
              * coordinates when the locate started where the first stage
              * started and no test met the tie */
-            const bool reuse = !ltie && start == is.mp[0].elem;
+            const bool reuse = !ltie && start == mpe[0];
+            pleft = 0;
             if (!begin_step()) break;
             if (!reuse) {
                 enter_probe(PH_F0);
                 if (err) break;
                 continue;
             }
-            tl_i = tloc_of(PH_F0);
-            if ((tl_i > 1 + TINY) || (tl_i < 0 - TINY)) {
-                err = E_TLOC;
-                break;
-            }
-            is.mp[0].elem = e;
             phase = PH_F0;
-        } else {
-            if (want) is.mp[phase].elem = e;
-            else { /* the probe left: zero velocity for this stage (velocity.c:588-592) */
-                is.mp[phase].elem = -1;
-                is.mp[phase].left = 1;
-            }
-            tl_i = tloc_of(phase);
+            from_loc = true;
         }
-        /* ---- this stage's velocity and the next phase */
+        const double tl_i = tloc_of(phase);
+        if (from_loc && ((tl_i > 1 + TINY) || (tl_i < 0 - TINY))) { /* the first stage taken from the locate */
+            err = E_TLOC;
+            break;
+        }
+        /* the probe's element, or -1 and its left flag when it left: zero
+         * velocity for this stage (velocity.c:588-592) */
+        for (int q = 0; q < 4; q++) mpe[q] = q == phase ? (want ? e : -1) : mpe[q];
+        if (!want) {
+            mpl |= 1 << phase;
+            pleft = 1;
+        }
+        /* ---- this stage's velocity and the next phase: the four stages
+         * share one update, S = k, S + 2 k, S + 2 k, S + k and the next
+         * probe's offset dq = D + w k (w = h/2, h/2, h) */
         float k[3] = {0.f, 0.f, 0.f};
         if (want) interp_nodes_f<MODE, DIM>(f, (float)tl_i, nd, r, s, t, k);
         const float hf = (float)is.h, hh = 0.5f * hf;
-        if (phase == PH_F0) {
-            for (int c = 0; c < 3; c++) {
-                S[c] = k[c];
-                dq[c] = D[c] + hh * k[c];
-            }
-            gdq = guard_offset(m, dq);
-            enter_probe(PH_K2);
-        } else if (phase == PH_K2 || phase == PH_K3) {
-            const float w = phase == PH_K2 ? hh : hf;
-            for (int c = 0; c < 3; c++) {
-                S[c] = S[c] + 2.0f * k[c];
-                dq[c] = D[c] + w * k[c];
-            }
+        const float ck = (phase == PH_K2 || phase == PH_K3) ? 2.0f : 1.0f;
+        const float w = phase == PH_K3 ? hf : hh;
+        for (int c = 0; c < 3; c++) {
+            const float kc = ck * k[c];
+            S[c] = phase == PH_F0 ? kc : S[c] + kc;
+            dq[c] = D[c] + w * k[c];
+        }
+        if (phase != PH_K4) {
             gdq = guard_offset(m, dq);
             enter_probe(phase + 1);
-        } else { /* PH_K4: the step's increment, position and time */
+        } else { /* the step's increment, position and time */
             const double h = is.h, h6 = h * (1.0 / 6.0);
-            for (int c = 0; c < 3; c++) {
-                S[c] = S[c] + k[c];
-                p.X[c] = p.X[c] + h6 * (double)S[c];
-            }
+            for (int c = 0; c < 3; c++) p.X[c] = p.X[c] + h6 * (double)S[c];
             is.tc = is.tc + h;
             if (!(fabs(h) <= hmin)) nsteps++;
             taken++;
@@ -1336,6 +1339,10 @@ __device__ int rk4_sm(const DevMesh &m, const DevField &f, PState &p, IState &is
             e = p.elem; /* a located element: the walks above keep p.elem >= 0 */
         }
         if (err) break;
+    }
+    for (int q = 0; q < 4; q++) {
+        is.mp[q].elem = mpe[q];
+        is.mp[q].left = (mpl >> q) & 1;
     }
     if (err) stop = b;
     return ret;
