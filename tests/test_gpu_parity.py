@@ -59,6 +59,43 @@ def test_locate_walk_and_global_bit_exact(pipe, precision):
     assert (glob_ref >= 0).sum() > q.shape[0] // 2
 
 
+def _stretch(p):
+    """z squeezed 50 times: tets 0.17 wide and 0.005 high (condition ~30 times the pipe's)"""
+    return p * [1.0, 1.0, 0.02]
+
+
+def _grade(p):
+    """z -> 10 (z/10)^3: layers from 2e-5 to 0.7 high along the pipe"""
+    q = p.copy()
+    q[:, 2] = 10.0 * (p[:, 2] / 10.0) ** 3
+    return q
+
+
+@pytest.mark.parametrize("shape", ["stretched", "graded"])
+@pytest.mark.parametrize("precision", ["f32", "bf16"])
+def test_locate_bit_exact_on_badly_shaped_meshes(pipe, shape, precision):
+    """The FP32 frames carry each element's own guard band, set by its
+    condition number: on flat and strongly graded tets the reduced-precision
+    walks must still return the FP64 walk's element for every probe (faces,
+    edges and vertices included: the probes are mapped like the nodes)."""
+    t, xyz, conn, adj, _ = pipe
+    f = _stretch if shape == "stretched" else _grade
+    X = f(xyz)
+    b = [X[:, 0].min(), X[:, 0].max(), X[:, 1].min(), X[:, 1].max(), X[:, 2].min(), X[:, 2].max()]
+    ctx = fvc.Context(3, b, precision, 2, coords=X, conn=conn, adj=adj)
+    om = oracle.Mesh(3, b, coords=X, conn=conn, adj=adj)
+    q = f(query_points(t, 4000, 17))
+    rng = np.random.default_rng(6)
+    guess = rng.integers(0, conn.shape[0], q.shape[0]).astype(np.int32)
+    print("locate %s %s n=%d" % (shape, precision, q.shape[0]), flush=True)
+    g_ref, rc = om.locate_local(q, guess)
+    assert rc == 0
+    np.testing.assert_array_equal(ctx.locate(q, guess, fvc.LOCATE_WALK), g_ref)
+    np.testing.assert_array_equal(ctx.locate(q, None, fvc.LOCATE_GLOBAL), om.locate_global(q))
+    assert (g_ref >= 0).sum() > q.shape[0] // 4
+    ctx.close()
+
+
 def test_seed_list_chain_matches_oracle(pipe):
     t, xyz, conn, adj, _ = pipe
     ctx = fvc.Context(3, PIPE_BOUNDS, "f64", 2, coords=xyz, conn=conn, adj=adj)
