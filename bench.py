@@ -136,6 +136,15 @@ def make_context(fvc, precision, xyz, conn, adj, slices):
     return ctx
 
 
+def element_order(np, recs):
+    """Tracer records in the order of the element holding each (stable, so
+    ties keep their Z-order; points outside last): the mesh is numbered along
+    its own Z-curve, so neighbouring lanes gather from neighbouring element
+    frames."""
+    key = np.where(recs["ElementIndex"] < 0, 1 << 30, recs["ElementIndex"])
+    return recs[np.argsort(key, kind="stable")]
+
+
 def frames():
     """(tmin, tmax, lo, hi) of the 50 frames of one period (main.c:188-195; the
     last frame's upper slice wraps to file 0, velocity.c:77)."""
@@ -590,7 +599,7 @@ def bench_traffic(args, np, fvc, host):
     ctx = make_context(fvc, args.precision, xyz, conn, adj, slices)
     lo, hi = np.array([-1.0, -1.0, 0.0]), np.array([1.0, 1.0, 10.0])
     seeds = seeds[ctx.zsort(seeds, lo, hi)]
-    recs = ctx.seed_list(seeds, 1)
+    recs = element_order(np, ctx.seed_list(seeds, 1))
     P = fvc.Particles(ctx, recs.shape[0])
     P.set(recs)
     ctx.copy_probe(1 << 30)
@@ -773,11 +782,7 @@ def main():
     lo, hi = np.array([-1.0, -1.0, 0.0]), np.array([1.0, 1.0, 10.0])
     perm = ctx.zsort(seeds, lo, hi)
     seeds = seeds[perm]
-    recs = ctx.seed_list(seeds, 1)
-    # tracers in the order of the element holding each (ties in Z-order; the
-    # points outside last): the mesh is numbered along its own Z-curve, so
-    # neighbouring lanes gather from neighbouring element frames
-    recs = recs[np.argsort(np.where(recs["ElementIndex"] < 0, 1 << 30, recs["ElementIndex"]), kind="stable")]
+    recs = element_order(np, ctx.seed_list(seeds, 1))
     n_tr = recs.shape[0]
     P_ic = fvc.Particles(ctx, n_tr)
     P_ic.set(recs)

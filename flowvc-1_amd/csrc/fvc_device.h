@@ -1219,8 +1219,7 @@ __device__ int rk4_sm(const DevMesh &m, const DevField &f, PState &p, IState &is
     };
     anchor();
     float r = 0.f, s = 0.f, t = 0.f;
-    int ltie = 0;             /* a locate-walk test hit d == -TINY */
-    int pleft = 0;            /* a probe walk of this step left the mesh */
+    int ltie = 0; /* a locate-walk test hit d == -TINY */
     if (!begin_step()) phase = PH_DONE;
     else enter_probe(PH_F0);
     if (err) phase = PH_DONE;
@@ -1269,13 +1268,13 @@ __device__ int rk4_sm(const DevMesh &m, const DevField &f, PState &p, IState &is
             }
         }
         /* ---- the walk ended: accepted at e, or left the mesh (e == -1) */
-        const bool want = e >= 0;
-        bool from_loc = false;
+        bool want = e >= 0;
+        double tl_i = 0.0;
         if (phase == PH_LOC) {
             p.elem = want ? e : -1;
-            /* a probe of this step that left (its element is -1 and its left
-             * flag set) ends the step as an exit too */
-            if (!want || pleft) { /* exit: back to the step's start (integration.c:430-440) */
+            /* a probe that left holds element -1 (and its left flag) */
+            const bool out = !want || mpe[0] < 0 || mpe[1] < 0 || mpe[2] < 0 || mpe[3] < 0;
+            if (out) { /* exit: back to the step's start (integration.c:430-440) */
                 const double h6 = is.h * (1.0 / 6.0);
                 p.left = 1;
                 p.left_time = is.tc - is.h;
@@ -1287,47 +1286,51 @@ __device__ int rk4_sm(const DevMesh &m, const DevField &f, PState &p, IState &is
              * coordinates when the locate started where the first stage
              * started and no test met the tie */
             const bool reuse = !ltie && start == mpe[0];
-            pleft = 0;
             if (!begin_step()) break;
             if (!reuse) {
                 enter_probe(PH_F0);
                 if (err) break;
                 continue;
             }
+            tl_i = tloc_of(PH_F0);
+            if ((tl_i > 1 + TINY) || (tl_i < 0 - TINY)) {
+                err = E_TLOC;
+                break;
+            }
+            mpe[0] = e;
             phase = PH_F0;
-            from_loc = true;
+        } else {
+            /* the probe's element, or -1 and its left flag when it left:
+             * zero velocity for this stage (velocity.c:588-592) */
+            for (int q = 0; q < 4; q++) mpe[q] = q == phase ? (want ? e : -1) : mpe[q];
+            if (!want) mpl |= 1 << phase;
+            tl_i = tloc_of(phase);
         }
-        const double tl_i = tloc_of(phase);
-        if (from_loc && ((tl_i > 1 + TINY) || (tl_i < 0 - TINY))) { /* the first stage taken from the locate */
-            err = E_TLOC;
-            break;
-        }
-        /* the probe's element, or -1 and its left flag when it left: zero
-         * velocity for this stage (velocity.c:588-592) */
-        for (int q = 0; q < 4; q++) mpe[q] = q == phase ? (want ? e : -1) : mpe[q];
-        if (!want) {
-            mpl |= 1 << phase;
-            pleft = 1;
-        }
-        /* ---- this stage's velocity and the next phase: the four stages
-         * share one update, S = k, S + 2 k, S + 2 k, S + k and the next
-         * probe's offset dq = D + w k (w = h/2, h/2, h) */
+        /* ---- this stage's velocity and the next phase */
         float k[3] = {0.f, 0.f, 0.f};
         if (want) interp_nodes_f<MODE, DIM>(f, (float)tl_i, nd, r, s, t, k);
         const float hf = (float)is.h, hh = 0.5f * hf;
-        const float ck = (phase == PH_K2 || phase == PH_K3) ? 2.0f : 1.0f;
-        const float w = phase == PH_K3 ? hf : hh;
-        for (int c = 0; c < 3; c++) {
-            const float kc = ck * k[c];
-            S[c] = phase == PH_F0 ? kc : S[c] + kc;
-            dq[c] = D[c] + w * k[c];
-        }
-        if (phase != PH_K4) {
+        if (phase == PH_F0) {
+            for (int c = 0; c < 3; c++) {
+                S[c] = k[c];
+                dq[c] = D[c] + hh * k[c];
+            }
+            gdq = guard_offset(m, dq);
+            enter_probe(PH_K2);
+        } else if (phase == PH_K2 || phase == PH_K3) {
+            const float w = phase == PH_K2 ? hh : hf;
+            for (int c = 0; c < 3; c++) {
+                S[c] = S[c] + 2.0f * k[c];
+                dq[c] = D[c] + w * k[c];
+            }
             gdq = guard_offset(m, dq);
             enter_probe(phase + 1);
-        } else { /* the step's increment, position and time */
+        } else { /* PH_K4: the step's increment, position and time */
             const double h = is.h, h6 = h * (1.0 / 6.0);
-            for (int c = 0; c < 3; c++) p.X[c] = p.X[c] + h6 * (double)S[c];
+            for (int c = 0; c < 3; c++) {
+                S[c] = S[c] + k[c];
+                p.X[c] = p.X[c] + h6 * (double)S[c];
+            }
             is.tc = is.tc + h;
             if (!(fabs(h) <= hmin)) nsteps++;
             taken++;
