@@ -1167,13 +1167,6 @@ __device__ int rk4_sm(const DevMesh &m, const DevField &f, PState &p, IState &is
     const double inv_span = 1.0 / (f.tmax - f.tmin);
     int taken = 0, ret = 1;
     int phase = PH_F0, e = 0, hops = 0, start = 0;
-    /* the stage probes' walk elements and left flags, in registers (selected
-     * by phase; written back to the interval state at the end) */
-    int mpe[4], mpl = 0;
-    for (int q = 0; q < 4; q++) {
-        mpe[q] = is.mp[q].elem;
-        mpl |= is.mp[q].left << q;
-    }
     /* S: f0, then f0 + 2 k2, f0 + 2 k2 + 2 k3, and from the k4 stage to the
      * locate's end the step's f0 + 2 k2 + 2 k3 + k4 (increment h/6 S) */
     float A[3], D[3], dq[3], S[3] = {0.f, 0.f, 0.f};
@@ -1197,11 +1190,10 @@ __device__ int rk4_sm(const DevMesh &m, const DevField &f, PState &p, IState &is
         phase = ph;
         hops = 0;
         const double tl = tloc_of(ph);
-        const int pe = ph == PH_F0 ? mpe[0] : (ph == PH_K2 ? mpe[1] : (ph == PH_K3 ? mpe[2] : mpe[3]));
-        if (pe == -1) err = E_ELEM;
+        if (is.mp[ph].elem == -1) err = E_ELEM;
         else if ((tl > 1 + TINY) || (tl < 0 - TINY)) err = E_TLOC;
-        else if (pe < 0 || pe >= m.ne) err = E_ELEM;
-        e = pe;
+        else if (is.mp[ph].elem < 0 || is.mp[ph].elem >= m.ne) err = E_ELEM;
+        e = is.mp[ph].elem;
     };
     /* start of a step (the loop head of integration.c:344); false when the
      * interval is done or the step budget spent */
@@ -1273,7 +1265,8 @@ __device__ int rk4_sm(const DevMesh &m, const DevField &f, PState &p, IState &is
         if (phase == PH_LOC) {
             p.elem = want ? e : -1;
             /* a probe that left holds element -1 (and its left flag) */
-            const bool out = !want || mpe[0] < 0 || mpe[1] < 0 || mpe[2] < 0 || mpe[3] < 0;
+            const bool out = !want || is.mp[0].elem < 0 || is.mp[1].elem < 0 || is.mp[2].elem < 0 ||
+                             is.mp[3].elem < 0;
             if (out) { /* exit: back to the step's start (integration.c:430-440) */
                 const double h6 = is.h * (1.0 / 6.0);
                 p.left = 1;
@@ -1285,7 +1278,7 @@ __device__ int rk4_sm(const DevMesh &m, const DevField &f, PState &p, IState &is
             /* the next step; its first stage takes this element and these
              * coordinates when the locate started where the first stage
              * started and no test met the tie */
-            const bool reuse = !ltie && start == mpe[0];
+            const bool reuse = !ltie && start == is.mp[0].elem;
             if (!begin_step()) break;
             if (!reuse) {
                 enter_probe(PH_F0);
@@ -1297,13 +1290,14 @@ __device__ int rk4_sm(const DevMesh &m, const DevField &f, PState &p, IState &is
                 err = E_TLOC;
                 break;
             }
-            mpe[0] = e;
+            is.mp[0].elem = e;
             phase = PH_F0;
         } else {
-            /* the probe's element, or -1 and its left flag when it left:
-             * zero velocity for this stage (velocity.c:588-592) */
-            for (int q = 0; q < 4; q++) mpe[q] = q == phase ? (want ? e : -1) : mpe[q];
-            if (!want) mpl |= 1 << phase;
+            if (want) is.mp[phase].elem = e;
+            else { /* the probe left: zero velocity for this stage (velocity.c:588-592) */
+                is.mp[phase].elem = -1;
+                is.mp[phase].left = 1;
+            }
             tl_i = tloc_of(phase);
         }
         /* ---- this stage's velocity and the next phase */
@@ -1343,10 +1337,6 @@ __device__ int rk4_sm(const DevMesh &m, const DevField &f, PState &p, IState &is
         }
         if (err) break;
     }
-    for (int q = 0; q < 4; q++) {
-        is.mp[q].elem = mpe[q];
-        is.mp[q].left = (mpl >> q) & 1;
-    }
     if (err) stop = b;
     return ret;
 }
@@ -1366,10 +1356,12 @@ __device__ int interval_run(const DevMesh &m, const DevField &f, PState &p, ISta
 #pragma clang fp contract(off)
     /* GEN = 0 compiles RK4 only (no multistep state in registers) */
     const int itype = GEN ? itype_rt : INT_RK4;
-    if constexpr (MODE != P_F64 && UNSTRUCT) {
-        /* GEN = 0 builds integrate neither strain rate nor exposure time */
-        if (itype == INT_RK4 && !extrap && (!GEN || (f.sr_lo == nullptr && f.cet_lo == nullptr)))
-            return rk4_sm<MODE, DIM>(m, f, p, is, a, b, dt, max_steps, err, nsteps, stop);
+    if constexpr (MODE != P_F64 && UNSTRUCT && !GEN) {
+        /* GEN = 0 builds integrate RK4 only, without strain rate or exposure
+         * time; the GEN = 1 builds (other integrators, RK4 with extrapolation
+         * or those integrals) keep the straight-line walks below and leave
+         * the state machine's registers out */
+        if (!extrap) return rk4_sm<MODE, DIM>(m, f, p, is, a, b, dt, max_steps, err, nsteps, stop);
     }
     if (!is.started) {
         if (p.left) {
