@@ -183,10 +183,11 @@ __device__ void batch_particle(const DevMesh &m, const BatchDev &b, int i, int b
     }
 }
 
-/* GEN = 0: RK4 releases only (no multistep state in registers); 1: any.
- * Held to 3 waves/SIMD (at most 168 VGPRs; a few spill slots). */
+/* GEN = 0: RK4 releases only (no multistep state in registers), held to
+ * 3 waves/SIMD (at most 168 VGPRs); 1: the other integrators, at 2 waves/SIMD
+ * (about 235 VGPRs, where 3 waves spill 70-87 of them). */
 template <int MODE, int BS, int GEN>
-__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(FVC_HOT_WAVES))) void k_batch(DevMesh m, BatchDev b,
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(GEN ? 2 : FVC_HOT_WAVES))) void k_batch(DevMesh m, BatchDev b,
                                                                                       int budget) {
     const int lane = threadIdx.x & 63;
     const int home = blockIdx.x % NQ;

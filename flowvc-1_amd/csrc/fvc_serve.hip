@@ -209,9 +209,10 @@ __device__ uint32_t serve_particle(const DevMesh &m, const SrvDev &S, int i, int
     return ns;
 }
 
-/* GEN = 0: RK4 requests (tracer or FTLE); 1: the other integrators */
+/* GEN = 0: RK4 requests (tracer or FTLE), 3 waves/SIMD; 1: the other
+ * integrators, 2 waves/SIMD (no spills; 3 waves spill 70 VGPRs) */
 template <int MODE, int BS, int GEN>
-__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(FVC_HOT_WAVES))) void k_serve(DevMesh m, SrvDev S,
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(GEN ? 2 : FVC_HOT_WAVES))) void k_serve(DevMesh m, SrvDev S,
                                                                                       int budget) {
     const int lane = threadIdx.x & 63;
     const int home = blockIdx.x % NQ;
