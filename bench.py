@@ -85,9 +85,10 @@ def measured_traffic(kernel):
 def roofline_valu(kernel, steps_per_launch, avg_ms, bpp, default_workload):
     """Roofline of the dominant RK4 kernel against the nearest ceiling.
     Counter passes on the FP32 pipe workload (profiles/pmc_valu_pipe.md,
-    pmc_l2_pipe.md, pmc_traffic_pipe.md) put VALU issue highest (56-64%
-    busy), L2-to-L1 reads at 38% of the L2's bandwidth and HBM near 3%; the
-    rest of each launch is gather latency at 4 waves per SIMD.  The bound
+    pmc_l2_pipe.md, pmc_traffic_pipe.md, pmc_wait_pipe.md) put VALU issue
+    highest (64-68% busy), L2-to-L1 reads at 29-38% of the L2's bandwidth
+    and HBM near 3%; the waves wait on their own gathers more than half of
+    their cycles (the limiter, reported beside the roofline).  The bound
     reported is VALU issue: achieved = VALU busy SIMD-cycles per
     particle-step (SQ_ACTIVE_INST_VALU x 4 over the particle-steps of the
     profiled launches) x particle-steps of this launch / its measured
@@ -109,6 +110,10 @@ def roofline_valu(kernel, steps_per_launch, avg_ms, bpp, default_workload):
     out.update(bound="valu", achieved=achieved / 1e9, peak=VALU_PEAK_SIMD_CYCLES / 1e9, unit="G SIMD-cycles/s",
                frac=achieved / VALU_PEAK_SIMD_CYCLES, valu_cycles_per_particle_step=cyc,
                valu_source="profiles/pmc_valu_pipe.md (SQ_ACTIVE_INST_VALU x 4 per particle-step)")
+    wait = profile_figure("pmc_wait_pipe.md", "wave_wait_share", kernel)
+    if wait is not None:
+        out.update(limiter="gather latency: the kernel's waves wait on their loads %.0f%% of their cycles "
+                           "(SQ_WAIT_ANY / SQ_WAVE_CYCLES, profiles/pmc_wait_pipe.md)" % (100 * wait))
     return out
 
 
