@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <mutex>
+#include <thread>
 #include <vector>
 
 #include "fvc_internal.h"
@@ -207,82 +208,101 @@ static int build_hash(fvc_ctx *c, const fvc_mesh_desc *md) {
     return FVC_OK;
 }
 
-/* FP32 affine frames of the elements (the layout fvc_device.h's ld_frame
- * reads: rows (M[a][0..2], o[a]), then (lo[0..2], g)), from the FP64 nodes
- * relative to the mesh centre.  Returns the largest row-sum norm of M (the
- * 1-r-s-t row included) over elements with a finite frame. */
-static double build_frames(const fvc_mesh_desc *md, const double centre[3], std::vector<float4> &ER) {
+/* FP32 affine frame of element e (the layout fvc_device.h's ld_frame
+ * reads: rows (M[a][0..2], o[a]), then (lo[0..2], g)) into R[0..3], from the
+ * FP64 nodes relative to the mesh centre; returns the row-sum norm of M (the
+ * 1-r-s-t row included), or 0 for an element without a finite frame. */
+static double frame_of(const fvc_mesh_desc *md, const double centre[3], int64_t e, float4 *R) {
     const int dims = md->dims;
     const double u = ldexp(1.0, -24);
-    ER.assign((size_t)4 * md->num_elems, make_float4(0.f, 0.f, 0.f, 0.f));
-    double maxrow = 0.0;
-    for (int64_t e = 0; e < md->num_elems; e++) {
-        const int32_t *nd = md->conn + 4 * e;
-        double P[4][3] = {{0}};
-        for (int q = 0; q <= dims; q++)
-            for (int a = 0; a < dims; a++) P[q][a] = md->coords[3 * nd[q] + a] - centre[a];
-        double E[3][3] = {{0}}, M[3][3] = {{0}}; /* E[a][q] = (node q+1 - node 0)[a] */
-        for (int q = 0; q < dims; q++)
-            for (int a = 0; a < dims; a++) E[a][q] = P[q + 1][a] - P[0][a];
-        double det;
-        if (dims == 3) {
-            det = E[0][0] * (E[1][1] * E[2][2] - E[1][2] * E[2][1]) - E[0][1] * (E[1][0] * E[2][2] - E[1][2] * E[2][0]) +
-                  E[0][2] * (E[1][0] * E[2][1] - E[1][1] * E[2][0]);
-            M[0][0] = (E[1][1] * E[2][2] - E[1][2] * E[2][1]) / det;
-            M[0][1] = (E[0][2] * E[2][1] - E[0][1] * E[2][2]) / det;
-            M[0][2] = (E[0][1] * E[1][2] - E[0][2] * E[1][1]) / det;
-            M[1][0] = (E[1][2] * E[2][0] - E[1][0] * E[2][2]) / det;
-            M[1][1] = (E[0][0] * E[2][2] - E[0][2] * E[2][0]) / det;
-            M[1][2] = (E[0][2] * E[1][0] - E[0][0] * E[1][2]) / det;
-            M[2][0] = (E[1][0] * E[2][1] - E[1][1] * E[2][0]) / det;
-            M[2][1] = (E[0][1] * E[2][0] - E[0][0] * E[2][1]) / det;
-            M[2][2] = (E[0][0] * E[1][1] - E[0][1] * E[1][0]) / det;
-        } else {
-            det = E[0][0] * E[1][1] - E[0][1] * E[1][0];
-            M[0][0] = E[1][1] / det;
-            M[0][1] = -E[0][1] / det;
-            M[1][0] = -E[1][0] / det;
-            M[1][1] = E[0][0] / det;
-        }
-        double rown = 0.0, enorm = 0.0, l0row = 0.0;
-        for (int i = 0; i < dims; i++) {
-            double rs = 0.0, es = 0.0;
-            for (int j = 0; j < dims; j++) {
-                rs += fabs(M[i][j]);
-                es += fabs(E[i][j]);
-            }
-            rown = std::max(rown, rs);
-            enorm = std::max(enorm, es);
-        }
-        for (int j = 0; j < dims; j++) {
-            double cs = 0.0;
-            for (int i = 0; i < dims; i++) cs += M[i][j];
-            l0row += fabs(cs);
-        }
-        rown = std::max(rown, l0row);
-        float o[3] = {0.f, 0.f, 0.f};
-        for (int a = 0; a < dims; a++) o[a] = (float)P[0][a];
-        double lo[3] = {0.0, 0.0, 0.0};
-        for (int i = 0; i < dims; i++)
-            for (int a = 0; a < dims; a++) lo[i] += M[i][a] * ((double)o[a] - P[0][a]);
-        const double g = 4.0 * (18.0 * rown * enorm + 12.0) * u;
-        bool ok = det != 0.0 && std::isfinite(g) && g < 0.25;
-        for (int i = 0; i < dims && ok; i++) {
-            ok = std::isfinite(lo[i]) && fabs(lo[i]) < 1e30;
-            for (int j = 0; j < dims; j++) ok = ok && fabs(M[i][j]) < 1e30;
-        }
-        float4 *R = ER.data() + 4 * e;
-        if (!ok) { /* every test on this element is decided in FP64 */
-            R[3].w = INFINITY;
-            continue;
-        }
-        R[0] = make_float4((float)M[0][0], (float)M[0][1], (float)M[0][2], o[0]);
-        R[1] = make_float4((float)M[1][0], (float)M[1][1], (float)M[1][2], o[1]);
-        R[2] = make_float4((float)M[2][0], (float)M[2][1], (float)M[2][2], o[2]);
-        R[3] = make_float4((float)lo[0], (float)lo[1], (float)lo[2], (float)g);
-        maxrow = std::max(maxrow, rown);
+    const int32_t *nd = md->conn + 4 * e;
+    double P[4][3] = {{0}};
+    for (int q = 0; q <= dims; q++)
+        for (int a = 0; a < dims; a++) P[q][a] = md->coords[3 * nd[q] + a] - centre[a];
+    double E[3][3] = {{0}}, M[3][3] = {{0}}; /* E[a][q] = (node q+1 - node 0)[a] */
+    for (int q = 0; q < dims; q++)
+        for (int a = 0; a < dims; a++) E[a][q] = P[q + 1][a] - P[0][a];
+    double det;
+    if (dims == 3) {
+        det = E[0][0] * (E[1][1] * E[2][2] - E[1][2] * E[2][1]) - E[0][1] * (E[1][0] * E[2][2] - E[1][2] * E[2][0]) +
+              E[0][2] * (E[1][0] * E[2][1] - E[1][1] * E[2][0]);
+        M[0][0] = (E[1][1] * E[2][2] - E[1][2] * E[2][1]) / det;
+        M[0][1] = (E[0][2] * E[2][1] - E[0][1] * E[2][2]) / det;
+        M[0][2] = (E[0][1] * E[1][2] - E[0][2] * E[1][1]) / det;
+        M[1][0] = (E[1][2] * E[2][0] - E[1][0] * E[2][2]) / det;
+        M[1][1] = (E[0][0] * E[2][2] - E[0][2] * E[2][0]) / det;
+        M[1][2] = (E[0][2] * E[1][0] - E[0][0] * E[1][2]) / det;
+        M[2][0] = (E[1][0] * E[2][1] - E[1][1] * E[2][0]) / det;
+        M[2][1] = (E[0][1] * E[2][0] - E[0][0] * E[2][1]) / det;
+        M[2][2] = (E[0][0] * E[1][1] - E[0][1] * E[1][0]) / det;
+    } else {
+        det = E[0][0] * E[1][1] - E[0][1] * E[1][0];
+        M[0][0] = E[1][1] / det;
+        M[0][1] = -E[0][1] / det;
+        M[1][0] = -E[1][0] / det;
+        M[1][1] = E[0][0] / det;
     }
-    return maxrow;
+    double rown = 0.0, enorm = 0.0, l0row = 0.0;
+    for (int i = 0; i < dims; i++) {
+        double rs = 0.0, es = 0.0;
+        for (int j = 0; j < dims; j++) {
+            rs += fabs(M[i][j]);
+            es += fabs(E[i][j]);
+        }
+        rown = std::max(rown, rs);
+        enorm = std::max(enorm, es);
+    }
+    for (int j = 0; j < dims; j++) {
+        double cs = 0.0;
+        for (int i = 0; i < dims; i++) cs += M[i][j];
+        l0row += fabs(cs);
+    }
+    rown = std::max(rown, l0row);
+    float o[3] = {0.f, 0.f, 0.f};
+    for (int a = 0; a < dims; a++) o[a] = (float)P[0][a];
+    double lo[3] = {0.0, 0.0, 0.0};
+    for (int i = 0; i < dims; i++)
+        for (int a = 0; a < dims; a++) lo[i] += M[i][a] * ((double)o[a] - P[0][a]);
+    const double g = 4.0 * (18.0 * rown * enorm + 12.0) * u;
+    bool ok = det != 0.0 && std::isfinite(g) && g < 0.25;
+    for (int i = 0; i < dims && ok; i++) {
+        ok = std::isfinite(lo[i]) && fabs(lo[i]) < 1e30;
+        for (int j = 0; j < dims; j++) ok = ok && fabs(M[i][j]) < 1e30;
+    }
+    if (!ok) { /* every test on this element is decided in FP64 */
+        R[3].w = INFINITY;
+        return 0.0;
+    }
+    R[0] = make_float4((float)M[0][0], (float)M[0][1], (float)M[0][2], o[0]);
+    R[1] = make_float4((float)M[1][0], (float)M[1][1], (float)M[1][2], o[1]);
+    R[2] = make_float4((float)M[2][0], (float)M[2][1], (float)M[2][2], o[2]);
+    R[3] = make_float4((float)lo[0], (float)lo[1], (float)lo[2], (float)g);
+    return rown;
+}
+
+/* all frames, over up to 16 host threads; returns the largest row norm */
+static double build_frames(const fvc_mesh_desc *md, const double centre[3], std::vector<float4> &ER) {
+    ER.assign((size_t)4 * md->num_elems, make_float4(0.f, 0.f, 0.f, 0.f));
+    const int64_t ne = md->num_elems;
+    const int nt = (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)std::max(1u, std::thread::hardware_concurrency()),
+                                                                16, ne / 65536 + 1}));
+    std::vector<double> part(nt, 0.0);
+    auto work = [&](int t) {
+        double mx = 0.0;
+        for (int64_t e = ne * t / nt; e < ne * (t + 1) / nt; e++)
+            mx = std::max(mx, frame_of(md, centre, e, ER.data() + 4 * e));
+        part[t] = mx;
+    };
+    std::vector<std::thread> pool;
+    int started = 1;
+    try {
+        for (; started < nt; started++) pool.emplace_back(work, started);
+    } catch (...) { /* no more threads: the rest runs here */
+    }
+    for (int t = started; t < nt; t++) work(t);
+    work(0);
+    for (auto &th : pool) th.join();
+    return *std::max_element(part.begin(), part.end());
 }
 
 /* ------------------------------------------------------------------------ */
