@@ -172,7 +172,7 @@ int launch_ftle_fill(fvc_ftle *f, double int_t);
 int launch_ftle_mask(fvc_ftle *f);
 /* the same on any FTLE grid view (enqueue only, no synchronisation) */
 int launch_ftle_early(fvc_ctx *c, const FtleDev &d, int64_t n, const DevField &fld, double t1, int itype, double dt);
-int launch_ftle_update_all(fvc_ctx *c, const FtleDev &d, int64_t n);
+int launch_ftle_update_all(fvc_ctx *c, const FtleDev &d, int64_t n, int extrap);
 int launch_ftle_fill_dev(fvc_ctx *c, const FtleDev &d, int64_t n, double int_t);
 int launch_ftle_mask_dev(fvc_ctx *c, const FtleDev &d, int64_t n);
 /* indices i (or map[i]) of the nonzero flags, in order */

@@ -1537,25 +1537,6 @@ __global__ __launch_bounds__(BLOCK) void k_early_gen(DevMesh m, DevField f, Ftle
     early_body<MODE, DIM, UNS, 1>(m, f, d, list, d_nt, t1, itype, dt, eb);
 }
 
-/* position update (ftle.c:160-169, UpdateFTLELocations ftle.c:1229-1242) */
-__global__ __launch_bounds__(BLOCK) void k_ftle_update(FtleDev d, const int *__restrict__ active, int64_t nactive,
-                                                      int uns, int extrap) {
-    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nactive) return;
-    int64_t i = active[q];
-    if (d.exited[i]) {
-        d.p.left[i] = 1;
-    } else {
-        for (int c = 0; c < 3; c++) {
-            d.p.X[c][i] = d.Xn[c][i];
-            if (extrap) d.p.V[c][i] = d.Vn[c][i];
-        }
-        if (uns) d.p.elem[i] = d.elem_new[i];
-    }
-    d.exited[i] = 0;
-    d.advected[i] = 0;
-}
-
 /* GetFTLEForPoint (ftle.c:1050-1203) */
 __global__ __launch_bounds__(BLOCK) void k_ftle_fill(FtleDev d, int64_t n, int dims, double int_t, ErrBuf eb) {
 #pragma clang fp contract(off)
@@ -1660,23 +1641,28 @@ int launch_ftle_early(fvc_ctx *c, const FtleDev &d, int64_t n, const DevField &f
     return FVC_OK;
 }
 
-/* position update over a whole grid (points advected this frame) */
-__global__ __launch_bounds__(BLOCK) void k_ftle_update_all(FtleDev d, int64_t n, int uns) {
+/* position update (ftle.c:160-169, UpdateFTLELocations ftle.c:1229-1242)
+ * of the points advected this frame, over the whole grid in grid order:
+ * coalesced, where the active list's Z-order scatters the copies */
+__global__ __launch_bounds__(BLOCK) void k_ftle_update_all(FtleDev d, int64_t n, int uns, int extrap) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n || !d.advected[i]) return;
     if (d.exited[i]) {
         d.p.left[i] = 1;
     } else {
-        for (int c = 0; c < 3; c++) d.p.X[c][i] = d.Xn[c][i];
+        for (int c = 0; c < 3; c++) {
+            d.p.X[c][i] = d.Xn[c][i];
+            if (extrap) d.p.V[c][i] = d.Vn[c][i];
+        }
         if (uns) d.p.elem[i] = d.elem_new[i];
     }
     d.exited[i] = 0;
     d.advected[i] = 0;
 }
 
-int launch_ftle_update_all(fvc_ctx *c, const FtleDev &d, int64_t n) {
+int launch_ftle_update_all(fvc_ctx *c, const FtleDev &d, int64_t n, int extrap) {
     if (n <= 0) return FVC_OK;
-    k_ftle_update_all<<<grid_for(n, BLOCK), BLOCK, 0, c->stream>>>(d, n, c->mesh.mesh_type);
+    k_ftle_update_all<<<grid_for(n, BLOCK), BLOCK, 0, c->stream>>>(d, n, c->mesh.mesh_type, extrap);
     FVC_HIP(hipGetLastError());
     return FVC_OK;
 }
@@ -1714,9 +1700,7 @@ int launch_ftle_frame(fvc_ftle *f, int lo, int hi, double tmin, double tmax, dou
         timing_end(c);
         FVC_HIP(hipGetLastError());
         if (!ip.extrapolate && (rc = launch_ftle_early(c, d, f->n, fld, t1, itype, ip.time_step))) return rc;
-        k_ftle_update<<<grid_for(f->nactive, BLOCK), BLOCK, 0, c->stream>>>(d, f->active, f->nactive,
-                                                                            c->mesh.mesh_type, ip.extrapolate);
-        FVC_HIP(hipGetLastError());
+        if ((rc = launch_ftle_update_all(c, d, f->n, ip.extrapolate))) return rc;
     }
     unsigned long long steps = 0;
     FVC_HIP(hipMemcpyAsync(&steps, c->d_steps, sizeof steps, hipMemcpyDeviceToHost, c->stream));

@@ -803,7 +803,7 @@ static int finish_ftle_frame(fvc_server *S, HostReq &R, bool *complete) {
     const DevField fld = srv_field_host(S, R.frame);
     const double t1 = R.d.t_start > fld.tmin ? R.d.t_start : fld.tmin;
     int rc = launch_ftle_early(c, v, R.n, fld, t1, INT_RK4, S->sp.time_step);
-    if (!rc) rc = launch_ftle_update_all(c, v, R.n);
+    if (!rc) rc = launch_ftle_update_all(c, v, R.n, 0);
     if (rc) return rc;
     *complete = R.d.t_stop - fld.tmax < TINY; /* StopTime reached (ftle.c:352-377) */
     if (*complete) {
