@@ -30,16 +30,17 @@ using namespace fvc;
 namespace fvc {
 
 int check_err(fvc_ctx *c, const char *what) {
-    hipError_t e = hipStreamSynchronize(c->stream);
+    /* one copy of the whole status block (error, location, step counters)
+     * into pinned memory; callers read the counters from h_status after it */
+    hipError_t e = hipMemcpyAsync(c->h_status, c->d_status, sizeof(fvc_status), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
         c->last_error = std::string(what) + ": " + hipGetErrorString(e);
         return FVC_E_HIP;
     }
-    int code = 0;
-    int64_t where = -1;
-    if (hipMemcpy(&code, c->d_err, sizeof code, hipMemcpyDeviceToHost) != hipSuccess) return FVC_E_HIP;
+    const int code = c->h_status->err;
     if (code) {
-        hipMemcpy(&where, c->d_where, sizeof where, hipMemcpyDeviceToHost);
+        const int64_t where = c->h_status->where;
         int zero = 0;
         hipMemcpy(c->d_err, &zero, sizeof zero, hipMemcpyHostToDevice);
         char buf[256];
@@ -337,9 +338,17 @@ extern "C" int fvc_create(const fvc_mesh_desc *md, const fvc_params *pp, fvc_ctx
         delete c;
         return FVC_E_HIP;
     }
-    if ((rc = dev_alloc(c, &c->d_err, 1)) || (rc = dev_alloc(c, &c->d_where, 1)) || (rc = dev_alloc(c, &c->d_steps, 2))) {
+    if ((rc = dev_alloc(c, &c->d_status, 1))) {
         fvc_destroy(c);
         return rc;
+    }
+    c->d_err = &c->d_status->err;
+    c->d_where = &c->d_status->where;
+    c->d_steps = c->d_status->steps;
+    if (hipHostMalloc((void **)&c->h_status, sizeof(fvc_status), hipHostMallocDefault) != hipSuccess) {
+        c->h_status = nullptr;
+        fvc_destroy(c);
+        return FVC_E_NOMEM;
     }
     if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
         fvc_destroy(c);
@@ -352,7 +361,7 @@ extern "C" int fvc_create(const fvc_mesh_desc *md, const fvc_params *pp, fvc_ctx
             return FVC_E_HIP;
         }
     }
-    if (hipMemset(c->d_err, 0, sizeof(int)) != hipSuccess) {
+    if (hipMemset(c->d_status, 0, sizeof(fvc_status)) != hipSuccess) {
         fvc_destroy(c);
         return FVC_E_HIP;
     }
@@ -475,9 +484,8 @@ extern "C" void fvc_destroy(fvc_ctx *c) {
     hipFree(c->adj);
     hipFree(c->cell_start);
     hipFree(c->cell_elems);
-    hipFree(c->d_err);
-    hipFree(c->d_where);
-    hipFree(c->d_steps);
+    hipFree(c->d_status);
+    if (c->h_status) hipHostFree(c->h_status);
     if (c->scratch) hipFree(c->scratch);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);

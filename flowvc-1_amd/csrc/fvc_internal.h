@@ -13,6 +13,14 @@
 #include "fvc.h"
 #include "fvc_device.h"
 
+/* what the kernels report to the host after a launch */
+struct fvc_status {
+    unsigned long long steps[2]; /* particle-steps; idle lanes (fvc_advect) */
+    int64_t where;               /* point of the first error */
+    int err;                     /* first error code, 0 if none */
+    int count;                   /* length of the last compacted list */
+};
+
 struct fvc_ctx {
     int device;
     int precision;
@@ -37,7 +45,10 @@ struct fvc_ctx {
     /* exposure time: per element 128-bit fixed-point sums (low, high words) and encounters */
     unsigned long long *cet_lo, *cet_hi;
     int *cet_enc;
-    /* errors and counters */
+    /* errors and counters: d_err, d_where and d_steps point into d_status,
+     * which check_err copies back whole into the pinned h_status with one
+     * copy and one stream synchronisation */
+    fvc_status *d_status, *h_status;
     int *d_err;
     int64_t *d_where;
     unsigned long long *d_steps;

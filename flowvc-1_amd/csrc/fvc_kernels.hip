@@ -312,10 +312,8 @@ int launch_advect_times(fvc_ctx *c, PtrSet p, int64_t begin, int64_t count, cons
                                                                            errbuf(c), c->d_steps));
         FVC_HIP(hipGetLastError());
     }
-    unsigned long long steps = 0;
-    FVC_HIP(hipMemcpyAsync(&steps, c->d_steps, sizeof steps, hipMemcpyDeviceToHost, c->stream));
     int rc = check_err(c, "fvc_advect_times");
-    c->last_steps = (int64_t)steps;
+    c->last_steps = (int64_t)c->h_status->steps[0];
     return rc;
 }
 
@@ -373,12 +371,10 @@ int launch_advect(fvc_ctx *c, PtrSet p, const int *index, int64_t begin, int64_t
         timing_end(c);
         FVC_HIP(hipGetLastError());
     }
-    unsigned long long steps[2] = {0, 0};
-    FVC_HIP(hipMemcpyAsync(steps, c->d_steps, sizeof steps, hipMemcpyDeviceToHost, c->stream));
     int rc = check_err(c, "fvc_advect");
-    c->last_steps = (int64_t)steps[0];
-    c->last_idle = (int64_t)steps[1];
-    if (count > 0) timing_collect(c, (int64_t)steps[0]);
+    c->last_steps = (int64_t)c->h_status->steps[0];
+    c->last_idle = (int64_t)c->h_status->steps[1];
+    if (count > 0) timing_collect(c, c->last_steps);
     return rc;
 }
 
@@ -1684,9 +1680,11 @@ int launch_ftle_frame(fvc_ftle *f, int lo, int hi, double tmin, double tmax, dou
     if (!flags) return FVC_E_NOMEM;
     k_ftle_flags<<<grid_for(f->n, BLOCK), BLOCK, 0, c->stream>>>(d, f->n, ip.extrapolate, f->zperm, flags);
     FVC_HIP(hipGetLastError());
+    /* the counter is cleared before the compaction's count read-back, so the
+     * advection launches right after that synchronisation */
+    FVC_HIP(hipMemsetAsync(c->d_steps, 0, sizeof(unsigned long long), c->stream));
     int rc = launch_compact(c, flags, f->n, f->active, &f->nactive, f->zperm);
     if (rc) return rc;
-    FVC_HIP(hipMemsetAsync(c->d_steps, 0, sizeof(unsigned long long), c->stream));
     if (f->nactive > 0) {
         timing_begin(c);
         if (itype == 1 && !ip.extrapolate)
@@ -1706,11 +1704,9 @@ int launch_ftle_frame(fvc_ftle *f, int lo, int hi, double tmin, double tmax, dou
         if (!ip.extrapolate && (rc = launch_ftle_early(c, d, f->n, fld, t1, itype, ip.time_step))) return rc;
         if ((rc = launch_ftle_update_all(c, d, f->n, ip.extrapolate))) return rc;
     }
-    unsigned long long steps = 0;
-    FVC_HIP(hipMemcpyAsync(&steps, c->d_steps, sizeof steps, hipMemcpyDeviceToHost, c->stream));
     rc = check_err(c, "fvc_ftle_frame");
-    c->last_steps = (int64_t)steps;
-    if (f->nactive > 0) timing_collect(c, (int64_t)steps);
+    c->last_steps = (int64_t)c->h_status->steps[0];
+    if (f->nactive > 0) timing_collect(c, c->last_steps);
     return rc;
 }
 
@@ -1756,12 +1752,12 @@ int launch_compact(fvc_ctx *c, const int *flags, int64_t n, int *out_idx, int64_
     k_cmp_count<<<nb, BLOCK, 0, c->stream>>>(flags, n, offs);
     k_scan_small<<<1, 1024, 0, c->stream>>>(offs, nb);
     k_cmp_scatter<<<nb, BLOCK, 0, c->stream>>>(flags, n, offs, map, out_idx);
-    int total = 0;
-    FVC_HIP(hipMemcpyAsync(&total, offs + nb, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    /* the count lands in pinned host memory (the status block's slot) */
+    FVC_HIP(hipMemcpyAsync(&c->h_status->count, offs + nb, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     FVC_HIP(hipFreeAsync(offs, c->stream));
     FVC_HIP(hipStreamSynchronize(c->stream));
     FVC_HIP(hipGetLastError());
-    *count = total;
+    *count = c->h_status->count;
     return FVC_OK;
 }
 
