@@ -15,6 +15,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <mutex>
 #include <thread>
@@ -48,6 +49,82 @@ int check_err(fvc_ctx *c, const char *what) {
         c->last_error = buf;
     }
     return code;
+}
+
+/* Device -> host copy of a large block into caller (pageable) memory.  The
+ * runtime moves pageable destinations through its own small staging buffer,
+ * about 12 GB/s on a 168 MB copy; DMA into pinned memory runs at the link's
+ * rate.  So the block goes in chunks into the context's pinned staging
+ * buffer (grown to the copy's size, at most STAGE_MAX; larger copies go in
+ * rounds), an event after each chunk, and host threads copy each chunk out,
+ * each its own stripe, as soon as its event has completed: the host copy
+ * overlaps the DMA of the chunks behind it; dst is complete on return.
+ * Blocks under STAGE_MIN, or all blocks when the pinned buffer cannot be
+ * had, are only enqueued on the stream as a direct copy: the caller
+ * synchronises the stream before reading them. */
+int copy_to_host(fvc_ctx *c, void *dst, const void *src, size_t bytes) {
+    constexpr size_t STAGE_MIN = (size_t)4 << 20, STAGE_MAX = (size_t)64 << 20, CHUNK = (size_t)8 << 20;
+    if (bytes == 0) return FVC_OK;
+    if (bytes >= STAGE_MIN && c->stage_bytes < std::min(bytes, STAGE_MAX)) {
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return FVC_E_HIP;
+        if (c->stage) hipHostFree(c->stage);
+        c->stage = nullptr;
+        c->stage_bytes = 0;
+        const size_t want = std::min(bytes, STAGE_MAX);
+        if (hipHostMalloc((void **)&c->stage, want, hipHostMallocDefault) == hipSuccess)
+            c->stage_bytes = want;
+        else
+            c->stage = nullptr;
+    }
+    if (bytes < STAGE_MIN || !c->stage) {
+        FVC_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+        return FVC_OK;
+    }
+    const size_t max_chunks = (c->stage_bytes + CHUNK - 1) / CHUNK;
+    while (c->stage_ev.size() < max_chunks) {
+        hipEvent_t ev;
+        FVC_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        c->stage_ev.push_back(ev);
+    }
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int nthr = (int)std::min(8u, std::max(1u, hw / 2));
+    for (size_t done = 0; done < bytes;) {
+        const size_t round = std::min(bytes - done, c->stage_bytes);
+        const size_t nch = (round + CHUNK - 1) / CHUNK;
+        for (size_t k = 0; k < nch; k++) {
+            const size_t off = k * CHUNK, len = std::min(CHUNK, round - off);
+            FVC_HIP(hipMemcpyAsync(c->stage + off, (const char *)src + done + off, len, hipMemcpyDeviceToHost,
+                                   c->stream));
+            FVC_HIP(hipEventRecord(c->stage_ev[k], c->stream));
+        }
+        std::atomic<int> failed{0};
+        auto worker = [&](int t) {
+            for (size_t k = 0; k < nch; k++) {
+                const size_t off = k * CHUNK, len = std::min(CHUNK, round - off);
+                const size_t lo = (len * t / nthr) & ~(size_t)63, hi = t + 1 == nthr ? len : (len * (t + 1) / nthr) & ~(size_t)63;
+                if (hipEventSynchronize(c->stage_ev[k]) != hipSuccess) {
+                    failed = 1;
+                    return;
+                }
+                if (hi > lo) memcpy((char *)dst + done + off + lo, c->stage + off + lo, hi - lo);
+            }
+        };
+        std::vector<std::thread> pool;
+        try {
+            for (int t = 1; t < nthr; t++) pool.emplace_back(worker, t);
+        } catch (...) { /* fewer threads: the calling thread copies the rest */
+        }
+        const int started = (int)pool.size() + 1;
+        worker(0);
+        for (auto &th : pool) th.join();
+        for (int t = started; t < nthr; t++) worker(t);
+        if (failed) {
+            c->last_error = "device to host copy failed";
+            return FVC_E_HIP;
+        }
+        done += round;
+    }
+    return FVC_OK;
 }
 
 void *scratch(fvc_ctx *c, size_t bytes) {
@@ -486,6 +563,8 @@ extern "C" void fvc_destroy(fvc_ctx *c) {
     hipFree(c->cell_elems);
     hipFree(c->d_status);
     if (c->h_status) hipHostFree(c->h_status);
+    if (c->stage) hipHostFree(c->stage);
+    for (hipEvent_t ev : c->stage_ev) hipEventDestroy(ev);
     if (c->scratch) hipFree(c->scratch);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
@@ -796,11 +875,11 @@ extern "C" int fvc_particles_get(fvc_particles *p, int64_t begin, fvc_point *rec
     std::vector<double> d(8 * (size_t)n);
     std::vector<int> iv(3 * (size_t)n);
     double *src[8] = {p->X[0], p->X[1], p->X[2], p->V[0], p->V[1], p->V[2], p->left_time, p->scalar};
-    for (int a = 0; a < 8; a++)
-        FVC_HIP(hipMemcpyAsync(d.data() + a * n, src[a] + begin, 8 * n, hipMemcpyDeviceToHost, c->stream));
+    int rc = FVC_OK;
+    for (int a = 0; a < 8 && !rc; a++) rc = copy_to_host(c, d.data() + a * n, src[a] + begin, 8 * (size_t)n);
     int *isrc[3] = {p->elem, p->aux, p->left};
-    for (int a = 0; a < 3; a++)
-        FVC_HIP(hipMemcpyAsync(iv.data() + a * n, isrc[a] + begin, 4 * n, hipMemcpyDeviceToHost, c->stream));
+    for (int a = 0; a < 3 && !rc; a++) rc = copy_to_host(c, iv.data() + a * n, isrc[a] + begin, 4 * (size_t)n);
+    if (rc) return rc;
     FVC_HIP(hipStreamSynchronize(c->stream));
     for (int64_t i = 0; i < n; i++) {
         for (int a = 0; a < 3; a++) {
@@ -1367,15 +1446,13 @@ int fvc::copy_ftle_kji(fvc_ctx *c, const double *d_wt, const double *d_wot, cons
     FVC_HIP(hipMallocAsync((void **)&tmp, sizeof(double) * 2 * n, c->stream));
     k_ftle_kji<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(d_wt, d_wot, res[0], res[1], res[2], tmp);
     hipError_t e = hipGetLastError();
-    if (e == hipSuccess && wt) e = hipMemcpyAsync(wt, tmp, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess && wot) e = hipMemcpyAsync(wot, tmp + n, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream);
+    int rc = e == hipSuccess ? FVC_OK : FVC_E_HIP;
+    if (!rc && wt) rc = copy_to_host(c, wt, tmp, sizeof(double) * n);
+    if (!rc && wot) rc = copy_to_host(c, wot, tmp + n, sizeof(double) * n);
     (void)hipFreeAsync(tmp, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    if (e != hipSuccess) {
-        c->last_error = std::string("FTLE output copy: ") + hipGetErrorString(e);
-        return FVC_E_HIP;
-    }
-    return FVC_OK;
+    if (!rc) rc = hipStreamSynchronize(c->stream) == hipSuccess ? FVC_OK : FVC_E_HIP;
+    if (rc) c->last_error = std::string("FTLE output copy: ") + hipGetErrorString(hipGetLastError());
+    return rc;
 }
 
 extern "C" int fvc_ftle_output(fvc_ftle *f, double int_t, double *wt, double *wot) {

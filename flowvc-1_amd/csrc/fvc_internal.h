@@ -59,6 +59,11 @@ struct fvc_ctx {
     /* scratch (grown on demand) */
     void *scratch;
     size_t scratch_bytes;
+    /* pinned staging of large device -> host copies (copy_to_host), grown on
+     * demand, with one event per chunk */
+    char *stage;
+    size_t stage_bytes;
+    std::vector<hipEvent_t> stage_ev;
     /* kernel timing (HIP events on `stream`) */
     int timing;
     int chain_replay; /* seed chains: 1 = always the point-by-point replay */
@@ -197,6 +202,7 @@ int launch_copy_probe(fvc_ctx *c, void *dst, const void *src, int64_t bytes);
 int copy_ftle_kji(fvc_ctx *c, const double *d_wt, const double *d_wot, const int32_t res[3], double *wt, double *wot);
 
 int check_err(fvc_ctx *c, const char *what);
+int copy_to_host(fvc_ctx *c, void *dst, const void *src, size_t bytes);
 /* release an object's device memory and detach it from its context (the
  * handle stays valid for its *_destroy call only); no-ops when detached */
 void particles_release(fvc_particles *p);

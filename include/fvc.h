@@ -132,7 +132,10 @@ typedef struct fvc_ftle fvc_ftle;
  * spatial hash.  Replaces the mesh globals of ompVC/globals.h:38-47.
  * Device memory of an unstructured mesh: 32 B per node, 96 B per element
  * (connectivity, adjacency and the 64-byte FP32 frame the reduced-precision
- * walks test against), plus the hash; at most 2^26 elements and 2^27 nodes. */
+ * walks test against), plus the hash; at most 2^26 elements and 2^27 nodes.
+ * Host memory: device -> host copies of 4 MB or more (fvc_particles_get,
+ * fvc_ftle_output) stage through up to 64 MB of pinned memory that the
+ * context allocates on first use and holds until fvc_destroy. */
 int fvc_create(const fvc_mesh_desc *mesh, const fvc_params *params, fvc_ctx **out);
 /* Frees the context; objects still alive on it are released and detached
  * (see Lifetimes above). */
