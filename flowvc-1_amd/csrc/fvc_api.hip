@@ -51,6 +51,22 @@ int check_err(fvc_ctx *c, const char *what) {
     return code;
 }
 
+int side_join(fvc_ctx *c) {
+    if (!c->side_pending) return FVC_OK;
+    if (hipStreamWaitEvent(c->stream, c->side_done, 0) != hipSuccess) {
+        c->last_error = "side stream join failed";
+        return FVC_E_HIP;
+    }
+    c->side_pending = 0;
+    return FVC_OK;
+}
+
+int side_sync(fvc_ctx *c) {
+    if (!c->side) return FVC_OK;
+    c->side_pending = 0;
+    return hipStreamSynchronize(c->side) == hipSuccess ? FVC_OK : FVC_E_HIP;
+}
+
 /* Device -> host copy of a large block into caller (pageable) memory.  The
  * runtime moves pageable destinations through its own small staging buffer,
  * about 12 GB/s on a 168 MB copy; DMA into pinned memory runs at the link's
@@ -427,7 +443,10 @@ extern "C" int fvc_create(const fvc_mesh_desc *md, const fvc_params *pp, fvc_ctx
         fvc_destroy(c);
         return FVC_E_NOMEM;
     }
-    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->side_ready, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->side_done, hipEventDisableTiming) != hipSuccess) {
         fvc_destroy(c);
         return FVC_E_HIP;
     }
@@ -541,6 +560,7 @@ extern "C" int fvc_create(const fvc_mesh_desc *md, const fvc_params *pp, fvc_ctx
 extern "C" void fvc_destroy(fvc_ctx *c) {
     if (!c) return;
     hipSetDevice(c->device);
+    side_sync(c);
     if (c->stream) hipStreamSynchronize(c->stream);
     /* objects still alive on the context: servers and batches first (they
      * hold particle sets), then FTLE slides, then particle sets.  Each release
@@ -568,6 +588,9 @@ extern "C" void fvc_destroy(fvc_ctx *c) {
     if (c->scratch) hipFree(c->scratch);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->side_ready) hipEventDestroy(c->side_ready);
+    if (c->side_done) hipEventDestroy(c->side_done);
+    if (c->side) hipStreamDestroy(c->side);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -614,6 +637,7 @@ extern "C" int fvc_copy_probe(fvc_ctx *c, int64_t bytes) {
 
 extern "C" int fvc_sync(fvc_ctx *c) {
     if (!c) return FVC_E_INVALID;
+    if (side_join(c)) return FVC_E_HIP;
     return hipStreamSynchronize(c->stream) == hipSuccess ? FVC_OK : FVC_E_HIP;
 }
 
@@ -647,6 +671,7 @@ static inline uint16_t to_f16(float v) {
 extern "C" int fvc_load_slice(fvc_ctx *c, int32_t slot, double timestamp, const double *uvw, int64_t n) {
     if (!c || !uvw || slot < 0 || slot >= c->max_slices) return FVC_E_INVALID;
     if (n != c->nvel) return set_err(c, FVC_E_INVALID, "slice length differs from the node count");
+    if (side_join(c)) return FVC_E_HIP; /* an early FTLE may still read the slot */
     if (!c->slices[slot]) {
         void *p = nullptr;
         if (hipMalloc(&p, c->vel_stride * (size_t)n) != hipSuccess) return set_err(c, FVC_E_NOMEM, "slice alloc");
@@ -686,6 +711,7 @@ extern "C" int fvc_load_strain_slice(fvc_ctx *c, int32_t slot, double timestamp,
     (void)timestamp; /* read but unused, as the reference's (strainrate.c:112-115) */
     if (!c || !sr || slot < 0 || slot >= c->max_slices) return FVC_E_INVALID;
     if (n != c->nsr) return set_err(c, FVC_E_INVALID, "strain-rate slice length differs from the element (cell) count");
+    if (side_join(c)) return FVC_E_HIP;
     if (!c->sr_slices[slot]) {
         if (hipMalloc((void **)&c->sr_slices[slot], sizeof(double) * (size_t)std::max<int64_t>(n, 1)) != hipSuccess)
             return set_err(c, FVC_E_NOMEM, "strain-rate slice alloc");
@@ -1361,7 +1387,13 @@ extern "C" int fvc_ftle_create_from(fvc_ctx *c, const double mn[3], const double
 void fvc::ftle_release(fvc_ftle *f) {
     fvc_ctx *c = f->ctx;
     if (!c) return;
+    side_sync(c);
     hipStreamSynchronize(c->stream);
+    for (void *&sb : f->snap)
+        if (sb) {
+            hipFree(sb);
+            sb = nullptr;
+        }
     if (f->buf) hipFree(f->buf);
     if (f->init_have) hipFree(f->init_have);
     if (f->init_w) hipFree(f->init_w);
@@ -1385,7 +1417,9 @@ extern "C" int fvc_ftle_reset(fvc_ftle *f) {
     if (!f || !f->ctx || !f->init_pts) return FVC_E_INVALID;
     fvc_ctx *c = f->ctx;
     size_t n = (size_t)f->n;
-    int rc = particles_d2d(f->pts, 0, f->init_pts, 0, f->n);
+    int rc = side_join(c);
+    if (rc) return rc;
+    rc = particles_d2d(f->pts, 0, f->init_pts, 0, f->n);
     if (rc) return rc;
     FVC_HIP(hipMemcpyAsync(f->have, f->init_have, sizeof(int) * n, hipMemcpyDeviceToDevice, c->stream));
     FVC_HIP(hipMemcpyAsync(f->wt, f->init_w, sizeof(double) * 2 * n, hipMemcpyDeviceToDevice, c->stream));
@@ -1457,7 +1491,8 @@ int fvc::copy_ftle_kji(fvc_ctx *c, const double *d_wt, const double *d_wot, cons
 
 extern "C" int fvc_ftle_output(fvc_ftle *f, double int_t, double *wt, double *wot) {
     if (!f || !f->ctx) return FVC_E_INVALID;
-    int rc = launch_ftle_fill(f, int_t);
+    int rc = side_join(f->ctx);
+    if (!rc) rc = launch_ftle_fill(f, int_t);
     return rc ? rc : copy_ftle_kji(f->ctx, f->wt, f->wot, f->res, wt, wot);
 }
 

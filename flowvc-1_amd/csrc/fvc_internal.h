@@ -64,6 +64,13 @@ struct fvc_ctx {
     char *stage;
     size_t stage_bytes;
     std::vector<hipEvent_t> stage_ev;
+    /* side stream: an FTLE slide frame's early FTLE (from a snapshot of its
+     * stencils) runs there beside the next frame's advection; side_done
+     * follows the last launch on it, side_pending until the main stream or
+     * the host has waited for it */
+    hipStream_t side;
+    hipEvent_t side_ready, side_done;
+    int side_pending;
     /* kernel timing (HIP events on `stream`) */
     int timing;
     int chain_replay; /* seed chains: 1 = always the point-by-point replay */
@@ -117,6 +124,10 @@ struct fvc_ftle {
     int *active;    /* compacted indices of points to integrate (Z-ordered) */
     int *zperm;     /* grid points in Z-order */
     int64_t nactive;
+    /* stencil snapshots of the early FTLE, used by alternate frames */
+    void *snap[2];
+    int64_t snap_cap;
+    int snap_sel;
     void *buf;
     int64_t inside;
     /* state right after creation, for fvc_ftle_reset */
@@ -202,6 +213,9 @@ int launch_copy_probe(fvc_ctx *c, void *dst, const void *src, int64_t bytes);
 int copy_ftle_kji(fvc_ctx *c, const double *d_wt, const double *d_wot, const int32_t res[3], double *wt, double *wot);
 
 int check_err(fvc_ctx *c, const char *what);
+/* order the main stream after the side stream's last launch / wait for it on the host */
+int side_join(fvc_ctx *c);
+int side_sync(fvc_ctx *c);
 int copy_to_host(fvc_ctx *c, void *dst, const void *src, size_t bytes);
 /* release an object's device memory and detach it from its context (the
  * handle stays valid for its *_destroy call only); no-ops when detached */

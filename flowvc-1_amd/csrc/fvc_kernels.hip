@@ -1339,6 +1339,54 @@ __device__ void advect_copy(const DevMesh &m, const DevField &f, const FtleDev &
     advect<MODE, DIM, UNS, GEN>(m, f, s, t1, t2, itype, dt, 0, err, ns);
 }
 
+/* Frame-start state of the stencils of one frame's triggered points, taken
+ * before the frame's position update so that their early FTLE can run on a
+ * second stream beside the next frame's advection.  Slot q = 0-5 is
+ * neighbour nb[q] (grid_nbrs), q = 6 the point itself; elem is the element
+ * advect_copy would start from (the loop-order rule already applied). */
+struct EarlySnap {
+    int *nt;          /* triggered points of the frame (also when above cap) */
+    int *P;
+    double *te;
+    double *X;        /* [(q * 3 + c) * cap + t] */
+    int *left, *elem; /* [q * cap + t] */
+    int64_t cap;
+};
+
+static size_t early_snap_bytes(int64_t cap) { return 256 + (size_t)cap * (4 + 8 + 7 * (24 + 4 + 4)); }
+
+static EarlySnap early_snap_view(void *buf, int64_t cap) {
+    EarlySnap s;
+    char *b = (char *)buf;
+    s.nt = (int *)b;
+    b += 256;
+    s.X = (double *)b;
+    b += sizeof(double) * 21 * (size_t)cap;
+    s.te = (double *)b;
+    b += sizeof(double) * (size_t)cap;
+    s.P = (int *)b;
+    b += sizeof(int) * (size_t)cap;
+    s.left = (int *)b;
+    b += sizeof(int) * 7 * (size_t)cap;
+    s.elem = (int *)b;
+    s.cap = cap;
+    return s;
+}
+
+template <int MODE, int DIM, int UNS, int GEN>
+__device__ void advect_snap(const DevMesh &m, const DevField &f, const EarlySnap &sn, int q, int64_t t, double t1,
+                            double t2, int itype, double dt, PState &s, int &err) {
+    for (int c = 0; c < 3; c++) {
+        s.X[c] = sn.X[(q * 3 + c) * sn.cap + t];
+        s.V[c] = 0.0;
+    }
+    s.left = sn.left[q * sn.cap + t];
+    s.elem = sn.elem[q * sn.cap + t];
+    s.left_time = 0.0;
+    uint32_t ns = 0;
+    advect<MODE, DIM, UNS, GEN>(m, f, s, t1, t2, itype, dt, 0, err, ns);
+}
+
 __device__ inline void ftle_value(double D[3][3], double dt_int, double &wt, double &wot, int &err, bool zero_time) {
 #pragma clang fp contract(off)
     double C[3][3];
@@ -1426,6 +1474,36 @@ __global__ __launch_bounds__(BLOCK) void k_early_flag(FtleDev d, int64_t n, int 
     flags[P] = fl;
 }
 
+/* the stencil snapshot of the first cap triggered points (EarlySnap) */
+__global__ __launch_bounds__(BLOCK) void k_early_snap(FtleDev d, const int *__restrict__ list,
+                                                     const int *__restrict__ d_nt, EarlySnap s) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int nt = *d_nt;
+    if (t == 0) *s.nt = nt;
+    if (t >= nt || t >= s.cap) return;
+    const int64_t P = list[t];
+    int cc[3];
+    int64_t nb[6];
+    bool has[6];
+    grid_nbrs(d, P, cc, nb, has);
+    const int64_t trig = early_trigger(d, P, nb, has);
+    s.P[t] = (int)P;
+    s.te[t] = d.exit_time[trig];
+    for (int q = 0; q < 7; q++) {
+        const int64_t idx = q < 6 ? nb[q] : P;
+        const bool ok = q == 6 || has[q];
+        for (int c = 0; c < 3; c++) s.X[(q * 3 + c) * s.cap + t] = ok ? d.p.X[c][idx] : 0.0;
+        int left = 0, elem = -1;
+        if (ok) {
+            left = d.p.left[idx];
+            const bool updated = idx < trig && d.advected[idx] && !d.exited[idx];
+            elem = updated ? d.elem_new[idx] : d.p.elem[idx];
+        }
+        s.left[q * s.cap + t] = left;
+        s.elem[q * s.cap + t] = elem;
+    }
+}
+
 /* One triggered point per group of eight lanes (grid-stride over the list,
  * whose length is read from device memory):
  *   lanes 0-5 advect the six neighbours to te -> their exit times; every lane
@@ -1434,22 +1512,39 @@ __global__ __launch_bounds__(BLOCK) void k_early_flag(FtleDev d, int64_t n, int 
  *   (even) neighbour along each axis that the difference formula uses, to
  *   MinExitTime; lane 0 gathers the positions, forms the gradient (one-sided
  *   at the grid edges, ftle.c:994-1020) and the FTLE. */
-template <int MODE, int DIM, int UNS, int GEN>
+template <int MODE, int DIM, int UNS, int GEN, int SNAP>
 __device__ __forceinline__ void early_body(const DevMesh &m, const DevField &f, const FtleDev &d,
-                                           const int *__restrict__ list, const int *__restrict__ d_nt, double t1,
-                                           int itype, double dt, ErrBuf eb) {
+                                           const int *__restrict__ list, const int *__restrict__ d_nt,
+                                           const EarlySnap &sn, int64_t above, double t1, int itype, double dt,
+                                           ErrBuf eb) {
 #pragma clang fp contract(off)
-    const int nt = *d_nt;
+    /* live input (list, d_nt, the slide's state): runs when the frame has
+     * more than `above` triggered points; snapshot input: when it has at
+     * most sn.cap (the two launches of a frame split the cases) */
+    int nt;
+    if constexpr (SNAP) {
+        nt = *sn.nt;
+        if (nt > sn.cap) return;
+    } else {
+        nt = *d_nt;
+        if (nt <= above) return;
+    }
     const int sub = threadIdx.x & 7;
     const int64_t ngroups = ((int64_t)gridDim.x * blockDim.x) >> 3;
     for (int64_t t = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 3; t < nt; t += ngroups) {
-        const int64_t P = list[t];
+        const int64_t P = SNAP ? (int64_t)sn.P[t] : (int64_t)list[t];
         int cc[3];
         int64_t nb[6];
         bool has[6];
         grid_nbrs(d, P, cc, nb, has);
-        const int64_t trig = early_trigger(d, P, nb, has);
-        const double te = d.exit_time[trig];
+        int64_t trig = 0;
+        double te;
+        if constexpr (SNAP) {
+            te = sn.te[t];
+        } else {
+            trig = early_trigger(d, P, nb, has);
+            te = d.exit_time[trig];
+        }
         int err = 0;
         double mt = te;
         double X[3] = {0.0, 0.0, 0.0};
@@ -1457,18 +1552,22 @@ __device__ __forceinline__ void early_body(const DevMesh &m, const DevField &f, 
         for (int phase = 0; phase < 2; phase++) {
             bool need;
             int64_t who;
+            int q; /* snapshot slot of `who` */
             if (phase == 0) {
                 need = sub < 6 && has[sub];
                 who = sub < 6 ? nb[sub] : P;
+                q = sub < 6 ? sub : 6;
             } else {
                 need = sub == 0;
                 who = P;
+                q = 6;
                 if (sub >= 1 && sub <= 6) {
                     const int ax = (sub - 1) >> 1, side = (sub - 1) & 1;
                     if (ax < DIM) {
                         const int last = d.res[ax] - 1;
                         need = !(side == 0 && cc[ax] == 0) && !(side == 1 && cc[ax] != 0 && cc[ax] == last);
                         who = nb[2 * ax + side];
+                        q = 2 * ax + side;
                     }
                 }
             }
@@ -1476,7 +1575,10 @@ __device__ __forceinline__ void early_body(const DevMesh &m, const DevField &f, 
             s.left = 0;
             s.left_time = 0.0;
             if (need) {
-                advect_copy<MODE, DIM, UNS, GEN>(m, f, d, who, trig, t1, phase == 0 ? te : mt, itype, dt, s, err);
+                if constexpr (SNAP)
+                    advect_snap<MODE, DIM, UNS, GEN>(m, f, sn, q, t, t1, phase == 0 ? te : mt, itype, dt, s, err);
+                else
+                    advect_copy<MODE, DIM, UNS, GEN>(m, f, d, who, trig, t1, phase == 0 ? te : mt, itype, dt, s, err);
                 for (int c = 0; c < 3; c++) X[c] = s.X[c];
             }
             if (phase == 0) {
@@ -1521,12 +1623,19 @@ __device__ __forceinline__ void early_body(const DevMesh &m, const DevField &f, 
     }
 }
 
-/* RK4 frames */
+/* RK4 frames, live input: the frame's points when it has more than `above` */
 template <int MODE, int DIM, int UNS>
 __global__ __launch_bounds__(BLOCK) void k_early(DevMesh m, DevField f, FtleDev d, const int *__restrict__ list,
-                                                const int *__restrict__ d_nt, double t1, int itype, double dt,
-                                                ErrBuf eb) {
-    early_body<MODE, DIM, UNS, 0>(m, f, d, list, d_nt, t1, itype, dt, eb);
+                                                const int *__restrict__ d_nt, int64_t above, double t1, int itype,
+                                                double dt, ErrBuf eb) {
+    early_body<MODE, DIM, UNS, 0, 0>(m, f, d, list, d_nt, EarlySnap{}, above, t1, itype, dt, eb);
+}
+
+/* RK4 frames, from the stencil snapshot (at most cap triggered points) */
+template <int MODE, int DIM, int UNS>
+__global__ __launch_bounds__(BLOCK) void k_early_snapped(DevMesh m, DevField f, FtleDev d, EarlySnap sn, double t1,
+                                                        int itype, double dt, ErrBuf eb) {
+    early_body<MODE, DIM, UNS, 0, 1>(m, f, d, nullptr, nullptr, sn, 0, t1, itype, dt, eb);
 }
 
 /* Euler frames */
@@ -1534,7 +1643,7 @@ template <int MODE, int DIM, int UNS>
 __global__ __launch_bounds__(BLOCK) void k_early_gen(DevMesh m, DevField f, FtleDev d, const int *__restrict__ list,
                                                     const int *__restrict__ d_nt, double t1, int itype, double dt,
                                                     ErrBuf eb) {
-    early_body<MODE, DIM, UNS, 1>(m, f, d, list, d_nt, t1, itype, dt, eb);
+    early_body<MODE, DIM, UNS, 1, 0>(m, f, d, list, d_nt, EarlySnap{}, -1, t1, itype, dt, eb);
 }
 
 /* GetFTLEForPoint (ftle.c:1050-1203) */
@@ -1633,11 +1742,76 @@ int launch_ftle_early(fvc_ctx *c, const FtleDev &d, int64_t n, const DevField &f
     const unsigned grid = (unsigned)(((int64_t)groups * 8 + BLOCK - 1) / BLOCK);
     if (itype == INT_RK4)
         FVC_DISPATCH(c->precision, c->mesh.dims, c->mesh.mesh_type, k_early,
-                     <<<grid, BLOCK, 0, c->stream>>>(c->mesh, fld, d, elist, offs + nb, t1, itype, dt, errbuf(c)));
+                     <<<grid, BLOCK, 0, c->stream>>>(c->mesh, fld, d, elist, offs + nb, (int64_t)-1, t1, itype, dt,
+                                                     errbuf(c)));
     else
         FVC_DISPATCH(c->precision, c->mesh.dims, c->mesh.mesh_type, k_early_gen,
                      <<<grid, BLOCK, 0, c->stream>>>(c->mesh, fld, d, elist, offs + nb, t1, itype, dt, errbuf(c)));
     FVC_HIP(hipGetLastError());
+    return FVC_OK;
+}
+
+/* Early FTLE of an RK4 slide frame, overlapped with what follows it.  On the
+ * context's stream: the triggered list, then the frame-start state of the
+ * first EARLY_CAP triggered points' stencils into one of the slide's two
+ * snapshot buffers (they alternate by frame), then the live kernel, which
+ * runs only when the frame triggered more points than that.  On the side
+ * stream, after the snapshot: the early FTLE from the snapshot, which reads
+ * nothing the position update or the next frame's advection writes, and
+ * writes only have / wt / wot of the triggered points.  Everything that
+ * reads those or changes the slices joins the side stream first (side_join);
+ * so does the next frame, before its triggered list, which also keeps a
+ * snapshot buffer from being rewritten while its early FTLE still reads it. */
+int launch_ftle_early_async(fvc_ftle *fo, const FtleDev &d, const DevField &fld, double t1, int itype, double dt) {
+    fvc_ctx *c = fo->ctx;
+    const int64_t n = fo->n;
+    constexpr int64_t EARLY_CAP = 1 << 18;
+    if (itype != INT_RK4) {
+        int rc = side_join(c);
+        return rc ? rc : launch_ftle_early(c, d, n, fld, t1, itype, dt);
+    }
+    if (!fo->snap[0]) {
+        const int64_t cap = std::min<int64_t>(n, EARLY_CAP);
+        const size_t bytes = early_snap_bytes(cap);
+        if (hipMalloc(&fo->snap[0], bytes) != hipSuccess) fo->snap[0] = nullptr;
+        if (fo->snap[0] && hipMalloc(&fo->snap[1], bytes) != hipSuccess) {
+            hipFree(fo->snap[0]);
+            fo->snap[0] = fo->snap[1] = nullptr;
+        }
+        if (!fo->snap[0]) {
+            (void)hipGetLastError();
+            c->last_error = "early FTLE snapshot alloc";
+            return FVC_E_NOMEM;
+        }
+        fo->snap_cap = cap;
+        c->bytes_used += 2 * (int64_t)bytes;
+    }
+    int rc = side_join(c);
+    if (rc) return rc;
+    const int nb = (int)((n + CMP_TILE - 1) / CMP_TILE);
+    int *eflags = (int *)scratch(c, sizeof(int) * (2 * (size_t)n + nb + 1));
+    if (!eflags) return FVC_E_NOMEM;
+    int *elist = eflags + n, *offs = elist + n;
+    k_early_flag<<<grid_for(n, BLOCK), BLOCK, 0, c->stream>>>(d, n, eflags);
+    k_cmp_count<<<nb, BLOCK, 0, c->stream>>>(eflags, n, offs);
+    k_scan_small<<<1, 1024, 0, c->stream>>>(offs, nb);
+    k_cmp_scatter<<<nb, BLOCK, 0, c->stream>>>(eflags, n, offs, nullptr, elist);
+    const EarlySnap sn = early_snap_view(fo->snap[fo->snap_sel], fo->snap_cap);
+    k_early_snap<<<grid_for(fo->snap_cap, BLOCK), BLOCK, 0, c->stream>>>(d, elist, offs + nb, sn);
+    const unsigned groups = (unsigned)std::min<int64_t>(std::max<int64_t>(n / 8, 1), 16384);
+    const unsigned grid = (unsigned)(((int64_t)groups * 8 + BLOCK - 1) / BLOCK);
+    FVC_DISPATCH(c->precision, c->mesh.dims, c->mesh.mesh_type, k_early,
+                 <<<grid, BLOCK, 0, c->stream>>>(c->mesh, fld, d, elist, offs + nb, fo->snap_cap, t1, itype, dt,
+                                                 errbuf(c)));
+    FVC_HIP(hipGetLastError());
+    FVC_HIP(hipEventRecord(c->side_ready, c->stream));
+    FVC_HIP(hipStreamWaitEvent(c->side, c->side_ready, 0));
+    FVC_DISPATCH(c->precision, c->mesh.dims, c->mesh.mesh_type, k_early_snapped,
+                 <<<grid, BLOCK, 0, c->side>>>(c->mesh, fld, d, sn, t1, itype, dt, errbuf(c)));
+    FVC_HIP(hipGetLastError());
+    FVC_HIP(hipEventRecord(c->side_done, c->side));
+    c->side_pending = 1;
+    fo->snap_sel ^= 1;
     return FVC_OK;
 }
 
@@ -1701,7 +1875,7 @@ int launch_ftle_frame(fvc_ftle *f, int lo, int hi, double tmin, double tmax, dou
                                                                                 errbuf(c), c->d_steps));
         timing_end(c);
         FVC_HIP(hipGetLastError());
-        if (!ip.extrapolate && (rc = launch_ftle_early(c, d, f->n, fld, t1, itype, ip.time_step))) return rc;
+        if (!ip.extrapolate && (rc = launch_ftle_early_async(f, d, fld, t1, itype, ip.time_step))) return rc;
         if ((rc = launch_ftle_update_all(c, d, f->n, ip.extrapolate))) return rc;
     }
     rc = check_err(c, "fvc_ftle_frame");
