@@ -13,7 +13,12 @@
  * names it.  A caller keeping ompVC's behaviour turns non-zero into exit(1).
  * Threading: one host thread per context; calls on one context are ordered on
  * the context's stream, and every call returns when its device work is done
- * (fvc_server_poll returns with one tick of the server enqueued).
+ * (fvc_server_poll returns with one tick of the server enqueued), except the
+ * early FTLE of an RK4 fvc_ftle_frame: it runs on the context's second
+ * stream beside what follows, and every call that depends on it (the next
+ * frame, fvc_ftle_output / _reset / _destroy, fvc_load_slice,
+ * fvc_load_strain_slice, fvc_sync, fvc_destroy) waits for it first; an error
+ * it raises is returned by a later call on the context.
  *
  * Lifetimes: particle sets, FTLE slides, batches and servers belong to the
  * context they were created on.  They may be destroyed in any order, before
@@ -250,7 +255,10 @@ void fvc_ftle_destroy(fvc_ftle *f);
  * not yet launched, advect, early FTLE for exits (ftle.c:788-1048) with the
  * reference's loop-order semantics, position update.  *done receives 0 when
  * the slide was not integrated in this interval (complete, or not launched
- * yet), 1 when it was, 2 when it was and reached its stop time. */
+ * yet), 1 when it was, 2 when it was and reached its stop time.  With RK4 the
+ * early FTLE reads a snapshot of the triggered points' stencils (240 B per
+ * point in two buffers of up to 2^18 points, allocated at the first frame)
+ * and overlaps the next frame's advection (see Threading). */
 int fvc_ftle_frame(fvc_ftle *f, int32_t direction, int32_t lo, int32_t hi, double tmin, double tmax,
                    const fvc_int_params *ip, int32_t *done);
 /* Missing values via GetFTLEForPoint (ftle.c:1050-1203) with integration time
