@@ -60,3 +60,37 @@ def test_slot_reload_waits_for_the_early_ftle(pipe, precision):
     assert left_a == left_b and left_a > 1000, (left_a, left_b)
     np.testing.assert_array_equal(wt_a, wt_b)
     np.testing.assert_array_equal(wot_a, wot_b)
+
+
+def test_frames_beyond_the_snapshot_cap_match_the_server(pipe):
+    """A frame that triggers more than 2^18 points takes the live launch on
+    the main stream instead of the snapshot.  A thin grid at the outlet loses
+    most of its 500k inside points in the first frame; its FTLE must equal
+    the same grid served as a request (the server's early FTLE always reads
+    the live state)."""
+    xyz, conn, adj, slices = pipe
+    ctx = fvc.Context(3, PIPE_BOUNDS, "f32", NF + 1, coords=xyz, conn=conn, adj=adj)
+    for s in range(NF + 1):
+        ctx.load_slice(s, s * FDT, slices[s])
+    mn, mx, res = [-0.99, -0.99, 9.99], [0.99, 0.99, 10.0], [128, 128, 40]
+    ts, te = 0.0, 3 * FDT
+    print("outlet FTLE slide, %d points" % (128 * 128 * 40), flush=True)
+    F = fvc.FtleSlide(ctx, mn, mx, res, 1, ts, te)
+    outside0 = F.records()["LeftDomain"] != 0
+    F.frame(1, 0, 1, 0.0, FDT, 1e-3)
+    left0 = int(((F.records()["LeftDomain"] != 0) & ~outside0).sum())
+    assert left0 > 300_000, left0  # each of them triggers its own early FTLE
+    for f in range(1, 3):
+        F.frame(1, f, f + 1, f * FDT, (f + 1) * FDT, 1e-3)
+    wt, wot = F.output(te - ts)
+    F.close()
+    print("the same grid as a served request", flush=True)
+    S = fvc.Server(ctx, 0.0, FDT, NF + 1, 1e-3, pool_particles=128 * 128 * 40, steps_per_launch=16)
+    rid = S.submit_ftle(mn, mx, res, ts, te)
+    S.run_until_idle()
+    assert S.info(rid)["state"] == fvc.RS_DONE
+    swt, swot = S.take(rid)
+    S.close()
+    ctx.close()
+    np.testing.assert_array_equal(wt, swt)
+    np.testing.assert_array_equal(wot, swot)
