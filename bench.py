@@ -714,6 +714,26 @@ def bench_dg(args, np, fvc, host):
                                "host": host_info(),
                                "sample": "%d of the 512x256 FTLE grid points (every 16th), T=10, FP64 oracle"
                                          % pts.shape[0]}
+        # ompVC as configured for this deck: its FTLE loop parallelises over
+        # output slides (UPPERLOOP, parameters.c:400-401, ftle.c:68), and one
+        # slide runs on one thread
+        pts1 = np.zeros(512 * 256 // 64, oracle.POINT_DTYPE)
+        pts1["X"][:, :2] = g.reshape(-1, 2)[::64]
+        t_one, steps_one = 0.0, 0
+        for f in range(20):
+            om.set_slot(0, slices[f])
+            om.set_slot(1, slices[f + 1])
+            om.set_interval(0.5 * f, 0.5 * (f + 1))
+            t1 = time.perf_counter()
+            rc = om.advect(pts1, 0.5 * f, 0.5 * (f + 1), 0.01, nthreads=1, chunk=pts1.shape[0])
+            t_one += time.perf_counter() - t1
+            if rc:
+                raise RuntimeError("oracle advect failed (%d)" % rc)
+            steps_one += om.last_steps.value
+        out["cpu_baseline_as_configured"] = {
+            "value": steps_one / t_one, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": "%d of the 512x256 FTLE grid points (every 64th), T=10, FP64 oracle on one thread, as "
+                      "ompVC runs one FTLE slide" % pts1.shape[0]}
     print(json.dumps(out), flush=True)
 
 
