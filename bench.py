@@ -86,8 +86,8 @@ def roofline_valu(kernel, steps_per_launch, avg_ms, bpp, default_workload):
     """Roofline of the dominant RK4 kernel against the nearest ceiling.
     Counter passes on the FP32 pipe workload (profiles/pmc_valu_pipe.md,
     pmc_l2_pipe.md, pmc_traffic_pipe.md, pmc_wait_pipe.md) put VALU issue
-    highest (64-68% busy), L2-to-L1 reads at 29-38% of the L2's bandwidth
-    and HBM near 3%; the waves wait on their own gathers more than half of
+    highest (64-68% busy), L2-to-L1 reads at 28-37% of the L2's bandwidth
+    and HBM at 2-4%; the waves wait on their own gathers more than half of
     their cycles (the limiter, reported beside the roofline).  The bound
     reported is VALU issue: achieved = VALU busy SIMD-cycles per
     particle-step (SQ_ACTIVE_INST_VALU x 4 over the particle-steps of the
@@ -209,12 +209,13 @@ def ctx_sync(ctx):
     fvc.lib().fvc_sync(ctx.h)
 
 
-def cpu_baseline(args, np, xyz, conn, adj, slices, recs_sample):
+def cpu_baseline(args, np, xyz, conn, adj, slices, recs_sample, threads=None):
     """The oracle (CPU FP64 restatement, ompVC semantics, OpenMP dynamic chunks
-    as tracers.c:222-223) on a bounded sample of the tracers."""
+    as tracers.c:222-223) on a bounded sample of the tracers, on every host
+    core, or on `threads` of them."""
     sys.path.insert(0, os.path.join(HERE, "oracle"))
     import oracle
-    cores = host_cores()
+    cores = threads or host_cores()
     om = oracle.Mesh(3, [-1.0, 1.0, -1.0, 1.0, 0.0, 10.0], coords=xyz, conn=conn, adj=adj)
     pts = np.ascontiguousarray(recs_sample.copy())
     chunk = max(1, pts.shape[0] // cores)
@@ -917,6 +918,15 @@ def main():
                                             "exit_agreement": float((gpu_left[sample_idx] ==
                                                                      pts["LeftDomain"]).mean())}
         result["speedup_vs_cpu"] = (tr_steps / tr_t) / cb["value"] if tr_t > 0 else None
+        # ompVC as configured for the FTLE slide: its FTLE loop parallelises
+        # over output slides (UPPERLOOP, parameters.c:400-401), one slide here
+        # runs on one thread; a sixteenth of the sample on one core
+        c1 = cpu_baseline(args, np, xyz, conn, adj, slices, recs[sample_idx[::16]], threads=1)
+        result["cpu_baseline_as_configured"] = {
+            "value": c1["value"], "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": "%d tracers (every 16th of the all-core sample), one period, integrate-only time %.1f s, one "
+                      "thread, as ompVC runs the FTLE slide" % (c1["pts"].shape[0], c1["seconds"])}
+        result["ftle_speedup_vs_cpu_as_configured"] = (ft_steps / ft_t) / c1["value"] if ft_t > 0 else None
     if not args.no_f64_check and args.precision != "f64":
         t0 = time.time()
         red = P.get()
