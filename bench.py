@@ -268,7 +268,8 @@ def bench_batch(args, np, fvc, host):
     for r, c in enumerate(counts):
         sd = host.tube_seeds(tube, int(c), 0.0, 10.0 / 3.0, 1000 + r)
         sd = sd[ctx.zsort(sd, lo, hi)]
-        recs.append(ctx.seed_list(sd, 1))
+        r_recs = ctx.seed_list(sd, 1)
+        recs.append(element_order(np, r_recs) if args.batch_order == "element" else r_recs)
     recs = np.concatenate(recs)
     rel = release_table(counts, starts, horizons, types)
     n = recs.shape[0]
@@ -336,6 +337,8 @@ def bench_batch(args, np, fvc, host):
                         "periods, RK4/AB3 mix, dt=1e-3, autotuned block/steps-per-launch, %s" %
                         (conn.shape[0], len(rel), n, "hipGraph replay" if args.batch_graph else "direct launches"),
             "releases": len(rel), "seeds": n, "precision": args.precision,
+            "particle_order": "each release's seeds by the element holding each" if args.batch_order == "element"
+            else "each release's seeds in Z-order",
         },
         "batch": {"tuned": best, "tune_ms_eighth_batch": best_ms, "launch_groups": launches,
                   "particle_steps_per_run": tot_steps // K},
@@ -775,6 +778,9 @@ def main():
                     help="also run every request alone (batch-at-a-time comparison)")
     ap.add_argument("--releases", type=int, default=64)
     ap.add_argument("--batch-cpu-stride", type=int, default=10)
+    ap.add_argument("--batch-order", default="element", choices=["element", "zorder"],
+                    help="batch workload: each release's seeds in Z-order, then ordered by the element holding each "
+                         "(as the pipe tracers), or in Z-order only")
     ap.add_argument("--batch-config", default="",
                     help="batch workload: 'block,steps_per_launch' to run without the autotuner sweep")
     ap.add_argument("--batch-graph", type=int, default=1, choices=[0, 1],
