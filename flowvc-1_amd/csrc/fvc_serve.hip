@@ -701,6 +701,43 @@ static int set_remaining(fvc_server *S, int slot, int64_t n) {
     return FVC_OK;
 }
 
+/* sort keys of the pool slots: the element holding each slot's seed, seeds
+ * outside the mesh last */
+__global__ void k_srv_elem_keys(const int *__restrict__ elem, const int *__restrict__ perm, int64_t n,
+                                uint64_t *__restrict__ key, int *__restrict__ val) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int p = perm[k];
+    const int e = elem[p];
+    key[k] = e < 0 ? (uint64_t)1 << 31 : (uint64_t)(uint32_t)e;
+    val[k] = p;
+}
+
+/* A request's pool order: Z-order of its seeds, then (stable) the element
+ * holding each, so that the lanes of a wave gather from neighbouring
+ * elements (the mesh is numbered along its own curve when renumbered);
+ * results return in input order through perm either way. */
+static int order_by_element(fvc_ctx *c, const int *elem, int *perm, int64_t n) {
+    if (n <= 1) return FVC_OK;
+    uint64_t *k[2] = {nullptr, nullptr};
+    int *v[2] = {nullptr, nullptr};
+    FVC_HIP(hipMallocAsync((void **)&k[0], sizeof(uint64_t) * n, c->stream));
+    FVC_HIP(hipMallocAsync((void **)&k[1], sizeof(uint64_t) * n, c->stream));
+    FVC_HIP(hipMallocAsync((void **)&v[0], sizeof(int) * n, c->stream));
+    FVC_HIP(hipMallocAsync((void **)&v[1], sizeof(int) * n, c->stream));
+    k_srv_elem_keys<<<grid256(n), 256, 0, c->stream>>>(elem, perm, n, k[0], v[0]);
+    int which = 0;
+    int rc = hipGetLastError() == hipSuccess ? FVC_OK : FVC_E_HIP;
+    if (!rc) rc = radix_sort_pairs(c->stream, k[0], v[0], k[1], v[1], n, 32, &which);
+    if (!rc && hipMemcpyAsync(perm, v[which], sizeof(int) * n, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+        rc = FVC_E_HIP;
+    for (int a = 0; a < 2; a++) {
+        (void)hipFreeAsync(k[a], c->stream);
+        (void)hipFreeAsync(v[a], c->stream);
+    }
+    return rc;
+}
+
 static int admit(fvc_server *S, HostReq &R) {
     fvc_ctx *c = S->ctx;
     int rc;
@@ -724,6 +761,7 @@ static int admit(fvc_server *S, HostReq &R) {
         double lo[3] = {c->mesh.bounds[0], c->mesh.bounds[2], c->mesh.bounds[4]};
         double hi[3] = {c->mesh.bounds[1], c->mesh.bounds[3], c->mesh.bounds[5]};
         if (!rc) rc = launch_zsort(c, dx, n, lo, hi, dperm);
+        if (!rc && c->mesh.mesh_type == 1) rc = order_by_element(c, de, dperm, n);
         if (!rc) {
             k_srv_tracer_init<<<grid256(n), 256, 0, c->stream>>>(ptrs(S->pool), R.base, n, dx, dperm, de, dl, dlt,
                                                                 S->pool->aux, S->pool->scalar, S->req_id, S->frame,
